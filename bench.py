@@ -1,0 +1,167 @@
+"""Throughput of the explicit finite-volume step on one MI355X.
+
+Workload (default): wet/dry circular dam break on a 16M-triangle seeded
+Delaunay mesh of [0, 1]^2 (n = 2828 jittered grid, Morton-ordered), random
+Fourier bathymetry (64 modes, amplitude 0.1), h = 1 inside r < 0.2, h = 0.1
+outside, 5 % dry pockets; FP64, HLLC<Einfeldt>, SSP-RK2, CFL mode 0.15.
+
+One step = one SSP-RK2 time step of every cell; value = cell-updates/s with
+the state resident in HBM.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 2828]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+# Compulsory bytes per cell-update of SSP-RK2 (state in/out per stage, U^n in
+# stage 2, int32 cell indices, node records at V/T ~ 0.5).
+STEP_BYTES_PER_CELL = 192.0
+
+
+def kernel_alg_bytes(T, E, V):
+    """Algorithmic bytes per launch of each kernel: every array the kernel
+    must touch, read or written once (int32 indices, FP64 values)."""
+    nodes = 24.0 * V
+    return {
+        # w, tp, tt, cflag, nodes in; MUSCL (w0, gx, gy) + class out
+        "recon1": T * (8 + 12 + 12 + 4) + nodes + T * (24 + 1),
+        # class of every cell (pass 2 itself is sparse)
+        "recon2": T * 1.0,
+        # edge ends/cells, per-cell MUSCL + class + w, u, v + tp, nodes in; F out
+        "flux": E * 16 + T * (24 + 1 + 24 + 12) + nodes + E * 24,
+        # w, tp, te, cflag, F0, nodes in; dtd out
+        "drain": T * (8 + 12 + 12 + 4) + E * 8 + nodes + T * 8,
+        # state (+ U0), tp, te, tt, cflag, MUSCL, class, F, dtd, nodes in; state out
+        "update": T * (24 + 24 + 36 + 4 + 24 + 1 + 8) + E * 24 + nodes + T * 24,
+    }
+
+
+KERNEL_NAMES = ["recon1", "recon2", "flux", "drain", "update"]
+
+
+def load_traffic(workload, kernel):
+    """HBM bytes per launch from committed rocprofv3 --pmc summaries, if any."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    ent = data.get(workload, {}).get(kernel)
+    if not ent:
+        return None
+    return ent.get("bytes_per_launch")
+
+
+def cpu_baseline(dom, prim, flux, wave, seconds_hint=""):
+    """The oracle (scalar C++ restatement, 1 thread) on the same state: one
+    SSP-RK2 step in CFL mode."""
+    from oracle import oracle as O
+    orc = O.OracleSolver(dom.geometry, dom.topology.tables(), flux=flux, wave=wave)
+    orc.set_state(prim)
+    el = orc.time_run("SSPRK2", 1, dt=0.0, cfl=0.15)
+    T = prim.shape[0]
+    return {"value": T / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"1 SSP-RK2 step (CFL mode) of the same {T}-cell state, "
+                      f"oracle C++ restatement -O3 -ffp-contract=off, 1 thread, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=2828, help="jittered grid size (T = 2 n^2)")
+    ap.add_argument("--kernel-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-only", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import swe_fvm_amd as swe
+    from swe_fvm_amd import workloads as W
+
+    t0 = time.time()
+    dom, prim = W.dam_break(n=args.n, seed=3)
+    T = prim.shape[0]
+    E = dom.topology.NumEdges()
+    V = dom.topology.NumNodes()
+    print(f"[bench] mesh+state {T} cells {E} edges {V} nodes in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    workload = f"dam_break_T{T}"
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(dom, prim, "HLLC", "Einfeldt")
+        print(f"[bench] cpu baseline {cpu['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
+    if args.cpu_only:
+        print(json.dumps({"cpu_baseline": cpu}))
+        return
+
+    fl = swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt]
+    sd = swe.SpaceDisc(fl, dom, prim, device=0)
+    if args.warmup:
+        sd.run("SSPRK2", args.warmup, dt=0.0, cfl=0.15)
+    sd.synchronize()
+    t1 = time.perf_counter()
+    ms = sd.time_run("SSPRK2", args.steps, dt=0.0, cfl=0.15)
+    sd.synchronize()
+    wall = time.perf_counter() - t1
+    ms_per_step = ms / args.steps
+    value = T * args.steps / (ms * 1e-3)
+    print(f"[bench] {args.steps} steps {ms:.1f} ms (wall {wall * 1e3:.1f} ms)", file=sys.stderr, flush=True)
+
+    # per-kernel average launch durations (events bracket each launch)
+    kms_total, kms = sd.time_run("SSPRK2", args.kernel_steps, dt=0.0, cfl=0.15, per_kernel=True)
+    alg = kernel_alg_bytes(T, E, V)
+    per = {k: {"avg_ms": float(kms[i]), "alg_bytes": alg[k],
+               "gbs": alg[k] / (kms[i] * 1e-3) / 1e9 if kms[i] > 0 else None}
+           for i, k in enumerate(KERNEL_NAMES)}
+    dom_k = max(KERNEL_NAMES, key=lambda k: per[k]["avg_ms"])
+    achieved = per[dom_k]["gbs"]
+    traffic = load_traffic(workload, dom_k)
+    step_gbs = STEP_BYTES_PER_CELL * value / 1e9
+    h = sd.GetVolField()
+    out = {
+        "metric": "FP64 cell-updates/s/GPU",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": workload, "cells": T, "edges": E, "nodes": V,
+                   "mesh": f"seeded Delaunay of {args.n}x{args.n} jittered grid, Morton order",
+                   "scheme": "SSPRK2", "flux": "HLLC<Einfeldt>", "dt": "CFL 0.15",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom_k, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                     "traffic": traffic},
+        "step_roofline": {"bytes_per_cell_update": STEP_BYTES_PER_CELL, "achieved": step_gbs,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
+        "kernels": per,
+        "cpu_baseline": cpu,
+        "host": platform.processor() or platform.machine(),
+        "finite": bool(np.isfinite(h).all()),
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

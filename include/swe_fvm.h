@@ -1,0 +1,171 @@
+/*
+ * swe_fvm.h — C ABI of the MI355X (gfx950) explicit finite-volume step for
+ * the 2-D shallow-water equations on unstructured triangles.
+ *
+ * Plain pointers and sizes only.  Every call returns an int status
+ * (SWE_OK = 0); on failure swe_last_error() names the cause.  Host arrays are
+ * row-major: node xyz is V x 3, cell primitives (w, u, v) are T x 3, edge
+ * tables are E x 2, cell tables are T x 3, all indices int64 like the
+ * reference's Idx (include/Includes.h:18).
+ *
+ * A solver handle owns its device buffers and one HIP stream; calls on one
+ * handle are not thread-safe.  Different handles are independent.
+ *
+ * Each entry point cites the reference interface it replaces.
+ */
+#ifndef SWE_FVM_H
+#define SWE_FVM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SWE_OK 0
+#define SWE_ERR_ARG 1      /* invalid argument (reference: DomainError)      */
+#define SWE_ERR_DEVICE 2   /* HIP runtime failure                             */
+#define SWE_ERR_SOLVER 3   /* numerical failure (reference: SolverError)     */
+#define SWE_ERR_DOMAIN 4   /* unsupported mesh / boundary tag (DomainError)  */
+
+/* Fluxes::HLL<W> / Fluxes::HLLC<W> (reference include/Fluxes.h:14-15,56-57) */
+#define SWE_FLUX_HLL 0
+#define SWE_FLUX_HLLC 1
+/* Wavespeeds::Rusanov / Davis / Einfeldt (reference include/Fluxes.h:4-10)  */
+#define SWE_WAVE_RUSANOV 0
+#define SWE_WAVE_DAVIS 1
+#define SWE_WAVE_EINFELDT 2
+/* Solvers::Euler / SSPRK2 / SSPRK3 (reference include/Solvers.h:6-8)        */
+#define SWE_SCHEME_EULER 0
+#define SWE_SCHEME_SSPRK2 1
+#define SWE_SCHEME_SSPRK3 2
+
+/* Option bits restoring literal HEAD behaviour where the reference is broken */
+#define SWE_OPT_HEAD_GRADIENT 1  /* full-wet plane through bathymetry, not w (src/MUSCLObject.cpp:45-64) */
+#define SWE_OPT_HEAD_PARTWET2 2  /* keep the transposed, lost writes (src/MUSCLObject.cpp:143,158-159,180) */
+
+typedef struct swe_topology swe_topology;
+typedef struct swe_solver swe_solver;
+
+/* Last error message of the calling thread (reference: exception what()). */
+const char* swe_last_error(void);
+
+/* ---------------------------------------------------------------- mesh --- */
+
+/* Topology::create / Topology ctor (reference include/TriangMesh.h:16-40,
+ * pybind/Topology.cpp:21-27).  Copies (owns) the tables: edge_nodes and
+ * edge_elements are E x 2 (edge_elements[e][1] < 0 on the boundary, only
+ * SOLID_WALL = -1 is accepted), element_* are T x 3. */
+int swe_topology_create(int64_t num_nodes, int64_t num_edges, const int64_t* edge_nodes,
+                        const int64_t* edge_elements, int64_t num_elements,
+                        const int64_t* element_nodes, const int64_t* element_edges,
+                        const int64_t* element_neighbours, swe_topology** out);
+
+/* Builds the tables of the reference's mesh reader (old TriangMesh(filename),
+ * docs/TriangMesh_8h_source.html l.17; conventions fixed by
+ * notebooks/topology.dat) from triangles (T x 3, CCW) and boundary lines
+ * (B x 2): boundary edges first in line order, the others by first
+ * appearance over cells and k = 0..2 of (tp[k], tp[k+1]). */
+int swe_topology_build(int64_t num_nodes, int64_t num_triangles, const int64_t* triangle_nodes,
+                       int64_t num_boundary_lines, const int64_t* boundary_lines,
+                       swe_topology** out);
+
+/* Gmsh 4.1 ASCII reader (old TriangMesh(filename); README.md:21): node xyz
+ * (V x 3, tag order) into *xyz (malloc'ed, free with swe_free) and the
+ * topology of the 2-D triangles with the 1-D lines as walls. */
+int swe_topology_read_msh(const char* path, double** xyz, int64_t* num_nodes,
+                          swe_topology** out);
+
+/* Topology::NumNodes / NumEdges / NumTriangles (include/TriangMesh.h:42-44) */
+int64_t swe_topology_num_nodes(const swe_topology* t);
+int64_t swe_topology_num_edges(const swe_topology* t);
+int64_t swe_topology_num_elements(const swe_topology* t);
+/* EdgePoints / EdgeTriangs / TriangPoints / TriangEdges / TriangTriangs
+ * (include/TriangMesh.h:46-52), whole tables; any pointer may be NULL. */
+int swe_topology_copy(const swe_topology* t, int64_t* edge_nodes, int64_t* edge_elements,
+                      int64_t* element_nodes, int64_t* element_edges,
+                      int64_t* element_neighbours);
+void swe_topology_destroy(swe_topology* t);
+void swe_free(void* p);
+
+/* StructTriangMesh(ni, nj, h) (reference include/StructTriangMesh.h:4-15):
+ * ni x nj squares of side h from (x0, y0), four CCW triangles per square
+ * (bottom, right, top, left) around a centre node.  Grid nodes first
+ * (row-major, (ni+1)(nj+1)), then centre nodes; walls are the outer edges. */
+int swe_mesh_struct(int64_t ni, int64_t nj, double h, double x0, double y0, double** xyz,
+                    int64_t* num_nodes, swe_topology** out);
+
+/* Seeded Delaunay triangulation of an n x n jittered grid on
+ * [x0, x0+L] x [y0, y0+L] (interior nodes moved by U(-j, j) * L / n), cells
+ * reordered along a Morton curve of their centroids and nodes renumbered by
+ * first appearance; the tables are then built as swe_topology_build does. */
+int swe_mesh_jittered(int64_t n, double jitter, uint64_t seed, double x0, double y0, double L,
+                      double** xyz, int64_t* num_nodes, swe_topology** out);
+
+/* -------------------------------------------------------------- solver --- */
+
+/* SpaceDisc(fluxer, Domain(geometry, topology), v0, cor, tau)
+ * (reference include/SpaceDisc.h:24, include/Bathymetry.h:14).  xyz is the
+ * node geometry (x, y, bathymetry z), V x 3.  device: HIP ordinal. */
+int swe_create(const swe_topology* topo, const double* xyz, int flux, int wavespeed, double cor,
+               double tau, int options, int device, swe_solver** out);
+void swe_destroy(swe_solver* s);
+
+/* VolumeField write / read (include/ValueField.h:33-44): primitive (w, u, v),
+ * T x 3 host arrays.  Writes store the values as given (no assigner). */
+int swe_set_state(swe_solver* s, const double* prim);
+int swe_get_state(swe_solver* s, double* prim);
+
+/* SpaceDisc::ComputeInterfaceValues (src/SpaceDisc.cpp:33-52). */
+int swe_compute_interface_values(swe_solver* s);
+/* SpaceDisc::ComputeFluxes (src/SpaceDisc.cpp:54-74); *r receives
+ * GetMinLenToWavespeed() (include/SpaceDisc.h:31) when non-NULL. */
+int swe_compute_fluxes(swe_solver* s, double* r);
+
+/* Getters of SpaceDisc (include/SpaceDisc.h:26-31), downloaded to host:
+ * edge slots (w, u, v) 2E x 3 (slot 2e + (from < to), include/ValueField.h:70-75),
+ * source slots 2E x 3 (column 0 unused), fluxes E x 3, node maximum V. */
+int swe_get_edge_field(swe_solver* s, double* edg);
+int swe_get_src_field(swe_solver* s, double* src);
+int swe_get_fluxes(swe_solver* s, double* f);
+int swe_get_max_wp(swe_solver* s, double* max_wp);
+/* MUSCLObject::IsDryCell / IsFullWetCell (src/MUSCLObject.cpp:13-29) for
+ * every cell: 0 dry, 1 partially wet, 2 fully wet. */
+int swe_get_cell_class(swe_solver* s, int8_t* cls);
+
+/* TimeDisc::RHS(i, dt) (src/TimeDisc.cpp:3-41) after ComputeFluxes. */
+int swe_rhs(swe_solver* s, int64_t cell, double dt, double* out3);
+
+/* Solvers::Euler / SSPRK2 / SSPRK3 (src/Solvers.cpp:5-59) repeated nsteps
+ * times on the device without host synchronisation.  dt > 0: fixed step;
+ * dt <= 0: CFL mode, each step uses cfl * r of its first flux pass.
+ * Asynchronous: call swe_synchronize (or swe_get_*) before reading. */
+int swe_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nsteps);
+/* dt of each step of the last swe_run (nsteps doubles) and simulated time. */
+int swe_get_dts(swe_solver* s, double* dts, int64_t nsteps);
+int swe_synchronize(swe_solver* s);
+/* TimeDisc::CFLdt (include/TimeDisc.h:13): 0.15 * last r. */
+int swe_cfl_dt(swe_solver* s, double* dt);
+/* SpaceDisc::GetMinLenToWavespeed (include/SpaceDisc.h:31): r of the last
+ * flux pass (of swe_compute_fluxes or of the last stage of swe_run). */
+int swe_min_len_to_wavespeed(swe_solver* s, double* r);
+
+/* Sizes and raw device pointers (state w, u, v SoA) for zero-copy interop. */
+int64_t swe_num_cells(const swe_solver* s);
+int64_t swe_num_edges(const swe_solver* s);
+int64_t swe_num_nodes(const swe_solver* s);
+int swe_device_state(swe_solver* s, double** w, double** u, double** v);
+
+/* Time, with HIP events on the solver's stream, nsteps steps of swe_run:
+ * *ms = total milliseconds; kernel_ms (optional, 5 doubles) = average
+ * per-launch duration of the reconstruction, wet/dry, flux, draining and
+ * update kernels, measured by bracketing each launch with events. */
+int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nsteps, double* ms,
+                 double* kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWE_FVM_H */

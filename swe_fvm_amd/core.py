@@ -1,0 +1,393 @@
+"""Host-side mirror of SWE_FVM's operator interface over the C ABI.
+
+Names, argument meaning and error behaviour follow the reference headers:
+``Topology`` (include/TriangMesh.h:14-65), ``Domain`` (include/Bathymetry.h:12-49),
+``SpaceDisc`` (include/SpaceDisc.h:20-48), ``TimeDisc`` (include/TimeDisc.h:4-23),
+``Solvers`` (include/Solvers.h:4-10), ``Fluxes``/``Wavespeeds``
+(include/Fluxes.h:4-113).  All numerics run in the HIP library; this module
+only moves arrays and maps status codes to the reference's exception types
+(include/Exceptions.h:4-10).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+SWE_OK = 0
+SWE_ERR_ARG = 1
+SWE_ERR_DEVICE = 2
+SWE_ERR_SOLVER = 3
+SWE_ERR_DOMAIN = 4
+
+SCHEMES = {"Euler": 0, "SSPRK2": 1, "SSPRK3": 2}
+OPT_HEAD_GRADIENT = 1
+OPT_HEAD_PARTWET2 = 2
+
+
+class DomainError(RuntimeError):
+    """include/Exceptions.h:4-6"""
+
+
+class SolverError(RuntimeError):
+    """include/Exceptions.h:8-10"""
+
+
+class DeviceError(RuntimeError):
+    """HIP runtime failure."""
+
+
+def _check(rc):
+    if rc == SWE_OK:
+        return
+    msg = _lib.load().swe_last_error().decode(errors="replace")
+    if rc == SWE_ERR_DOMAIN:
+        raise DomainError(msg)
+    if rc == SWE_ERR_SOLVER:
+        raise SolverError(msg)
+    if rc == SWE_ERR_DEVICE:
+        raise DeviceError(msg)
+    raise ValueError(msg)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _i64(a, shape):
+    a = np.ascontiguousarray(a, dtype=np.int64)
+    return a.reshape(shape)
+
+
+# ------------------------------------------------------------------ fluxes --
+
+class _WaveSpeed:
+    def __init__(self, name, code):
+        self.name, self.code = name, code
+
+    def __repr__(self):
+        return f"Wavespeeds::{self.name}"
+
+
+class Wavespeeds:
+    """Wave-speed estimates (src/Fluxes.cpp:5-26)."""
+    Rusanov = _WaveSpeed("Rusanov", 0)
+    Davis = _WaveSpeed("Davis", 1)
+    Einfeldt = _WaveSpeed("Einfeldt", 2)
+
+
+class Fluxer:
+    """A numerical flux instantiation, e.g. ``Fluxes.HLLC[Wavespeeds.Einfeldt]``."""
+
+    def __init__(self, name, code, ws):
+        if not isinstance(ws, _WaveSpeed):
+            raise TypeError("flux template argument must be a Wavespeeds member")
+        self.name, self.code, self.ws = name, code, ws
+
+    def __repr__(self):
+        return f"Fluxes::{self.name}<{self.ws!r}>"
+
+
+class _FluxTemplate:
+    def __init__(self, name, code):
+        self.name, self.code = name, code
+
+    def __getitem__(self, ws):
+        return Fluxer(self.name, self.code, ws)
+
+
+class Fluxes:
+    """``Fluxes::HLL<W>`` and ``Fluxes::HLLC<W>`` (include/Fluxes.h:14-111)."""
+    HLL = _FluxTemplate("HLL", 0)
+    HLLC = _FluxTemplate("HLLC", 1)
+
+
+# ---------------------------------------------------------------- topology --
+
+class Topology:
+    """Owning mesh tables (include/TriangMesh.h:14-65).
+
+    Unlike the reference, which keeps references to the caller's arrays, the
+    tables are copied into the library.
+    """
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @staticmethod
+    def create(numNodes, edgeNodes, edgeElements, elementNodes, elementEdges, elementNeighbours):
+        """Topology::create (include/TriangMesh.h:16-25, pybind/Topology.cpp:21-27)."""
+        L = _lib.load()
+        en = _i64(edgeNodes, (-1, 2))
+        ee = _i64(edgeElements, (-1, 2))
+        tn = _i64(elementNodes, (-1, 3))
+        te = _i64(elementEdges, (-1, 3))
+        tt = _i64(elementNeighbours, (-1, 3))
+        if en.shape != ee.shape or tn.shape != te.shape or tn.shape != tt.shape:
+            raise ValueError("table shapes disagree")
+        h = ctypes.c_void_p()
+        _check(L.swe_topology_create(int(numNodes), en.shape[0], _ptr(en), _ptr(ee), tn.shape[0],
+                                     _ptr(tn), _ptr(te), _ptr(tt), ctypes.byref(h)))
+        return Topology(h)
+
+    @staticmethod
+    def build(numNodes, triangles, boundaryLines):
+        """Tables by the reference mesh reader's conventions from raw triangles."""
+        L = _lib.load()
+        tri = _i64(triangles, (-1, 3))
+        bnd = _i64(boundaryLines, (-1, 2))
+        h = ctypes.c_void_p()
+        _check(L.swe_topology_build(int(numNodes), tri.shape[0], _ptr(tri), bnd.shape[0],
+                                    _ptr(bnd), ctypes.byref(h)))
+        return Topology(h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.load().swe_topology_destroy(self._h)
+            self._h = None
+
+    def NumNodes(self):
+        return _lib.load().swe_topology_num_nodes(self._h)
+
+    def NumEdges(self):
+        return _lib.load().swe_topology_num_edges(self._h)
+
+    def NumTriangles(self):
+        return _lib.load().swe_topology_num_elements(self._h)
+
+    # pybind names (pybind/Topology.cpp:28-30)
+    num_nodes = NumNodes
+    num_edges = NumEdges
+    num_elements = NumTriangles
+
+    def tables(self):
+        """dict of EdgePoints, EdgeTriangs, TriangPoints, TriangEdges, TriangTriangs tables."""
+        E, T = self.NumEdges(), self.NumTriangles()
+        ep = np.empty((E, 2), np.int64)
+        et = np.empty((E, 2), np.int64)
+        tp = np.empty((T, 3), np.int64)
+        te = np.empty((T, 3), np.int64)
+        tt = np.empty((T, 3), np.int64)
+        _check(_lib.load().swe_topology_copy(self._h, _ptr(ep), _ptr(et), _ptr(tp), _ptr(te), _ptr(tt)))
+        return dict(ep=ep, et=et, tp=tp, te=te, tt=tt)
+
+    def IsEdgeBoundary(self, e):
+        return bool(self.tables()["et"][e, 1] < 0)
+
+
+def _take_mesh(rc, xyz_p, nn, h):
+    _check(rc)
+    L = _lib.load()
+    n = nn.value
+    buf = (ctypes.c_double * (3 * n)).from_address(xyz_p.value)
+    xyz = np.array(np.frombuffer(buf, dtype=np.float64).reshape(n, 3))
+    L.swe_free(xyz_p)
+    return xyz, Topology(h)
+
+
+def read_msh(path):
+    """Gmsh 4.1 reader (old TriangMesh(filename)); returns (xyz V x 3, Topology)."""
+    L = _lib.load()
+    xyz_p, nn, h = ctypes.c_void_p(), ctypes.c_int64(), ctypes.c_void_p()
+    rc = L.swe_topology_read_msh(str(path).encode(), ctypes.byref(xyz_p), ctypes.byref(nn), ctypes.byref(h))
+    return _take_mesh(rc, xyz_p, nn, h)
+
+
+def StructTriangMesh(ni, nj, h, x0=0.0, y0=0.0):
+    """StructTriangMesh(ni, nj, h) (include/StructTriangMesh.h:4-15): 4 triangles per square."""
+    L = _lib.load()
+    xyz_p, nn, hh = ctypes.c_void_p(), ctypes.c_int64(), ctypes.c_void_p()
+    rc = L.swe_mesh_struct(int(ni), int(nj), float(h), float(x0), float(y0),
+                           ctypes.byref(xyz_p), ctypes.byref(nn), ctypes.byref(hh))
+    return _take_mesh(rc, xyz_p, nn, hh)
+
+
+def jittered_mesh(n, jitter=0.25, seed=0, x0=0.0, y0=0.0, L=1.0):
+    """Seeded Delaunay mesh of an n x n jittered grid, Morton-ordered cells."""
+    lib = _lib.load()
+    xyz_p, nn, hh = ctypes.c_void_p(), ctypes.c_int64(), ctypes.c_void_p()
+    rc = lib.swe_mesh_jittered(int(n), float(jitter), int(seed) & 0xFFFFFFFFFFFFFFFF, float(x0),
+                               float(y0), float(L), ctypes.byref(xyz_p), ctypes.byref(nn), ctypes.byref(hh))
+    return _take_mesh(rc, xyz_p, nn, hh)
+
+
+class Domain:
+    """Node geometry (x, y, bathymetry z) on a Topology (include/Bathymetry.h:12-49)."""
+
+    def __init__(self, geometry, topology):
+        g = np.ascontiguousarray(geometry, dtype=np.float64)
+        if g.ndim != 2 or g.shape[1] != 3:
+            raise ValueError("geometry must be V x 3 (x, y, z)")
+        if g.shape[0] != topology.NumNodes():
+            raise DomainError("geometry and topology disagree on the number of nodes")
+        self.geometry = g
+        self.topology = topology
+
+    def GetTopology(self):
+        return self.topology
+
+    def GetGeometry(self):
+        return self.geometry
+
+    def centroids(self):
+        """Domain::T for every cell ((p0 c + p1 c) + p2 c, c = 1/3)."""
+        tp = self.topology.tables()["tp"]
+        c = 1.0 / 3.0
+        P = self.geometry
+        return (P[tp[:, 0]] * c + P[tp[:, 1]] * c) + P[tp[:, 2]] * c
+
+
+# --------------------------------------------------------------- SpaceDisc --
+
+class SpaceDisc:
+    """SpaceDisc(fluxer, domain, v0, cor, tau) (include/SpaceDisc.h:20-48).
+
+    v0 is the primitive VolumeField (w, u, v) per cell, T x 3.  The field and
+    all work buffers live on the GPU; getters download them.
+    """
+
+    def __init__(self, fluxer, domain, v0, cor=0.0, tau=0.0, options=0, device=0):
+        if not isinstance(fluxer, Fluxer):
+            raise SolverError(f"unsupported fluxer {fluxer!r}: use Fluxes.HLL[...] or Fluxes.HLLC[...]")
+        L = _lib.load()
+        self.fluxer = fluxer
+        self.domain = domain
+        self.cor, self.tau = float(cor), float(tau)
+        h = ctypes.c_void_p()
+        _check(L.swe_create(domain.topology._h, _ptr(domain.geometry), fluxer.code, fluxer.ws.code,
+                            self.cor, self.tau, int(options), int(device), ctypes.byref(h)))
+        self._h = h
+        self.T = L.swe_num_cells(h)
+        self.E = L.swe_num_edges(h)
+        self.V = L.swe_num_nodes(h)
+        self.SetVolField(v0)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.load().swe_destroy(self._h)
+            self._h = None
+
+    # VolumeField access
+    def SetVolField(self, prim):
+        p = np.ascontiguousarray(prim, dtype=np.float64).reshape(self.T, 3)
+        _check(_lib.load().swe_set_state(self._h, _ptr(p)))
+
+    def GetVolField(self):
+        out = np.empty((self.T, 3), np.float64)
+        _check(_lib.load().swe_get_state(self._h, _ptr(out)))
+        return out
+
+    def GetDomain(self):
+        return self.domain
+
+    def ComputeInterfaceValues(self):
+        _check(_lib.load().swe_compute_interface_values(self._h))
+
+    def ComputeFluxes(self):
+        _check(_lib.load().swe_compute_fluxes(self._h, None))
+
+    def GetMinLenToWavespeed(self):
+        r = ctypes.c_double()
+        _check(_lib.load().swe_min_len_to_wavespeed(self._h, ctypes.byref(r)))
+        return r.value
+
+    def GetEdgField(self):
+        out = np.empty((2 * self.E, 3), np.float64)
+        _check(_lib.load().swe_get_edge_field(self._h, _ptr(out)))
+        return out
+
+    def GetSrcField(self):
+        out = np.empty((2 * self.E, 3), np.float64)
+        _check(_lib.load().swe_get_src_field(self._h, _ptr(out)))
+        return out
+
+    def GetFluxes(self):
+        out = np.empty((self.E, 3), np.float64)
+        _check(_lib.load().swe_get_fluxes(self._h, _ptr(out)))
+        return out
+
+    def GetMaxWp(self):
+        out = np.empty(self.V, np.float64)
+        _check(_lib.load().swe_get_max_wp(self._h, _ptr(out)))
+        return out
+
+    def CellClass(self):
+        """0 dry, 1 partially wet, 2 fully wet (src/MUSCLObject.cpp:13-29)."""
+        out = np.empty(self.T, np.int8)
+        _check(_lib.load().swe_get_cell_class(self._h, _ptr(out)))
+        return out
+
+    def GetCor(self):
+        return self.cor
+
+    def GetTau(self):
+        return self.tau
+
+    # whole-run driver (no host synchronisation between steps)
+    def run(self, scheme="SSPRK2", nsteps=1, dt=1e-3, cfl=0.15, sync=True):
+        """nsteps steps; dt > 0 fixed, dt <= 0 CFL mode (dt = cfl * r)."""
+        _check(_lib.load().swe_run(self._h, SCHEMES[scheme], float(dt), float(cfl), int(nsteps)))
+        if sync:
+            self.synchronize()
+
+    def dts(self, nsteps):
+        out = np.empty(nsteps, np.float64)
+        _check(_lib.load().swe_get_dts(self._h, _ptr(out), int(nsteps)))
+        return out
+
+    def synchronize(self):
+        _check(_lib.load().swe_synchronize(self._h))
+
+    def time_run(self, scheme="SSPRK2", nsteps=1, dt=1e-3, cfl=0.15, per_kernel=False):
+        """Milliseconds of nsteps steps (HIP events on the solver stream).
+
+        per_kernel: also return the average per-launch milliseconds of the
+        recon1, recon2, flux, drain and update kernels."""
+        ms = ctypes.c_double()
+        if per_kernel:
+            k = np.zeros(5, np.float64)
+            _check(_lib.load().swe_time_run(self._h, SCHEMES[scheme], float(dt), float(cfl),
+                                            int(nsteps), ctypes.byref(ms), _ptr(k)))
+            return ms.value, k
+        _check(_lib.load().swe_time_run(self._h, SCHEMES[scheme], float(dt), float(cfl), int(nsteps),
+                                        ctypes.byref(ms), None))
+        return ms.value
+
+
+class TimeDisc:
+    """TimeDisc(SpaceDisc*) (include/TimeDisc.h:4-23)."""
+
+    m_constCFL = 0.15
+
+    def __init__(self, sd=None):
+        self.m_sd = sd
+
+    def GetSpaceDisc(self):
+        return self.m_sd
+
+    def SetSpaceDisc(self, sd):
+        self.m_sd = sd
+
+    def CFLdt(self):
+        return self.m_constCFL * self.m_sd.GetMinLenToWavespeed()
+
+    def RHS(self, i, dt):
+        out = np.empty(3, np.float64)
+        _check(_lib.load().swe_rhs(self.m_sd._h, int(i), float(dt), _ptr(out)))
+        return out
+
+
+class Solvers:
+    """Solvers::Euler / SSPRK2 / SSPRK3 (include/Solvers.h:6-8)."""
+
+    @staticmethod
+    def Euler(td, dt):
+        td.GetSpaceDisc().run("Euler", 1, dt)
+
+    @staticmethod
+    def SSPRK2(td, dt):
+        td.GetSpaceDisc().run("SSPRK2", 1, dt)
+
+    @staticmethod
+    def SSPRK3(td, dt):
+        td.GetSpaceDisc().run("SSPRK3", 1, dt)

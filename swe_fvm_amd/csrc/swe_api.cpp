@@ -1,0 +1,525 @@
+// Solver runtime: device-resident mesh and state, stage sequencing of the
+// SSP Runge-Kutta step on one HIP stream, getters serving the reference's
+// SpaceDisc accessors, and event-based timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/swe_fvm.h"
+#include "swe_internal.h"
+#include "swe_launch.h"
+
+using namespace swe;
+
+struct swe_solver {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int flux = 1, ws = 2, opts = 0;
+  double cor = 0., tau = 0.;
+  int64_t T = 0, E = 0, V = 0;
+  std::vector<void*> allocs;
+  DevMesh M{};
+  double* sw[3] = {nullptr, nullptr, nullptr};
+  double* su[3] = {nullptr, nullptr, nullptr};
+  double* sv[3] = {nullptr, nullptr, nullptr};
+  int cur = 0;
+  DevWork W{};
+  unsigned long long* rbits = nullptr;  // [4]
+  int last_r_slot = 3;
+  double* dts = nullptr;
+  int64_t dts_cap = 0;
+  int64_t last_nsteps = 0;
+};
+
+namespace {
+
+int dev_fail(hipError_t e, const char* what) {
+  return fail(SWE_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+hipError_t dalloc(swe_solver* s, T** p, int64_t n) {
+  size_t bytes = sizeof(T) * size_t(std::max<int64_t>(n, 1));
+  hipError_t e = hipMalloc((void**)p, bytes);
+  if (e == hipSuccess) s->allocs.push_back(*p);
+  return e;
+}
+
+DevState state_of(const swe_solver* s, int b) { return DevState{s->sw[b], s->su[b], s->sv[b]}; }
+DevOut out_of(const swe_solver* s, int b) { return DevOut{s->sw[b], s->su[b], s->sv[b]}; }
+
+#define SWE_CHECK(call, what)                 \
+  do {                                        \
+    hipError_t e_ = (call);                   \
+    if (e_ != hipSuccess) return dev_fail(e_, what); \
+  } while (0)
+
+int set_device(swe_solver* s) {
+  SWE_CHECK(hipSetDevice(s->device), "hipSetDevice");
+  return SWE_OK;
+}
+
+// One stage: ComputeInterfaceValues, ComputeFluxes, update loop.
+int run_stage(swe_solver* s, int in, int u0, int out, double a0, double a1, double coef,
+              double dt_fixed, double cfl, int slot, double* dts_log) {
+  DevState S = state_of(s, in);
+  SWE_CHECK(launch_recon1(s->M, S, s->W, s->opts, s->stream), "recon1");
+  SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
+  SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + slot, s->stream), "flux");
+  SWE_CHECK(launch_drain(s->M, S, s->W, s->stream), "drain");
+  StageArgs A{a0, a1, coef, dt_fixed, cfl, s->cor, s->rbits, dts_log};
+  SWE_CHECK(launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->stream), "update");
+  s->last_r_slot = slot;
+  return SWE_OK;
+}
+
+int run_step(swe_solver* s, int scheme, double dt, double cfl, double* dts_log) {
+  int a = s->cur;
+  int b = (a + 1) % 3, c = (a + 2) % 3;
+  int rc;
+  if (scheme == SWE_SCHEME_EULER) {
+    rc = run_stage(s, a, a, b, 0., 0., 1., dt, cfl, 0, dts_log);
+    if (rc) return rc;
+    s->cur = b;
+  } else if (scheme == SWE_SCHEME_SSPRK2) {
+    rc = run_stage(s, a, a, b, 0., 0., 1., dt, cfl, 0, dts_log);
+    if (rc) return rc;
+    rc = run_stage(s, b, a, a, 0.5, 0.5, 0.5, dt, cfl, 1, nullptr);
+    if (rc) return rc;
+  } else {
+    rc = run_stage(s, a, a, b, 0., 0., 1., dt, cfl, 0, dts_log);
+    if (rc) return rc;
+    rc = run_stage(s, b, a, c, 0.75, 0.25, 0.25, dt, cfl, 1, nullptr);
+    if (rc) return rc;
+    rc = run_stage(s, c, a, a, 1. / 3., 2. / 3., 2. / 3., dt, cfl, 2, nullptr);
+    if (rc) return rc;
+  }
+  return SWE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int swe_create(const swe_topology* topo, const double* xyz, int flux, int wavespeed, double cor,
+               double tau, int options, int device, swe_solver** out) {
+  if (!topo || !xyz || !out) return fail(SWE_ERR_ARG, "null argument");
+  if (flux < 0 || flux > 1 || wavespeed < 0 || wavespeed > 2)
+    return fail(SWE_ERR_ARG, "unknown flux or wave-speed estimate");
+  const Topo& m = topo->m;
+  if (m.T > 0x7fffffff || m.E > 0x7fffffff || m.V > 0x7fffffff)
+    return fail(SWE_ERR_ARG, "mesh too large for int32 device indices");
+  const int64_t T = m.T, E = m.E, V = m.V;
+  // host-side validation and SoA packing
+  std::vector<int> tp(size_t(3 * T)), te(size_t(3 * T)), tt(size_t(3 * T));
+  std::vector<uint32_t> flag(size_t(T), 0u);
+  std::vector<int> ep0(static_cast<size_t>(E)), ep1(static_cast<size_t>(E)), et0(static_cast<size_t>(E)), et1(static_cast<size_t>(E));
+  for (int64_t e = 0; e < E; ++e) {
+    int64_t a = m.ep[2 * e], b = m.ep[2 * e + 1], f = m.et[2 * e], t = m.et[2 * e + 1];
+    if (a < 0 || a >= V || b < 0 || b >= V) return fail(SWE_ERR_ARG, "edge node out of range");
+    if (f < 0 || f >= T || t >= T) return fail(SWE_ERR_ARG, "edge element out of range");
+    if (t < -1)
+      return fail(SWE_ERR_DOMAIN, "boundary tag other than SOLID_WALL (-1) is not supported");
+    ep0[size_t(e)] = int(a); ep1[size_t(e)] = int(b); et0[size_t(e)] = int(f); et1[size_t(e)] = int(t);
+  }
+  std::vector<int> deg(size_t(V) + 1, 0);
+  for (int64_t i = 0; i < T; ++i) {
+    uint32_t fl = 0;
+    for (int k = 0; k < 3; ++k) {
+      int64_t p = m.tp[3 * i + k], e = m.te[3 * i + k], n = m.tt[3 * i + k];
+      if (p < 0 || p >= V) return fail(SWE_ERR_ARG, "element node out of range");
+      if (e < 0 || e >= E) return fail(SWE_ERR_ARG, "element edge out of range");
+      if (n < -1 || n >= T) return fail(SWE_ERR_ARG, "element neighbour out of range");
+      tp[size_t(k * T + i)] = int(p);
+      te[size_t(k * T + i)] = int(e);
+      tt[size_t(k * T + i)] = int(n);
+      deg[size_t(p)]++;
+    }
+    for (int k = 0; k < 3; ++k) {
+      int64_t e = m.te[3 * i + k];
+      // the kernels index neighbours through tt: it must name the other cell
+      // of edge te[k] (or -1 on a wall), as the reference builder writes it
+      int64_t a = m.et[2 * e], b = m.et[2 * e + 1];
+      if (a != i && b != i) return fail(SWE_ERR_ARG, "element edge does not list the element");
+      int64_t other = (a == i) ? b : a;
+      if (m.tt[3 * i + k] != (other < 0 ? -1 : other))
+        return fail(SWE_ERR_ARG, "element neighbour disagrees with the edge's elements");
+      if (m.et[2 * e] == i) fl |= 1u << k;
+      if (m.et[2 * e + 1] < 0) fl |= 1u << 3;
+      int la = -1, lb = -1;
+      for (int q = 0; q < 3; ++q) {
+        if (m.tp[3 * i + q] == m.ep[2 * e]) la = q;
+        if (m.tp[3 * i + q] == m.ep[2 * e + 1]) lb = q;
+      }
+      if (la < 0 || lb < 0 || la == lb)
+        return fail(SWE_ERR_ARG, "element edge does not join two of the element's nodes");
+      fl |= uint32_t(la) << (4 + 4 * k);
+      fl |= uint32_t(lb) << (6 + 4 * k);
+    }
+    flag[size_t(i)] = fl;
+  }
+  std::vector<int> n2c_off(size_t(V) + 1, 0), n2c(size_t(3 * T));
+  for (int64_t n = 0; n < V; ++n) n2c_off[size_t(n) + 1] = n2c_off[size_t(n)] + deg[size_t(n)];
+  {
+    std::vector<int> fillp(n2c_off.begin(), n2c_off.end() - 1);
+    for (int64_t i = 0; i < T; ++i)
+      for (int k = 0; k < 3; ++k) n2c[size_t(fillp[size_t(m.tp[3 * i + k])]++)] = int(i);
+  }
+  std::vector<double> px(static_cast<size_t>(V)), py(static_cast<size_t>(V)), pz(static_cast<size_t>(V));
+  for (int64_t n = 0; n < V; ++n) {
+    px[size_t(n)] = xyz[3 * n]; py[size_t(n)] = xyz[3 * n + 1]; pz[size_t(n)] = xyz[3 * n + 2];
+  }
+
+  // tables are validated first, so mesh errors are reported without a device
+  int ndev = 0;
+  hipError_t he = hipGetDeviceCount(&ndev);
+  if (he != hipSuccess || ndev <= 0) return fail(SWE_ERR_DEVICE, "no HIP device available");
+  if (device < 0 || device >= ndev) return fail(SWE_ERR_ARG, "invalid device ordinal");
+
+  auto* s = new swe_solver;
+  s->device = device;
+  s->flux = flux; s->ws = wavespeed; s->opts = options; s->cor = cor; s->tau = tau;
+  s->T = T; s->E = E; s->V = V;
+  auto bail = [&](hipError_t e, const char* what) {
+    int rc = dev_fail(e, what);
+    swe_destroy(s);
+    return rc;
+  };
+  hipError_t e;
+  if ((e = hipSetDevice(device)) != hipSuccess) return bail(e, "hipSetDevice");
+  if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess)
+    return bail(e, "hipStreamCreate");
+  double *dpx, *dpy, *dpz;
+  int *dtp, *dte, *dtt, *dep0, *dep1, *det0, *det1, *doff, *dn2c;
+  uint32_t* dfl;
+#define ALLOC(ptr, n) if ((e = dalloc(s, &ptr, n)) != hipSuccess) return bail(e, "hipMalloc")
+  ALLOC(dpx, V); ALLOC(dpy, V); ALLOC(dpz, V);
+  ALLOC(dtp, 3 * T); ALLOC(dte, 3 * T); ALLOC(dtt, 3 * T); ALLOC(dfl, T);
+  ALLOC(dep0, E); ALLOC(dep1, E); ALLOC(det0, E); ALLOC(det1, E);
+  ALLOC(doff, V + 1); ALLOC(dn2c, 3 * T);
+  for (int b = 0; b < 3; ++b) { ALLOC(s->sw[b], T); ALLOC(s->su[b], T); ALLOC(s->sv[b], T); }
+  ALLOC(s->W.mw, T); ALLOC(s->W.mgx, T); ALLOC(s->W.mgy, T);
+  ALLOC(s->W.aw, T); ALLOC(s->W.agx, T); ALLOC(s->W.agy, T);
+  ALLOC(s->W.cls, T);
+  ALLOC(s->W.f0, E); ALLOC(s->W.f1, E); ALLOC(s->W.f2, E);
+  ALLOC(s->W.dtd, T);
+  ALLOC(s->rbits, 4);
+#undef ALLOC
+#define H2D(dst, src, n)                                                                     \
+  if ((e = hipMemcpy(dst, src, sizeof(*(src)) * size_t(n), hipMemcpyHostToDevice)) != hipSuccess) \
+    return bail(e, "hipMemcpy");
+  H2D(dpx, px.data(), V); H2D(dpy, py.data(), V); H2D(dpz, pz.data(), V);
+  H2D(dtp, tp.data(), 3 * T); H2D(dte, te.data(), 3 * T); H2D(dtt, tt.data(), 3 * T);
+  H2D(dfl, flag.data(), T);
+  H2D(dep0, ep0.data(), E); H2D(dep1, ep1.data(), E); H2D(det0, et0.data(), E); H2D(det1, et1.data(), E);
+  H2D(doff, n2c_off.data(), V + 1); H2D(dn2c, n2c.data(), 3 * T);
+#undef H2D
+  unsigned long long one;
+  double d1 = 1.0;
+  std::memcpy(&one, &d1, 8);
+  unsigned long long rinit[4] = {one, one, one, one};
+  if ((e = hipMemcpy(s->rbits, rinit, sizeof(rinit), hipMemcpyHostToDevice)) != hipSuccess)
+    return bail(e, "hipMemcpy");
+  for (int b = 0; b < 3; ++b) {
+    if ((e = hipMemset(s->sw[b], 0, sizeof(double) * size_t(std::max<int64_t>(T, 1)))) != hipSuccess) return bail(e, "hipMemset");
+    if ((e = hipMemset(s->su[b], 0, sizeof(double) * size_t(std::max<int64_t>(T, 1)))) != hipSuccess) return bail(e, "hipMemset");
+    if ((e = hipMemset(s->sv[b], 0, sizeof(double) * size_t(std::max<int64_t>(T, 1)))) != hipSuccess) return bail(e, "hipMemset");
+  }
+  s->M = DevMesh{dpx, dpy, dpz, dtp, dte, dtt, dfl, dep0, dep1, det0, det1, doff, dn2c, T, E, V};
+  *out = s;
+  return SWE_OK;
+}
+
+void swe_destroy(swe_solver* s) {
+  if (!s) return;
+  hipSetDevice(s->device);
+  if (s->stream) hipStreamSynchronize(s->stream);
+  for (void* p : s->allocs) hipFree(p);
+  if (s->dts) hipFree(s->dts);
+  if (s->stream) hipStreamDestroy(s->stream);
+  delete s;
+}
+
+int swe_set_state(swe_solver* s, const double* prim) {
+  if (!s || !prim) return fail(SWE_ERR_ARG, "null argument");
+  if (int rc = set_device(s)) return rc;
+  std::vector<double> w(size_t(s->T)), u(size_t(s->T)), v(size_t(s->T));
+  for (int64_t i = 0; i < s->T; ++i) {
+    w[size_t(i)] = prim[3 * i]; u[size_t(i)] = prim[3 * i + 1]; v[size_t(i)] = prim[3 * i + 2];
+  }
+  int b = s->cur;
+  SWE_CHECK(hipMemcpyAsync(s->sw[b], w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice, s->stream), "hipMemcpy");
+  SWE_CHECK(hipMemcpyAsync(s->su[b], u.data(), sizeof(double) * u.size(), hipMemcpyHostToDevice, s->stream), "hipMemcpy");
+  SWE_CHECK(hipMemcpyAsync(s->sv[b], v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, s->stream), "hipMemcpy");
+  SWE_CHECK(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  return SWE_OK;
+}
+
+int swe_get_state(swe_solver* s, double* prim) {
+  if (!s || !prim) return fail(SWE_ERR_ARG, "null argument");
+  if (int rc = set_device(s)) return rc;
+  std::vector<double> w(size_t(s->T)), u(size_t(s->T)), v(size_t(s->T));
+  int b = s->cur;
+  SWE_CHECK(hipMemcpyAsync(w.data(), s->sw[b], sizeof(double) * w.size(), hipMemcpyDeviceToHost, s->stream), "hipMemcpy");
+  SWE_CHECK(hipMemcpyAsync(u.data(), s->su[b], sizeof(double) * u.size(), hipMemcpyDeviceToHost, s->stream), "hipMemcpy");
+  SWE_CHECK(hipMemcpyAsync(v.data(), s->sv[b], sizeof(double) * v.size(), hipMemcpyDeviceToHost, s->stream), "hipMemcpy");
+  SWE_CHECK(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  for (int64_t i = 0; i < s->T; ++i) {
+    prim[3 * i] = w[size_t(i)]; prim[3 * i + 1] = u[size_t(i)]; prim[3 * i + 2] = v[size_t(i)];
+  }
+  return SWE_OK;
+}
+
+int swe_compute_interface_values(swe_solver* s) {
+  if (!s) return fail(SWE_ERR_ARG, "null solver");
+  if (int rc = set_device(s)) return rc;
+  DevState S = state_of(s, s->cur);
+  SWE_CHECK(launch_recon1(s->M, S, s->W, s->opts, s->stream), "recon1");
+  SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
+  SWE_CHECK(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  return SWE_OK;
+}
+
+int swe_compute_fluxes(swe_solver* s, double* r) {
+  if (!s) return fail(SWE_ERR_ARG, "null solver");
+  if (int rc = set_device(s)) return rc;
+  DevState S = state_of(s, s->cur);
+  SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + 3, s->stream), "flux");
+  s->last_r_slot = 3;
+  unsigned long long bits;
+  SWE_CHECK(hipMemcpyAsync(&bits, s->rbits + 3, 8, hipMemcpyDeviceToHost, s->stream), "hipMemcpy");
+  SWE_CHECK(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  if (r) std::memcpy(r, &bits, 8);
+  return SWE_OK;
+}
+
+static int download(swe_solver* s, void* dst, const void* src, size_t bytes) {
+  SWE_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->stream), "hipMemcpy");
+  SWE_CHECK(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  return SWE_OK;
+}
+
+static int edge_and_src(swe_solver* s, double* edg, double* src) {
+  if (int rc = set_device(s)) return rc;
+  double *de = nullptr, *ds = nullptr;
+  size_t n = size_t(6 * std::max<int64_t>(s->E, 1));
+  SWE_CHECK(hipMalloc((void**)&de, sizeof(double) * n), "hipMalloc");
+  hipError_t e = hipMalloc((void**)&ds, sizeof(double) * n);
+  if (e != hipSuccess) { hipFree(de); return dev_fail(e, "hipMalloc"); }
+  hipMemsetAsync(de, 0, sizeof(double) * n, s->stream);
+  hipMemsetAsync(ds, 0, sizeof(double) * n, s->stream);
+  e = launch_edge_field(s->M, state_of(s, s->cur), s->W, s->cor, de, ds, s->stream);
+  int rc = SWE_OK;
+  if (e != hipSuccess) rc = dev_fail(e, "edge_field");
+  if (!rc && edg) rc = download(s, edg, de, sizeof(double) * size_t(6 * s->E));
+  if (!rc && src) rc = download(s, src, ds, sizeof(double) * size_t(6 * s->E));
+  hipStreamSynchronize(s->stream);
+  hipFree(de);
+  hipFree(ds);
+  return rc;
+}
+
+int swe_get_edge_field(swe_solver* s, double* edg) {
+  if (!s || !edg) return fail(SWE_ERR_ARG, "null argument");
+  return edge_and_src(s, edg, nullptr);
+}
+int swe_get_src_field(swe_solver* s, double* src) {
+  if (!s || !src) return fail(SWE_ERR_ARG, "null argument");
+  return edge_and_src(s, nullptr, src);
+}
+
+int swe_get_fluxes(swe_solver* s, double* f) {
+  if (!s || !f) return fail(SWE_ERR_ARG, "null argument");
+  if (int rc = set_device(s)) return rc;
+  std::vector<double> a(size_t(s->E)), b(size_t(s->E)), c(size_t(s->E));
+  if (int rc = download(s, a.data(), s->W.f0, sizeof(double) * a.size())) return rc;
+  if (int rc = download(s, b.data(), s->W.f1, sizeof(double) * b.size())) return rc;
+  if (int rc = download(s, c.data(), s->W.f2, sizeof(double) * c.size())) return rc;
+  for (int64_t e = 0; e < s->E; ++e) {
+    f[3 * e] = a[size_t(e)]; f[3 * e + 1] = b[size_t(e)]; f[3 * e + 2] = c[size_t(e)];
+  }
+  return SWE_OK;
+}
+
+int swe_get_max_wp(swe_solver* s, double* max_wp) {
+  if (!s || !max_wp) return fail(SWE_ERR_ARG, "null argument");
+  if (int rc = set_device(s)) return rc;
+  double* d = nullptr;
+  SWE_CHECK(hipMalloc((void**)&d, sizeof(double) * size_t(std::max<int64_t>(s->V, 1))), "hipMalloc");
+  hipError_t e = launch_max_wp(s->M, s->W, d, s->stream);
+  int rc = (e != hipSuccess) ? dev_fail(e, "max_wp") : download(s, max_wp, d, sizeof(double) * size_t(s->V));
+  hipStreamSynchronize(s->stream);
+  hipFree(d);
+  return rc;
+}
+
+int swe_get_cell_class(swe_solver* s, int8_t* cls) {
+  if (!s || !cls) return fail(SWE_ERR_ARG, "null argument");
+  if (int rc = set_device(s)) return rc;
+  return download(s, cls, s->W.cls, size_t(s->T));
+}
+
+int swe_rhs(swe_solver* s, int64_t cell, double dt, double* out3) {
+  if (!s || !out3) return fail(SWE_ERR_ARG, "null argument");
+  if (cell < 0 || cell >= s->T) return fail(SWE_ERR_ARG, "cell index out of range");
+  if (int rc = set_device(s)) return rc;
+  DevState S = state_of(s, s->cur);
+  double* d = nullptr;
+  SWE_CHECK(hipMalloc((void**)&d, 3 * sizeof(double)), "hipMalloc");
+  hipError_t e = launch_drain(s->M, S, s->W, s->stream);
+  if (e == hipSuccess) e = launch_rhs_one(s->M, S, s->W, int(cell), dt, s->cor, d, s->stream);
+  int rc = (e != hipSuccess) ? dev_fail(e, "rhs") : download(s, out3, d, 3 * sizeof(double));
+  hipStreamSynchronize(s->stream);
+  hipFree(d);
+  return rc;
+}
+
+int swe_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nsteps) {
+  if (!s) return fail(SWE_ERR_ARG, "null solver");
+  if (scheme < 0 || scheme > 2) return fail(SWE_ERR_ARG, "unknown scheme");
+  if (nsteps < 0) return fail(SWE_ERR_ARG, "negative step count");
+  if (!(dt > 0.) && !(cfl > 0.)) return fail(SWE_ERR_ARG, "CFL mode needs cfl > 0");
+  if (int rc = set_device(s)) return rc;
+  if (nsteps > s->dts_cap) {
+    if (s->dts) { hipStreamSynchronize(s->stream); hipFree(s->dts); s->dts = nullptr; }
+    SWE_CHECK(hipMalloc((void**)&s->dts, sizeof(double) * size_t(nsteps)), "hipMalloc");
+    s->dts_cap = nsteps;
+  }
+  for (int64_t k = 0; k < nsteps; ++k)
+    if (int rc = run_step(s, scheme, dt, cfl, s->dts + k)) return rc;
+  s->last_nsteps = nsteps;
+  return SWE_OK;
+}
+
+int swe_get_dts(swe_solver* s, double* dts, int64_t nsteps) {
+  if (!s || !dts) return fail(SWE_ERR_ARG, "null argument");
+  if (nsteps > s->last_nsteps) return fail(SWE_ERR_ARG, "more steps requested than run");
+  if (int rc = set_device(s)) return rc;
+  return download(s, dts, s->dts, sizeof(double) * size_t(nsteps));
+}
+
+int swe_synchronize(swe_solver* s) {
+  if (!s) return fail(SWE_ERR_ARG, "null solver");
+  if (int rc = set_device(s)) return rc;
+  SWE_CHECK(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  return SWE_OK;
+}
+
+int swe_cfl_dt(swe_solver* s, double* dt) {
+  if (!s || !dt) return fail(SWE_ERR_ARG, "null argument");
+  if (int rc = set_device(s)) return rc;
+  unsigned long long bits;
+  if (int rc = download(s, &bits, s->rbits + s->last_r_slot, 8)) return rc;
+  double r;
+  std::memcpy(&r, &bits, 8);
+  *dt = 0.15 * r;  // include/TimeDisc.h:13,22
+  return SWE_OK;
+}
+
+int swe_min_len_to_wavespeed(swe_solver* s, double* r) {
+  if (!s || !r) return fail(SWE_ERR_ARG, "null argument");
+  if (int rc = set_device(s)) return rc;
+  unsigned long long bits;
+  if (int rc = download(s, &bits, s->rbits + s->last_r_slot, 8)) return rc;
+  std::memcpy(r, &bits, 8);
+  return SWE_OK;
+}
+
+int64_t swe_num_cells(const swe_solver* s) { return s ? s->T : -1; }
+int64_t swe_num_edges(const swe_solver* s) { return s ? s->E : -1; }
+int64_t swe_num_nodes(const swe_solver* s) { return s ? s->V : -1; }
+
+int swe_device_state(swe_solver* s, double** w, double** u, double** v) {
+  if (!s) return fail(SWE_ERR_ARG, "null solver");
+  if (w) *w = s->sw[s->cur];
+  if (u) *u = s->su[s->cur];
+  if (v) *v = s->sv[s->cur];
+  return SWE_OK;
+}
+
+int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nsteps, double* ms,
+                 double* kernel_ms) {
+  if (!s || !ms) return fail(SWE_ERR_ARG, "null argument");
+  if (int rc = set_device(s)) return rc;
+  hipEvent_t e0, e1;
+  SWE_CHECK(hipEventCreate(&e0), "hipEventCreate");
+  SWE_CHECK(hipEventCreate(&e1), "hipEventCreate");
+  int rc = SWE_OK;
+  if (!kernel_ms) {
+    hipEventRecord(e0, s->stream);
+    rc = swe_run(s, scheme, dt, cfl, nsteps);
+    hipEventRecord(e1, s->stream);
+    hipEventSynchronize(e1);
+    float f = 0.f;
+    hipEventElapsedTime(&f, e0, e1);
+    *ms = f;
+  } else {
+    // bracket each kernel of each stage with its own event pair
+    const int nk = 5;
+    double acc[nk] = {0, 0, 0, 0, 0};
+    int64_t cnt = 0;
+    std::vector<hipEvent_t> ev(2 * nk);
+    for (auto& x : ev) hipEventCreate(&x);
+    float total = 0.f;
+    int nst = scheme == SWE_SCHEME_EULER ? 1 : (scheme == SWE_SCHEME_SSPRK2 ? 2 : 3);
+    if (nsteps > s->dts_cap) {
+      if (s->dts) { hipStreamSynchronize(s->stream); hipFree(s->dts); s->dts = nullptr; }
+      SWE_CHECK(hipMalloc((void**)&s->dts, sizeof(double) * size_t(nsteps)), "hipMalloc");
+      s->dts_cap = nsteps;
+    }
+    for (int64_t k = 0; k < nsteps && !rc; ++k) {
+      int a = s->cur, b = (a + 1) % 3, c = (a + 2) % 3;
+      for (int st = 0; st < nst && !rc; ++st) {
+        int in, u0, out;
+        double a0, a1, coef;
+        if (st == 0) { in = a; u0 = a; out = b; a0 = 0; a1 = 0; coef = 1; }
+        else if (scheme == SWE_SCHEME_SSPRK2) { in = b; u0 = a; out = a; a0 = .5; a1 = .5; coef = .5; }
+        else if (st == 1) { in = b; u0 = a; out = c; a0 = .75; a1 = .25; coef = .25; }
+        else { in = c; u0 = a; out = a; a0 = 1. / 3.; a1 = 2. / 3.; coef = 2. / 3.; }
+        DevState S = state_of(s, in);
+        hipEventRecord(ev[0], s->stream);
+        launch_recon1(s->M, S, s->W, s->opts, s->stream);
+        hipEventRecord(ev[1], s->stream);
+        launch_recon2(s->M, S, s->W, s->opts, s->stream);
+        hipEventRecord(ev[2], s->stream);
+        hipEventRecord(ev[3], s->stream);
+        launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + st, s->stream);
+        hipEventRecord(ev[4], s->stream);
+        hipEventRecord(ev[5], s->stream);
+        launch_drain(s->M, S, s->W, s->stream);
+        hipEventRecord(ev[6], s->stream);
+        hipEventRecord(ev[7], s->stream);
+        StageArgs A{a0, a1, coef, dt, cfl, s->cor, s->rbits, st == 0 ? s->dts + k : nullptr};
+        launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->stream);
+        hipEventRecord(ev[8], s->stream);
+        hipError_t e = hipEventSynchronize(ev[8]);
+        if (e != hipSuccess) { rc = dev_fail(e, "kernel"); break; }
+        float f;
+        hipEventElapsedTime(&f, ev[0], ev[1]); acc[0] += f; total += f;
+        hipEventElapsedTime(&f, ev[1], ev[2]); acc[1] += f; total += f;
+        hipEventElapsedTime(&f, ev[3], ev[4]); acc[2] += f; total += f;
+        hipEventElapsedTime(&f, ev[5], ev[6]); acc[3] += f; total += f;
+        hipEventElapsedTime(&f, ev[7], ev[8]); acc[4] += f; total += f;
+        ++cnt;
+        s->last_r_slot = st;
+      }
+      if (scheme == SWE_SCHEME_EULER) s->cur = b;
+    }
+    s->last_nsteps = nsteps;
+    for (int q = 0; q < nk; ++q) kernel_ms[q] = cnt ? acc[q] / double(cnt) : 0.;
+    *ms = total;
+    for (auto& x : ev) hipEventDestroy(x);
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (!rc) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = dev_fail(e, "run");
+  }
+  return rc;
+}
+
+}  // extern "C"

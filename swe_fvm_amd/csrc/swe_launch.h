@@ -1,0 +1,90 @@
+// Device data layout and kernel launchers of the explicit step.
+//
+// HBM layout (structure of arrays, int32 indices, Morton-ordered cells):
+//   nodes  px, py, pz                     double[V]
+//   cells  tp, te, tt                     int32[3][T]   (column k at k*T)
+//          cflag                          uint32[T]
+//            bits 0-2  et0(te[k]) == cell (flux orientation of edge k)
+//            bit  3    boundary triangle (some edge has et1 < 0)
+//            bits 4+4k, 6+4k  local vertex of ep0 / ep1 of edge k
+//   edges  ep0, ep1, et0, et1             int32[E]
+//   node -> cell map  n2c_off[V+1], n2c[3T]
+//   state  w, u, v                        double[T] per buffer
+//   work   MUSCL pass 1 (mw, mgx, mgy), pass 2 (aw, agx, agy), cls int8,
+//          fluxes f0, f1, f2 double[E], draining dt dtd double[T]
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace swe {
+
+struct DevMesh {
+  const double* px;
+  const double* py;
+  const double* pz;
+  const int* tp;
+  const int* te;
+  const int* tt;
+  const uint32_t* cflag;
+  const int* ep0;
+  const int* ep1;
+  const int* et0;
+  const int* et1;
+  const int* n2c_off;
+  const int* n2c;
+  int64_t T, E, V;
+};
+
+struct DevState {
+  const double* w;
+  const double* u;
+  const double* v;
+};
+
+struct DevOut {
+  double* w;
+  double* u;
+  double* v;
+};
+
+struct DevWork {
+  double* mw;
+  double* mgx;
+  double* mgy;
+  double* aw;
+  double* agx;
+  double* agy;
+  int8_t* cls;
+  double* f0;
+  double* f1;
+  double* f2;
+  double* dtd;
+};
+
+struct StageArgs {
+  double a0, a1;       // Shu-Osher weights of U0 and the stage state (a0 = 0: U + RHS)
+  double coef;         // stage dt = coef * dt
+  double dt_fixed;     // > 0 fixed step, else cfl * r of the step's first flux pass
+  double cfl;
+  double cor;
+  const unsigned long long* rbits;  // r of the step's first flux pass (double bits)
+  double* dts_log;     // receives the step's dt (first stage only), may be null
+};
+
+hipError_t launch_recon1(const DevMesh& M, const DevState& S, const DevWork& W, int opts,
+                         hipStream_t st);
+hipError_t launch_recon2(const DevMesh& M, const DevState& S, const DevWork& W, int opts,
+                         hipStream_t st);
+hipError_t launch_flux(const DevMesh& M, const DevState& S, const DevWork& W, int flux, int ws,
+                       double cor, unsigned long long* rbits, hipStream_t st);
+hipError_t launch_drain(const DevMesh& M, const DevState& S, const DevWork& W, hipStream_t st);
+hipError_t launch_update(const DevMesh& M, const DevState& S, const DevState& U0, const DevOut& O,
+                         const DevWork& W, const StageArgs& A, hipStream_t st);
+hipError_t launch_edge_field(const DevMesh& M, const DevState& S, const DevWork& W, double cor,
+                             double* edg, double* src, hipStream_t st);
+hipError_t launch_max_wp(const DevMesh& M, const DevWork& W, double* out, hipStream_t st);
+hipError_t launch_rhs_one(const DevMesh& M, const DevState& S, const DevWork& W, int i, double dt,
+                          double cor, double* out, hipStream_t st);
+
+}  // namespace swe

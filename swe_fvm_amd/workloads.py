@@ -1,0 +1,203 @@
+"""Seeded synthetic workloads: mesh, bathymetry and initial state.
+
+Equations are in the reference's non-dimensional form with g = 1
+(include/SpaceDisc.h:14, src/Fluxes.cpp:6-23).  All generators are
+deterministic in their seed.
+
+* thacker      — classic Thacker paraboloid on StructTriangMesh
+                 (examples/Tests.h:237-280, examples/Main.cpp:197-223)
+* lake_at_rest — flat surface over Gaussian bumps, two emerging
+* dam_break    — circular wet/dry dam break over a random Fourier bathymetry
+"""
+import math
+
+import numpy as np
+
+from .core import Domain, StructTriangMesh, jittered_mesh
+
+
+def triang_average(p0, p1, p2, f, n):
+    """TriangAverage<k, n> (include/PointOperations.h:20-44), vectorised over cells.
+
+    p0, p1, p2: (T, 2) vertex arrays; f maps (x, y) arrays to a (k, ...) array.
+    """
+    h = 1.0 / n
+    di = h * (p1 - p0)
+    dj = h * (p2 - p0)
+    dt = (1.0 / 3.0) * (di + dj)
+    total = None
+    pi = p0.copy()
+    for i in range(n):
+        pt = pi + dt
+        for _ in range(n - i - 1):
+            a = h * np.asarray(f(pt[:, 0], pt[:, 1]))
+            q = pt + dt
+            b = h * np.asarray(f(q[:, 0], q[:, 1]))
+            total = (a if total is None else total + a) + b
+            pt = pt + dj
+        a = h * np.asarray(f(pt[:, 0], pt[:, 1]))
+        total = a if total is None else total + a
+        pi = pi + di
+    return h * total
+
+
+class ClassicThacker:
+    """ClassicThackerTest (examples/Tests.h:237-280) on BowlTest bathymetry
+    b = delta ((x - mx)^2 + (y - my)^2 - 1) (examples/Tests.h:54-56)."""
+
+    def __init__(self, H0=1.25, p0=0.0, q0=0.0, cor=0.0, delta=1.0, mid=(0.0, 0.0)):
+        self.H0, self.p0, self.q0, self.cor, self.delta = H0, p0, q0, cor, delta
+        self.mx, self.my = mid
+        self.w = math.sqrt(cor * cor + 8.0 * delta)
+        qz = (q0 - 0.5 * cor) ** 2
+        rz = qz + 2.0 * H0 * H0 + p0 * p0 - 0.25 * self.w * self.w
+        self.a = math.sqrt(rz * rz + self.w * self.w * p0 * p0) / (rz + 0.5 * self.w * self.w)
+        self.b = math.atan(self.w * p0 / rz)
+        self.period = 2.0 * math.pi / self.w
+
+    def bath(self, x, y):
+        return self.delta * ((x - self.mx) ** 2 + (y - self.my) ** 2 - 1.0)
+
+    def _den(self, t):
+        return 1.0 - self.a * math.cos(self.w * t + self.b)
+
+    def p(self, t):
+        return 0.5 * self.w * self.a * math.sin(self.w * t + self.b) / self._den(t)
+
+    def q(self, t):
+        return (self.q0 - 0.5 * self.cor) * (1.0 - self.a * math.cos(self.b)) / self._den(t) + 0.5 * self.cor
+
+    def Hc(self, t):
+        return self.H0 * (1.0 - self.a * math.cos(self.b)) / self._den(t)
+
+    def Hxx(self, t):
+        qz = (self.q0 - 0.5 * self.cor) ** 2
+        az0 = (1.0 - self.a * math.cos(self.b)) ** 2
+        azt = self._den(t) ** 2
+        return (0.25 * self.w * self.w * (self.a * self.a - 1.0) + qz * az0) / azt
+
+    def h(self, x, y, t):
+        dx, dy = x - self.mx, y - self.my
+        return np.maximum(0.0, self.Hc(t) + 0.5 * self.Hxx(t) * (dx * dx + dy * dy))
+
+    def u(self, x, y, t):
+        return self.p(t) * (x - self.mx) + self.q(t) * (y - self.my)
+
+    def v(self, x, y, t):
+        return self.q(t) * (self.mx - x) + self.p(t) * (y - self.my)
+
+
+def _cell_vertices(xyz, topo):
+    tp = topo.tables()["tp"]
+    return xyz[tp[:, 0], :2], xyz[tp[:, 1], :2], xyz[tp[:, 2], :2]
+
+
+def thacker(n=100, L=4.0, quad=100, t=0.0):
+    """StructTriangMesh n x n on [-L/2, L/2]^2 (4 triangles per square), bowl
+    bathymetry, Thacker state averaged with TriangAverage<3, quad> and
+    w = <h> + b_cell (examples/Main.cpp:213-222)."""
+    th = ClassicThacker()
+    hh = L / n
+    xyz, topo = StructTriangMesh(n, n, hh, -0.5 * L, -0.5 * L)
+    xyz[:, 2] = th.bath(xyz[:, 0], xyz[:, 1])
+    dom = Domain(xyz, topo)
+    p0, p1, p2 = _cell_vertices(xyz, topo)
+    avg = triang_average(p0, p1, p2, lambda x, y: np.stack([th.h(x, y, t), th.u(x, y, t), th.v(x, y, t)]), quad)
+    bc = dom.centroids()[:, 2]
+    prim = np.stack([avg[0] + bc, avg[1], avg[2]], axis=1)
+    return dom, prim, th
+
+
+def _tri_wet_volume(z, level):
+    """Exact mean of max(0, level - b) over a triangle with linear b through
+    the vertex values z (T, 3)."""
+    zs = np.sort(z, axis=1)
+    a, b, c = zs[:, 0], zs[:, 1], zs[:, 2]
+    d1, d2, d3 = level - a, level - b, level - c
+    out = np.zeros(z.shape[0])
+    full = d3 >= 0
+    out[full] = (d1[full] + d2[full] + d3[full]) / 3.0
+    # one vertex wet: the wet region is a corner triangle
+    m1 = (d1 > 0) & (d2 <= 0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        out[m1] = d1[m1] ** 3 / (3.0 * (c[m1] - a[m1]) * (b[m1] - a[m1]))
+        # two vertices wet: full mean minus the dry corner at c
+        m2 = (d2 > 0) & (d3 < 0)
+        mean = (d1[m2] + d2[m2] + d3[m2]) / 3.0
+        out[m2] = mean + (-d3[m2]) ** 3 / (3.0 * (c[m2] - a[m2]) * (c[m2] - b[m2]))
+    return out
+
+
+def wall_taper(t, margin=0.05):
+    """0 within `margin` of the walls of [0, 1], 1 beyond 2 * margin, C1 smoothstep between."""
+    u = np.clip((np.minimum(t, 1.0 - t) - margin) / margin, 0.0, 1.0)
+    return u * u * (3.0 - 2.0 * u)
+
+
+def lake_at_rest(n=707, seed=1, level=1.0, jitter=0.25):
+    """Flat surface w = level over 8 Gaussian bumps A ~ U(0.2, 1.2),
+    s ~ U(0.02, 0.1); exactly two bumps emerge (A > level).  Partially wet
+    cells hold the exact wet volume of the linear bathymetry.
+
+    The bathymetry is tapered to exactly 0 near the walls: the reference's
+    wall flux uses the cell-mean depth (src/SpaceDisc.cpp:67-69) while the
+    pressure correction uses the edge depth (src/TimeDisc.cpp:35), so a lake
+    at rest is steady only where the bottom is flat along the walls."""
+    xyz, topo = jittered_mesh(n, jitter, seed)
+    rng = np.random.default_rng(seed)
+    A = np.concatenate([rng.uniform(1.05, 1.2, 2), rng.uniform(0.2, 0.95, 6)])
+    s = rng.uniform(0.02, 0.1, 8)
+    c = rng.uniform(0.15, 0.85, (8, 2))
+    x, y = xyz[:, 0], xyz[:, 1]
+    z = np.zeros_like(x)
+    for k in range(8):
+        z += A[k] * np.exp(-((x - c[k, 0]) ** 2 + (y - c[k, 1]) ** 2) / (2 * s[k] ** 2))
+    z = z * wall_taper(x) * wall_taper(y)
+    xyz[:, 2] = z
+    dom = Domain(xyz, topo)
+    tp = topo.tables()["tp"]
+    zc = z[tp]
+    bc = dom.centroids()[:, 2]
+    zmax = zc.max(axis=1)
+    prim = np.zeros((tp.shape[0], 3))
+    wet = zmax < level
+    prim[wet, 0] = level
+    part = ~wet
+    hbar = _tri_wet_volume(zc[part], level)
+    prim[part, 0] = bc[part] + hbar
+    return dom, prim
+
+
+def dam_break(n=2828, seed=3, modes=64, amp=0.1, dry_frac=0.05, jitter=0.25):
+    """Circular dam break (h = 1 for r < 0.2 around (0.5, 0.5), h = 0.1
+    elsewhere) over z = amp * S / max|S|, S a sum of `modes` random Fourier
+    modes with wave numbers in {-8..8}; circular dry pockets (h = 0) are added
+    until dry_frac of the cells are dry; u = v = 0."""
+    xyz, topo = jittered_mesh(n, jitter, seed)
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(-1.0, 1.0, modes)
+    m = rng.integers(-8, 9, modes)
+    k = rng.integers(-8, 9, modes)
+    ph = rng.uniform(0.0, 2.0 * np.pi, modes)
+    x, y = xyz[:, 0], xyz[:, 1]
+    S = np.zeros_like(x)
+    for q in range(modes):
+        S += a[q] * np.cos(2.0 * np.pi * (m[q] * x + k[q] * y) + ph[q])
+    xyz[:, 2] = amp * S / np.abs(S).max()
+    dom = Domain(xyz, topo)
+    cen = dom.centroids()
+    bc = cen[:, 2]
+    r = np.hypot(cen[:, 0] - 0.5, cen[:, 1] - 0.5)
+    h = np.where(r < 0.2, 1.0, 0.1)
+    T = h.shape[0]
+    dry = np.zeros(T, bool)
+    target = int(dry_frac * T)
+    while dry.sum() < target:
+        pc = rng.uniform(0.05, 0.95, 2)
+        rad = rng.uniform(0.01, 0.04)
+        if math.hypot(pc[0] - 0.5, pc[1] - 0.5) < 0.25 + rad:
+            continue
+        dry |= np.hypot(cen[:, 0] - pc[0], cen[:, 1] - pc[1]) < rad
+    h[dry] = 0.0
+    prim = np.stack([bc + h, np.zeros(T), np.zeros(T)], axis=1)
+    return dom, prim
