@@ -84,7 +84,18 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-only", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device-to-device copy of 2 GiB x 5 (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(2 ** 28, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": 2 * a.numel() * 8, "launches": 5}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))

@@ -207,6 +207,12 @@ int swe_create(const swe_topology* topo, const double* xyz, int flux, int wavesp
   ALLOC(s->W.f0, E); ALLOC(s->W.f1, E); ALLOC(s->W.f2, E);
   ALLOC(s->W.dtd, T);
   ALLOC(s->rbits, 4);
+  DevGeomOut G{};
+  ALLOC(G.cx, T); ALLOC(G.cy, T); ALLOC(G.bc, T); ALLOC(G.zmin, T); ALLOC(G.zmax, T);
+  ALLOC(G.area, T); ALLOC(G.iarea, T);
+  ALLOC(G.el, E); ALLOC(G.etx, E); ALLOC(G.ety, E); ALLOC(G.ex, E); ALLOC(G.ey, E); ALLOC(G.ez, E);
+  ALLOC(G.edmin, E);
+  ALLOC(G.bad, 1);
 #undef ALLOC
 #define H2D(dst, src, n)                                                                     \
   if ((e = hipMemcpy(dst, src, sizeof(*(src)) * size_t(n), hipMemcpyHostToDevice)) != hipSuccess) \
@@ -228,7 +234,19 @@ int swe_create(const swe_topology* topo, const double* xyz, int flux, int wavesp
     if ((e = hipMemset(s->su[b], 0, sizeof(double) * size_t(std::max<int64_t>(T, 1)))) != hipSuccess) return bail(e, "hipMemset");
     if ((e = hipMemset(s->sv[b], 0, sizeof(double) * size_t(std::max<int64_t>(T, 1)))) != hipSuccess) return bail(e, "hipMemset");
   }
-  s->M = DevMesh{dpx, dpy, dpz, dtp, dte, dtt, dfl, dep0, dep1, det0, det1, doff, dn2c, T, E, V};
+  s->M = DevMesh{dpx, dpy, dpz, dtp, dte, dtt, dfl, dep0, dep1, det0, det1, doff, dn2c, T, E, V,
+                 G.cx, G.cy, G.bc, G.zmin, G.zmax, G.area, G.iarea,
+                 G.el, G.etx, G.ety, G.ex, G.ey, G.ez, G.edmin};
+  if ((e = hipMemset(G.bad, 0, sizeof(unsigned int))) != hipSuccess) return bail(e, "hipMemset");
+  if ((e = launch_setup_geometry(s->M, G, s->stream)) != hipSuccess) return bail(e, "setup_geometry");
+  unsigned int bad = 0;
+  if ((e = hipMemcpyAsync(&bad, G.bad, sizeof(bad), hipMemcpyDeviceToHost, s->stream)) != hipSuccess)
+    return bail(e, "hipMemcpy");
+  if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) return bail(e, "hipStreamSynchronize");
+  if (bad) {
+    swe_destroy(s);
+    return fail(SWE_ERR_DOMAIN, "degenerate triangle: an edge's outward tangents are not opposite");
+  }
   *out = s;
   return SWE_OK;
 }

@@ -202,11 +202,12 @@ struct Slot {
 };
 
 // Fluxes::HLL<W> (include/Fluxes.h:14-54).  lw: length-to-wavespeed
-// candidate for the CFL minimum (unchanged when the flux returns early).
+// candidate for the CFL minimum (unchanged when the flux returns early);
+// dmn = min(2 Area(from) / L, 2 Area(to) / L).
 template <int WS>
 __device__ __forceinline__ void flux_hll(double nx, double ny, double be, const Slot& L,
-                                         const Slot& R, double areaL, double areaR, double le,
-                                         double cor, double& lw, double F[3]) {
+                                         const Slot& R, double dmn, double cor, double& lw,
+                                         double F[3]) {
   double ul = L.u * nx + L.v * ny;
   double hl = L.w - be;
   double ur = R.u * nx + R.v * ny;
@@ -217,9 +218,8 @@ __device__ __forceinline__ void flux_hll(double nx, double ny, double be, const 
   wavespeed<WS>(ul, hl, ur, hr, a0, a1);
   double al = dmin(0., a0), ar = dmax(0., a1);
   if (ar - al <= 1e-10) return;
-  double dl = 2. * areaL / le, dr = 2. * areaR / le;
   double c = fabs(cor);
-  lw = dmin(dl, dr) / (c + dmax(-al, ar));
+  lw = dmn / (c + dmax(-al, ar));
   double Ul0 = hl, Ul1 = hl * L.u, Ul2 = hl * L.v;
   double Ur0 = hr, Ur1 = hr * R.u, Ur2 = hr * R.v;
   double Fl0, Fl1, Fl2, Fr0, Fr1, Fr2;
@@ -234,8 +234,8 @@ __device__ __forceinline__ void flux_hll(double nx, double ny, double be, const 
 // Fluxes::HLLC<W> (include/Fluxes.h:56-111)
 template <int WS>
 __device__ __forceinline__ void flux_hllc(double tx, double ty, double be, const Slot& L,
-                                          const Slot& R, double areaL, double areaR, double le,
-                                          double cor, double& lw, double F[3]) {
+                                          const Slot& R, double dmn, double cor, double& lw,
+                                          double F[3]) {
   double nx = ty, ny = -tx;
   double ul = L.u * nx + L.v * ny;
   double vl = L.u * tx + L.v * ty;
@@ -249,9 +249,8 @@ __device__ __forceinline__ void flux_hllc(double tx, double ty, double be, const
   wavespeed<WS>(ul, hl, ur, hr, al, ar);
   double ustar = (ar - ur) * hr * ur - (al - ul) * hl * ul + 0.5 * (hl * hl - hr * hr);
   ustar /= (hr * (ar - ur) - hl * (al - ul));
-  double dl = 2. * areaL / le, dr = 2. * areaR / le;
   double c = fabs(cor);
-  lw = dmin(dl, dr) / (c + dmax(kTol, dmax(al, ar)));
+  lw = dmn / (c + dmax(kTol, dmax(al, ar)));
   double U0, U1, U2, Fe0, Fe1, Fe2;
   if (ustar <= 0) {
     double us = vr * tx + ustar * ty;

@@ -17,8 +17,13 @@
 //   k_update  per cell  — fixed-order gather of the three edge fluxes, the
 //                         well-balanced source, SSP-RK stage combination and
 //                         ConsAssigner write-back.
+// Static geometry (centroids, areas, edge lengths, tangents, midpoints) is
+// computed once by k_setup_geometry with the same device functions, so the
+// step reads it instead of chasing cell -> node indirections.
 // Every reduction is either order-independent (max, min) or a fixed-order
-// per-cell gather, so results are deterministic run to run.
+// per-cell gather, so results are deterministic run to run.  Local arrays
+// are only indexed by compile-time constants (unrolled loops) or through
+// register selects, so nothing is placed in scratch or LDS.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -45,21 +50,22 @@ __device__ __forceinline__ int64_t global_index() {
 __device__ __forceinline__ V3 node(const DevMesh& M, int n) {
   return {M.px[n], M.py[n], M.pz[n]};
 }
-__device__ __forceinline__ void cell_nodes(const DevMesh& M, int i, V3 p[3]) {
-  p[0] = node(M, M.tp[i]);
-  p[1] = node(M, M.tp[M.T + i]);
-  p[2] = node(M, M.tp[2 * M.T + i]);
-}
 __device__ __forceinline__ int ep0_local(uint32_t fl, int k) { return (fl >> (4 + 4 * k)) & 3; }
 __device__ __forceinline__ int ep1_local(uint32_t fl, int k) { return (fl >> (6 + 4 * k)) & 3; }
 __device__ __forceinline__ bool sgn_pos(uint32_t fl, int k) { return (fl >> k) & 1u; }
+__device__ __forceinline__ V3 pick(const V3& a, const V3& b, const V3& c, int i) {
+  return i == 0 ? a : (i == 1 ? b : c);
+}
+// midpoint of edge k from the cell's own vertices
+__device__ __forceinline__ V3 edge_mid(const V3& p0, const V3& p1, const V3& p2, uint32_t fl, int k) {
+  return midpoint(pick(p0, p1, p2, ep0_local(fl, k)), pick(p0, p1, p2, ep1_local(fl, k)));
+}
 
 // Final MUSCL of cell c: pass-2 result for partially wet cells.
 struct Muscl {
   double w0, gx, gy, u0, v0;
 };
-__device__ __forceinline__ Muscl load_muscl(const DevMesh& M, const DevState& S,
-                                            const DevWork& W, int c) {
+__device__ __forceinline__ Muscl load_muscl(const DevState& S, const DevWork& W, int c) {
   int k = W.cls[c];
   Muscl m;
   if (k == 1) {
@@ -76,14 +82,60 @@ __device__ __forceinline__ Muscl load_muscl(const DevMesh& M, const DevState& S,
 }
 
 // AtPoint + PrimAssigner: the edge slot written by UpdateInterfaceValues
-// (src/SpaceDisc.cpp:24-25).
-__device__ __forceinline__ Slot slot_value(const Muscl& m, const V3& c, const V3& E) {
-  bool wet;
-  double aw = at_point_w(m.w0, m.gx, m.gy, c, E, wet);
-  double au = wet ? m.u0 : 0., av = wet ? m.v0 : 0.;
+// (src/SpaceDisc.cpp:24-25); (cx, cy) is the cell centroid.
+__device__ __forceinline__ Slot slot_value(const Muscl& m, double cx, double cy, const V3& E) {
+  double dx = E.x - cx, dy = E.y - cy;
+  double aw = m.w0 + (m.gx * dx + m.gy * dy);
+  bool wet = (aw - E.z) >= 0;
   Slot s;
-  prim_assign(E.z, aw, au, av, s.w, s.u, s.v);
+  if (!wet) aw = E.z;
+  prim_assign(E.z, aw, wet ? m.u0 : 0., wet ? m.v0 : 0., s.w, s.u, s.v);
   return s;
+}
+
+// -------------------------------------------------------------- setup --
+// Static geometry with the formulas of Domain (src/Bathymetry.cpp:24-90).
+__global__ void __launch_bounds__(256) k_setup_cells(DevMesh M, DevGeomOut G) {
+  int64_t i = global_index();
+  if (i >= M.T) return;
+  V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[M.T + i]), p2 = node(M, M.tp[2 * M.T + i]);
+  V3 c = centroid(p0, p1, p2);
+  G.cx[i] = c.x;
+  G.cy[i] = c.y;
+  G.bc[i] = c.z;
+  G.zmin[i] = dmin(dmin(p0.z, p1.z), p2.z);
+  G.zmax[i] = dmax(dmax(p0.z, p1.z), p2.z);
+  double a = tri_area(p0, p1, p2);
+  G.area[i] = a;
+  G.iarea[i] = 1. / a;
+}
+
+__global__ void __launch_bounds__(256) k_setup_edges(DevMesh M, DevGeomOut G) {
+  int64_t e = global_index();
+  if (e >= M.E) return;
+  V3 Pa = node(M, M.ep0[e]), Pb = node(M, M.ep1[e]);
+  double le = edge_len(Pa, Pb);
+  V3 E = midpoint(Pa, Pb);
+  int f = M.et0[e], t = M.et1[e];
+  V3 cf = {G.cx[f], G.cy[f], 0.};
+  double tx, ty;
+  tangent(Pa, Pb, le, cf, tx, ty);
+  G.el[e] = le;
+  G.etx[e] = tx;
+  G.ety[e] = ty;
+  G.ex[e] = E.x;
+  G.ey[e] = E.y;
+  G.ez[e] = E.z;
+  double dm = 0.;
+  if (t >= 0) {
+    V3 ct = {G.cx[t], G.cy[t], 0.};
+    double ux, uy;
+    tangent(Pa, Pb, le, ct, ux, uy);
+    if (!(ux == -tx && uy == -ty)) atomicAdd(G.bad, 1u);
+    double dl = 2. * G.area[f] / le, dr = 2. * G.area[t] / le;
+    dm = dmin(dl, dr);
+  }
+  G.edmin[e] = dm;
 }
 
 // ---------------------------------------------------------------- pass 1 --
@@ -94,63 +146,69 @@ template <bool HEAD_GRAD>
 __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W) {
   int64_t i = global_index();
   if (i >= M.T) return;
-  V3 p[3];
-  cell_nodes(M, (int)i, p);
-  V3 c = centroid(p[0], p[1], p[2]);
-  double w = S.w[i];
-  double bc = c.z;
-  uint32_t fl = M.cflag[i];
+  const double w = S.w[i];
+  const double bc = M.bc[i];
+  const uint32_t fl = M.cflag[i];
   double w0 = w, gx = 0., gy = 0.;
   int8_t k;
   if (!is_wet(w - bc)) {
     k = 0;
     w0 = bc;
-    gradient(p[0], p[1], p[2], gx, gy);
-  } else if ((fl & CELL_BND) || !(dmax(dmax(p[0].z, p[1].z), p[2].z) < w)) {
+    V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[M.T + i]), p2 = node(M, M.tp[2 * M.T + i]);
+    gradient(p0, p1, p2, gx, gy);
+  } else if ((fl & CELL_BND) || !(M.zmax[i] < w)) {
     k = 1;
-    w0 = partwet1_w(p[0].z, p[1].z, p[2].z, bc, w);
+    const double zmx = M.zmax[i], zmn = M.zmin[i];
+    // b13 = zmax, b23 = zmin; partwet1_w takes the three vertex z's only
+    // through max/min and 3 bc - b23 - b13
+    w0 = partwet1_w(zmn, zmn, zmx, bc, w);
   } else {
     k = 2;
+    const double cx = M.cx[i], cy = M.cy[i];
+    V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[M.T + i]), p2 = node(M, M.tp[2 * M.T + i]);
     V3 pts[3];
     V3 E[3];
-    bool any_dry = false;
     double wn[3];
+    bool any_dry = false;
 #pragma unroll
     for (int kk = 0; kk < 3; ++kk) {
-      E[kk] = midpoint(p[ep0_local(fl, kk)], p[ep1_local(fl, kk)]);
+      E[kk] = edge_mid(p0, p1, p2, fl, kk);
       int j = M.tt[kk * M.T + i];
-      V3 q[3];
-      cell_nodes(M, j, q);
-      V3 cj = centroid(q[0], q[1], q[2]);
       double wj = S.w[j];
       wn[kk] = wj;
+      double bcj = M.bc[j];
       uint32_t flj = M.cflag[j];
-      if (!(flj & CELL_BND) && dmax(dmax(q[0].z, q[1].z), q[2].z) < wj) {
-        pts[kk] = {cj.x, cj.y, HEAD_GRAD ? cj.z : wj};
-      } else if (!is_wet(wj - cj.z)) {
+      double zmxj = M.zmax[j];
+      if (!(flj & CELL_BND) && zmxj < wj) {
+        pts[kk] = {M.cx[j], M.cy[j], HEAD_GRAD ? bcj : wj};
+      } else if (!is_wet(wj - bcj)) {
         any_dry = true;
+        pts[kk] = {0., 0., 0.};
       } else {
-        double w0j = partwet1_w(q[0].z, q[1].z, q[2].z, cj.z, wj);
-        bool wet;
-        double a = at_point_w(w0j, 0., 0., cj, E[kk], wet);
+        double w0j = partwet1_w(M.zmin[j], M.zmin[j], zmxj, bcj, wj);
+        double dx = E[kk].x - M.cx[j], dy = E[kk].y - M.cy[j];
+        double a = w0j + (0. * dx + 0. * dy);
+        if (!((a - E[kk].z) >= 0)) a = E[kk].z;
         double val = 0.5 * (w + a);
         pts[kk] = {E[kk].x, E[kk].y, HEAD_GRAD ? E[kk].z : val};
       }
-      if (any_dry) break;
     }
     if (!any_dry) {
       gradient(pts[0], pts[1], pts[2], gx, gy);
       // vertex positivity (src/MUSCLObject.cpp:66-72)
       const double c23 = 1. - 1. / 3., c13 = -1. / 3.;
       bool all_wet = true;
-#pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        double wx0 = p[0].x * (m == 0 ? c23 : c13), wy0 = p[0].y * (m == 0 ? c23 : c13);
-        double wx1 = p[1].x * (m == 1 ? c23 : c13), wy1 = p[1].y * (m == 1 ? c23 : c13);
-        double wx2 = p[2].x * (m == 2 ? c23 : c13), wy2 = p[2].y * (m == 2 ? c23 : c13);
-        double dxm = (wx0 + wx1) + wx2, dym = (wy0 + wy1) + wy2;
-        double wp = (gx * dxm + gy * dym) + w;
-        if (!is_wet(wp - p[m].z)) all_wet = false;
+      {
+        double dxm = (p0.x * c23 + p1.x * c13) + p2.x * c13, dym = (p0.y * c23 + p1.y * c13) + p2.y * c13;
+        if (!is_wet(((gx * dxm + gy * dym) + w) - p0.z)) all_wet = false;
+      }
+      {
+        double dxm = (p0.x * c13 + p1.x * c23) + p2.x * c13, dym = (p0.y * c13 + p1.y * c23) + p2.y * c13;
+        if (!is_wet(((gx * dxm + gy * dym) + w) - p1.z)) all_wet = false;
+      }
+      {
+        double dxm = (p0.x * c13 + p1.x * c13) + p2.x * c23, dym = (p0.y * c13 + p1.y * c13) + p2.y * c23;
+        if (!is_wet(((gx * dxm + gy * dym) + w) - p2.z)) all_wet = false;
       }
       if (!all_wet) { gx = 0.; gy = 0.; }
       // TVD limiter on w (src/MUSCLObject.cpp:74-81)
@@ -158,7 +216,7 @@ __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W
 #pragma unroll
       for (int kk = 0; kk < 3; ++kk) {
         double vtmin = dmin(w, wn[kk]), vtmax = dmax(w, wn[kk]);
-        double vek = w + (gx * (E[kk].x - c.x) + gy * (E[kk].y - c.y));
+        double vek = w + (gx * (E[kk].x - cx) + gy * (E[kk].y - cy));
         if (!((vtmin <= vek) && (vek <= vtmax))) tvd = 0.;
       }
       gx = tvd * gx;
@@ -175,17 +233,14 @@ __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W
 // ReconstructPartWetCell2 (src/MUSCLObject.cpp:114-191) for every partially
 // wet cell (src/SpaceDisc.cpp:47-51).  The node maximum m_max_wp at the
 // lowest vertex is max(z, AtPoint of each incident cell's pass-1 MUSCL).
-__device__ inline double node_max_pass1(const DevMesh& M, const DevWork& W, int n) {
-  V3 P = node(M, n);
+__device__ inline double node_max_pass1(const DevMesh& M, const DevWork& W, int n, const V3& P) {
   double mx = P.z;
   int b = M.n2c_off[n], e = M.n2c_off[n + 1];
   for (int q = b; q < e; ++q) {
     int c = M.n2c[q];
-    V3 r[3];
-    cell_nodes(M, c, r);
-    V3 cc = centroid(r[0], r[1], r[2]);
-    bool wet;
-    double a = at_point_w(W.mw[c], W.mgx[c], W.mgy[c], cc, P, wet);
+    double dx = P.x - M.cx[c], dy = P.y - M.cy[c];
+    double a = W.mw[c] + (W.mgx[c] * dx + W.mgy[c] * dy);
+    if (!((a - P.z) >= 0)) a = P.z;
     mx = dmax(mx, a);
   }
   return mx;
@@ -196,42 +251,42 @@ __global__ void __launch_bounds__(256) k_recon2(DevMesh M, DevState S, DevWork W
   int64_t i = global_index();
   if (i >= M.T) return;
   if (W.cls[i] != 1) return;
-  int ip[3] = {M.tp[i], M.tp[M.T + i], M.tp[2 * M.T + i]};
-  V3 Q[3] = {node(M, ip[0]), node(M, ip[1]), node(M, ip[2])};
-  V3 c = centroid(Q[0], Q[1], Q[2]);
+  int i0 = M.tp[i], i1 = M.tp[M.T + i], i2 = M.tp[2 * M.T + i];
+  V3 Q0 = node(M, i0), Q1 = node(M, i1), Q2 = node(M, i2);
+  const double cx = M.cx[i], cy = M.cy[i], bc = M.bc[i];
   // sort by z with the reference's three conditional swaps
   {
     V3 t; int ti;
-    if (Q[0].z > Q[1].z) { t = Q[0]; Q[0] = Q[1]; Q[1] = t; ti = ip[0]; ip[0] = ip[1]; ip[1] = ti; }
-    if (Q[1].z > Q[2].z) { t = Q[1]; Q[1] = Q[2]; Q[2] = t; ti = ip[1]; ip[1] = ip[2]; ip[2] = ti; }
-    if (Q[0].z > Q[1].z) { t = Q[0]; Q[0] = Q[1]; Q[1] = t; ti = ip[0]; ip[0] = ip[1]; ip[1] = ti; }
+    if (Q0.z > Q1.z) { t = Q0; Q0 = Q1; Q1 = t; ti = i0; i0 = i1; i1 = ti; }
+    if (Q1.z > Q2.z) { t = Q1; Q1 = Q2; Q2 = t; ti = i1; i1 = i2; i2 = ti; }
+    if (Q0.z > Q1.z) { t = Q0; Q0 = Q1; Q1 = t; ti = i0; i0 = i1; i1 = ti; }
   }
+  (void)i1;
+  (void)i2;
   double w = S.w[i];
-  double b23 = Q[0].z, b12 = Q[1].z, b13 = Q[2].z;
+  double b23 = Q0.z, b12 = Q1.z, b13 = Q2.z;
   double w0 = W.mw[i], gx = 0., gy = 0.;  // fallback: pass-1 (PartWet1) result
   bool fallback = (w > b13) || (b13 - b23 < kTol);
   if (!fallback) {
-    double w23 = node_max_pass1(M, W, ip[0]);
+    double w23 = node_max_pass1(M, W, i0, Q0);
     double h23 = w23 - b23;
     double ratio_b = (b12 - b23) / (b13 - b23);
     double hd1 = 1. / 3. * h23 * ratio_b;
     double hd2 = 1. / 3. * h23 * (2. * b13 - b12 - b23) / (b13 - b23);
-    double hi = w - c.z;
+    double hi = w - bc;
     double ratio_h = hi / h23;
-    V3 p0 = Q[0], p1, p2;
+    V3 p0 = Q0, p1, p2;
     if (!HEAD_PW2) p0.z = w23;
     if (hi <= hd1) {
       double k2 = sqrt(3. * ratio_h / ratio_b);
-      p1 = {k2 * Q[1].x + (1. - k2) * Q[0].x, k2 * Q[1].y + (1. - k2) * Q[0].y,
-            k2 * Q[1].z + (1. - k2) * Q[0].z};
+      p1 = {k2 * Q1.x + (1. - k2) * Q0.x, k2 * Q1.y + (1. - k2) * Q0.y, k2 * Q1.z + (1. - k2) * Q0.z};
       double k3 = sqrt(3. * ratio_h * ratio_b);
-      p2 = {k3 * Q[2].x + (1. - k3) * Q[0].x, k3 * Q[2].y + (1. - k3) * Q[0].y,
-            k3 * Q[2].z + (1. - k3) * Q[0].z};
+      p2 = {k3 * Q2.x + (1. - k3) * Q0.x, k3 * Q2.y + (1. - k3) * Q0.y, k3 * Q2.z + (1. - k3) * Q0.z};
     } else if (hi >= hd2) {
       double delta_w = 1.5 * (hi - hd2);
-      p1 = Q[1];
+      p1 = Q1;
       if (!HEAD_PW2) { p1.z += delta_w; p1.z += (1. - ratio_b) * h23; }
-      p2 = Q[2];
+      p2 = Q2;
       p2.z += delta_w;
     } else {
       double alpha = 3. * ratio_h;
@@ -243,16 +298,15 @@ __global__ void __launch_bounds__(256) k_recon2(DevMesh M, DevState S, DevWork W
       } else {
         double k3 = 1. - beta * (1. - k1);
         double bp1 = k1 * b13 + (1. - k1) * b12;
-        p1 = Q[1];
+        p1 = Q1;
         if (!HEAD_PW2) p1.z = b12 + (k1 / k3) * beta * h23;
-        p2 = {k1 * Q[2].x + (1. - k1) * Q[1].x, k1 * Q[2].y + (1. - k1) * Q[1].y,
-              k1 * Q[2].z + (1. - k1) * Q[1].z};
+        p2 = {k1 * Q2.x + (1. - k1) * Q1.x, k1 * Q2.y + (1. - k1) * Q1.y, k1 * Q2.z + (1. - k1) * Q1.z};
         p2.z = bp1;
       }
     }
     if (!fallback) {
       gradient(p0, p1, p2, gx, gy);
-      w0 = w23 + (gx * (c.x - Q[0].x) + gy * (c.y - Q[0].y));
+      w0 = w23 + (gx * (cx - Q0.x) + gy * (cy - Q0.y));
     }
   }
   W.aw[i] = w0;
@@ -262,7 +316,7 @@ __global__ void __launch_bounds__(256) k_recon2(DevMesh M, DevState S, DevWork W
 
 // ------------------------------------------------------------------ flux --
 __device__ __forceinline__ void block_min_bits(double v, unsigned long long* target) {
-  // wave-level min by DPP/shuffle, then one LDS slot per wave
+  // wave-level min by cross-lane shuffles, then one LDS slot per wave
   __shared__ double red[16];
   for (int off = 32; off > 0; off >>= 1) v = dmin(v, __shfl_xor(v, off, 64));
   int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -284,30 +338,19 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, 
   int64_t e = global_index();
   double lw = 1.0;
   if (e < M.E) {
-    int a = M.ep0[e], b = M.ep1[e];
-    V3 Pa = node(M, a), Pb = node(M, b);
-    double le = edge_len(Pa, Pb);
-    V3 E = midpoint(Pa, Pb);
-    int f = M.et0[e], t = M.et1[e];
-    V3 pf[3];
-    cell_nodes(M, f, pf);
-    V3 cf = centroid(pf[0], pf[1], pf[2]);
-    double tx, ty;
-    tangent(Pa, Pb, le, cf, tx, ty);
-    double nx = ty, ny = -tx;
+    const double tx = M.etx[e], ty = M.ety[e];
+    const double nx = ty, ny = -tx;
+    const int f = M.et0[e], t = M.et1[e];
     double F[3];
     if (t < 0) {
-      double h = S.w[f] - cf.z;
+      double h = S.w[f] - M.bc[f];
       elem_flux(nx, ny, h, 0., 0., F[0], F[1], F[2]);
     } else {
-      V3 pt[3];
-      cell_nodes(M, t, pt);
-      V3 ct = centroid(pt[0], pt[1], pt[2]);
-      double af = tri_area(pf[0], pf[1], pf[2]), at = tri_area(pt[0], pt[1], pt[2]);
-      Slot L = slot_value(load_muscl(M, S, W, f), cf, E);
-      Slot R = slot_value(load_muscl(M, S, W, t), ct, E);
-      if (FLUX == 0) flux_hll<WS>(nx, ny, E.z, L, R, af, at, le, cor, lw, F);
-      else flux_hllc<WS>(tx, ty, E.z, L, R, af, at, le, cor, lw, F);
+      const V3 E = {M.ex[e], M.ey[e], M.ez[e]};
+      Slot L = slot_value(load_muscl(S, W, f), M.cx[f], M.cy[f], E);
+      Slot R = slot_value(load_muscl(S, W, t), M.cx[t], M.cy[t], E);
+      if (FLUX == 0) flux_hll<WS>(nx, ny, E.z, L, R, M.edmin[e], cor, lw, F);
+      else flux_hllc<WS>(tx, ty, E.z, L, R, M.edmin[e], cor, lw, F);
     }
     W.f0[e] = F[0];
     W.f1[e] = F[1];
@@ -322,10 +365,7 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, 
 __global__ void __launch_bounds__(256) k_drain(DevMesh M, DevState S, DevWork W) {
   int64_t i = global_index();
   if (i >= M.T) return;
-  V3 p[3];
-  cell_nodes(M, (int)i, p);
-  V3 c = centroid(p[0], p[1], p[2]);
-  double h = S.w[i] - c.z;
+  double h = S.w[i] - M.bc[i];
   double d;
   if (!is_wet(h)) {
     d = 0.;
@@ -337,7 +377,7 @@ __global__ void __launch_bounds__(256) k_drain(DevMesh M, DevState S, DevWork W)
       double fe = W.f0[M.te[k * M.T + i]];
       sum += dmax(0., sgn_pos(fl, k) ? fe : -fe);
     }
-    d = sum > kTol ? tri_area(p[0], p[1], p[2]) * h / sum : INFINITY;
+    d = sum > kTol ? M.area[i] * h / sum : INFINITY;
   }
   W.dtd[i] = d;
 }
@@ -346,44 +386,45 @@ __global__ void __launch_bounds__(256) k_drain(DevMesh M, DevState S, DevWork W)
 // TimeDisc::RHS (src/TimeDisc.cpp:3-41) + stage combination
 // (src/Solvers.cpp:11-13,23-25,30-32,42-58) + ConsAssigner.
 __device__ __forceinline__ void rhs_cell(const DevMesh& M, const DevState& S, const DevWork& W,
-                                         int i, const V3 p[3], const V3& c, double dts,
-                                         double cor, double res[3]) {
-  uint32_t fl = M.cflag[i];
-  Muscl mu = load_muscl(M, S, W, i);
-  double i_area = 1. / tri_area(p[0], p[1], p[2]);
-  double dti = W.dtd[i];
-  res[0] = res[1] = res[2] = 0.;
+                                         int i, double dts, double cor, double& r0, double& r1,
+                                         double& r2) {
+  const uint32_t fl = M.cflag[i];
+  const Muscl mu = load_muscl(S, W, i);
+  const double cx = M.cx[i], cy = M.cy[i];
+  const double i_area = M.iarea[i];
+  const double dti = W.dtd[i];
+  r0 = 0.; r1 = 0.; r2 = 0.;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    int e = M.te[k * M.T + i];
-    int nb = M.tt[k * M.T + i];
-    double sg = sgn_pos(fl, k) ? 1. : -1.;
-    double F0 = W.f0[e], F1 = W.f1[e], F2 = W.f2[e];
-    double dtik = nb < 0 ? INFINITY : W.dtd[nb];
-    double dtk = (sg * F0) > 0. ? dmin(dts, dti) : dmin(dts, dtik);
-    const V3& Pa = p[ep0_local(fl, k)];
-    const V3& Pb = p[ep1_local(fl, k)];
-    double le = edge_len(Pa, Pb);
-    double c_ek = i_area * le;
-    V3 E = midpoint(Pa, Pb);
-    Slot s = slot_value(mu, c, E);
-    double h_ek = s.w - E.z;
+    const int e = M.te[k * M.T + i];
+    const int nb = M.tt[k * M.T + i];
+    const bool own = sgn_pos(fl, k);
+    const double sg = own ? 1. : -1.;
+    const double F0 = W.f0[e], F1 = W.f1[e], F2 = W.f2[e];
+    const double dtik = nb < 0 ? INFINITY : W.dtd[nb];
+    const double dtk = (sg * F0) > 0. ? dmin(dts, dti) : dmin(dts, dtik);
+    const double le = M.el[e];
+    const double c_ek = i_area * le;
+    const V3 E = {M.ex[e], M.ey[e], M.ez[e]};
+    const Slot s = slot_value(mu, cx, cy, E);
+    const double h_ek = s.w - E.z;
     double src1 = mu.gx, src2 = mu.gy;
     src1 += cor * (-s.v);
     src2 += cor * s.u;
-    double tx, ty;
-    tangent(Pa, Pb, le, c, tx, ty);
-    double nx = ty, ny = -tx;
-    double cf = dtk * sg * c_ek;
-    res[0] -= cf * F0;
-    res[1] -= cf * F1;
-    res[2] -= cf * F2;
-    double cs = dts * (1. / 3.);
-    res[1] -= cs * src1 * h_ek;
-    res[2] -= cs * src2 * h_ek;
-    double q = 0.5 * h_ek * h_ek;
-    res[1] += dtk * (nx * c_ek * q);
-    res[2] += dtk * (ny * c_ek * q);
+    // Norm(e, i): the stored tangent is outward of et0; exact negation for et1
+    const double tx = own ? M.etx[e] : -M.etx[e];
+    const double ty = own ? M.ety[e] : -M.ety[e];
+    const double nx = ty, ny = -tx;
+    const double cf = dtk * sg * c_ek;
+    r0 -= cf * F0;
+    r1 -= cf * F1;
+    r2 -= cf * F2;
+    const double cs = dts * (1. / 3.);
+    r1 -= cs * src1 * h_ek;
+    r2 -= cs * src2 * h_ek;
+    const double q = 0.5 * h_ek * h_ek;
+    r1 += dtk * (nx * c_ek * q);
+    r2 += dtk * (ny * c_ek * q);
   }
 }
 
@@ -396,30 +437,27 @@ __device__ __forceinline__ double stage_dt(const StageArgs& A) {
 __global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, DevState U0, DevOut O,
                                                 DevWork W, StageArgs A) {
   int64_t i = global_index();
-  double dt = stage_dt(A);
+  const double dt = stage_dt(A);
   if (A.dts_log && blockIdx.x == 0 && threadIdx.x == 0) A.dts_log[0] = dt;
   if (i >= M.T) return;
-  V3 p[3];
-  cell_nodes(M, (int)i, p);
-  V3 c = centroid(p[0], p[1], p[2]);
-  double bc = c.z;
-  double R[3];
-  rhs_cell(M, S, W, (int)i, p, c, A.coef * dt, A.cor, R);
-  double w = S.w[i], u = S.u[i], v = S.v[i];
-  double h = w - bc;
-  double Uc0 = h, Uc1 = h * u, Uc2 = h * v;
+  const double bc = M.bc[i];
+  double R0, R1, R2;
+  rhs_cell(M, S, W, (int)i, A.coef * dt, A.cor, R0, R1, R2);
+  const double w = S.w[i], u = S.u[i], v = S.v[i];
+  const double h = w - bc;
+  const double Uc0 = h, Uc1 = h * u, Uc2 = h * v;
   double r0, r1, r2;
   if (A.a0 != 0.) {
-    double w0 = U0.w[i], u0 = U0.u[i], v0 = U0.v[i];
-    double h0 = w0 - bc;
-    double Ua0 = h0, Ua1 = h0 * u0, Ua2 = h0 * v0;
-    r0 = (A.a0 * Ua0 + A.a1 * Uc0) + R[0];
-    r1 = (A.a0 * Ua1 + A.a1 * Uc1) + R[1];
-    r2 = (A.a0 * Ua2 + A.a1 * Uc2) + R[2];
+    const double w0 = U0.w[i], u0 = U0.u[i], v0 = U0.v[i];
+    const double h0 = w0 - bc;
+    const double Ua0 = h0, Ua1 = h0 * u0, Ua2 = h0 * v0;
+    r0 = (A.a0 * Ua0 + A.a1 * Uc0) + R0;
+    r1 = (A.a0 * Ua1 + A.a1 * Uc1) + R1;
+    r2 = (A.a0 * Ua2 + A.a1 * Uc2) + R2;
   } else {
-    r0 = Uc0 + R[0];
-    r1 = Uc1 + R[1];
-    r2 = Uc2 + R[2];
+    r0 = Uc0 + R0;
+    r1 = Uc1 + R1;
+    r2 = Uc2 + R2;
   }
   double ow, ou, ov;
   cons_assign(bc, r0, r1, r2, ow, ou, ov);
@@ -439,17 +477,15 @@ __global__ void __launch_bounds__(256) k_edge_field(DevMesh M, DevState S, DevWo
                                                     double* edg, double* src) {
   int64_t i = global_index();
   if (i >= M.T) return;
-  V3 p[3];
-  cell_nodes(M, (int)i, p);
-  V3 c = centroid(p[0], p[1], p[2]);
-  uint32_t fl = M.cflag[i];
-  Muscl mu = load_muscl(M, S, W, (int)i);
+  const Muscl mu = load_muscl(S, W, (int)i);
+  const double cx = M.cx[i], cy = M.cy[i];
+#pragma unroll
   for (int k = 0; k < 3; ++k) {
     int e = M.te[k * M.T + i];
     int nb = M.tt[k * M.T + i];
     int64_t s = 2 * (int64_t)e + (((int)i < nb) ? 1 : 0);
-    V3 E = midpoint(p[ep0_local(fl, k)], p[ep1_local(fl, k)]);
-    Slot sv = slot_value(mu, c, E);
+    V3 E = {M.ex[e], M.ey[e], M.ez[e]};
+    Slot sv = slot_value(mu, cx, cy, E);
     edg[3 * s] = sv.w;
     edg[3 * s + 1] = sv.u;
     edg[3 * s + 2] = sv.v;
@@ -471,12 +507,15 @@ __global__ void __launch_bounds__(256) k_max_wp(DevMesh M, DevWork W, double* ou
   double mx = P.z;
   for (int q = M.n2c_off[n]; q < M.n2c_off[n + 1]; ++q) {
     int c = M.n2c[q];
-    V3 r[3];
-    cell_nodes(M, c, r);
-    V3 cc = centroid(r[0], r[1], r[2]);
-    bool wet;
-    mx = dmax(mx, at_point_w(W.mw[c], W.mgx[c], W.mgy[c], cc, P, wet));
-    if (W.cls[c] == 1) mx = dmax(mx, at_point_w(W.aw[c], W.agx[c], W.agy[c], cc, P, wet));
+    double dx = P.x - M.cx[c], dy = P.y - M.cy[c];
+    double a = W.mw[c] + (W.mgx[c] * dx + W.mgy[c] * dy);
+    if (!((a - P.z) >= 0)) a = P.z;
+    mx = dmax(mx, a);
+    if (W.cls[c] == 1) {
+      double b = W.aw[c] + (W.agx[c] * dx + W.agy[c] * dy);
+      if (!((b - P.z) >= 0)) b = P.z;
+      mx = dmax(mx, b);
+    }
   }
   out[n] = mx;
 }
@@ -484,10 +523,7 @@ __global__ void __launch_bounds__(256) k_max_wp(DevMesh M, DevWork W, double* ou
 __global__ void k_rhs_one(DevMesh M, DevState S, DevWork W, int i, double dt, double cor,
                           double* out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  V3 p[3];
-  cell_nodes(M, i, p);
-  V3 c = centroid(p[0], p[1], p[2]);
-  rhs_cell(M, S, W, i, p, c, dt, cor, out);
+  rhs_cell(M, S, W, i, dt, cor, out[0], out[1], out[2]);
 }
 
 // -------------------------------------------------------------- launch --
@@ -498,6 +534,11 @@ static inline dim3 grid_for(int64_t n) {
   return dim3((unsigned)nb);
 }
 
+hipError_t launch_setup_geometry(const DevMesh& M, const DevGeomOut& G, hipStream_t st) {
+  hipLaunchKernelGGL(k_setup_cells, grid_for(M.T), dim3(256), 0, st, M, G);
+  hipLaunchKernelGGL(k_setup_edges, grid_for(M.E), dim3(256), 0, st, M, G);
+  return hipGetLastError();
+}
 hipError_t launch_recon1(const DevMesh& M, const DevState& S, const DevWork& W, int opts,
                          hipStream_t st) {
   if (opts & 1) hipLaunchKernelGGL((k_recon1<true>), grid_for(M.T), dim3(256), 0, st, M, S, W);

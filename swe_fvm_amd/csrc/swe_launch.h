@@ -9,6 +9,11 @@
 //            bits 4+4k, 6+4k  local vertex of ep0 / ep1 of edge k
 //   edges  ep0, ep1, et0, et1             int32[E]
 //   node -> cell map  n2c_off[V+1], n2c[3T]
+//   static geometry (computed once by k_setup_geometry with the step's own
+//   formulas, so every use is bit-identical to recomputing it):
+//     cells  cx, cy, bc (centroid x, y, z), zmin, zmax, area, iarea (1/area)
+//     edges  el (length), etx, ety (tangent outward of et0), ex, ey, ez
+//            (midpoint), edmin (min over the two cells of 2 area / length)
 //   state  w, u, v                        double[T] per buffer
 //   work   MUSCL pass 1 (mw, mgx, mgy), pass 2 (aw, agx, agy), cls int8,
 //          fluxes f0, f1, f2 double[E], draining dt dtd double[T]
@@ -34,6 +39,28 @@ struct DevMesh {
   const int* n2c_off;
   const int* n2c;
   int64_t T, E, V;
+  // static geometry
+  const double* cx;
+  const double* cy;
+  const double* bc;
+  const double* zmin;
+  const double* zmax;
+  const double* area;
+  const double* iarea;
+  const double* el;
+  const double* etx;
+  const double* ety;
+  const double* ex;
+  const double* ey;
+  const double* ez;
+  const double* edmin;
+};
+
+// writable view of the static geometry arrays, for the setup kernel
+struct DevGeomOut {
+  double *cx, *cy, *bc, *zmin, *zmax, *area, *iarea;
+  double *el, *etx, *ety, *ex, *ey, *ez, *edmin;
+  unsigned int* bad;  // count of edges whose two tangents are not exact opposites
 };
 
 struct DevState {
@@ -72,6 +99,7 @@ struct StageArgs {
   double* dts_log;     // receives the step's dt (first stage only), may be null
 };
 
+hipError_t launch_setup_geometry(const DevMesh& M, const DevGeomOut& G, hipStream_t st);
 hipError_t launch_recon1(const DevMesh& M, const DevState& S, const DevWork& W, int opts,
                          hipStream_t st);
 hipError_t launch_recon2(const DevMesh& M, const DevState& S, const DevWork& W, int opts,

@@ -40,8 +40,12 @@ def test_facade_gauss_wave_matches_oracle(tmp_path, golden):
     topo = O.build_topology(xyz.shape[0], tri, bnd)
     xyz = xyz.copy()
     xyz[:, 2] = 0.0
+    # the driver evaluates exp with the C library; numpy's exp may differ in
+    # the last bit, so the oracle starts from the driver's own initial state
     prim = C.gauss_wave_state(xyz, topo["tp"])
-    assert np.array_equal(out0[:, 0], prim[:, 0])
+    assert np.max(np.abs(out0[:, 0] - prim[:, 0])) <= 4e-16
+    assert np.all(out0[:, 1:] == 0.0)
+    prim = np.stack([out0[:, 0], np.zeros(len(out0)), np.zeros(len(out0))], axis=1)
     s = O.OracleSolver(xyz, topo, "HLL", "Einfeldt")
     s.set_state(prim)
     s.run("Euler", 1, dt=1e-3)
