@@ -202,7 +202,7 @@ int swe_create(const swe_topology* topo, const double* xyz, int flux, int wavesp
   ALLOC(doff, V + 1); ALLOC(dn2c, 3 * T);
   for (int b = 0; b < 3; ++b) { ALLOC(s->sw[b], T); ALLOC(s->su[b], T); ALLOC(s->sv[b], T); }
   ALLOC(s->W.mw, T); ALLOC(s->W.mgx, T); ALLOC(s->W.mgy, T);
-  ALLOC(s->W.aw, T); ALLOC(s->W.agx, T); ALLOC(s->W.agy, T);
+  ALLOC(s->W.wrec1, T);
   ALLOC(s->W.cls, T);
   ALLOC(s->W.f0, E); ALLOC(s->W.f1, E); ALLOC(s->W.f2, E);
   ALLOC(s->W.dtd, T);

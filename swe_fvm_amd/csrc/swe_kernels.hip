@@ -43,8 +43,9 @@ __device__ __forceinline__ int logical_block() {
   int b = blockIdx.x;
   return (b & 7) * per + (b >> 3);
 }
+constexpr int kBlock = 256;  // every launch uses 256-thread workgroups
 __device__ __forceinline__ int64_t global_index() {
-  return (int64_t)logical_block() * blockDim.x + threadIdx.x;
+  return (int64_t)logical_block() * kBlock + threadIdx.x;
 }
 
 __device__ __forceinline__ V3 node(const DevMesh& M, int n) {
@@ -61,22 +62,18 @@ __device__ __forceinline__ V3 edge_mid(const V3& p0, const V3& p1, const V3& p2,
   return midpoint(pick(p0, p1, p2, ep0_local(fl, k)), pick(p0, p1, p2, ep1_local(fl, k)));
 }
 
-// Final MUSCL of cell c: pass-2 result for partially wet cells.
+// Final MUSCL of cell c (pass 2 has overwritten the partially wet cells in
+// place); every load is issued unconditionally so they overlap.
 struct Muscl {
   double w0, gx, gy, u0, v0;
 };
 __device__ __forceinline__ Muscl load_muscl(const DevState& S, const DevWork& W, int c) {
-  int k = W.cls[c];
-  Muscl m;
-  if (k == 1) {
-    m.w0 = W.aw[c]; m.gx = W.agx[c]; m.gy = W.agy[c];
-  } else {
-    m.w0 = W.mw[c]; m.gx = W.mgx[c]; m.gy = W.mgy[c];
-  }
+  const int k = W.cls[c];
+  const double w0 = W.mw[c], gx = W.mgx[c], gy = W.mgy[c];
+  const double u = S.u[c], v = S.v[c];
+  Muscl m{w0, gx, gy, u, v};
   if (k == 0) {
     m.u0 = 0.; m.v0 = 0.;
-  } else {
-    m.u0 = S.u[c]; m.v0 = S.v[c];
   }
   return m;
 }
@@ -146,47 +143,58 @@ template <bool HEAD_GRAD>
 __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W) {
   int64_t i = global_index();
   if (i >= M.T) return;
+  // every input of every branch is loaded up front, so the gathers overlap
   const double w = S.w[i];
   const double bc = M.bc[i];
   const uint32_t fl = M.cflag[i];
+  const double zmx = M.zmax[i], zmn = M.zmin[i];
+  const double cx = M.cx[i], cy = M.cy[i];
+  const V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[M.T + i]), p2 = node(M, M.tp[2 * M.T + i]);
+  int jn[3];
+  double wn[3], bcn[3], zmxn[3], zmnn[3], cxn[3], cyn[3];
+  uint32_t fln[3];
+#pragma unroll
+  for (int kk = 0; kk < 3; ++kk) {
+    int j = M.tt[kk * M.T + i];
+    j = j < 0 ? (int)i : j;  // boundary cells never take the full-wet branch
+    jn[kk] = j;
+    wn[kk] = S.w[j];
+    bcn[kk] = M.bc[j];
+    zmxn[kk] = M.zmax[j];
+    zmnn[kk] = M.zmin[j];
+    cxn[kk] = M.cx[j];
+    cyn[kk] = M.cy[j];
+    fln[kk] = M.cflag[j];
+  }
+  (void)jn;
   double w0 = w, gx = 0., gy = 0.;
   int8_t k;
   if (!is_wet(w - bc)) {
     k = 0;
     w0 = bc;
-    V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[M.T + i]), p2 = node(M, M.tp[2 * M.T + i]);
     gradient(p0, p1, p2, gx, gy);
-  } else if ((fl & CELL_BND) || !(M.zmax[i] < w)) {
+  } else if ((fl & CELL_BND) || !(zmx < w)) {
     k = 1;
-    const double zmx = M.zmax[i], zmn = M.zmin[i];
-    // b13 = zmax, b23 = zmin; partwet1_w takes the three vertex z's only
-    // through max/min and 3 bc - b23 - b13
+    // b13 = zmax, b23 = zmin enter partwet1_w only through max / min
     w0 = partwet1_w(zmn, zmn, zmx, bc, w);
+    W.wrec1[i] = w0;
   } else {
     k = 2;
-    const double cx = M.cx[i], cy = M.cy[i];
-    V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[M.T + i]), p2 = node(M, M.tp[2 * M.T + i]);
     V3 pts[3];
     V3 E[3];
-    double wn[3];
     bool any_dry = false;
 #pragma unroll
     for (int kk = 0; kk < 3; ++kk) {
       E[kk] = edge_mid(p0, p1, p2, fl, kk);
-      int j = M.tt[kk * M.T + i];
-      double wj = S.w[j];
-      wn[kk] = wj;
-      double bcj = M.bc[j];
-      uint32_t flj = M.cflag[j];
-      double zmxj = M.zmax[j];
-      if (!(flj & CELL_BND) && zmxj < wj) {
-        pts[kk] = {M.cx[j], M.cy[j], HEAD_GRAD ? bcj : wj};
+      const double wj = wn[kk], bcj = bcn[kk];
+      if (!(fln[kk] & CELL_BND) && zmxn[kk] < wj) {
+        pts[kk] = {cxn[kk], cyn[kk], HEAD_GRAD ? bcj : wj};
       } else if (!is_wet(wj - bcj)) {
         any_dry = true;
         pts[kk] = {0., 0., 0.};
       } else {
-        double w0j = partwet1_w(M.zmin[j], M.zmin[j], zmxj, bcj, wj);
-        double dx = E[kk].x - M.cx[j], dy = E[kk].y - M.cy[j];
+        double w0j = partwet1_w(zmnn[kk], zmnn[kk], zmxn[kk], bcj, wj);
+        double dx = E[kk].x - cxn[kk], dy = E[kk].y - cyn[kk];
         double a = w0j + (0. * dx + 0. * dy);
         if (!((a - E[kk].z) >= 0)) a = E[kk].z;
         double val = 0.5 * (w + a);
@@ -237,9 +245,14 @@ __device__ inline double node_max_pass1(const DevMesh& M, const DevWork& W, int 
   double mx = P.z;
   int b = M.n2c_off[n], e = M.n2c_off[n + 1];
   for (int q = b; q < e; ++q) {
-    int c = M.n2c[q];
+    const int c = M.n2c[q];
+    const int k = W.cls[c];
+    // pass-1 reconstruction of c: partially wet cells hold their pass-1
+    // level in wrec1 (zero gradient) while pass 2 rewrites mw in place
+    double w0 = W.mw[c], gx = W.mgx[c], gy = W.mgy[c];
+    if (k == 1) { w0 = W.wrec1[c]; gx = 0.; gy = 0.; }
     double dx = P.x - M.cx[c], dy = P.y - M.cy[c];
-    double a = W.mw[c] + (W.mgx[c] * dx + W.mgy[c] * dy);
+    double a = w0 + (gx * dx + gy * dy);
     if (!((a - P.z) >= 0)) a = P.z;
     mx = dmax(mx, a);
   }
@@ -265,7 +278,7 @@ __global__ void __launch_bounds__(256) k_recon2(DevMesh M, DevState S, DevWork W
   (void)i2;
   double w = S.w[i];
   double b23 = Q0.z, b12 = Q1.z, b13 = Q2.z;
-  double w0 = W.mw[i], gx = 0., gy = 0.;  // fallback: pass-1 (PartWet1) result
+  double w0 = W.wrec1[i], gx = 0., gy = 0.;  // fallback: pass-1 (PartWet1) result
   bool fallback = (w > b13) || (b13 - b23 < kTol);
   if (!fallback) {
     double w23 = node_max_pass1(M, W, i0, Q0);
@@ -309,9 +322,10 @@ __global__ void __launch_bounds__(256) k_recon2(DevMesh M, DevState S, DevWork W
       w0 = w23 + (gx * (cx - Q0.x) + gy * (cy - Q0.y));
     }
   }
-  W.aw[i] = w0;
-  W.agx[i] = gx;
-  W.agy[i] = gy;
+  // in place: readers of pass-1 values take partially wet cells from wrec1
+  W.mw[i] = w0;
+  W.mgx[i] = gx;
+  W.mgy[i] = gy;
 }
 
 // ------------------------------------------------------------------ flux --
@@ -323,7 +337,7 @@ __device__ __forceinline__ void block_min_bits(double v, unsigned long long* tar
   if (lane == 0) red[wid] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
-    int nw = (blockDim.x + 63) >> 6;
+    const int nw = kBlock / 64;
     double m = red[0];
     for (int q = 1; q < nw; ++q) m = dmin(m, red[q]);
     // positive doubles order like their bit patterns
@@ -341,14 +355,17 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, 
     const double tx = M.etx[e], ty = M.ety[e];
     const double nx = ty, ny = -tx;
     const int f = M.et0[e], t = M.et1[e];
+    const int tr = t < 0 ? f : t;  // wall edges read their own cell twice
+    const Muscl mf = load_muscl(S, W, f), mt = load_muscl(S, W, tr);
+    const double cxf = M.cx[f], cyf = M.cy[f], cxt = M.cx[tr], cyt = M.cy[tr];
+    const double hf = S.w[f] - M.bc[f];
     double F[3];
     if (t < 0) {
-      double h = S.w[f] - M.bc[f];
-      elem_flux(nx, ny, h, 0., 0., F[0], F[1], F[2]);
+      elem_flux(nx, ny, hf, 0., 0., F[0], F[1], F[2]);
     } else {
       const V3 E = {M.ex[e], M.ey[e], M.ez[e]};
-      Slot L = slot_value(load_muscl(S, W, f), M.cx[f], M.cy[f], E);
-      Slot R = slot_value(load_muscl(S, W, t), M.cx[t], M.cy[t], E);
+      Slot L = slot_value(mf, cxf, cyf, E);
+      Slot R = slot_value(mt, cxt, cyt, E);
       if (FLUX == 0) flux_hll<WS>(nx, ny, E.z, L, R, M.edmin[e], cor, lw, F);
       else flux_hllc<WS>(tx, ty, E.z, L, R, M.edmin[e], cor, lw, F);
     }
@@ -508,11 +525,14 @@ __global__ void __launch_bounds__(256) k_max_wp(DevMesh M, DevWork W, double* ou
   for (int q = M.n2c_off[n]; q < M.n2c_off[n + 1]; ++q) {
     int c = M.n2c[q];
     double dx = P.x - M.cx[c], dy = P.y - M.cy[c];
-    double a = W.mw[c] + (W.mgx[c] * dx + W.mgy[c] * dy);
+    const bool pw = W.cls[c] == 1;
+    // pass 1 (a partially wet cell's level is in wrec1) ...
+    double a = pw ? W.wrec1[c] + (0. * dx + 0. * dy) : W.mw[c] + (W.mgx[c] * dx + W.mgy[c] * dy);
     if (!((a - P.z) >= 0)) a = P.z;
     mx = dmax(mx, a);
-    if (W.cls[c] == 1) {
-      double b = W.aw[c] + (W.agx[c] * dx + W.agy[c] * dy);
+    // ... and pass 2 of partially wet cells
+    if (pw) {
+      double b = W.mw[c] + (W.mgx[c] * dx + W.mgy[c] * dy);
       if (!((b - P.z) >= 0)) b = P.z;
       mx = dmax(mx, b);
     }

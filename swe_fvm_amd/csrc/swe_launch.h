@@ -15,8 +15,10 @@
 //     edges  el (length), etx, ety (tangent outward of et0), ex, ey, ez
 //            (midpoint), edmin (min over the two cells of 2 area / length)
 //   state  w, u, v                        double[T] per buffer
-//   work   MUSCL pass 1 (mw, mgx, mgy), pass 2 (aw, agx, agy), cls int8,
-//          fluxes f0, f1, f2 double[E], draining dt dtd double[T]
+//   work   MUSCL (mw, mgx, mgy): pass 1, overwritten in place by pass 2 for
+//          partially wet cells, whose pass-1 level is kept in wrec1 for the
+//          node-maximum readers; cls int8; fluxes f0, f1, f2 double[E];
+//          draining dt dtd double[T]
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -79,9 +81,7 @@ struct DevWork {
   double* mw;
   double* mgx;
   double* mgy;
-  double* aw;
-  double* agx;
-  double* agy;
+  double* wrec1;  // PartWet1 level of partially wet cells (pass 1)
   int8_t* cls;
   double* f0;
   double* f1;
