@@ -7,7 +7,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libswe_fvm_amd.so")
+# SWE_FVM_AMD_LIB may name another build of the same sources (tuning variants)
+LIB_PATH = os.environ.get("SWE_FVM_AMD_LIB", os.path.join(HERE, "libswe_fvm_amd.so"))
 
 _d = ctypes.c_double
 _i64 = ctypes.c_int64

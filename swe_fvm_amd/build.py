@@ -25,18 +25,26 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
-    """Compiles every source into LIB; raises CalledProcessError on failure."""
-    cmd = [HIPCC] + FLAGS + ["-o", LIB + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES]
+def build(verbose=False, out=None, defines=()):
+    """Compiles every source into LIB (or `out`); `defines` are extra -D
+    tuning macros (work items per thread).  Raises CalledProcessError."""
+    target = out or LIB
+    cmd = [HIPCC] + FLAGS + [f"-D{d}" for d in defines] + ["-o", target + ".tmp"] + \
+        [os.path.join(CSRC, s) for s in SOURCES]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise subprocess.CalledProcessError(res.returncode, cmd, res.stdout, res.stderr)
-    os.replace(LIB + ".tmp", LIB)
-    return LIB
+    os.replace(target + ".tmp", target)
+    return target
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=None)
+    ap.add_argument("-D", dest="defines", action="append", default=[])
+    a = ap.parse_args()
+    print(build(verbose=True, out=a.out, defines=a.defines))

@@ -345,33 +345,68 @@ __device__ __forceinline__ void block_min_bits(double v, unsigned long long* tar
   }
 }
 
-// SpaceDisc::ComputeFluxes (src/SpaceDisc.cpp:54-74)
+// SpaceDisc::ComputeFluxes (src/SpaceDisc.cpp:54-74).  Each thread handles
+// kFluxItems edges kBlock apart; all their loads are issued before any flux
+// is computed, so a wave keeps twice as many gathers in flight.
+#ifndef SWE_FLUX_ITEMS
+#define SWE_FLUX_ITEMS 2
+#endif
+constexpr int kFluxItems = SWE_FLUX_ITEMS;
+struct FluxIn {
+  double tx, ty, ex, ey, ez, edmin, hf, cxf, cyf, cxt, cyt;
+  Muscl mf, mt;
+  int64_t e;
+  int t;
+};
 template <int FLUX, int WS>
 __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, double cor,
                                               unsigned long long* rbits) {
-  int64_t e = global_index();
-  double lw = 1.0;
-  if (e < M.E) {
-    const double tx = M.etx[e], ty = M.ety[e];
-    const double nx = ty, ny = -tx;
-    const int f = M.et0[e], t = M.et1[e];
+  const int64_t base = (int64_t)logical_block() * (kBlock * kFluxItems) + threadIdx.x;
+  FluxIn in[kFluxItems];
+#pragma unroll
+  for (int it = 0; it < kFluxItems; ++it) {
+    const int64_t e = base + it * kBlock;
+    const int64_t es = e < M.E ? e : 0;  // out-of-range items read edge 0 and are dropped
+    FluxIn& q = in[it];
+    q.e = e;
+    q.tx = M.etx[es];
+    q.ty = M.ety[es];
+    q.ex = M.ex[es];
+    q.ey = M.ey[es];
+    q.ez = M.ez[es];
+    q.edmin = M.edmin[es];
+    const int f = M.et0[es], t = M.et1[es];
     const int tr = t < 0 ? f : t;  // wall edges read their own cell twice
-    const Muscl mf = load_muscl(S, W, f), mt = load_muscl(S, W, tr);
-    const double cxf = M.cx[f], cyf = M.cy[f], cxt = M.cx[tr], cyt = M.cy[tr];
-    const double hf = S.w[f] - M.bc[f];
+    q.t = t;
+    q.mf = load_muscl(S, W, f);
+    q.mt = load_muscl(S, W, tr);
+    q.cxf = M.cx[f];
+    q.cyf = M.cy[f];
+    q.cxt = M.cx[tr];
+    q.cyt = M.cy[tr];
+    q.hf = S.w[f] - M.bc[f];
+  }
+  double lw = 1.0;
+#pragma unroll
+  for (int it = 0; it < kFluxItems; ++it) {
+    const FluxIn& q = in[it];
+    if (q.e >= M.E) continue;
+    const double nx = q.ty, ny = -q.tx;
     double F[3];
-    if (t < 0) {
-      elem_flux(nx, ny, hf, 0., 0., F[0], F[1], F[2]);
+    double lwi = 1.0;
+    if (q.t < 0) {
+      elem_flux(nx, ny, q.hf, 0., 0., F[0], F[1], F[2]);
     } else {
-      const V3 E = {M.ex[e], M.ey[e], M.ez[e]};
-      Slot L = slot_value(mf, cxf, cyf, E);
-      Slot R = slot_value(mt, cxt, cyt, E);
-      if (FLUX == 0) flux_hll<WS>(nx, ny, E.z, L, R, M.edmin[e], cor, lw, F);
-      else flux_hllc<WS>(tx, ty, E.z, L, R, M.edmin[e], cor, lw, F);
+      const V3 E = {q.ex, q.ey, q.ez};
+      Slot L = slot_value(q.mf, q.cxf, q.cyf, E);
+      Slot R = slot_value(q.mt, q.cxt, q.cyt, E);
+      if (FLUX == 0) flux_hll<WS>(nx, ny, E.z, L, R, q.edmin, cor, lwi, F);
+      else flux_hllc<WS>(q.tx, q.ty, E.z, L, R, q.edmin, cor, lwi, F);
     }
-    W.f0[e] = F[0];
-    W.f1[e] = F[1];
-    W.f2[e] = F[2];
+    lw = dmin(lw, lwi);
+    W.f0[q.e] = F[0];
+    W.f1[q.e] = F[1];
+    W.f2[q.e] = F[2];
   }
   block_min_bits(lw, rbits);
 }
@@ -401,36 +436,58 @@ __global__ void __launch_bounds__(256) k_drain(DevMesh M, DevState S, DevWork W)
 
 // -------------------------------------------------------------- update --
 // TimeDisc::RHS (src/TimeDisc.cpp:3-41) + stage combination
-// (src/Solvers.cpp:11-13,23-25,30-32,42-58) + ConsAssigner.
-__device__ __forceinline__ void rhs_cell(const DevMesh& M, const DevState& S, const DevWork& W,
-                                         int i, double dts, double cor, double& r0, double& r1,
-                                         double& r2) {
-  const uint32_t fl = M.cflag[i];
-  const Muscl mu = load_muscl(S, W, i);
-  const double cx = M.cx[i], cy = M.cy[i];
-  const double i_area = M.iarea[i];
-  const double dti = W.dtd[i];
-  r0 = 0.; r1 = 0.; r2 = 0.;
+// (src/Solvers.cpp:11-13,23-25,30-32,42-58) + ConsAssigner.  The inputs of a
+// cell are gathered first (RhsIn), the arithmetic follows.
+struct RhsIn {
+  uint32_t fl;
+  Muscl mu;
+  double cx, cy, i_area, dti;
+  double F0[3], F1[3], F2[3], dtik[3], le[3], ex[3], ey[3], ez[3], tx[3], ty[3];
+};
+__device__ __forceinline__ void rhs_load(const DevMesh& M, const DevState& S, const DevWork& W,
+                                         int i, RhsIn& q) {
+  q.fl = M.cflag[i];
+  q.mu = load_muscl(S, W, i);
+  q.cx = M.cx[i];
+  q.cy = M.cy[i];
+  q.i_area = M.iarea[i];
+  q.dti = W.dtd[i];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int e = M.te[k * M.T + i];
     const int nb = M.tt[k * M.T + i];
-    const bool own = sgn_pos(fl, k);
+    q.F0[k] = W.f0[e];
+    q.F1[k] = W.f1[e];
+    q.F2[k] = W.f2[e];
+    const double d = W.dtd[nb < 0 ? i : nb];
+    q.dtik[k] = nb < 0 ? INFINITY : d;
+    q.le[k] = M.el[e];
+    q.ex[k] = M.ex[e];
+    q.ey[k] = M.ey[e];
+    q.ez[k] = M.ez[e];
+    q.tx[k] = M.etx[e];
+    q.ty[k] = M.ety[e];
+  }
+}
+__device__ __forceinline__ void rhs_compute(const RhsIn& q, double dts, double cor, double& r0,
+                                            double& r1, double& r2) {
+  r0 = 0.; r1 = 0.; r2 = 0.;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const bool own = sgn_pos(q.fl, k);
     const double sg = own ? 1. : -1.;
-    const double F0 = W.f0[e], F1 = W.f1[e], F2 = W.f2[e];
-    const double dtik = nb < 0 ? INFINITY : W.dtd[nb];
-    const double dtk = (sg * F0) > 0. ? dmin(dts, dti) : dmin(dts, dtik);
-    const double le = M.el[e];
-    const double c_ek = i_area * le;
-    const V3 E = {M.ex[e], M.ey[e], M.ez[e]};
-    const Slot s = slot_value(mu, cx, cy, E);
+    const double F0 = q.F0[k], F1 = q.F1[k], F2 = q.F2[k];
+    const double dtk = (sg * F0) > 0. ? dmin(dts, q.dti) : dmin(dts, q.dtik[k]);
+    const double c_ek = q.i_area * q.le[k];
+    const V3 E = {q.ex[k], q.ey[k], q.ez[k]};
+    const Slot s = slot_value(q.mu, q.cx, q.cy, E);
     const double h_ek = s.w - E.z;
-    double src1 = mu.gx, src2 = mu.gy;
+    double src1 = q.mu.gx, src2 = q.mu.gy;
     src1 += cor * (-s.v);
     src2 += cor * s.u;
     // Norm(e, i): the stored tangent is outward of et0; exact negation for et1
-    const double tx = own ? M.etx[e] : -M.etx[e];
-    const double ty = own ? M.ety[e] : -M.ety[e];
+    const double tx = own ? q.tx[k] : -q.tx[k];
+    const double ty = own ? q.ty[k] : -q.ty[k];
     const double nx = ty, ny = -tx;
     const double cf = dtk * sg * c_ek;
     r0 -= cf * F0;
@@ -439,10 +496,17 @@ __device__ __forceinline__ void rhs_cell(const DevMesh& M, const DevState& S, co
     const double cs = dts * (1. / 3.);
     r1 -= cs * src1 * h_ek;
     r2 -= cs * src2 * h_ek;
-    const double q = 0.5 * h_ek * h_ek;
-    r1 += dtk * (nx * c_ek * q);
-    r2 += dtk * (ny * c_ek * q);
+    const double q2 = 0.5 * h_ek * h_ek;
+    r1 += dtk * (nx * c_ek * q2);
+    r2 += dtk * (ny * c_ek * q2);
   }
+}
+__device__ __forceinline__ void rhs_cell(const DevMesh& M, const DevState& S, const DevWork& W,
+                                         int i, double dts, double cor, double& r0, double& r1,
+                                         double& r2) {
+  RhsIn q;
+  rhs_load(M, S, W, i, q);
+  rhs_compute(q, dts, cor, r0, r1, r2);
 }
 
 __device__ __forceinline__ double stage_dt(const StageArgs& A) {
@@ -451,36 +515,62 @@ __device__ __forceinline__ double stage_dt(const StageArgs& A) {
   return dt;
 }
 
+// kUpdItems cells per thread, kBlock apart, loads of all of them first.
+#ifndef SWE_UPDATE_ITEMS
+#define SWE_UPDATE_ITEMS 1
+#endif
+constexpr int kUpdItems = SWE_UPDATE_ITEMS;
 __global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, DevState U0, DevOut O,
                                                 DevWork W, StageArgs A) {
-  int64_t i = global_index();
+  const int64_t base = (int64_t)logical_block() * (kBlock * kUpdItems) + threadIdx.x;
   const double dt = stage_dt(A);
   if (A.dts_log && blockIdx.x == 0 && threadIdx.x == 0) A.dts_log[0] = dt;
-  if (i >= M.T) return;
-  const double bc = M.bc[i];
-  double R0, R1, R2;
-  rhs_cell(M, S, W, (int)i, A.coef * dt, A.cor, R0, R1, R2);
-  const double w = S.w[i], u = S.u[i], v = S.v[i];
-  const double h = w - bc;
-  const double Uc0 = h, Uc1 = h * u, Uc2 = h * v;
-  double r0, r1, r2;
-  if (A.a0 != 0.) {
-    const double w0 = U0.w[i], u0 = U0.u[i], v0 = U0.v[i];
-    const double h0 = w0 - bc;
-    const double Ua0 = h0, Ua1 = h0 * u0, Ua2 = h0 * v0;
-    r0 = (A.a0 * Ua0 + A.a1 * Uc0) + R0;
-    r1 = (A.a0 * Ua1 + A.a1 * Uc1) + R1;
-    r2 = (A.a0 * Ua2 + A.a1 * Uc2) + R2;
-  } else {
-    r0 = Uc0 + R0;
-    r1 = Uc1 + R1;
-    r2 = Uc2 + R2;
+  RhsIn in[kUpdItems];
+  double w[kUpdItems], u[kUpdItems], v[kUpdItems], bc[kUpdItems];
+  double w0[kUpdItems], u0[kUpdItems], v0[kUpdItems];
+  const bool comb = A.a0 != 0.;
+#pragma unroll
+  for (int it = 0; it < kUpdItems; ++it) {
+    const int64_t i = base + it * kBlock;
+    const int is = (int)(i < M.T ? i : 0);  // out-of-range items read cell 0 and are dropped
+    rhs_load(M, S, W, is, in[it]);
+    w[it] = S.w[is];
+    u[it] = S.u[is];
+    v[it] = S.v[is];
+    bc[it] = M.bc[is];
+    if (comb) {
+      w0[it] = U0.w[is];
+      u0[it] = U0.u[is];
+      v0[it] = U0.v[is];
+    }
   }
-  double ow, ou, ov;
-  cons_assign(bc, r0, r1, r2, ow, ou, ov);
-  O.w[i] = ow;
-  O.u[i] = ou;
-  O.v[i] = ov;
+  const double dts = A.coef * dt;
+#pragma unroll
+  for (int it = 0; it < kUpdItems; ++it) {
+    const int64_t i = base + it * kBlock;
+    if (i >= M.T) continue;
+    double R0, R1, R2;
+    rhs_compute(in[it], dts, A.cor, R0, R1, R2);
+    const double h = w[it] - bc[it];
+    const double Uc0 = h, Uc1 = h * u[it], Uc2 = h * v[it];
+    double r0, r1, r2;
+    if (comb) {
+      const double h0 = w0[it] - bc[it];
+      const double Ua0 = h0, Ua1 = h0 * u0[it], Ua2 = h0 * v0[it];
+      r0 = (A.a0 * Ua0 + A.a1 * Uc0) + R0;
+      r1 = (A.a0 * Ua1 + A.a1 * Uc1) + R1;
+      r2 = (A.a0 * Ua2 + A.a1 * Uc2) + R2;
+    } else {
+      r0 = Uc0 + R0;
+      r1 = Uc1 + R1;
+      r2 = Uc2 + R2;
+    }
+    double ow, ou, ov;
+    cons_assign(bc[it], r0, r1, r2, ow, ou, ov);
+    O.w[i] = ow;
+    O.u[i] = ou;
+    O.v[i] = ov;
+  }
 }
 
 __global__ void k_reset_r(unsigned long long* rbits) {
@@ -574,7 +664,8 @@ hipError_t launch_recon2(const DevMesh& M, const DevState& S, const DevWork& W, 
 hipError_t launch_flux(const DevMesh& M, const DevState& S, const DevWork& W, int flux, int ws,
                        double cor, unsigned long long* rbits, hipStream_t st) {
   hipLaunchKernelGGL(k_reset_r, dim3(1), dim3(1), 0, st, rbits);
-  dim3 g = grid_for(M.E), b(256);
+  if (M.E == 0) return hipGetLastError();
+  dim3 g = grid_for((M.E + kFluxItems - 1) / kFluxItems), b(256);
 #define SWE_FLUX_CASE(F, Wv)                                                           \
   if (flux == F && ws == Wv) {                                                         \
     hipLaunchKernelGGL((k_flux<F, Wv>), g, b, 0, st, M, S, W, cor, rbits);             \
@@ -591,7 +682,8 @@ hipError_t launch_drain(const DevMesh& M, const DevState& S, const DevWork& W, h
 }
 hipError_t launch_update(const DevMesh& M, const DevState& S, const DevState& U0, const DevOut& O,
                          const DevWork& W, const StageArgs& A, hipStream_t st) {
-  hipLaunchKernelGGL(k_update, grid_for(M.T), dim3(256), 0, st, M, S, U0, O, W, A);
+  hipLaunchKernelGGL(k_update, grid_for((M.T + kUpdItems - 1) / kUpdItems), dim3(256), 0, st, M, S,
+                     U0, O, W, A);
   return hipGetLastError();
 }
 hipError_t launch_edge_field(const DevMesh& M, const DevState& S, const DevWork& W, double cor,
