@@ -75,6 +75,66 @@ def cpu_baseline(dom, prim, flux, wave, seconds_hint=""):
                       f"oracle C++ restatement -O3 -ffp-contract=off, 1 thread, {el:.1f} s"}
 
 
+def bench_ensemble(args):
+    """256 seeded members of 10k-340k cells (~20M in total), HLLC<Einfeldt>,
+    SSP-RK2, per-member CFL 0.15 steps to t_end: the batched ensemble (one
+    graph-captured step for all members) against the same members advanced
+    one after the other with the single-mesh solver."""
+    import gc
+    import swe_fvm_amd as swe
+    from swe_fvm_amd import workloads as W
+    from swe_fvm_amd.ensemble import Ensemble
+    t0 = time.time()
+    mem = W.ensemble(n_inst=args.members, total=args.ensemble_cells, seed=11)
+    T = np.array([p.shape[0] for _, p in mem])
+    print(f"[bench] ensemble {len(mem)} members, {T.sum()} cells ({T.min()}..{T.max()}) in "
+          f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    fl = swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt]
+    ens = Ensemble(fl, [d for d, _ in mem], [p for _, p in mem])
+    ens.run("SSPRK2", cfl=0.15, t_end=args.t_end, max_steps=2, graph=True)  # warm-up + capture
+    ens.reset(0.0)
+    ens.set_states([p for _, p in mem])
+    ens.synchronize()
+    t1 = time.perf_counter()
+    done, act = ens.run("SSPRK2", cfl=0.15, t_end=args.t_end, graph=True)
+    ens.synchronize()
+    wall_b = time.perf_counter() - t1
+    tt, steps = ens.times()
+    updates = float(np.sum(T * steps))
+    batched = updates / wall_b
+    print(f"[bench] batched: {done} steps, {updates:.3e} cell-updates in {wall_b:.2f}s "
+          f"-> {batched:.3e}/s, all at t_end: {bool(np.all(tt == args.t_end))}", file=sys.stderr, flush=True)
+    del ens
+    gc.collect()
+    # sequential: each member alone, args.seq_steps CFL steps (a bounded sample)
+    seq_updates = 0.0
+    seq_ms = 0.0
+    for dom, prim in mem:
+        sd = swe.SpaceDisc(fl, dom, prim)
+        sd.run("SSPRK2", 1, dt=0.0, cfl=0.15)
+        sd.synchronize()
+        t2 = time.perf_counter()
+        sd.run("SSPRK2", args.seq_steps, dt=0.0, cfl=0.15)
+        seq_ms += (time.perf_counter() - t2) * 1e3
+        seq_updates += prim.shape[0] * args.seq_steps
+        del sd
+    sequential = seq_updates / (seq_ms * 1e-3)
+    print(f"[bench] sequential: {seq_updates:.3e} cell-updates in {seq_ms / 1e3:.2f}s -> "
+          f"{sequential:.3e}/s", file=sys.stderr, flush=True)
+    print(json.dumps({
+        "metric": "batched/sequential aggregate FP64 cell-updates/s",
+        "value": batched / sequential, "unit": "ratio", "higher_is_better": True,
+        "batched_cell_updates_per_s": batched, "sequential_cell_updates_per_s": sequential,
+        "batched": {"steps": done, "members_active_at_end": act, "cell_updates": updates,
+                    "seconds": wall_b, "t_end": args.t_end, "graph": True},
+        "sequential": {"steps_per_member": args.seq_steps, "cell_updates": seq_updates,
+                       "seconds": seq_ms / 1e3},
+        "config": {"workload": f"ensemble_{len(mem)}_members_T{int(T.sum())}", "members": len(mem),
+                   "cells_min": int(T.min()), "cells_max": int(T.max()), "scheme": "SSPRK2",
+                   "flux": "HLLC<Einfeldt>", "dt": "per-member CFL 0.15"},
+        "dtype": "f64", "data": "synthetic", "n_gpus": 1}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,7 +146,16 @@ def main():
     ap.add_argument("--cpu-only", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of 2 GiB x 5 (counter calibration)")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="batched ensemble vs sequential single-mesh solves (one JSON line)")
+    ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--ensemble-cells", type=int, default=20_000_000)
+    ap.add_argument("--t-end", type=float, default=1.0)
+    ap.add_argument("--seq-steps", type=int, default=40)
     args = ap.parse_args()
+    if args.ensemble:
+        bench_ensemble(args)
+        return
     if args.copy_calibration:
         import torch
         a = torch.ones(2 ** 28, dtype=torch.float64, device="cuda")

@@ -157,6 +157,30 @@ int64_t swe_num_edges(const swe_solver* s);
 int64_t swe_num_nodes(const swe_solver* s);
 int swe_device_state(swe_solver* s, double** w, double** u, double** v);
 
+/* ------------------------------------------------------------ ensembles --- */
+
+/* Ensemble of n independent meshes advanced together (no reference
+ * counterpart: the reference advances one SpaceDisc at a time).  The
+ * members are concatenated; state arrays (swe_set_state / swe_get_state)
+ * hold the members one after another (swe_batch_info gives the offsets).
+ * Each member has its own CFL step dt_k = cfl * r_k of its own first flux
+ * pass, clipped to land on t_end, and stops there. */
+int swe_batch_create(int64_t n, const swe_topology* const* topos, const double* const* xyz,
+                     int flux, int wavespeed, double cor, double tau, int options, int device,
+                     swe_solver** out);
+/* Sets every member's time to t0 and makes it active again. */
+int swe_batch_reset(swe_solver* s, double t0);
+/* Steps until every member reached t_end or max_steps steps were taken
+ * (checked every 64 steps).  use_graph: replay one captured HIP graph per
+ * step instead of launching the kernels one by one. */
+int swe_batch_run(swe_solver* s, int scheme, double cfl, double t_end, int64_t max_steps,
+                  int use_graph, int64_t* steps_done, int64_t* n_active);
+/* Member count and cell / edge / node offsets (n + 1 entries each). */
+int swe_batch_info(swe_solver* s, int64_t* n, int64_t* cell_off, int64_t* edge_off,
+                   int64_t* node_off);
+/* Each member's simulated time and number of steps taken. */
+int swe_batch_time(swe_solver* s, double* t, int64_t* steps);
+
 /* Time, with HIP events on the solver's stream, nsteps steps of swe_run:
  * *ms = total milliseconds; kernel_ms (optional, 5 doubles) = average
  * per-launch duration of the reconstruction, wet/dry, flux, draining and

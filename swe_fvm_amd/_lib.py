@@ -54,6 +54,11 @@ SIGNATURES = [
     ("swe_num_nodes", _i64, [_p]),
     ("swe_device_state", _int, [_p, _pp, _pp, _pp]),
     ("swe_time_run", _int, [_p, _int, _d, _d, _i64, _p, _p]),
+    ("swe_batch_create", _int, [_i64, _p, _p, _int, _int, _d, _d, _int, _int, _pp]),
+    ("swe_batch_reset", _int, [_p, _d]),
+    ("swe_batch_run", _int, [_p, _int, _d, _d, _i64, _int, _pi64, _pi64]),
+    ("swe_batch_info", _int, [_p, _pi64, _p, _p, _p]),
+    ("swe_batch_time", _int, [_p, _p, _p]),
 ]
 
 _lib = None

@@ -32,6 +32,14 @@ struct swe_solver {
   double* dts = nullptr;
   int64_t dts_cap = 0;
   int64_t last_nsteps = 0;
+  // batched ensemble (n_inst > 0)
+  int n_inst = 0;
+  DevBatch B{};
+  std::vector<int64_t> cell_off, edge_off, node_off;
+  hipGraphExec_t graph = nullptr;
+  int graph_scheme = -1;
+  double graph_cfl = 0., graph_tend = 0.;
+  int64_t graph_c0 = -1, graph_c1 = -1;
 };
 
 namespace {
@@ -102,14 +110,12 @@ int run_step(swe_solver* s, int scheme, double dt, double cfl, double* dts_log) 
 
 }  // namespace
 
-extern "C" {
-
-int swe_create(const swe_topology* topo, const double* xyz, int flux, int wavespeed, double cor,
-               double tau, int options, int device, swe_solver** out) {
-  if (!topo || !xyz || !out) return fail(SWE_ERR_ARG, "null argument");
+static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed, double cor,
+                       double tau, int options, int device, const std::vector<int>* cell_inst,
+                       int n_inst, swe_solver** out) {
+  if (!xyz || !out) return fail(SWE_ERR_ARG, "null argument");
   if (flux < 0 || flux > 1 || wavespeed < 0 || wavespeed > 2)
     return fail(SWE_ERR_ARG, "unknown flux or wave-speed estimate");
-  const Topo& m = topo->m;
   if (m.T > 0x7fffffff || m.E > 0x7fffffff || m.V > 0x7fffffff)
     return fail(SWE_ERR_ARG, "mesh too large for int32 device indices");
   const int64_t T = m.T, E = m.E, V = m.V;
@@ -237,6 +243,10 @@ int swe_create(const swe_topology* topo, const double* xyz, int flux, int wavesp
   s->M = DevMesh{dpx, dpy, dpz, dtp, dte, dtt, dfl, dep0, dep1, det0, det1, doff, dn2c, T, E, V,
                  G.cx, G.cy, G.bc, G.zmin, G.zmax, G.area, G.iarea,
                  G.el, G.etx, G.ety, G.ex, G.ey, G.ez, G.edmin};
+  s->M.c0 = 0;
+  s->M.c1 = T;
+  s->M.e0 = 0;
+  s->M.e1 = E;
   if ((e = hipMemset(G.bad, 0, sizeof(unsigned int))) != hipSuccess) return bail(e, "hipMemset");
   if ((e = launch_setup_geometry(s->M, G, s->stream)) != hipSuccess) return bail(e, "setup_geometry");
   unsigned int bad = 0;
@@ -247,14 +257,77 @@ int swe_create(const swe_topology* topo, const double* xyz, int flux, int wavesp
     swe_destroy(s);
     return fail(SWE_ERR_DOMAIN, "degenerate triangle: an edge's outward tangents are not opposite");
   }
+  if (n_inst > 0) {
+    s->n_inst = n_inst;
+    int* dci;
+    DevBatch& B = s->B;
+    B.n = n_inst;
+#define ALLOC(ptr, n) if ((e = dalloc(s, &ptr, n)) != hipSuccess) return bail(e, "hipMalloc")
+    ALLOC(dci, T);
+    ALLOC(B.t, n_inst); ALLOC(B.dt, n_inst); ALLOC(B.active, n_inst); ALLOC(B.steps, n_inst);
+    ALLOC(B.rbits, 4 * n_inst); ALLOC(B.n_active, 1);
+#undef ALLOC
+    if ((e = hipMemcpy(dci, cell_inst->data(), sizeof(int) * size_t(T), hipMemcpyHostToDevice)) != hipSuccess)
+      return bail(e, "hipMemcpy");
+    s->M.cinst = dci;
+    s->M.bdt = B.dt;
+    s->M.bactive = B.active;
+    if (int rc = swe_batch_reset(s, 0.0)) { swe_destroy(s); return rc; }
+  }
   *out = s;
   return SWE_OK;
+}
+
+extern "C" {
+
+int swe_create(const swe_topology* topo, const double* xyz, int flux, int wavespeed, double cor,
+               double tau, int options, int device, swe_solver** out) {
+  if (!topo) return fail(SWE_ERR_ARG, "null topology");
+  return create_impl(topo->m, xyz, flux, wavespeed, cor, tau, options, device, nullptr, 0, out);
+}
+
+int swe_batch_create(int64_t n, const swe_topology* const* topos, const double* const* xyzs,
+                     int flux, int wavespeed, double cor, double tau, int options, int device,
+                     swe_solver** out) {
+  if (n <= 0 || n > (1 << 20) || !topos || !xyzs || !out) return fail(SWE_ERR_ARG, "invalid ensemble");
+  Topo all;
+  std::vector<double> xyz;
+  std::vector<int> inst;
+  std::vector<int64_t> coff(1, 0), eoff(1, 0), noff(1, 0);
+  for (int64_t k = 0; k < n; ++k) {
+    if (!topos[k] || !xyzs[k]) return fail(SWE_ERR_ARG, "null ensemble member");
+    const Topo& m = topos[k]->m;
+    const int64_t V0 = all.V, E0 = all.E, T0 = all.T;
+    for (int64_t q = 0; q < 2 * m.E; ++q) all.ep.push_back(m.ep[size_t(q)] + V0);
+    for (int64_t q = 0; q < 2 * m.E; ++q) {
+      int64_t c = m.et[size_t(q)];
+      all.et.push_back(c >= 0 ? c + T0 : c);
+    }
+    for (int64_t q = 0; q < 3 * m.T; ++q) all.tp.push_back(m.tp[size_t(q)] + V0);
+    for (int64_t q = 0; q < 3 * m.T; ++q) all.te.push_back(m.te[size_t(q)] + E0);
+    for (int64_t q = 0; q < 3 * m.T; ++q) {
+      int64_t c = m.tt[size_t(q)];
+      all.tt.push_back(c >= 0 ? c + T0 : c);
+    }
+    xyz.insert(xyz.end(), xyzs[k], xyzs[k] + 3 * m.V);
+    inst.insert(inst.end(), size_t(m.T), int(k));
+    all.V += m.V; all.E += m.E; all.T += m.T;
+    coff.push_back(all.T); eoff.push_back(all.E); noff.push_back(all.V);
+  }
+  int rc = create_impl(all, xyz.data(), flux, wavespeed, cor, tau, options, device, &inst, int(n), out);
+  if (rc == SWE_OK) {
+    (*out)->cell_off = coff;
+    (*out)->edge_off = eoff;
+    (*out)->node_off = noff;
+  }
+  return rc;
 }
 
 void swe_destroy(swe_solver* s) {
   if (!s) return;
   hipSetDevice(s->device);
   if (s->stream) hipStreamSynchronize(s->stream);
+  if (s->graph) hipGraphExecDestroy(s->graph);
   for (void* p : s->allocs) hipFree(p);
   if (s->dts) hipFree(s->dts);
   if (s->stream) hipStreamDestroy(s->stream);
@@ -455,6 +528,175 @@ int swe_device_state(swe_solver* s, double** w, double** u, double** v) {
   if (w) *w = s->sw[s->cur];
   if (u) *u = s->su[s->cur];
   if (v) *v = s->sv[s->cur];
+  return SWE_OK;
+}
+
+static int batch_step(swe_solver* s, int scheme, double cfl, double t_end) {
+  // Euler ping-pongs between two buffers so that a two-step graph returns
+  // the state to the buffer it started from
+  const int a = s->cur;
+  const int b = (scheme == SWE_SCHEME_EULER) ? (a == 0 ? 1 : 0) : (a + 1) % 3;
+  const int c = (a + 2) % 3;
+  struct St { int in, u0, out; double a0, a1, coef; };
+  St st[3];
+  int nst;
+  if (scheme == SWE_SCHEME_EULER) {
+    st[0] = {a, a, b, 0., 0., 1.}; nst = 1;
+  } else if (scheme == SWE_SCHEME_SSPRK2) {
+    st[0] = {a, a, b, 0., 0., 1.}; st[1] = {b, a, a, .5, .5, .5}; nst = 2;
+  } else {
+    st[0] = {a, a, b, 0., 0., 1.}; st[1] = {b, a, c, .75, .25, .25};
+    st[2] = {c, a, a, 1. / 3., 2. / 3., 2. / 3.}; nst = 3;
+  }
+  for (int q = 0; q < nst; ++q) {
+    DevState S = state_of(s, st[q].in);
+    SWE_CHECK(launch_batch_reset_r(s->B, q, s->stream), "batch_reset_r");
+    SWE_CHECK(launch_recon1(s->M, S, s->W, s->opts, s->stream), "recon1");
+    SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
+    SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->B.rbits + q, s->stream), "flux");
+    if (q == 0) SWE_CHECK(launch_batch_dt(s->B, cfl, t_end, s->stream), "batch_dt");
+    SWE_CHECK(launch_drain(s->M, S, s->W, s->stream), "drain");
+    StageArgs A{st[q].a0, st[q].a1, st[q].coef, 0., cfl, s->cor, nullptr, nullptr};
+    SWE_CHECK(launch_update(s->M, S, state_of(s, st[q].u0), out_of(s, st[q].out), s->W, A, s->stream),
+              "update");
+  }
+  SWE_CHECK(launch_batch_advance(s->B, t_end, s->stream), "batch_advance");
+  if (scheme == SWE_SCHEME_EULER) s->cur = b;
+  return SWE_OK;
+}
+
+int swe_batch_reset(swe_solver* s, double t0) {
+  if (!s || s->n_inst <= 0) return fail(SWE_ERR_ARG, "not an ensemble solver");
+  if (int rc = set_device(s)) return rc;
+  const int n = s->n_inst;
+  std::vector<double> t(size_t(n), t0), dt(size_t(n), 0.);
+  std::vector<int> act(size_t(n), 1);
+  std::vector<long long> steps(size_t(n), 0);
+  SWE_CHECK(hipMemcpy(s->B.t, t.data(), sizeof(double) * size_t(n), hipMemcpyHostToDevice), "hipMemcpy");
+  SWE_CHECK(hipMemcpy(s->B.dt, dt.data(), sizeof(double) * size_t(n), hipMemcpyHostToDevice), "hipMemcpy");
+  SWE_CHECK(hipMemcpy(s->B.active, act.data(), sizeof(int) * size_t(n), hipMemcpyHostToDevice), "hipMemcpy");
+  SWE_CHECK(hipMemcpy(s->B.steps, steps.data(), sizeof(long long) * size_t(n), hipMemcpyHostToDevice), "hipMemcpy");
+  SWE_CHECK(hipMemcpy(s->B.n_active, &n, sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+  return SWE_OK;
+}
+
+// Cells and edges of the members still running: finished members at the
+// two ends of the member list are left out of the launches.
+static void batch_window(swe_solver* s, const std::vector<int>& act) {
+  int k0 = 0, k1 = s->n_inst - 1;
+  while (k0 <= k1 && !act[size_t(k0)]) ++k0;
+  while (k1 >= k0 && !act[size_t(k1)]) --k1;
+  if (k0 > k1) {
+    s->M.c0 = s->M.c1 = 0;
+    s->M.e0 = s->M.e1 = 0;
+    return;
+  }
+  s->M.c0 = s->cell_off[size_t(k0)];
+  s->M.c1 = s->cell_off[size_t(k1) + 1];
+  s->M.e0 = s->edge_off[size_t(k0)];
+  s->M.e1 = s->edge_off[size_t(k1) + 1];
+}
+
+static int batch_capture(swe_solver* s, int scheme, double cfl, double t_end, int per_graph) {
+  if (s->graph) { hipGraphExecDestroy(s->graph); s->graph = nullptr; }
+  const int cur0 = s->cur;
+  hipGraph_t g;
+  SWE_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  int rc = SWE_OK;
+  for (int q = 0; q < per_graph && !rc; ++q) rc = batch_step(s, scheme, cfl, t_end);
+  hipError_t ce = hipStreamEndCapture(s->stream, &g);
+  s->cur = cur0;
+  if (rc) return rc;
+  if (ce != hipSuccess) return dev_fail(ce, "hipStreamEndCapture");
+  hipError_t ie = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (ie != hipSuccess) return dev_fail(ie, "hipGraphInstantiate");
+  s->graph_scheme = scheme;
+  s->graph_cfl = cfl;
+  s->graph_tend = t_end;
+  s->graph_c0 = s->M.c0;
+  s->graph_c1 = s->M.c1;
+  return SWE_OK;
+}
+
+int swe_batch_run(swe_solver* s, int scheme, double cfl, double t_end, int64_t max_steps,
+                  int use_graph, int64_t* steps_done, int64_t* n_active) {
+  if (!s || s->n_inst <= 0) return fail(SWE_ERR_ARG, "not an ensemble solver");
+  if (scheme < 0 || scheme > 2) return fail(SWE_ERR_ARG, "unknown scheme");
+  if (!(cfl > 0.)) return fail(SWE_ERR_ARG, "ensembles run in CFL mode: cfl > 0");
+  if (int rc = set_device(s)) return rc;
+  // Euler ping-pongs between two state buffers: a graph holds two steps
+  const int per_graph = (scheme == SWE_SCHEME_EULER) ? 2 : 1;
+  std::vector<int> act_flags(size_t(s->n_inst));
+  SWE_CHECK(hipMemcpyAsync(act_flags.data(), s->B.active, sizeof(int) * act_flags.size(),
+                           hipMemcpyDeviceToHost, s->stream), "hipMemcpy");
+  SWE_CHECK(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  batch_window(s, act_flags);
+  int64_t done = 0;
+  int act = 0;
+  for (int f : act_flags) act += f;
+  const int64_t chunk = 64;
+  while (done < max_steps && act > 0) {
+    if (use_graph) {
+      // re-capture when the scheme changed or the window shrank by a fifth
+      const int64_t w = s->M.c1 - s->M.c0, wg = s->graph_c1 - s->graph_c0;
+      if (s->graph == nullptr || s->graph_scheme != scheme || s->graph_cfl != cfl ||
+          s->graph_tend != t_end || s->M.c0 < s->graph_c0 || s->M.c1 > s->graph_c1 || 5 * w < 4 * wg)
+        if (int rc = batch_capture(s, scheme, cfl, t_end, per_graph)) return rc;
+    }
+    int64_t k = std::min<int64_t>(chunk, max_steps - done);
+    if (use_graph) k = (k / per_graph) * per_graph;
+    if (k == 0) break;
+    for (int64_t q = 0; q < k;) {
+      if (use_graph) {
+        SWE_CHECK(hipGraphLaunch(s->graph, s->stream), "hipGraphLaunch");
+        q += per_graph;
+      } else {
+        if (int rc = batch_step(s, scheme, cfl, t_end)) return rc;
+        q += 1;
+      }
+    }
+    done += k;
+    SWE_CHECK(hipMemcpyAsync(act_flags.data(), s->B.active, sizeof(int) * act_flags.size(),
+                             hipMemcpyDeviceToHost, s->stream), "hipMemcpy");
+    SWE_CHECK(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    act = 0;
+    for (int f : act_flags) act += f;
+    // a captured graph keeps its own window until the next capture; members
+    // inside it that finished are skipped by the kernels
+    batch_window(s, act_flags);
+  }
+  // restore the whole-mesh window for the single-mesh entry points
+  s->M.c0 = 0;
+  s->M.c1 = s->T;
+  s->M.e0 = 0;
+  s->M.e1 = s->E;
+  if (steps_done) *steps_done = done;
+  if (n_active) *n_active = act;
+  return SWE_OK;
+}
+
+int swe_batch_info(swe_solver* s, int64_t* n, int64_t* cell_off, int64_t* edge_off,
+                   int64_t* node_off) {
+  if (!s || s->n_inst <= 0) return fail(SWE_ERR_ARG, "not an ensemble solver");
+  if (n) *n = s->n_inst;
+  const size_t m = size_t(s->n_inst) + 1;
+  if (cell_off) std::memcpy(cell_off, s->cell_off.data(), sizeof(int64_t) * m);
+  if (edge_off) std::memcpy(edge_off, s->edge_off.data(), sizeof(int64_t) * m);
+  if (node_off) std::memcpy(node_off, s->node_off.data(), sizeof(int64_t) * m);
+  return SWE_OK;
+}
+
+int swe_batch_time(swe_solver* s, double* t, int64_t* steps) {
+  if (!s || s->n_inst <= 0) return fail(SWE_ERR_ARG, "not an ensemble solver");
+  if (int rc = set_device(s)) return rc;
+  const size_t n = size_t(s->n_inst);
+  if (t) if (int rc = download(s, t, s->B.t, sizeof(double) * n)) return rc;
+  if (steps) {
+    std::vector<long long> st(n);
+    if (int rc = download(s, st.data(), s->B.steps, sizeof(long long) * n)) return rc;
+    for (size_t k = 0; k < n; ++k) steps[k] = st[k];
+  }
   return SWE_OK;
 }
 

@@ -141,8 +141,9 @@ __global__ void __launch_bounds__(256) k_setup_edges(DevMesh M, DevGeomOut G) {
 // (src/MUSCLObject.cpp:31-112).
 template <bool HEAD_GRAD>
 __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W) {
-  int64_t i = global_index();
-  if (i >= M.T) return;
+  int64_t i = M.c0 + global_index();
+  if (i >= M.c1) return;
+  if (M.cinst && !M.bactive[M.cinst[i]]) return;  // finished ensemble member
   // every input of every branch is loaded up front, so the gathers overlap
   const double w = S.w[i];
   const double bc = M.bc[i];
@@ -261,8 +262,9 @@ __device__ inline double node_max_pass1(const DevMesh& M, const DevWork& W, int 
 
 template <bool HEAD_PW2>
 __global__ void __launch_bounds__(256) k_recon2(DevMesh M, DevState S, DevWork W) {
-  int64_t i = global_index();
-  if (i >= M.T) return;
+  int64_t i = M.c0 + global_index();
+  if (i >= M.c1) return;
+  if (M.cinst && !M.bactive[M.cinst[i]]) return;  // finished ensemble member
   if (W.cls[i] != 1) return;
   int i0 = M.tp[i], i1 = M.tp[M.T + i], i2 = M.tp[2 * M.T + i];
   V3 Q0 = node(M, i0), Q1 = node(M, i1), Q2 = node(M, i2);
@@ -356,17 +358,18 @@ struct FluxIn {
   double tx, ty, ex, ey, ez, edmin, hf, cxf, cyf, cxt, cyt;
   Muscl mf, mt;
   int64_t e;
-  int t;
+  int t, inst;
+  bool on;
 };
 template <int FLUX, int WS>
 __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, double cor,
                                               unsigned long long* rbits) {
-  const int64_t base = (int64_t)logical_block() * (kBlock * kFluxItems) + threadIdx.x;
+  const int64_t base = M.e0 + (int64_t)logical_block() * (kBlock * kFluxItems) + threadIdx.x;
   FluxIn in[kFluxItems];
 #pragma unroll
   for (int it = 0; it < kFluxItems; ++it) {
     const int64_t e = base + it * kBlock;
-    const int64_t es = e < M.E ? e : 0;  // out-of-range items read edge 0 and are dropped
+    const int64_t es = e < M.e1 ? e : M.e0;  // out-of-range items read a valid edge and are dropped
     FluxIn& q = in[it];
     q.e = e;
     q.tx = M.etx[es];
@@ -385,12 +388,16 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, 
     q.cxt = M.cx[tr];
     q.cyt = M.cy[tr];
     q.hf = S.w[f] - M.bc[f];
+    q.inst = M.cinst ? M.cinst[f] : 0;
+    q.on = e < M.e1 && (!M.cinst || M.bactive[q.inst]);
   }
   double lw = 1.0;
+  double lwi_item[kFluxItems];
 #pragma unroll
   for (int it = 0; it < kFluxItems; ++it) {
     const FluxIn& q = in[it];
-    if (q.e >= M.E) continue;
+    lwi_item[it] = 1.0;
+    if (!q.on) continue;
     const double nx = q.ty, ny = -q.tx;
     double F[3];
     double lwi = 1.0;
@@ -404,19 +411,44 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, 
       else flux_hllc<WS>(q.tx, q.ty, E.z, L, R, q.edmin, cor, lwi, F);
     }
     lw = dmin(lw, lwi);
+    lwi_item[it] = lwi;
     W.f0[q.e] = F[0];
     W.f1[q.e] = F[1];
     W.f2[q.e] = F[2];
   }
-  block_min_bits(lw, rbits);
+  if (!M.cinst) {
+    block_min_bits(lw, rbits);
+    return;
+  }
+  // ensembles: one r per instance (4 slots apart); a block inside one
+  // instance reduces as usual, a block straddling instances uses atomics
+  __shared__ int s_inst, s_mixed;
+  if (threadIdx.x == 0) {
+    s_inst = in[0].inst;
+    s_mixed = 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < kFluxItems; ++it)
+    if (in[it].e < M.e1 && in[it].inst != s_inst) s_mixed = 1;
+  __syncthreads();
+  if (!s_mixed) {
+    block_min_bits(lw, rbits + 4 * (int64_t)s_inst);
+  } else {
+#pragma unroll
+    for (int it = 0; it < kFluxItems; ++it)
+      if (lwi_item[it] < 1.0)
+        atomicMin(rbits + 4 * (int64_t)in[it].inst, (unsigned long long)__double_as_longlong(lwi_item[it]));
+  }
 }
 
 // ----------------------------------------------------------- draining dt --
 // TimeDisc::ComputeDrainingDt (src/TimeDisc.cpp:43-66) on the stage-start
 // state.
 __global__ void __launch_bounds__(256) k_drain(DevMesh M, DevState S, DevWork W) {
-  int64_t i = global_index();
-  if (i >= M.T) return;
+  int64_t i = M.c0 + global_index();
+  if (i >= M.c1) return;
+  if (M.cinst && !M.bactive[M.cinst[i]]) return;  // finished ensemble member
   double h = S.w[i] - M.bc[i];
   double d;
   if (!is_wet(h)) {
@@ -522,8 +554,8 @@ __device__ __forceinline__ double stage_dt(const StageArgs& A) {
 constexpr int kUpdItems = SWE_UPDATE_ITEMS;
 __global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, DevState U0, DevOut O,
                                                 DevWork W, StageArgs A) {
-  const int64_t base = (int64_t)logical_block() * (kBlock * kUpdItems) + threadIdx.x;
-  const double dt = stage_dt(A);
+  const int64_t base = M.c0 + (int64_t)logical_block() * (kBlock * kUpdItems) + threadIdx.x;
+  const double dt = M.cinst ? 0. : stage_dt(A);
   if (A.dts_log && blockIdx.x == 0 && threadIdx.x == 0) A.dts_log[0] = dt;
   RhsIn in[kUpdItems];
   double w[kUpdItems], u[kUpdItems], v[kUpdItems], bc[kUpdItems];
@@ -532,7 +564,7 @@ __global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, DevState 
 #pragma unroll
   for (int it = 0; it < kUpdItems; ++it) {
     const int64_t i = base + it * kBlock;
-    const int is = (int)(i < M.T ? i : 0);  // out-of-range items read cell 0 and are dropped
+    const int is = (int)(i < M.c1 ? i : M.c0);  // out-of-range items read a valid cell and are dropped
     rhs_load(M, S, W, is, in[it]);
     w[it] = S.w[is];
     u[it] = S.u[is];
@@ -544,11 +576,22 @@ __global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, DevState 
       v0[it] = U0.v[is];
     }
   }
-  const double dts = A.coef * dt;
 #pragma unroll
   for (int it = 0; it < kUpdItems; ++it) {
     const int64_t i = base + it * kBlock;
-    if (i >= M.T) continue;
+    if (i >= M.c1) continue;
+    double dts = A.coef * dt;
+    if (M.cinst) {
+      const int inst = M.cinst[i];
+      if (!M.bactive[inst]) {
+        // finished ensemble member: the stage leaves its state unchanged
+        O.w[i] = comb ? w0[it] : w[it];
+        O.u[i] = comb ? u0[it] : u[it];
+        O.v[i] = comb ? v0[it] : v[it];
+        continue;
+      }
+      dts = A.coef * M.bdt[inst];
+    }
     double R0, R1, R2;
     rhs_compute(in[it], dts, A.cor, R0, R1, R2);
     const double h = w[it] - bc[it];
@@ -575,6 +618,49 @@ __global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, DevState 
 
 __global__ void k_reset_r(unsigned long long* rbits) {
   *rbits = (unsigned long long)__double_as_longlong(1.0);
+}
+
+// ------------------------------------------------------------ ensembles --
+__global__ void k_batch_reset_r(DevBatch B, int slot) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < B.n) B.rbits[4 * k + slot] = (unsigned long long)__double_as_longlong(1.0);
+}
+
+// dt of every active member after the step's first flux pass: cfl * r,
+// clipped so the last step lands on t_end.
+__global__ void k_batch_dt(DevBatch B, double cfl, double t_end) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= B.n) return;
+  double d = 0.;
+  if (B.active[k]) {
+    d = cfl * __longlong_as_double((long long)B.rbits[4 * k]);
+    double rest = t_end - B.t[k];
+    if (!(d < rest)) d = rest;
+  }
+  B.dt[k] = d;
+}
+
+// end of a step: advance time, retire members that reached t_end, count
+// the active ones (one workgroup of 1024 threads, strided over members)
+__global__ void __launch_bounds__(1024) k_batch_advance(DevBatch B, double t_end) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int k = threadIdx.x; k < B.n; k += blockDim.x) {
+    if (B.active[k]) {
+      double d = B.dt[k];
+      double t = B.t[k];
+      double rest = t_end - t;
+      B.t[k] = (d < rest) ? t + d : t_end;
+      B.steps[k] += 1;
+      B.active[k] = (d < rest) ? 1 : 0;
+    }
+    mine += B.active[k];
+  }
+  atomicAdd(&cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) *B.n_active = cnt;
 }
 
 // ------------------------------------------------------------- getters --
@@ -651,21 +737,21 @@ hipError_t launch_setup_geometry(const DevMesh& M, const DevGeomOut& G, hipStrea
 }
 hipError_t launch_recon1(const DevMesh& M, const DevState& S, const DevWork& W, int opts,
                          hipStream_t st) {
-  if (opts & 1) hipLaunchKernelGGL((k_recon1<true>), grid_for(M.T), dim3(256), 0, st, M, S, W);
-  else hipLaunchKernelGGL((k_recon1<false>), grid_for(M.T), dim3(256), 0, st, M, S, W);
+  if (opts & 1) hipLaunchKernelGGL((k_recon1<true>), grid_for(M.c1 - M.c0), dim3(256), 0, st, M, S, W);
+  else hipLaunchKernelGGL((k_recon1<false>), grid_for(M.c1 - M.c0), dim3(256), 0, st, M, S, W);
   return hipGetLastError();
 }
 hipError_t launch_recon2(const DevMesh& M, const DevState& S, const DevWork& W, int opts,
                          hipStream_t st) {
-  if (opts & 2) hipLaunchKernelGGL((k_recon2<true>), grid_for(M.T), dim3(256), 0, st, M, S, W);
-  else hipLaunchKernelGGL((k_recon2<false>), grid_for(M.T), dim3(256), 0, st, M, S, W);
+  if (opts & 2) hipLaunchKernelGGL((k_recon2<true>), grid_for(M.c1 - M.c0), dim3(256), 0, st, M, S, W);
+  else hipLaunchKernelGGL((k_recon2<false>), grid_for(M.c1 - M.c0), dim3(256), 0, st, M, S, W);
   return hipGetLastError();
 }
 hipError_t launch_flux(const DevMesh& M, const DevState& S, const DevWork& W, int flux, int ws,
                        double cor, unsigned long long* rbits, hipStream_t st) {
-  hipLaunchKernelGGL(k_reset_r, dim3(1), dim3(1), 0, st, rbits);
-  if (M.E == 0) return hipGetLastError();
-  dim3 g = grid_for((M.E + kFluxItems - 1) / kFluxItems), b(256);
+  if (!M.cinst) hipLaunchKernelGGL(k_reset_r, dim3(1), dim3(1), 0, st, rbits);
+  if (M.e1 <= M.e0) return hipGetLastError();
+  dim3 g = grid_for((M.e1 - M.e0 + kFluxItems - 1) / kFluxItems), b(256);
 #define SWE_FLUX_CASE(F, Wv)                                                           \
   if (flux == F && ws == Wv) {                                                         \
     hipLaunchKernelGGL((k_flux<F, Wv>), g, b, 0, st, M, S, W, cor, rbits);             \
@@ -677,13 +763,25 @@ hipError_t launch_flux(const DevMesh& M, const DevState& S, const DevWork& W, in
   return hipErrorInvalidValue;
 }
 hipError_t launch_drain(const DevMesh& M, const DevState& S, const DevWork& W, hipStream_t st) {
-  hipLaunchKernelGGL(k_drain, grid_for(M.T), dim3(256), 0, st, M, S, W);
+  hipLaunchKernelGGL(k_drain, grid_for(M.c1 - M.c0), dim3(256), 0, st, M, S, W);
   return hipGetLastError();
 }
 hipError_t launch_update(const DevMesh& M, const DevState& S, const DevState& U0, const DevOut& O,
                          const DevWork& W, const StageArgs& A, hipStream_t st) {
-  hipLaunchKernelGGL(k_update, grid_for((M.T + kUpdItems - 1) / kUpdItems), dim3(256), 0, st, M, S,
+  hipLaunchKernelGGL(k_update, grid_for((M.c1 - M.c0 + kUpdItems - 1) / kUpdItems), dim3(256), 0, st, M, S,
                      U0, O, W, A);
+  return hipGetLastError();
+}
+hipError_t launch_batch_reset_r(const DevBatch& B, int slot, hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_reset_r, dim3((B.n + 255) / 256), dim3(256), 0, st, B, slot);
+  return hipGetLastError();
+}
+hipError_t launch_batch_dt(const DevBatch& B, double cfl, double t_end, hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_dt, dim3((B.n + 255) / 256), dim3(256), 0, st, B, cfl, t_end);
+  return hipGetLastError();
+}
+hipError_t launch_batch_advance(const DevBatch& B, double t_end, hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1024), 0, st, B, t_end);
   return hipGetLastError();
 }
 hipError_t launch_edge_field(const DevMesh& M, const DevState& S, const DevWork& W, double cor,

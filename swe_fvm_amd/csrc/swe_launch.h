@@ -56,6 +56,25 @@ struct DevMesh {
   const double* ey;
   const double* ez;
   const double* edmin;
+  // batched ensembles (null for a single mesh): instance of every cell and
+  // the per-instance step size and activity flag
+  const int* cinst;
+  const double* bdt;
+  const int* bactive;
+  // cells [c0, c1) and edges [e0, e1) the step kernels visit (the whole mesh
+  // except when finished ensemble members are trimmed off the front)
+  int64_t c0, c1, e0, e1;
+};
+
+// per-instance control of a batched ensemble
+struct DevBatch {
+  int n;                      // instances
+  double* t;                  // simulated time
+  double* dt;                 // step size of the current step
+  int* active;                // t < t_end
+  long long* steps;           // steps taken
+  unsigned long long* rbits;  // r of each flux pass, 4 slots per instance
+  int* n_active;              // number of active instances after the last step
 };
 
 // writable view of the static geometry arrays, for the setup kernel
@@ -109,6 +128,9 @@ hipError_t launch_flux(const DevMesh& M, const DevState& S, const DevWork& W, in
 hipError_t launch_drain(const DevMesh& M, const DevState& S, const DevWork& W, hipStream_t st);
 hipError_t launch_update(const DevMesh& M, const DevState& S, const DevState& U0, const DevOut& O,
                          const DevWork& W, const StageArgs& A, hipStream_t st);
+hipError_t launch_batch_reset_r(const DevBatch& B, int slot, hipStream_t st);
+hipError_t launch_batch_dt(const DevBatch& B, double cfl, double t_end, hipStream_t st);
+hipError_t launch_batch_advance(const DevBatch& B, double t_end, hipStream_t st);
 hipError_t launch_edge_field(const DevMesh& M, const DevState& S, const DevWork& W, double cor,
                              double* edg, double* src, hipStream_t st);
 hipError_t launch_max_wp(const DevMesh& M, const DevWork& W, double* out, hipStream_t st);

@@ -201,3 +201,45 @@ def dam_break(n=2828, seed=3, modes=64, amp=0.1, dry_frac=0.05, jitter=0.25):
     h[dry] = 0.0
     prim = np.stack([bc + h, np.zeros(T), np.zeros(T)], axis=1)
     return dom, prim
+
+
+def ensemble_sizes(n_inst=256, total=20_000_000, tmin=10_000, tmax=500_000, seed=11):
+    """Member sizes (triangles): log-uniform on [tmin, tmax], then rescaled
+    (and clipped to the same range) so they sum to about `total`; each is
+    rounded to a jittered grid 2 n^2."""
+    rng = np.random.default_rng(seed)
+    t = np.exp(rng.uniform(np.log(tmin), np.log(tmax), n_inst))
+    for _ in range(50):
+        t = np.clip(t * (total / t.sum()), tmin, tmax)
+    n = np.maximum(2, np.round(np.sqrt(t / 2.0))).astype(int)
+    return n
+
+
+def ensemble_member(n, seed):
+    """One ensemble member on [0, 1]^2: 1-3 Gaussian bumps of the bottom
+    (amplitude U(0.05, 0.25), width U(0.05, 0.2)), a Gaussian surface hump
+    of amplitude U(0.01, 0.2) and width U(0.05, 0.2) on w = 1, u = v = 0."""
+    xyz, topo = jittered_mesh(int(n), 0.25, seed)
+    rng = np.random.default_rng(seed + 1_000_003)
+    x, y = xyz[:, 0], xyz[:, 1]
+    z = np.zeros_like(x)
+    for _ in range(int(rng.integers(1, 4))):
+        cx, cy = rng.uniform(0.1, 0.9, 2)
+        A, s = rng.uniform(0.05, 0.25), rng.uniform(0.05, 0.2)
+        z += A * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * s * s))
+    xyz[:, 2] = z
+    dom = Domain(xyz, topo)
+    cen = dom.centroids()
+    A, s = rng.uniform(0.01, 0.2), rng.uniform(0.05, 0.2)
+    hx, hy = rng.uniform(0.3, 0.7, 2)
+    w = 1.0 + A * np.exp(-((cen[:, 0] - hx) ** 2 + (cen[:, 1] - hy) ** 2) / (2 * s * s))
+    prim = np.stack([w, np.zeros_like(w), np.zeros_like(w)], axis=1)
+    return dom, prim
+
+
+def ensemble(n_inst=256, total=20_000_000, seed=11, sizes=None):
+    """Seeded ensemble: list of (Domain, prim) of log-uniform sizes, smallest
+    first (small members reach t_end first, so the running ones stay a
+    contiguous block of cells)."""
+    ns = np.sort(ensemble_sizes(n_inst, total, seed=seed)) if sizes is None else sizes
+    return [ensemble_member(n, seed * 7919 + k) for k, n in enumerate(ns)]
