@@ -215,10 +215,8 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
   ALLOC(s->rbits, 4);
   DevGeomOut G{};
   ALLOC(G.cx, T); ALLOC(G.cy, T); ALLOC(G.bc, T); ALLOC(G.zmin, T); ALLOC(G.zmax, T);
-  ALLOC(G.area, T); ALLOC(G.iarea, T);
-  ALLOC(G.el, E); ALLOC(G.etx, E); ALLOC(G.ety, E); ALLOC(G.ex, E); ALLOC(G.ey, E); ALLOC(G.ez, E);
+  ALLOC(G.area, T);
   ALLOC(G.edmin, E);
-  ALLOC(G.bad, 1);
 #undef ALLOC
 #define H2D(dst, src, n)                                                                     \
   if ((e = hipMemcpy(dst, src, sizeof(*(src)) * size_t(n), hipMemcpyHostToDevice)) != hipSuccess) \
@@ -241,22 +239,13 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
     if ((e = hipMemset(s->sv[b], 0, sizeof(double) * size_t(std::max<int64_t>(T, 1)))) != hipSuccess) return bail(e, "hipMemset");
   }
   s->M = DevMesh{dpx, dpy, dpz, dtp, dte, dtt, dfl, dep0, dep1, det0, det1, doff, dn2c, T, E, V,
-                 G.cx, G.cy, G.bc, G.zmin, G.zmax, G.area, G.iarea,
-                 G.el, G.etx, G.ety, G.ex, G.ey, G.ez, G.edmin};
+                 G.cx, G.cy, G.bc, G.zmin, G.zmax, G.area, G.edmin};
   s->M.c0 = 0;
   s->M.c1 = T;
   s->M.e0 = 0;
   s->M.e1 = E;
-  if ((e = hipMemset(G.bad, 0, sizeof(unsigned int))) != hipSuccess) return bail(e, "hipMemset");
   if ((e = launch_setup_geometry(s->M, G, s->stream)) != hipSuccess) return bail(e, "setup_geometry");
-  unsigned int bad = 0;
-  if ((e = hipMemcpyAsync(&bad, G.bad, sizeof(bad), hipMemcpyDeviceToHost, s->stream)) != hipSuccess)
-    return bail(e, "hipMemcpy");
   if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) return bail(e, "hipStreamSynchronize");
-  if (bad) {
-    swe_destroy(s);
-    return fail(SWE_ERR_DOMAIN, "degenerate triangle: an edge's outward tangents are not opposite");
-  }
   if (n_inst > 0) {
     s->n_inst = n_inst;
     int* dci;

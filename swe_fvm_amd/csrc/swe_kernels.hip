@@ -17,9 +17,11 @@
 //   k_update  per cell  — fixed-order gather of the three edge fluxes, the
 //                         well-balanced source, SSP-RK stage combination and
 //                         ConsAssigner write-back.
-// Static geometry (centroids, areas, edge lengths, tangents, midpoints) is
-// computed once by k_setup_geometry with the same device functions, so the
-// step reads it instead of chasing cell -> node indirections.
+// Cell geometry that neighbours gather (centroid, bathymetry range, area) and
+// the flux's CFL length are computed once by the setup kernels; the edge
+// geometry (length, midpoint, tangent) and the updated cell's own centroid and
+// area are recomputed from the node coordinates with the same device
+// functions (bit-identical), which costs less than streaming them from HBM.
 // Every reduction is either order-independent (max, min) or a fixed-order
 // per-cell gather, so results are deterministic run to run.  Local arrays
 // are only indexed by compile-time constants (unrolled loops) or through
@@ -102,34 +104,18 @@ __global__ void __launch_bounds__(256) k_setup_cells(DevMesh M, DevGeomOut G) {
   G.bc[i] = c.z;
   G.zmin[i] = dmin(dmin(p0.z, p1.z), p2.z);
   G.zmax[i] = dmax(dmax(p0.z, p1.z), p2.z);
-  double a = tri_area(p0, p1, p2);
-  G.area[i] = a;
-  G.iarea[i] = 1. / a;
+  G.area[i] = tri_area(p0, p1, p2);
 }
 
+// min over the edge's two cells of 2 area / length (the flux's CFL length)
 __global__ void __launch_bounds__(256) k_setup_edges(DevMesh M, DevGeomOut G) {
   int64_t e = global_index();
   if (e >= M.E) return;
-  V3 Pa = node(M, M.ep0[e]), Pb = node(M, M.ep1[e]);
-  double le = edge_len(Pa, Pb);
-  V3 E = midpoint(Pa, Pb);
-  int f = M.et0[e], t = M.et1[e];
-  V3 cf = {G.cx[f], G.cy[f], 0.};
-  double tx, ty;
-  tangent(Pa, Pb, le, cf, tx, ty);
-  G.el[e] = le;
-  G.etx[e] = tx;
-  G.ety[e] = ty;
-  G.ex[e] = E.x;
-  G.ey[e] = E.y;
-  G.ez[e] = E.z;
+  const int f = M.et0[e], t = M.et1[e];
   double dm = 0.;
   if (t >= 0) {
-    V3 ct = {G.cx[t], G.cy[t], 0.};
-    double ux, uy;
-    tangent(Pa, Pb, le, ct, ux, uy);
-    if (!(ux == -tx && uy == -ty)) atomicAdd(G.bad, 1u);
-    double dl = 2. * G.area[f] / le, dr = 2. * G.area[t] / le;
+    const double le = edge_len(node(M, M.ep0[e]), node(M, M.ep1[e]));
+    const double dl = 2. * G.area[f] / le, dr = 2. * G.area[t] / le;
     dm = dmin(dl, dr);
   }
   G.edmin[e] = dm;
@@ -146,28 +132,25 @@ __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W
   if (M.cinst && !M.bactive[M.cinst[i]]) return;  // finished ensemble member
   // every input of every branch is loaded up front, so the gathers overlap
   const double w = S.w[i];
-  const double bc = M.bc[i];
   const uint32_t fl = M.cflag[i];
-  const double zmx = M.zmax[i], zmn = M.zmin[i];
-  const double cx = M.cx[i], cy = M.cy[i];
   const V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[M.T + i]), p2 = node(M, M.tp[2 * M.T + i]);
-  int jn[3];
   double wn[3], bcn[3], zmxn[3], zmnn[3], cxn[3], cyn[3];
   uint32_t fln[3];
 #pragma unroll
   for (int kk = 0; kk < 3; ++kk) {
     int j = M.tt[kk * M.T + i];
     j = j < 0 ? (int)i : j;  // boundary cells never take the full-wet branch
-    jn[kk] = j;
     wn[kk] = S.w[j];
+    fln[kk] = M.cflag[j];
     bcn[kk] = M.bc[j];
     zmxn[kk] = M.zmax[j];
     zmnn[kk] = M.zmin[j];
     cxn[kk] = M.cx[j];
     cyn[kk] = M.cy[j];
-    fln[kk] = M.cflag[j];
   }
-  (void)jn;
+  const double bc = M.bc[i];
+  const double zmx = M.zmax[i], zmn = M.zmin[i];
+  const double cx = M.cx[i], cy = M.cy[i];
   double w0 = w, gx = 0., gy = 0.;
   int8_t k;
   if (!is_wet(w - bc)) {
@@ -355,7 +338,8 @@ __device__ __forceinline__ void block_min_bits(double v, unsigned long long* tar
 #endif
 constexpr int kFluxItems = SWE_FLUX_ITEMS;
 struct FluxIn {
-  double tx, ty, ex, ey, ez, edmin, hf, cxf, cyf, cxt, cyt;
+  V3 Pa, Pb;  // P(ep0), P(ep1)
+  double edmin, hf, cxf, cyf, cxt, cyt;
   Muscl mf, mt;
   int64_t e;
   int t, inst;
@@ -372,11 +356,8 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, 
     const int64_t es = e < M.e1 ? e : M.e0;  // out-of-range items read a valid edge and are dropped
     FluxIn& q = in[it];
     q.e = e;
-    q.tx = M.etx[es];
-    q.ty = M.ety[es];
-    q.ex = M.ex[es];
-    q.ey = M.ey[es];
-    q.ez = M.ez[es];
+    q.Pa = node(M, M.ep0[es]);
+    q.Pb = node(M, M.ep1[es]);
     q.edmin = M.edmin[es];
     const int f = M.et0[es], t = M.et1[es];
     const int tr = t < 0 ? f : t;  // wall edges read their own cell twice
@@ -398,17 +379,21 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, 
     const FluxIn& q = in[it];
     lwi_item[it] = 1.0;
     if (!q.on) continue;
-    const double nx = q.ty, ny = -q.tx;
+    // Domain::L / E / Tang (src/Bathymetry.cpp:29-76), outward of et0
+    const double le = edge_len(q.Pa, q.Pb);
+    const V3 E = midpoint(q.Pa, q.Pb);
+    double tx, ty;
+    tangent(q.Pa, q.Pb, le, V3{q.cxf, q.cyf, 0.}, tx, ty);
+    const double nx = ty, ny = -tx;
     double F[3];
     double lwi = 1.0;
     if (q.t < 0) {
       elem_flux(nx, ny, q.hf, 0., 0., F[0], F[1], F[2]);
     } else {
-      const V3 E = {q.ex, q.ey, q.ez};
       Slot L = slot_value(q.mf, q.cxf, q.cyf, E);
       Slot R = slot_value(q.mt, q.cxt, q.cyt, E);
       if (FLUX == 0) flux_hll<WS>(nx, ny, E.z, L, R, q.edmin, cor, lwi, F);
-      else flux_hllc<WS>(q.tx, q.ty, E.z, L, R, q.edmin, cor, lwi, F);
+      else flux_hllc<WS>(tx, ty, E.z, L, R, q.edmin, cor, lwi, F);
     }
     lw = dmin(lw, lwi);
     lwi_item[it] = lwi;
@@ -473,16 +458,17 @@ __global__ void __launch_bounds__(256) k_drain(DevMesh M, DevState S, DevWork W)
 struct RhsIn {
   uint32_t fl;
   Muscl mu;
-  double cx, cy, i_area, dti;
-  double F0[3], F1[3], F2[3], dtik[3], le[3], ex[3], ey[3], ez[3], tx[3], ty[3];
+  V3 p[3];  // the cell's vertices
+  double dti;
+  double F0[3], F1[3], F2[3], dtik[3];
 };
 __device__ __forceinline__ void rhs_load(const DevMesh& M, const DevState& S, const DevWork& W,
                                          int i, RhsIn& q) {
   q.fl = M.cflag[i];
   q.mu = load_muscl(S, W, i);
-  q.cx = M.cx[i];
-  q.cy = M.cy[i];
-  q.i_area = M.iarea[i];
+  q.p[0] = node(M, M.tp[i]);
+  q.p[1] = node(M, M.tp[M.T + i]);
+  q.p[2] = node(M, M.tp[2 * M.T + i]);
   q.dti = W.dtd[i];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -493,33 +479,34 @@ __device__ __forceinline__ void rhs_load(const DevMesh& M, const DevState& S, co
     q.F2[k] = W.f2[e];
     const double d = W.dtd[nb < 0 ? i : nb];
     q.dtik[k] = nb < 0 ? INFINITY : d;
-    q.le[k] = M.el[e];
-    q.ex[k] = M.ex[e];
-    q.ey[k] = M.ey[e];
-    q.ez[k] = M.ez[e];
-    q.tx[k] = M.etx[e];
-    q.ty[k] = M.ety[e];
   }
 }
 __device__ __forceinline__ void rhs_compute(const RhsIn& q, double dts, double cor, double& r0,
                                             double& r1, double& r2) {
   r0 = 0.; r1 = 0.; r2 = 0.;
+  // Domain::T / Area (src/Bathymetry.cpp:24-27,84-90)
+  const V3 c = centroid(q.p[0], q.p[1], q.p[2]);
+  const double i_area = 1. / tri_area(q.p[0], q.p[1], q.p[2]);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const bool own = sgn_pos(q.fl, k);
     const double sg = own ? 1. : -1.;
     const double F0 = q.F0[k], F1 = q.F1[k], F2 = q.F2[k];
     const double dtk = (sg * F0) > 0. ? dmin(dts, q.dti) : dmin(dts, q.dtik[k]);
-    const double c_ek = q.i_area * q.le[k];
-    const V3 E = {q.ex[k], q.ey[k], q.ez[k]};
-    const Slot s = slot_value(q.mu, q.cx, q.cy, E);
+    // Domain::L / E / Tang of edge k (src/Bathymetry.cpp:29-76); the tangent
+    // is taken outward of this cell, as Norm(e, i) is
+    const V3 Pa = pick(q.p[0], q.p[1], q.p[2], ep0_local(q.fl, k));
+    const V3 Pb = pick(q.p[0], q.p[1], q.p[2], ep1_local(q.fl, k));
+    const double le = edge_len(Pa, Pb);
+    const double c_ek = i_area * le;
+    const V3 E = midpoint(Pa, Pb);
+    const Slot s = slot_value(q.mu, c.x, c.y, E);
     const double h_ek = s.w - E.z;
     double src1 = q.mu.gx, src2 = q.mu.gy;
     src1 += cor * (-s.v);
     src2 += cor * s.u;
-    // Norm(e, i): the stored tangent is outward of et0; exact negation for et1
-    const double tx = own ? q.tx[k] : -q.tx[k];
-    const double ty = own ? q.ty[k] : -q.ty[k];
+    double tx, ty;
+    tangent(Pa, Pb, le, c, tx, ty);
     const double nx = ty, ny = -tx;
     const double cf = dtk * sg * c_ek;
     r0 -= cf * F0;
@@ -569,7 +556,7 @@ __global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, DevState 
     w[it] = S.w[is];
     u[it] = S.u[is];
     v[it] = S.v[is];
-    bc[it] = M.bc[is];
+    bc[it] = centroid(in[it].p[0], in[it].p[1], in[it].p[2]).z;
     if (comb) {
       w0[it] = U0.w[is];
       u0[it] = U0.u[is];
@@ -677,7 +664,7 @@ __global__ void __launch_bounds__(256) k_edge_field(DevMesh M, DevState S, DevWo
     int e = M.te[k * M.T + i];
     int nb = M.tt[k * M.T + i];
     int64_t s = 2 * (int64_t)e + (((int)i < nb) ? 1 : 0);
-    V3 E = {M.ex[e], M.ey[e], M.ez[e]};
+    const V3 E = midpoint(node(M, M.ep0[e]), node(M, M.ep1[e]));
     Slot sv = slot_value(mu, cx, cy, E);
     edg[3 * s] = sv.w;
     edg[3 * s + 1] = sv.u;

@@ -9,11 +9,11 @@
 //            bits 4+4k, 6+4k  local vertex of ep0 / ep1 of edge k
 //   edges  ep0, ep1, et0, et1             int32[E]
 //   node -> cell map  n2c_off[V+1], n2c[3T]
-//   static geometry (computed once by k_setup_geometry with the step's own
+//   static geometry (computed once by the setup kernels with the step's own
 //   formulas, so every use is bit-identical to recomputing it):
-//     cells  cx, cy, bc (centroid x, y, z), zmin, zmax, area, iarea (1/area)
-//     edges  el (length), etx, ety (tangent outward of et0), ex, ey, ez
-//            (midpoint), edmin (min over the two cells of 2 area / length)
+//     cells  cx, cy, bc (centroid x, y, z), zmin, zmax, area
+//     edges  edmin (min over the two cells of 2 area / length)
+//   Edge lengths, midpoints and tangents are recomputed from px, py, pz.
 //   state  w, u, v                        double[T] per buffer
 //   work   MUSCL (mw, mgx, mgy): pass 1, overwritten in place by pass 2 for
 //          partially wet cells, whose pass-1 level is kept in wrec1 for the
@@ -48,13 +48,6 @@ struct DevMesh {
   const double* zmin;
   const double* zmax;
   const double* area;
-  const double* iarea;
-  const double* el;
-  const double* etx;
-  const double* ety;
-  const double* ex;
-  const double* ey;
-  const double* ez;
   const double* edmin;
   // batched ensembles (null for a single mesh): instance of every cell and
   // the per-instance step size and activity flag
@@ -79,9 +72,7 @@ struct DevBatch {
 
 // writable view of the static geometry arrays, for the setup kernel
 struct DevGeomOut {
-  double *cx, *cy, *bc, *zmin, *zmax, *area, *iarea;
-  double *el, *etx, *ety, *ex, *ey, *ez, *edmin;
-  unsigned int* bad;  // count of edges whose two tangents are not exact opposites
+  double *cx, *cy, *bc, *zmin, *zmax, *area, *edmin;
 };
 
 struct DevState {
