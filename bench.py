@@ -91,6 +91,9 @@ def bench_ensemble(args):
           f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     fl = swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt]
     ens = Ensemble(fl, [d for d, _ in mem], [p for _, p in mem])
+    tune = ens.autotune(reps=3)
+    print(f"[bench] autotune: flux x{tune['flux_items']} update x{tune['update_items']}",
+          file=sys.stderr, flush=True)
     ens.run("SSPRK2", cfl=0.15, t_end=args.t_end, max_steps=2, graph=True)  # warm-up + capture
     ens.reset(0.0)
     ens.set_states([p for _, p in mem])
@@ -153,7 +156,9 @@ def bench_ensemble(args):
                        "seconds": seq_ms / 1e3},
         "config": {"workload": f"ensemble_{len(mem)}_members_T{int(T.sum())}", "members": len(mem),
                    "cells_min": int(T.min()), "cells_max": int(T.max()), "scheme": "SSPRK2",
-                   "flux": "HLLC<Einfeldt>", "dt": "per-member CFL 0.15"},
+                   "flux": "HLLC<Einfeldt>", "dt": "per-member CFL 0.15",
+                   "tuning": {"flux_edges_per_thread": tune["flux_items"],
+                              "update_cells_per_thread": tune["update_items"]}},
         "dtype": "f64", "data": "synthetic", "n_gpus": 1}))
 
 
@@ -215,6 +220,10 @@ def main():
     sd = swe.SpaceDisc(fl, dom, prim, device=0)
     if args.warmup:
         sd.run("SSPRK2", args.warmup, dt=0.0, cfl=0.15)
+    # work items per thread: the fastest bit-identical kernel instantiation
+    tune = sd.autotune(reps=5)
+    print(f"[bench] autotune: flux x{tune['flux_items']} update x{tune['update_items']}",
+          file=sys.stderr, flush=True)
     sd.synchronize()
     t1 = time.perf_counter()
     ms = sd.time_run("SSPRK2", args.steps, dt=0.0, cfl=0.15)
@@ -251,7 +260,9 @@ def main():
         "config": {"workload": workload, "cells": T, "edges": E, "nodes": V,
                    "mesh": f"seeded Delaunay of {args.n}x{args.n} jittered grid, Morton order",
                    "scheme": "SSPRK2", "flux": "HLLC<Einfeldt>", "dt": "CFL 0.15",
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU",
+                   "tuning": {"flux_edges_per_thread": tune["flux_items"],
+                              "update_cells_per_thread": tune["update_items"]}},
         "roofline": {"bound": "hbm", "kernel": dom_k, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
                      "traffic": traffic},

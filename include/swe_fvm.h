@@ -188,6 +188,21 @@ int swe_batch_time(swe_solver* s, double* t, int64_t* steps);
 int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nsteps, double* ms,
                  double* kernel_ms);
 
+/* ---------------------------------------------------------------- tuning --- */
+
+/* Work items per thread of the flux kernel (1, 2 or 4 edges) and of the
+ * update kernel (1 or 2 cells); no reference counterpart (the reference is
+ * serial).  Results do not depend on the choice, only the speed does.
+ * Defaults: 2 and 1. */
+int swe_set_tuning(swe_solver* s, int flux_items, int update_items);
+int swe_get_tuning(const swe_solver* s, int* flux_items, int* update_items);
+/* Times every instantiation reps times on the current state, keeps the
+ * fastest and returns it; times (optional, 5 doubles) receives the average
+ * per-launch milliseconds of flux x1, x2, x4 and update x1, x2.  The state
+ * is unchanged; the reconstruction and flux fields afterwards are those of
+ * the current state. */
+int swe_autotune(swe_solver* s, int reps, int* flux_items, int* update_items, double* times);
+
 #ifdef __cplusplus
 }
 #endif

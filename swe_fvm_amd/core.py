@@ -353,6 +353,33 @@ class SpaceDisc:
                                         ctypes.byref(ms), None))
         return ms.value
 
+    def set_tuning(self, flux_items=2, update_items=1):
+        """Edges per thread of the flux kernel (1, 2, 4) and cells per thread
+        of the update kernel (1, 2); results are identical for every choice."""
+        _check(_lib.load().swe_set_tuning(self._h, int(flux_items), int(update_items)))
+
+    def tuning(self):
+        return _tuning(self._h)
+
+    def autotune(self, reps=10):
+        """Times every flux / update instantiation on the current state and
+        keeps the fastest; returns the choice and the per-launch times."""
+        return _autotune(self._h, reps)
+
+
+def _tuning(h):
+    fi, ui = ctypes.c_int(), ctypes.c_int()
+    _check(_lib.load().swe_get_tuning(h, ctypes.byref(fi), ctypes.byref(ui)))
+    return fi.value, ui.value
+
+
+def _autotune(h, reps):
+    fi, ui = ctypes.c_int(), ctypes.c_int()
+    t = np.zeros(5, np.float64)
+    _check(_lib.load().swe_autotune(h, int(reps), ctypes.byref(fi), ctypes.byref(ui), _ptr(t)))
+    return {"flux_items": fi.value, "update_items": ui.value,
+            "flux_ms": {1: t[0], 2: t[1], 4: t[2]}, "update_ms": {1: t[3], 2: t[4]}}
+
 
 class TimeDisc:
     """TimeDisc(SpaceDisc*) (include/TimeDisc.h:4-23)."""

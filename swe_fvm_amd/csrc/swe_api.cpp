@@ -18,6 +18,7 @@ struct swe_solver {
   int device = 0;
   hipStream_t stream = nullptr;
   int flux = 1, ws = 2, opts = 0;
+  int flux_items = 2, upd_items = 1;  // work items per thread (swe_autotune)
   double cor = 0., tau = 0.;
   int64_t T = 0, E = 0, V = 0;
   std::vector<void*> allocs;
@@ -43,6 +44,14 @@ struct swe_solver {
 };
 
 namespace {
+
+// entry points of the single-mesh step (one CFL slot per stage) refuse
+// ensemble handles, whose flux pass keeps one slot per member
+int single_mesh(const swe_solver* s) {
+  if (!s) return fail(SWE_ERR_ARG, "null solver");
+  if (s->n_inst > 0) return fail(SWE_ERR_ARG, "ensemble solver: use swe_batch_run");
+  return SWE_OK;
+}
 
 int dev_fail(hipError_t e, const char* what) {
   return fail(SWE_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
@@ -76,10 +85,10 @@ int run_stage(swe_solver* s, int in, int u0, int out, double a0, double a1, doub
   DevState S = state_of(s, in);
   SWE_CHECK(launch_recon1(s->M, S, s->W, s->opts, s->stream), "recon1");
   SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
-  SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + slot, s->stream), "flux");
+  SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + slot, s->flux_items, s->stream), "flux");
   SWE_CHECK(launch_drain(s->M, S, s->W, s->stream), "drain");
   StageArgs A{a0, a1, coef, dt_fixed, cfl, s->cor, s->rbits, dts_log};
-  SWE_CHECK(launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->stream), "update");
+  SWE_CHECK(launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->upd_items, s->stream), "update");
   s->last_r_slot = slot;
   return SWE_OK;
 }
@@ -214,9 +223,10 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
   ALLOC(s->W.dtd, T);
   ALLOC(s->rbits, 4);
   DevGeomOut G{};
-  ALLOC(G.cx, T); ALLOC(G.cy, T); ALLOC(G.bc, T); ALLOC(G.zmin, T); ALLOC(G.zmax, T);
+  ALLOC(G.cx, T); ALLOC(G.cy, T); ALLOC(G.bc, T);
   ALLOC(G.area, T);
   ALLOC(G.edmin, E);
+  ALLOC(G.crec, T);
 #undef ALLOC
 #define H2D(dst, src, n)                                                                     \
   if ((e = hipMemcpy(dst, src, sizeof(*(src)) * size_t(n), hipMemcpyHostToDevice)) != hipSuccess) \
@@ -239,7 +249,7 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
     if ((e = hipMemset(s->sv[b], 0, sizeof(double) * size_t(std::max<int64_t>(T, 1)))) != hipSuccess) return bail(e, "hipMemset");
   }
   s->M = DevMesh{dpx, dpy, dpz, dtp, dte, dtt, dfl, dep0, dep1, det0, det1, doff, dn2c, T, E, V,
-                 G.cx, G.cy, G.bc, G.zmin, G.zmax, G.area, G.edmin};
+                 G.cx, G.cy, G.bc, G.area, G.edmin, G.crec};
   s->M.c0 = 0;
   s->M.c1 = T;
   s->M.e0 = 0;
@@ -364,10 +374,10 @@ int swe_compute_interface_values(swe_solver* s) {
 }
 
 int swe_compute_fluxes(swe_solver* s, double* r) {
-  if (!s) return fail(SWE_ERR_ARG, "null solver");
+  if (int rc = single_mesh(s)) return rc;
   if (int rc = set_device(s)) return rc;
   DevState S = state_of(s, s->cur);
-  SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + 3, s->stream), "flux");
+  SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + 3, s->flux_items, s->stream), "flux");
   s->last_r_slot = 3;
   unsigned long long bits;
   SWE_CHECK(hipMemcpyAsync(&bits, s->rbits + 3, 8, hipMemcpyDeviceToHost, s->stream), "hipMemcpy");
@@ -458,7 +468,7 @@ int swe_rhs(swe_solver* s, int64_t cell, double dt, double* out3) {
 }
 
 int swe_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nsteps) {
-  if (!s) return fail(SWE_ERR_ARG, "null solver");
+  if (int rc = single_mesh(s)) return rc;
   if (scheme < 0 || scheme > 2) return fail(SWE_ERR_ARG, "unknown scheme");
   if (nsteps < 0) return fail(SWE_ERR_ARG, "negative step count");
   if (!(dt > 0.) && !(cfl > 0.)) return fail(SWE_ERR_ARG, "CFL mode needs cfl > 0");
@@ -489,7 +499,8 @@ int swe_synchronize(swe_solver* s) {
 }
 
 int swe_cfl_dt(swe_solver* s, double* dt) {
-  if (!s || !dt) return fail(SWE_ERR_ARG, "null argument");
+  if (int rc = single_mesh(s)) return rc;
+  if (!dt) return fail(SWE_ERR_ARG, "null argument");
   if (int rc = set_device(s)) return rc;
   unsigned long long bits;
   if (int rc = download(s, &bits, s->rbits + s->last_r_slot, 8)) return rc;
@@ -500,7 +511,8 @@ int swe_cfl_dt(swe_solver* s, double* dt) {
 }
 
 int swe_min_len_to_wavespeed(swe_solver* s, double* r) {
-  if (!s || !r) return fail(SWE_ERR_ARG, "null argument");
+  if (int rc = single_mesh(s)) return rc;
+  if (!r) return fail(SWE_ERR_ARG, "null argument");
   if (int rc = set_device(s)) return rc;
   unsigned long long bits;
   if (int rc = download(s, &bits, s->rbits + s->last_r_slot, 8)) return rc;
@@ -542,11 +554,11 @@ static int batch_step(swe_solver* s, int scheme, double cfl, double t_end) {
     SWE_CHECK(launch_batch_reset_r(s->B, q, s->stream), "batch_reset_r");
     SWE_CHECK(launch_recon1(s->M, S, s->W, s->opts, s->stream), "recon1");
     SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
-    SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->B.rbits + q, s->stream), "flux");
+    SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->B.rbits + q, s->flux_items, s->stream), "flux");
     if (q == 0) SWE_CHECK(launch_batch_dt(s->B, cfl, t_end, s->stream), "batch_dt");
     SWE_CHECK(launch_drain(s->M, S, s->W, s->stream), "drain");
     StageArgs A{st[q].a0, st[q].a1, st[q].coef, 0., cfl, s->cor, nullptr, nullptr};
-    SWE_CHECK(launch_update(s->M, S, state_of(s, st[q].u0), out_of(s, st[q].out), s->W, A, s->stream),
+    SWE_CHECK(launch_update(s->M, S, state_of(s, st[q].u0), out_of(s, st[q].out), s->W, A, s->upd_items, s->stream),
               "update");
   }
   SWE_CHECK(launch_batch_advance(s->B, t_end, s->stream), "batch_advance");
@@ -691,7 +703,8 @@ int swe_batch_time(swe_solver* s, double* t, int64_t* steps) {
 
 int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nsteps, double* ms,
                  double* kernel_ms) {
-  if (!s || !ms) return fail(SWE_ERR_ARG, "null argument");
+  if (int rc = single_mesh(s)) return rc;
+  if (!ms) return fail(SWE_ERR_ARG, "null argument");
   if (int rc = set_device(s)) return rc;
   hipEvent_t e0, e1;
   SWE_CHECK(hipEventCreate(&e0), "hipEventCreate");
@@ -735,14 +748,14 @@ int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nstep
         launch_recon2(s->M, S, s->W, s->opts, s->stream);
         hipEventRecord(ev[2], s->stream);
         hipEventRecord(ev[3], s->stream);
-        launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + st, s->stream);
+        launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + st, s->flux_items, s->stream);
         hipEventRecord(ev[4], s->stream);
         hipEventRecord(ev[5], s->stream);
         launch_drain(s->M, S, s->W, s->stream);
         hipEventRecord(ev[6], s->stream);
         hipEventRecord(ev[7], s->stream);
         StageArgs A{a0, a1, coef, dt, cfl, s->cor, s->rbits, st == 0 ? s->dts + k : nullptr};
-        launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->stream);
+        launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->upd_items, s->stream);
         hipEventRecord(ev[8], s->stream);
         hipError_t e = hipEventSynchronize(ev[8]);
         if (e != hipSuccess) { rc = dev_fail(e, "kernel"); break; }
@@ -769,6 +782,89 @@ int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nstep
     if (e != hipSuccess) rc = dev_fail(e, "run");
   }
   return rc;
+}
+
+int swe_set_tuning(swe_solver* s, int flux_items, int update_items) {
+  if (!s) return fail(SWE_ERR_ARG, "null solver");
+  if (flux_items != 1 && flux_items != 2 && flux_items != 4)
+    return fail(SWE_ERR_ARG, "flux_items must be 1, 2 or 4");
+  if (update_items != 1 && update_items != 2) return fail(SWE_ERR_ARG, "update_items must be 1 or 2");
+  if (s->flux_items != flux_items || s->upd_items != update_items) {
+    s->flux_items = flux_items;
+    s->upd_items = update_items;
+    if (s->graph) {  // a captured ensemble step holds the old launches
+      hipGraphExecDestroy(s->graph);
+      s->graph = nullptr;
+      s->graph_scheme = -1;
+    }
+  }
+  return SWE_OK;
+}
+
+int swe_get_tuning(const swe_solver* s, int* flux_items, int* update_items) {
+  if (!s) return fail(SWE_ERR_ARG, "null solver");
+  if (flux_items) *flux_items = s->flux_items;
+  if (update_items) *update_items = s->upd_items;
+  return SWE_OK;
+}
+
+// Times every work-items-per-thread instantiation of the flux and update
+// kernels on the current state (each launch is idempotent: the flux pass
+// rewrites F and a CFL slot no step reads, the update writes a free state
+// buffer), keeps the fastest, and reports the per-launch times in ms as
+// {flux x1, x2, x4, update x1, x2}.  The state is left unchanged.
+int swe_autotune(swe_solver* s, int reps, int* flux_items, int* update_items, double* times) {
+  if (!s) return fail(SWE_ERR_ARG, "null solver");
+  if (reps < 1) return fail(SWE_ERR_ARG, "reps must be >= 1");
+  if (int rc = set_device(s)) return rc;
+  DevState S = state_of(s, s->cur);
+  const int scratch = (s->cur + 1) % 3;
+  unsigned long long* rslot = (s->n_inst > 0 ? s->B.rbits : s->rbits) + 3;
+  SWE_CHECK(launch_recon1(s->M, S, s->W, s->opts, s->stream), "recon1");
+  SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
+  hipEvent_t e0, e1;
+  SWE_CHECK(hipEventCreate(&e0), "hipEventCreate");
+  SWE_CHECK(hipEventCreate(&e1), "hipEventCreate");
+  const int fi[3] = {1, 2, 4}, ui[2] = {1, 2};
+  double t[5];
+  int rc = SWE_OK;
+  auto timed = [&](auto&& launch, double& out) {
+    hipError_t e = launch();  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, s->stream);
+    for (int q = 0; q < reps && e == hipSuccess; ++q) e = launch();
+    if (e == hipSuccess) e = hipEventRecord(e1, s->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float f = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
+    if (e != hipSuccess) return dev_fail(e, "autotune");
+    out = double(f) / reps;
+    return SWE_OK;
+  };
+  for (int q = 0; q < 3 && !rc; ++q)
+    rc = timed([&] { return launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, rslot, fi[q], s->stream); },
+               t[q]);
+  if (!rc) {
+    hipError_t e = launch_drain(s->M, S, s->W, s->stream);
+    if (e != hipSuccess) rc = dev_fail(e, "drain");
+  }
+  // Euler-form stage with a small fixed step into the free buffer
+  StageArgs A{0., 0., 1., 1e-6, 0., s->cor, rslot, nullptr};
+  for (int q = 0; q < 2 && !rc; ++q)
+    rc = timed([&] {
+      return launch_update(s->M, S, S, out_of(s, scratch), s->W, A, ui[q], s->stream);
+    }, t[3 + q]);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  int bf = 0, bu = 0;
+  for (int q = 1; q < 3; ++q) if (t[q] < t[bf]) bf = q;
+  if (t[4] < t[3]) bu = 1;
+  if (int r2 = swe_set_tuning(s, fi[bf], ui[bu])) return r2;
+  if (s->n_inst == 0) s->last_r_slot = 3;
+  if (flux_items) *flux_items = fi[bf];
+  if (update_items) *update_items = ui[bu];
+  if (times) for (int q = 0; q < 5; ++q) times[q] = t[q];
+  return SWE_OK;
 }
 
 }  // extern "C"

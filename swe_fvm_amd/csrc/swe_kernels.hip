@@ -102,9 +102,14 @@ __global__ void __launch_bounds__(256) k_setup_cells(DevMesh M, DevGeomOut G) {
   G.cx[i] = c.x;
   G.cy[i] = c.y;
   G.bc[i] = c.z;
-  G.zmin[i] = dmin(dmin(p0.z, p1.z), p2.z);
-  G.zmax[i] = dmax(dmax(p0.z, p1.z), p2.z);
   G.area[i] = tri_area(p0, p1, p2);
+  CellRec r;
+  r.cx = c.x; r.cy = c.y; r.bc = c.z;
+  r.zmin = dmin(dmin(p0.z, p1.z), p2.z);
+  r.zmax = dmax(dmax(p0.z, p1.z), p2.z);
+  r.fl = M.cflag[i];
+  r.pad = 0;
+  G.crec[i] = r;
 }
 
 // min over the edge's two cells of 2 area / length (the flux's CFL length)
@@ -141,16 +146,18 @@ __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W
     int j = M.tt[kk * M.T + i];
     j = j < 0 ? (int)i : j;  // boundary cells never take the full-wet branch
     wn[kk] = S.w[j];
-    fln[kk] = M.cflag[j];
-    bcn[kk] = M.bc[j];
-    zmxn[kk] = M.zmax[j];
-    zmnn[kk] = M.zmin[j];
-    cxn[kk] = M.cx[j];
-    cyn[kk] = M.cy[j];
+    const CellRec rj = M.crec[j];
+    fln[kk] = rj.fl;
+    bcn[kk] = rj.bc;
+    zmxn[kk] = rj.zmax;
+    zmnn[kk] = rj.zmin;
+    cxn[kk] = rj.cx;
+    cyn[kk] = rj.cy;
   }
-  const double bc = M.bc[i];
-  const double zmx = M.zmax[i], zmn = M.zmin[i];
-  const double cx = M.cx[i], cy = M.cy[i];
+  const CellRec ri = M.crec[i];
+  const double bc = ri.bc;
+  const double zmx = ri.zmax, zmn = ri.zmin;
+  const double cx = ri.cx, cy = ri.cy;
   double w0 = w, gx = 0., gy = 0.;
   int8_t k;
   if (!is_wet(w - bc)) {
@@ -331,12 +338,9 @@ __device__ __forceinline__ void block_min_bits(double v, unsigned long long* tar
 }
 
 // SpaceDisc::ComputeFluxes (src/SpaceDisc.cpp:54-74).  Each thread handles
-// kFluxItems edges kBlock apart; all their loads are issued before any flux
-// is computed, so a wave keeps twice as many gathers in flight.
-#ifndef SWE_FLUX_ITEMS
-#define SWE_FLUX_ITEMS 2
-#endif
-constexpr int kFluxItems = SWE_FLUX_ITEMS;
+// NI edges kBlock apart; all their loads are issued before any flux is
+// computed, so a wave keeps NI times as many gathers in flight.  NI is a
+// launch-time choice (swe_autotune); the results do not depend on it.
 struct FluxIn {
   V3 Pa, Pb;  // P(ep0), P(ep1)
   double edmin, hf, cxf, cyf, cxt, cyt;
@@ -345,13 +349,13 @@ struct FluxIn {
   int t, inst;
   bool on;
 };
-template <int FLUX, int WS>
+template <int FLUX, int WS, int NI>
 __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, double cor,
                                               unsigned long long* rbits) {
-  const int64_t base = M.e0 + (int64_t)logical_block() * (kBlock * kFluxItems) + threadIdx.x;
-  FluxIn in[kFluxItems];
+  const int64_t base = M.e0 + (int64_t)logical_block() * (kBlock * NI) + threadIdx.x;
+  FluxIn in[NI];
 #pragma unroll
-  for (int it = 0; it < kFluxItems; ++it) {
+  for (int it = 0; it < NI; ++it) {
     const int64_t e = base + it * kBlock;
     const int64_t es = e < M.e1 ? e : M.e0;  // out-of-range items read a valid edge and are dropped
     FluxIn& q = in[it];
@@ -373,9 +377,9 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, 
     q.on = e < M.e1 && (!M.cinst || M.bactive[q.inst]);
   }
   double lw = 1.0;
-  double lwi_item[kFluxItems];
+  double lwi_item[NI];
 #pragma unroll
-  for (int it = 0; it < kFluxItems; ++it) {
+  for (int it = 0; it < NI; ++it) {
     const FluxIn& q = in[it];
     lwi_item[it] = 1.0;
     if (!q.on) continue;
@@ -414,14 +418,14 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, 
   }
   __syncthreads();
 #pragma unroll
-  for (int it = 0; it < kFluxItems; ++it)
+  for (int it = 0; it < NI; ++it)
     if (in[it].e < M.e1 && in[it].inst != s_inst) s_mixed = 1;
   __syncthreads();
   if (!s_mixed) {
     block_min_bits(lw, rbits + 4 * (int64_t)s_inst);
   } else {
 #pragma unroll
-    for (int it = 0; it < kFluxItems; ++it)
+    for (int it = 0; it < NI; ++it)
       if (lwi_item[it] < 1.0)
         atomicMin(rbits + 4 * (int64_t)in[it].inst, (unsigned long long)__double_as_longlong(lwi_item[it]));
   }
@@ -534,22 +538,20 @@ __device__ __forceinline__ double stage_dt(const StageArgs& A) {
   return dt;
 }
 
-// kUpdItems cells per thread, kBlock apart, loads of all of them first.
-#ifndef SWE_UPDATE_ITEMS
-#define SWE_UPDATE_ITEMS 1
-#endif
-constexpr int kUpdItems = SWE_UPDATE_ITEMS;
+// NI cells per thread, kBlock apart, loads of all of them first (NI is a
+// launch-time choice like the flux's).
+template <int NI>
 __global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, DevState U0, DevOut O,
                                                 DevWork W, StageArgs A) {
-  const int64_t base = M.c0 + (int64_t)logical_block() * (kBlock * kUpdItems) + threadIdx.x;
+  const int64_t base = M.c0 + (int64_t)logical_block() * (kBlock * NI) + threadIdx.x;
   const double dt = M.cinst ? 0. : stage_dt(A);
   if (A.dts_log && blockIdx.x == 0 && threadIdx.x == 0) A.dts_log[0] = dt;
-  RhsIn in[kUpdItems];
-  double w[kUpdItems], u[kUpdItems], v[kUpdItems], bc[kUpdItems];
-  double w0[kUpdItems], u0[kUpdItems], v0[kUpdItems];
+  RhsIn in[NI];
+  double w[NI], u[NI], v[NI], bc[NI];
+  double w0[NI], u0[NI], v0[NI];
   const bool comb = A.a0 != 0.;
 #pragma unroll
-  for (int it = 0; it < kUpdItems; ++it) {
+  for (int it = 0; it < NI; ++it) {
     const int64_t i = base + it * kBlock;
     const int is = (int)(i < M.c1 ? i : M.c0);  // out-of-range items read a valid cell and are dropped
     rhs_load(M, S, W, is, in[it]);
@@ -564,7 +566,7 @@ __global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, DevState 
     }
   }
 #pragma unroll
-  for (int it = 0; it < kUpdItems; ++it) {
+  for (int it = 0; it < NI; ++it) {
     const int64_t i = base + it * kBlock;
     if (i >= M.c1) continue;
     double dts = A.coef * dt;
@@ -735,17 +737,20 @@ hipError_t launch_recon2(const DevMesh& M, const DevState& S, const DevWork& W, 
   return hipGetLastError();
 }
 hipError_t launch_flux(const DevMesh& M, const DevState& S, const DevWork& W, int flux, int ws,
-                       double cor, unsigned long long* rbits, hipStream_t st) {
+                       double cor, unsigned long long* rbits, int items, hipStream_t st) {
   if (!M.cinst) hipLaunchKernelGGL(k_reset_r, dim3(1), dim3(1), 0, st, rbits);
   if (M.e1 <= M.e0) return hipGetLastError();
-  dim3 g = grid_for((M.e1 - M.e0 + kFluxItems - 1) / kFluxItems), b(256);
-#define SWE_FLUX_CASE(F, Wv)                                                           \
-  if (flux == F && ws == Wv) {                                                         \
-    hipLaunchKernelGGL((k_flux<F, Wv>), g, b, 0, st, M, S, W, cor, rbits);             \
+  dim3 g = grid_for((M.e1 - M.e0 + items - 1) / items), b(256);
+#define SWE_FLUX_CASE(F, Wv, N)                                                        \
+  if (flux == F && ws == Wv && items == N) {                                           \
+    hipLaunchKernelGGL((k_flux<F, Wv, N>), g, b, 0, st, M, S, W, cor, rbits);          \
     return hipGetLastError();                                                          \
   }
-  SWE_FLUX_CASE(0, 0) SWE_FLUX_CASE(0, 1) SWE_FLUX_CASE(0, 2)
-  SWE_FLUX_CASE(1, 0) SWE_FLUX_CASE(1, 1) SWE_FLUX_CASE(1, 2)
+#define SWE_FLUX_CASES(N)                                                              \
+  SWE_FLUX_CASE(0, 0, N) SWE_FLUX_CASE(0, 1, N) SWE_FLUX_CASE(0, 2, N)                 \
+  SWE_FLUX_CASE(1, 0, N) SWE_FLUX_CASE(1, 1, N) SWE_FLUX_CASE(1, 2, N)
+  SWE_FLUX_CASES(1) SWE_FLUX_CASES(2) SWE_FLUX_CASES(4)
+#undef SWE_FLUX_CASES
 #undef SWE_FLUX_CASE
   return hipErrorInvalidValue;
 }
@@ -754,9 +759,11 @@ hipError_t launch_drain(const DevMesh& M, const DevState& S, const DevWork& W, h
   return hipGetLastError();
 }
 hipError_t launch_update(const DevMesh& M, const DevState& S, const DevState& U0, const DevOut& O,
-                         const DevWork& W, const StageArgs& A, hipStream_t st) {
-  hipLaunchKernelGGL(k_update, grid_for((M.c1 - M.c0 + kUpdItems - 1) / kUpdItems), dim3(256), 0, st, M, S,
-                     U0, O, W, A);
+                         const DevWork& W, const StageArgs& A, int items, hipStream_t st) {
+  dim3 g = grid_for((M.c1 - M.c0 + items - 1) / items), b(256);
+  if (items == 1) hipLaunchKernelGGL((k_update<1>), g, b, 0, st, M, S, U0, O, W, A);
+  else if (items == 2) hipLaunchKernelGGL((k_update<2>), g, b, 0, st, M, S, U0, O, W, A);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 hipError_t launch_batch_reset_r(const DevBatch& B, int slot, hipStream_t st) {

@@ -11,7 +11,9 @@
 //   node -> cell map  n2c_off[V+1], n2c[3T]
 //   static geometry (computed once by the setup kernels with the step's own
 //   formulas, so every use is bit-identical to recomputing it):
-//     cells  cx, cy, bc (centroid x, y, z), zmin, zmax, area
+//     cells  cx, cy, bc (centroid x, y, z), area, and an AoS copy
+//            crec {cx, cy, bc, zmin, zmax, cflag} (48 B) for the
+//            reconstruction's neighbour gathers
 //     edges  edmin (min over the two cells of 2 area / length)
 //   Edge lengths, midpoints and tangents are recomputed from px, py, pz.
 //   state  w, u, v                        double[T] per buffer
@@ -25,6 +27,13 @@
 #include <stdint.h>
 
 namespace swe {
+
+// static record of a cell for the reconstruction's neighbour gathers: one
+// 48-byte load instead of six scattered ones
+struct __attribute__((aligned(16))) CellRec {
+  double cx, cy, bc, zmin, zmax;
+  uint32_t fl, pad;
+};
 
 struct DevMesh {
   const double* px;
@@ -45,10 +54,9 @@ struct DevMesh {
   const double* cx;
   const double* cy;
   const double* bc;
-  const double* zmin;
-  const double* zmax;
   const double* area;
   const double* edmin;
+  const CellRec* crec;
   // batched ensembles (null for a single mesh): instance of every cell and
   // the per-instance step size and activity flag
   const int* cinst;
@@ -72,7 +80,8 @@ struct DevBatch {
 
 // writable view of the static geometry arrays, for the setup kernel
 struct DevGeomOut {
-  double *cx, *cy, *bc, *zmin, *zmax, *area, *edmin;
+  double *cx, *cy, *bc, *area, *edmin;
+  CellRec* crec;
 };
 
 struct DevState {
@@ -114,11 +123,13 @@ hipError_t launch_recon1(const DevMesh& M, const DevState& S, const DevWork& W, 
                          hipStream_t st);
 hipError_t launch_recon2(const DevMesh& M, const DevState& S, const DevWork& W, int opts,
                          hipStream_t st);
+// items: edges per thread of the flux (1, 2 or 4), cells per thread of the
+// update (1 or 2); results are identical for every choice
 hipError_t launch_flux(const DevMesh& M, const DevState& S, const DevWork& W, int flux, int ws,
-                       double cor, unsigned long long* rbits, hipStream_t st);
+                       double cor, unsigned long long* rbits, int items, hipStream_t st);
 hipError_t launch_drain(const DevMesh& M, const DevState& S, const DevWork& W, hipStream_t st);
 hipError_t launch_update(const DevMesh& M, const DevState& S, const DevState& U0, const DevOut& O,
-                         const DevWork& W, const StageArgs& A, hipStream_t st);
+                         const DevWork& W, const StageArgs& A, int items, hipStream_t st);
 hipError_t launch_batch_reset_r(const DevBatch& B, int slot, hipStream_t st);
 hipError_t launch_batch_dt(const DevBatch& B, double cfl, double t_end, hipStream_t st);
 hipError_t launch_batch_advance(const DevBatch& B, double t_end, hipStream_t st);

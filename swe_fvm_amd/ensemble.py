@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .core import SCHEMES, Fluxer, SolverError, _check, _ptr
+from .core import SCHEMES, Fluxer, SolverError, _autotune, _check, _ptr, _tuning
 
 
 class Ensemble:
@@ -72,6 +72,17 @@ class Ensemble:
         st = np.empty(self.n, np.int64)
         _check(_lib.load().swe_batch_time(self._h, _ptr(t), _ptr(st)))
         return t, st
+
+    def set_tuning(self, flux_items=2, update_items=1):
+        _check(_lib.load().swe_set_tuning(self._h, int(flux_items), int(update_items)))
+
+    def tuning(self):
+        return _tuning(self._h)
+
+    def autotune(self, reps=10):
+        """Picks the fastest work items per thread on the current states
+        (as SpaceDisc.autotune); a captured step graph is re-captured."""
+        return _autotune(self._h, reps)
 
     def synchronize(self):
         _check(_lib.load().swe_synchronize(self._h))

@@ -1,0 +1,82 @@
+"""Work items per thread (swe_set_tuning / swe_autotune): every choice gives
+bit-identical results, the autotuner leaves the state alone, and ensemble
+handles are refused by the single-mesh entry points."""
+import itertools
+
+import numpy as np
+import pytest
+
+import swe_fvm_amd as swe
+from swe_fvm_amd import workloads as W
+from swe_fvm_amd.ensemble import Ensemble
+from tests import _common as C
+
+pytestmark = pytest.mark.gpu
+
+FL = swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt]
+
+
+def test_every_tuning_gives_identical_steps():
+    xyz, topo, prim = C.wet_dry_mesh(40, seed=9)
+    dom = swe.Domain(xyz, topo)
+    ref = None
+    for fi, ui in itertools.product((1, 2, 4), (1, 2)):
+        sd = swe.SpaceDisc(FL, dom, prim)
+        sd.set_tuning(fi, ui)
+        assert sd.tuning() == (fi, ui)
+        sd.run("SSPRK2", 6, dt=0.0, cfl=0.15)
+        got = (sd.GetVolField(), sd.dts(6), sd.GetFluxes())
+        print(f"tuning flux x{fi} update x{ui}", flush=True)
+        if ref is None:
+            ref = got
+        else:
+            for a, b in zip(got, ref):
+                assert np.array_equal(a, b), (fi, ui)
+
+
+def test_bad_tuning_rejected():
+    xyz, topo, prim = C.wet_dry_mesh(8, seed=1)
+    sd = swe.SpaceDisc(FL, swe.Domain(xyz, topo), prim)
+    for fi, ui in ((3, 1), (0, 1), (2, 3), (8, 2)):
+        with pytest.raises(ValueError):
+            sd.set_tuning(fi, ui)
+    assert sd.tuning() == (2, 1)
+
+
+def test_autotune_keeps_state_and_results():
+    xyz, topo, prim = C.wet_dry_mesh(60, seed=4)
+    dom = swe.Domain(xyz, topo)
+    base = swe.SpaceDisc(FL, dom, prim)
+    base.run("SSPRK2", 3, dt=0.0, cfl=0.15)
+    sd = swe.SpaceDisc(FL, dom, prim)
+    res = sd.autotune(reps=3)
+    print("autotune:", res, flush=True)
+    assert res["flux_items"] in (1, 2, 4) and res["update_items"] in (1, 2)
+    assert sd.tuning() == (res["flux_items"], res["update_items"])
+    assert all(t > 0 for t in res["flux_ms"].values()) and all(t > 0 for t in res["update_ms"].values())
+    assert np.array_equal(sd.GetVolField(), prim)
+    sd.run("SSPRK2", 3, dt=0.0, cfl=0.15)
+    assert np.array_equal(sd.GetVolField(), base.GetVolField())
+
+
+def test_ensemble_retune_recaptures_graph():
+    mem = [W.ensemble_member(n, 300 + k) for k, n in enumerate([7, 15, 11])]
+    doms, prims = [d for d, _ in mem], [p for _, p in mem]
+    ref = Ensemble(FL, doms, prims)
+    ref.run("SSPRK2", cfl=0.15, t_end=1e9, max_steps=8, graph=True)
+    ens = Ensemble(FL, doms, prims)
+    ens.run("SSPRK2", cfl=0.15, t_end=1e9, max_steps=4, graph=True)
+    ens.set_tuning(4, 2)
+    ens.run("SSPRK2", cfl=0.15, t_end=1e9, max_steps=4, graph=True)
+    for a, b in zip(ens.states(), ref.states()):
+        assert np.array_equal(a, b)
+    assert np.array_equal(ens.times()[0], ref.times()[0])
+
+
+def test_single_mesh_calls_refuse_ensembles():
+    mem = [W.ensemble_member(n, 400 + k) for k, n in enumerate([5, 6])]
+    ens = Ensemble(FL, [d for d, _ in mem], [p for _, p in mem])
+    L = swe._lib.load()
+    assert L.swe_run(ens._h, 1, 0.0, 0.15, 1) == swe.core.SWE_ERR_ARG
+    assert L.swe_compute_fluxes(ens._h, None) == swe.core.SWE_ERR_ARG
+    assert L.swe_time_run(ens._h, 1, 0.0, 0.15, 1, None, None) == swe.core.SWE_ERR_ARG
