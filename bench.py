@@ -179,6 +179,8 @@ def main():
     ap.add_argument("--ensemble-cells", type=int, default=20_000_000)
     ap.add_argument("--t-end", type=float, default=1.0)
     ap.add_argument("--seq-steps", type=int, default=40)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default work items per thread (profiling runs)")
     args = ap.parse_args()
     if args.ensemble:
         bench_ensemble(args)
@@ -221,7 +223,8 @@ def main():
     if args.warmup:
         sd.run("SSPRK2", args.warmup, dt=0.0, cfl=0.15)
     # work items per thread: the fastest bit-identical kernel instantiation
-    tune = sd.autotune(reps=5)
+    tune = {"flux_items": sd.tuning()[0], "update_items": sd.tuning()[1]} if args.no_autotune \
+        else sd.autotune(reps=5)
     print(f"[bench] autotune: flux x{tune['flux_items']} update x{tune['update_items']}",
           file=sys.stderr, flush=True)
     sd.synchronize()

@@ -221,6 +221,7 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
   ALLOC(s->W.cls, T);
   ALLOC(s->W.f0, E); ALLOC(s->W.f1, E); ALLOC(s->W.f2, E);
   ALLOC(s->W.dtd, T);
+  ALLOC(s->W.pw_list, T); ALLOC(s->W.pw_count, 1);
   ALLOC(s->rbits, 4);
   DevGeomOut G{};
   ALLOC(G.cx, T); ALLOC(G.cy, T); ALLOC(G.bc, T);
@@ -243,6 +244,7 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
   unsigned long long rinit[4] = {one, one, one, one};
   if ((e = hipMemcpy(s->rbits, rinit, sizeof(rinit), hipMemcpyHostToDevice)) != hipSuccess)
     return bail(e, "hipMemcpy");
+  if ((e = hipMemset(s->W.pw_count, 0, sizeof(unsigned int))) != hipSuccess) return bail(e, "hipMemset");
   for (int b = 0; b < 3; ++b) {
     if ((e = hipMemset(s->sw[b], 0, sizeof(double) * size_t(std::max<int64_t>(T, 1)))) != hipSuccess) return bail(e, "hipMemset");
     if ((e = hipMemset(s->su[b], 0, sizeof(double) * size_t(std::max<int64_t>(T, 1)))) != hipSuccess) return bail(e, "hipMemset");

@@ -5,10 +5,11 @@
 //                         level / full-wet limited plane), stored as
 //                         (w0, gx, gy) per cell: the reference's edge and
 //                         source slots are functions of it and are never
-//                         materialised on the hot path.
-//   k_recon2  per cell  — pass 2 for partially wet cells (Liu et al. plane)
-//                         with the node maximum gathered over the node->cell
-//                         map (no atomics, order-independent max).
+//                         materialised on the hot path.  Partially wet cells
+//                         are appended to a work list (one atomic per wave).
+//   k_recon2  per listed cell — pass 2 for partially wet cells (Liu et al.
+//                         plane) with the node maximum gathered over the
+//                         node->cell map (no atomics, order-independent max).
 //   k_flux    per edge  — HLL/HLLC numerical flux from the two cells' MUSCL
 //                         at the edge midpoint; wave-level + block min of the
 //                         CFL length-to-wavespeed, one 64-bit atomic min per
@@ -226,6 +227,22 @@ __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W
   W.mgx[i] = gx;
   W.mgy[i] = gy;
   W.cls[i] = k;
+  // append partially wet cells to pass 2's work list, one atomic per wave
+  // (the order of the list is irrelevant: pass 2 is independent per cell)
+  const bool pw = k == 1;
+  const unsigned long long m = __ballot(pw);
+  if (m) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned int base = 0;
+    if (lane == leader) base = atomicAdd(W.pw_count, (unsigned int)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (pw) {
+      const unsigned int below = __builtin_amdgcn_mbcnt_hi(
+          (unsigned int)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)m, 0u));
+      W.pw_list[base + below] = (int)i;
+    }
+  }
 }
 
 // ---------------------------------------------------------------- pass 2 --
@@ -251,11 +268,7 @@ __device__ inline double node_max_pass1(const DevMesh& M, const DevWork& W, int 
 }
 
 template <bool HEAD_PW2>
-__global__ void __launch_bounds__(256) k_recon2(DevMesh M, DevState S, DevWork W) {
-  int64_t i = M.c0 + global_index();
-  if (i >= M.c1) return;
-  if (M.cinst && !M.bactive[M.cinst[i]]) return;  // finished ensemble member
-  if (W.cls[i] != 1) return;
+__device__ inline void recon2_cell(const DevMesh& M, const DevState& S, const DevWork& W, int i) {
   int i0 = M.tp[i], i1 = M.tp[M.T + i], i2 = M.tp[2 * M.T + i];
   V3 Q0 = node(M, i0), Q1 = node(M, i1), Q2 = node(M, i2);
   const double cx = M.cx[i], cy = M.cy[i], bc = M.bc[i];
@@ -320,6 +333,15 @@ __global__ void __launch_bounds__(256) k_recon2(DevMesh M, DevState S, DevWork W
   W.mgy[i] = gy;
 }
 
+// over the list of partially wet cells written by k_recon1
+template <bool HEAD_PW2>
+__global__ void __launch_bounds__(256) k_recon2(DevMesh M, DevState S, DevWork W) {
+  const unsigned int n = *W.pw_count;
+  for (unsigned int q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock)
+    recon2_cell<HEAD_PW2>(M, S, W, W.pw_list[q]);
+}
+__global__ void k_zero_count(unsigned int* c) { *c = 0u; }
+
 // ------------------------------------------------------------------ flux --
 __device__ __forceinline__ void block_min_bits(double v, unsigned long long* target) {
   // wave-level min by cross-lane shuffles, then one LDS slot per wave
@@ -372,7 +394,8 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, 
     q.cyf = M.cy[f];
     q.cxt = M.cx[tr];
     q.cyt = M.cy[tr];
-    q.hf = S.w[f] - M.bc[f];
+    // the cell average enters only the wall flux: walls alone load it
+    q.hf = t < 0 ? S.w[f] - M.bc[f] : 0.;
     q.inst = M.cinst ? M.cinst[f] : 0;
     q.on = e < M.e1 && (!M.cinst || M.bactive[q.inst]);
   }
@@ -726,14 +749,17 @@ hipError_t launch_setup_geometry(const DevMesh& M, const DevGeomOut& G, hipStrea
 }
 hipError_t launch_recon1(const DevMesh& M, const DevState& S, const DevWork& W, int opts,
                          hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, st, W.pw_count);
   if (opts & 1) hipLaunchKernelGGL((k_recon1<true>), grid_for(M.c1 - M.c0), dim3(256), 0, st, M, S, W);
   else hipLaunchKernelGGL((k_recon1<false>), grid_for(M.c1 - M.c0), dim3(256), 0, st, M, S, W);
   return hipGetLastError();
 }
 hipError_t launch_recon2(const DevMesh& M, const DevState& S, const DevWork& W, int opts,
                          hipStream_t st) {
-  if (opts & 2) hipLaunchKernelGGL((k_recon2<true>), grid_for(M.c1 - M.c0), dim3(256), 0, st, M, S, W);
-  else hipLaunchKernelGGL((k_recon2<false>), grid_for(M.c1 - M.c0), dim3(256), 0, st, M, S, W);
+  dim3 g = grid_for(M.c1 - M.c0);
+  if (g.x > 1024) g.x = 1024;  // grid-stride over the list
+  if (opts & 2) hipLaunchKernelGGL((k_recon2<true>), g, dim3(256), 0, st, M, S, W);
+  else hipLaunchKernelGGL((k_recon2<false>), g, dim3(256), 0, st, M, S, W);
   return hipGetLastError();
 }
 hipError_t launch_flux(const DevMesh& M, const DevState& S, const DevWork& W, int flux, int ws,

@@ -106,6 +106,8 @@ struct DevWork {
   double* f1;
   double* f2;
   double* dtd;
+  int* pw_list;            // partially wet cells of the current pass 1
+  unsigned int* pw_count;  // their number
 };
 
 struct StageArgs {
