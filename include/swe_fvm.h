@@ -188,6 +188,21 @@ int swe_batch_time(swe_solver* s, double* t, int64_t* steps);
 int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nsteps, double* ms,
                  double* kernel_ms);
 
+/* ----------------------------------------------------------- diagnostics --- */
+
+/* ComputeIntegrals (src/SpaceDisc.cpp:77-103, commented out in the
+ * reference) of the current state: out3 = (mass, kinetic energy, potential
+ * energy) = sums over cells of area * (h, 1/2 h |u|^2, 1/2 h^2 + h b), with
+ * partially wet cells integrating the energies of their reconstruction by
+ * TriangAverage<2,10>.  Recomputes the interface values first.  The sum is
+ * a fixed-order tree (identical run to run); per_cell (optional, 3 T
+ * doubles) receives the cell terms.  Single-mesh handles only. */
+int swe_integrals(swe_solver* s, double* out3, double* per_cell);
+/* Final per-cell reconstruction after swe_compute_interface_values: level
+ * w0 at the centroid and gradient (gx, gy); velocity is the cell's (zero
+ * for dry cells).  Any pointer may be null. */
+int swe_get_muscl(swe_solver* s, double* w0, double* gx, double* gy);
+
 /* ---------------------------------------------------------------- tuning --- */
 
 /* Work items per thread of the flux kernel (1, 2 or 4 edges) and of the

@@ -61,6 +61,7 @@ def lib():
     L.orc_min_len_to_wavespeed.argtypes = [_p]
     L.orc_reconstruct.argtypes = [_p, _i64, _int, _p]
     L.orc_rhs.argtypes = [_p, _i64, _d, _p]
+    L.orc_integrals.argtypes = [_p, _p, _p]
     L.orc_run.argtypes = [_p, _int, _d, _d, _i64, _p]
     L.orc_time_run.restype = _d
     L.orc_time_run.argtypes = [_p, _int, _d, _d, _i64]
@@ -165,6 +166,14 @@ class OracleSolver:
         out = np.empty(5, np.float64)
         lib().orc_reconstruct(self.h, i, kind, _ptr(out))
         return out
+
+    def integrals(self, per_cell=False):
+        """(mass, kinetic, potential) of the current state's reconstruction,
+        summed in cell order; with per_cell also the (T, 3) cell terms."""
+        out = np.zeros(3)
+        pc = np.zeros((self.T, 3)) if per_cell else None
+        lib().orc_integrals(self.h, _ptr(out), _ptr(pc) if per_cell else None)
+        return (out, pc) if per_cell else out
 
     def rhs(self, i, dt):
         out = np.empty(3, np.float64)

@@ -512,6 +512,69 @@ struct Solver {
     }
   }
 
+  // TriangAverage<k, n> (include/PointOperations.h:24-44): mean of f over
+  // the n^2 sub-triangle centroids of (p0, p1, p2), points carried in 3-D.
+  template <class F>
+  void TriangAverage2(int n, const V3& p0, const V3& p1, const V3& p2, F f, double out[2]) const {
+    const double hs = 1. / n;
+    const V3 di = {hs * (p1.x - p0.x), hs * (p1.y - p0.y), hs * (p1.z - p0.z)};
+    const V3 dj = {hs * (p2.x - p0.x), hs * (p2.y - p0.y), hs * (p2.z - p0.z)};
+    const double c = 1. / 3.;
+    const V3 dt = {c * (di.x + dj.x), c * (di.y + dj.y), c * (di.z + dj.z)};
+    double sum[2] = {0., 0.};
+    V3 pi = p0;
+    auto acc = [&](const V3& q) {
+      double r[2];
+      f(q, r);
+      sum[0] += hs * r[0];
+      sum[1] += hs * r[1];
+    };
+    for (int i = 0; i < n; ++i) {
+      V3 pt = {pi.x + dt.x, pi.y + dt.y, pi.z + dt.z};
+      for (int j = 0; j < n - i - 1; ++j) {
+        acc(pt);
+        acc({pt.x + dt.x, pt.y + dt.y, pt.z + dt.z});
+        pt = {pt.x + dj.x, pt.y + dj.y, pt.z + dj.z};
+      }
+      acc(pt);
+      pi = {pi.x + di.x, pi.y + di.y, pi.z + di.z};
+    }
+    out[0] = hs * sum[0];
+    out[1] = hs * sum[1];
+  }
+
+  // Per-cell terms of ComputeIntegrals (src/SpaceDisc.cpp:77-103, commented
+  // out in the reference): area * (h, 1/2 h |u|^2, 1/2 h^2 + h b).  Full-wet
+  // cells use the cell average; partially wet cells add area * h and the
+  // TriangAverage<2,10> of the energies of their pass-2 reconstruction (the
+  // bathymetry at a quadrature point is the point's interpolated z); dry
+  // cells add nothing.  Call after ComputeInterfaceValues.
+  void CellIntegrals(Idx i, double out[3]) const {
+    out[0] = out[1] = out[2] = 0.;
+    if (IsDryCell(i)) return;
+    const double A = Area(i);
+    if (IsFullWetCell(i)) {
+      const double hh = h(i), b = bcell(i), uu = u(i), vv = v(i);
+      out[0] = A * hh;
+      out[1] = A * (0.5 * hh * (uu * uu + vv * vv));
+      out[2] = A * (0.5 * hh * hh + hh * b);
+      return;
+    }
+    out[0] = A * h(i);
+    const MUSCL mu = ReconstructPartWetCell2(i, (opts & OPT_SEQ_NODEMAX) ? max_wp : max_wp1);
+    double e[2];
+    TriangAverage2(10, P(m.tp[3 * i]), P(m.tp[3 * i + 1]), P(m.tp[3 * i + 2]),
+                   [&](const V3& q, double r[2]) {
+                     double z[3];
+                     AtPoint(mu, q, z);
+                     const double hq = z[0] - q.z;
+                     r[0] = 0.5 * hq * (z[1] * z[1] + z[2] * z[2]);
+                     r[1] = 0.5 * hq * hq + hq * q.z;
+                   }, e);
+    out[1] = A * e[0];
+    out[2] = A * e[1];
+  }
+
   // ElemFlux: include/SpaceDisc.h:4-18
   static void ElemFlux(const double n[2], const double U[3], double out[3]) {
     out[0] = 0.; out[1] = 0.; out[2] = 0.;
@@ -804,14 +867,31 @@ void orc_classify(const Solver* s, int8_t* out) {
 }
 
 // Reconstruction of one cell (kind: 0 dry, 1 partwet1, 2 fullwet, 3 partwet2
-// using the current max_wp).  out = origin (w, u, v), grad (gx, gy).
+// using the current max_wp, 4 partwet2 using the pass-1 node maximum, as the
+// step does).  out = origin (w, u, v), grad (gx, gy).
 void orc_reconstruct(const Solver* s, int64_t i, int kind, double* out) {
   MUSCL mu;
   if (kind == 0) mu = s->ReconstructDryCell(i);
   else if (kind == 1) mu = s->ReconstructPartWetCell1(i);
   else if (kind == 2) mu = s->ReconstructFullWetCell(i);
-  else mu = s->ReconstructPartWetCell2(i, s->max_wp);
+  else if (kind == 3) mu = s->ReconstructPartWetCell2(i, s->max_wp);
+  else mu = s->ReconstructPartWetCell2(i, s->max_wp1);
   out[0] = mu.o[0]; out[1] = mu.o[1]; out[2] = mu.o[2]; out[3] = mu.g[0]; out[4] = mu.g[1];
+}
+
+// Mass, kinetic and potential energy (sums over cells in index order) and,
+// optionally, the per-cell terms (3 per cell); recomputes the interface
+// values of the current state first.
+void orc_integrals(Solver* s, double* out, double* per_cell) {
+  s->ComputeInterfaceValues();
+  double tot[3] = {0., 0., 0.};
+  for (int64_t i = 0; i < s->m.T; ++i) {
+    double c[3];
+    s->CellIntegrals(i, c);
+    for (int k = 0; k < 3; ++k) tot[k] += c[k];
+    if (per_cell) for (int k = 0; k < 3; ++k) per_cell[3 * i + k] = c[k];
+  }
+  for (int k = 0; k < 3; ++k) out[k] = tot[k];
 }
 
 // TimeDisc::RHS with draining dt read from the current state.

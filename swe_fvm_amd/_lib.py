@@ -59,6 +59,8 @@ SIGNATURES = [
     ("swe_batch_run", _int, [_p, _int, _d, _d, _i64, _int, _pi64, _pi64]),
     ("swe_batch_info", _int, [_p, _pi64, _p, _p, _p]),
     ("swe_batch_time", _int, [_p, _p, _p]),
+    ("swe_integrals", _int, [_p, _p, _p]),
+    ("swe_get_muscl", _int, [_p, _p, _p, _p]),
     ("swe_set_tuning", _int, [_p, _int, _int]),
     ("swe_get_tuning", _int, [_p, _p, _p]),
     ("swe_autotune", _int, [_p, _int, _p, _p, _p]),

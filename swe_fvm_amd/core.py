@@ -236,6 +236,13 @@ class Domain:
         P = self.geometry
         return (P[tp[:, 0]] * c + P[tp[:, 1]] * c) + P[tp[:, 2]] * c
 
+    def areas(self):
+        """Domain::Area for every cell (TriangArea, src/PointOperations.cpp:12-14)."""
+        tp = self.topology.tables()["tp"]
+        a, b, c = (self.geometry[tp[:, k]] for k in range(3))
+        ux, uy, vx, vy = b[:, 0] - a[:, 0], b[:, 1] - a[:, 1], c[:, 0] - a[:, 0], c[:, 1] - a[:, 1]
+        return 0.5 * np.abs(ux * vy - vx * uy)
+
 
 # --------------------------------------------------------------- SpaceDisc --
 
@@ -316,6 +323,22 @@ class SpaceDisc:
         out = np.empty(self.T, np.int8)
         _check(_lib.load().swe_get_cell_class(self._h, _ptr(out)))
         return out
+
+    def GetMuscl(self):
+        """Final per-cell reconstruction after ComputeInterfaceValues:
+        (w0, gx, gy) arrays, origin at the centroid."""
+        w0, gx, gy = (np.empty(self.T, np.float64) for _ in range(3))
+        _check(_lib.load().swe_get_muscl(self._h, _ptr(w0), _ptr(gx), _ptr(gy)))
+        return w0, gx, gy
+
+    def Integrals(self, per_cell=False):
+        """ComputeIntegrals (src/SpaceDisc.cpp:77-103, commented out in the
+        reference): (mass, kinetic, potential) of the current state, computed
+        on the device; per_cell also returns the (T, 3) cell terms."""
+        out = np.zeros(3, np.float64)
+        pc = np.zeros((self.T, 3), np.float64) if per_cell else None
+        _check(_lib.load().swe_integrals(self._h, _ptr(out), _ptr(pc) if per_cell else None))
+        return (out, pc) if per_cell else out
 
     def GetCor(self):
         return self.cor

@@ -869,4 +869,42 @@ int swe_autotune(swe_solver* s, int reps, int* flux_items, int* update_items, do
   return SWE_OK;
 }
 
+// ComputeIntegrals of the current state: reconstruction, per-cell terms
+// and a fixed-order reduction (identical run to run).
+int swe_integrals(swe_solver* s, double* out3, double* per_cell) {
+  if (int rc = single_mesh(s)) return rc;
+  if (!out3) return fail(SWE_ERR_ARG, "null argument");
+  if (int rc = set_device(s)) return rc;
+  DevState S = state_of(s, s->cur);
+  SWE_CHECK(launch_recon1(s->M, S, s->W, s->opts, s->stream), "recon1");
+  SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
+  const int64_t nb = integrals_blocks(s->M);
+  double *d = nullptr, *pc = nullptr;
+  SWE_CHECK(hipMalloc((void**)&d, sizeof(double) * size_t(3 * nb + 3)), "hipMalloc");
+  hipError_t e = hipSuccess;
+  if (per_cell) e = hipMalloc((void**)&pc, sizeof(double) * size_t(3 * std::max<int64_t>(s->T, 1)));
+  if (e == hipSuccess) e = launch_integrals(s->M, S, s->W, d + 3, pc, d, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out3, d, 3 * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess && per_cell)
+    e = hipMemcpyAsync(per_cell, pc, sizeof(double) * size_t(3 * s->T), hipMemcpyDeviceToHost, s->stream);
+  hipError_t e2 = hipStreamSynchronize(s->stream);
+  hipFree(d);
+  if (pc) hipFree(pc);
+  if (e == hipSuccess) e = e2;
+  if (e != hipSuccess) return dev_fail(e, "integrals");
+  return SWE_OK;
+}
+
+// Final reconstruction of every cell after swe_compute_interface_values:
+// origin level w0 at the centroid and gradient (gx, gy) (any may be null).
+int swe_get_muscl(swe_solver* s, double* w0, double* gx, double* gy) {
+  if (!s) return fail(SWE_ERR_ARG, "null solver");
+  if (int rc = set_device(s)) return rc;
+  const size_t b = sizeof(double) * size_t(s->T);
+  if (w0) if (int rc = download(s, w0, s->W.mw, b)) return rc;
+  if (gx) if (int rc = download(s, gx, s->W.mgx, b)) return rc;
+  if (gy) if (int rc = download(s, gy, s->W.mgy, b)) return rc;
+  return SWE_OK;
+}
+
 }  // extern "C"

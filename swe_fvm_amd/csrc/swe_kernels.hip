@@ -728,6 +728,99 @@ __global__ void __launch_bounds__(256) k_max_wp(DevMesh M, DevWork W, double* ou
   out[n] = mx;
 }
 
+// ---------------------------------------------------------- integrals --
+// Per-cell terms of ComputeIntegrals (src/SpaceDisc.cpp:77-103, commented out
+// in the reference): area * (h, 1/2 h |u|^2, 1/2 h^2 + h b); partially wet
+// cells integrate the energies of their final reconstruction with
+// TriangAverage<2,10> (include/PointOperations.h:24-44) at points carried in
+// 3-D (the bathymetry at a point is its interpolated z).
+__device__ inline void cell_integrals(const DevMesh& M, const DevState& S, const DevWork& W, int i,
+                                      double out[3]) {
+  out[0] = 0.; out[1] = 0.; out[2] = 0.;
+  const int k = W.cls[i];
+  if (k == 0) return;
+  const double A = M.area[i];
+  const double hh = S.w[i] - M.bc[i];
+  if (k == 2) {
+    const double uu = S.u[i], vv = S.v[i];
+    out[0] = A * hh;
+    out[1] = A * (0.5 * hh * (uu * uu + vv * vv));
+    out[2] = A * (0.5 * hh * hh + hh * M.bc[i]);
+    return;
+  }
+  out[0] = A * hh;
+  const double w0 = W.mw[i], gx = W.mgx[i], gy = W.mgy[i], u0 = S.u[i], v0 = S.v[i];
+  const double cx = M.cx[i], cy = M.cy[i];
+  const V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[M.T + i]), p2 = node(M, M.tp[2 * M.T + i]);
+  const int n = 10;
+  const double hs = 1. / n;
+  const V3 di = {hs * (p1.x - p0.x), hs * (p1.y - p0.y), hs * (p1.z - p0.z)};
+  const V3 dj = {hs * (p2.x - p0.x), hs * (p2.y - p0.y), hs * (p2.z - p0.z)};
+  const double c = 1. / 3.;
+  const V3 dt = {c * (di.x + dj.x), c * (di.y + dj.y), c * (di.z + dj.z)};
+  double s0 = 0., s1 = 0.;
+  auto acc = [&](const V3& q) {
+    const double dx = q.x - cx, dy = q.y - cy;
+    double a = w0 + (gx * dx + gy * dy), uq = u0, vq = v0;
+    if (!((a - q.z) >= 0)) { a = q.z; uq = 0.; vq = 0.; }
+    const double hq = a - q.z;
+    s0 += hs * (0.5 * hq * (uq * uq + vq * vq));
+    s1 += hs * (0.5 * hq * hq + hq * q.z);
+  };
+  V3 pi = p0;
+  for (int a = 0; a < n; ++a) {
+    V3 pt = {pi.x + dt.x, pi.y + dt.y, pi.z + dt.z};
+    for (int b = 0; b < n - a - 1; ++b) {
+      acc(pt);
+      acc({pt.x + dt.x, pt.y + dt.y, pt.z + dt.z});
+      pt = {pt.x + dj.x, pt.y + dj.y, pt.z + dj.z};
+    }
+    acc(pt);
+    pi = {pi.x + di.x, pi.y + di.y, pi.z + di.z};
+  }
+  out[1] = A * (hs * s0);
+  out[2] = A * (hs * s1);
+}
+
+// fixed-shape tree sum over a 256-thread block (deterministic)
+__device__ inline void block_sum3(double v[3], double* red /*3*kBlock*/) {
+  const int t = threadIdx.x;
+  for (int q = 0; q < 3; ++q) red[q * kBlock + t] = v[q];
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (t < off)
+      for (int q = 0; q < 3; ++q) red[q * kBlock + t] += red[q * kBlock + t + off];
+    __syncthreads();
+  }
+  for (int q = 0; q < 3; ++q) v[q] = red[q * kBlock];
+}
+
+__global__ void __launch_bounds__(256) k_integrals(DevMesh M, DevState S, DevWork W, double* partial,
+                                                   double* per_cell) {
+  __shared__ double red[3 * kBlock];
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double v[3] = {0., 0., 0.};
+  if (i < M.T) {
+    cell_integrals(M, S, W, (int)i, v);
+    if (per_cell)
+      for (int q = 0; q < 3; ++q) per_cell[3 * i + q] = v[q];
+  }
+  block_sum3(v, red);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 3; ++q) partial[3 * (int64_t)blockIdx.x + q] = v[q];
+}
+
+// one block: strided sequential sums of the block partials, then a tree
+__global__ void __launch_bounds__(256) k_sum_partials(const double* partial, int64_t nb, double* out) {
+  __shared__ double red[3 * kBlock];
+  double v[3] = {0., 0., 0.};
+  for (int64_t b = threadIdx.x; b < nb; b += kBlock)
+    for (int q = 0; q < 3; ++q) v[q] += partial[3 * b + q];
+  block_sum3(v, red);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 3; ++q) out[q] = v[q];
+}
+
 __global__ void k_rhs_one(DevMesh M, DevState S, DevWork W, int i, double dt, double cor,
                           double* out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -811,6 +904,15 @@ hipError_t launch_edge_field(const DevMesh& M, const DevState& S, const DevWork&
 }
 hipError_t launch_max_wp(const DevMesh& M, const DevWork& W, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_max_wp, grid_for(M.V), dim3(256), 0, st, M, W, out);
+  return hipGetLastError();
+}
+int64_t integrals_blocks(const DevMesh& M) { return (M.T + kBlock - 1) / kBlock; }
+hipError_t launch_integrals(const DevMesh& M, const DevState& S, const DevWork& W, double* partial,
+                            double* per_cell, double* out, hipStream_t st) {
+  const int64_t nb = integrals_blocks(M);
+  if (nb > 0)
+    hipLaunchKernelGGL(k_integrals, dim3((unsigned)nb), dim3(256), 0, st, M, S, W, partial, per_cell);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, partial, nb, out);
   return hipGetLastError();
 }
 hipError_t launch_rhs_one(const DevMesh& M, const DevState& S, const DevWork& W, int i, double dt,

@@ -138,6 +138,12 @@ hipError_t launch_batch_advance(const DevBatch& B, double t_end, hipStream_t st)
 hipError_t launch_edge_field(const DevMesh& M, const DevState& S, const DevWork& W, double cor,
                              double* edg, double* src, hipStream_t st);
 hipError_t launch_max_wp(const DevMesh& M, const DevWork& W, double* out, hipStream_t st);
+// mass, kinetic and potential energy of the current reconstruction: block
+// partials (integrals_blocks(M) x 3 doubles) reduced in a fixed order into
+// out[3]; per_cell (optional, 3 T doubles) receives the cell terms
+int64_t integrals_blocks(const DevMesh& M);
+hipError_t launch_integrals(const DevMesh& M, const DevState& S, const DevWork& W, double* partial,
+                            double* per_cell, double* out, hipStream_t st);
 hipError_t launch_rhs_one(const DevMesh& M, const DevState& S, const DevWork& W, int i, double dt,
                           double cor, double* out, hipStream_t st);
 

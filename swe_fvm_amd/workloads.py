@@ -13,6 +13,7 @@ import math
 
 import numpy as np
 
+from .analytic import ClassicThacker
 from .core import Domain, StructTriangMesh, jittered_mesh
 
 
@@ -39,52 +40,6 @@ def triang_average(p0, p1, p2, f, n):
         total = a if total is None else total + a
         pi = pi + di
     return h * total
-
-
-class ClassicThacker:
-    """ClassicThackerTest (examples/Tests.h:237-280) on BowlTest bathymetry
-    b = delta ((x - mx)^2 + (y - my)^2 - 1) (examples/Tests.h:54-56)."""
-
-    def __init__(self, H0=1.25, p0=0.0, q0=0.0, cor=0.0, delta=1.0, mid=(0.0, 0.0)):
-        self.H0, self.p0, self.q0, self.cor, self.delta = H0, p0, q0, cor, delta
-        self.mx, self.my = mid
-        self.w = math.sqrt(cor * cor + 8.0 * delta)
-        qz = (q0 - 0.5 * cor) ** 2
-        rz = qz + 2.0 * H0 * H0 + p0 * p0 - 0.25 * self.w * self.w
-        self.a = math.sqrt(rz * rz + self.w * self.w * p0 * p0) / (rz + 0.5 * self.w * self.w)
-        self.b = math.atan(self.w * p0 / rz)
-        self.period = 2.0 * math.pi / self.w
-
-    def bath(self, x, y):
-        return self.delta * ((x - self.mx) ** 2 + (y - self.my) ** 2 - 1.0)
-
-    def _den(self, t):
-        return 1.0 - self.a * math.cos(self.w * t + self.b)
-
-    def p(self, t):
-        return 0.5 * self.w * self.a * math.sin(self.w * t + self.b) / self._den(t)
-
-    def q(self, t):
-        return (self.q0 - 0.5 * self.cor) * (1.0 - self.a * math.cos(self.b)) / self._den(t) + 0.5 * self.cor
-
-    def Hc(self, t):
-        return self.H0 * (1.0 - self.a * math.cos(self.b)) / self._den(t)
-
-    def Hxx(self, t):
-        qz = (self.q0 - 0.5 * self.cor) ** 2
-        az0 = (1.0 - self.a * math.cos(self.b)) ** 2
-        azt = self._den(t) ** 2
-        return (0.25 * self.w * self.w * (self.a * self.a - 1.0) + qz * az0) / azt
-
-    def h(self, x, y, t):
-        dx, dy = x - self.mx, y - self.my
-        return np.maximum(0.0, self.Hc(t) + 0.5 * self.Hxx(t) * (dx * dx + dy * dy))
-
-    def u(self, x, y, t):
-        return self.p(t) * (x - self.mx) + self.q(t) * (y - self.my)
-
-    def v(self, x, y, t):
-        return self.q(t) * (self.mx - x) + self.p(t) * (y - self.my)
 
 
 def _cell_vertices(xyz, topo):
