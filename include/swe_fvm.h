@@ -55,8 +55,9 @@ const char* swe_last_error(void);
 
 /* Topology::create / Topology ctor (reference include/TriangMesh.h:16-40,
  * pybind/Topology.cpp:21-27).  Copies (owns) the tables: edge_nodes and
- * edge_elements are E x 2 (edge_elements[e][1] < 0 on the boundary, only
- * SOLID_WALL = -1 is accepted), element_* are T x 3. */
+ * edge_elements are E x 2 (edge_elements[e][1] < 0 on the boundary: a tag,
+ * of which swe_create accepts only SOLID_WALL = -1), element_* are T x 3.
+ * Indices out of range give SWE_ERR_DOMAIN. */
 int swe_topology_create(int64_t num_nodes, int64_t num_edges, const int64_t* edge_nodes,
                         const int64_t* edge_elements, int64_t num_elements,
                         const int64_t* element_nodes, const int64_t* element_edges,

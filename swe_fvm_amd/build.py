@@ -41,6 +41,27 @@ def build(verbose=False, out=None, defines=()):
     return target
 
 
+def build_pybind(verbose=False):
+    """Compiles the Python module SWE_FVM (pybind11, host code) next to the
+    library it links, with an $ORIGIN rpath.  Needs LIB built first."""
+    import sysconfig
+
+    import pybind11
+    out = os.path.join(HERE, "SWE_FVM" + sysconfig.get_config_var("EXT_SUFFIX"))
+    cmd = ["g++", "-std=c++17", "-O2", "-shared", "-fPIC", "-Wall", "-Wextra",
+           "-I", os.path.join(os.path.dirname(HERE), "include"), "-I", pybind11.get_include(),
+           "-I", sysconfig.get_paths()["include"], os.path.join(CSRC, "swe_pybind.cpp"),
+           "-L", HERE, "-lswe_fvm_amd", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise subprocess.CalledProcessError(res.returncode, cmd, res.stdout, res.stderr)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser()

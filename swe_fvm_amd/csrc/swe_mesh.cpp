@@ -433,6 +433,18 @@ int swe_topology_create(int64_t num_nodes, int64_t num_edges, const int64_t* edg
   if ((num_edges && (!edge_nodes || !edge_elements)) ||
       (num_elements && (!element_nodes || !element_edges || !element_neighbours)))
     return fail(SWE_ERR_ARG, "null table");
+  // indices in range; negative element ids are boundary tags (only
+  // SOLID_WALL = -1 is accepted later by swe_create)
+  for (int64_t k = 0; k < 2 * num_edges; ++k) {
+    if (edge_nodes[k] < 0 || edge_nodes[k] >= num_nodes) return fail(SWE_ERR_DOMAIN, "edge node out of range");
+    if (edge_elements[k] >= num_elements || (k % 2 == 0 && edge_elements[k] < 0))
+      return fail(SWE_ERR_DOMAIN, "edge element out of range");
+  }
+  for (int64_t k = 0; k < 3 * num_elements; ++k) {
+    if (element_nodes[k] < 0 || element_nodes[k] >= num_nodes) return fail(SWE_ERR_DOMAIN, "element node out of range");
+    if (element_edges[k] < 0 || element_edges[k] >= num_edges) return fail(SWE_ERR_DOMAIN, "element edge out of range");
+    if (element_neighbours[k] >= num_elements) return fail(SWE_ERR_DOMAIN, "element neighbour out of range");
+  }
   auto* t = new swe_topology;
   Topo& m = t->m;
   m.V = num_nodes; m.E = num_edges; m.T = num_elements;
