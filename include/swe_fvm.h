@@ -44,6 +44,23 @@ extern "C" {
 /* Option bits restoring literal HEAD behaviour where the reference is broken */
 #define SWE_OPT_HEAD_GRADIENT 1  /* full-wet plane through bathymetry, not w (src/MUSCLObject.cpp:45-64) */
 #define SWE_OPT_HEAD_PARTWET2 2  /* keep the transposed, lost writes (src/MUSCLObject.cpp:143,158-159,180) */
+/* Linear friction -tau (hu, hv) of the cell average in TimeDisc::RHS; the
+ * reference stores tau but never uses it (include/SpaceDisc.h:47,
+ * src/TimeDisc.cpp:38-39), so without this bit tau has no effect. */
+#define SWE_OPT_FRICTION 16
+/* Draining time step with the outflow through each edge, F0 * L, instead of
+ * the reference's sum of F0 alone (src/TimeDisc.cpp:56-62), which is a flux
+ * per unit length: its A h / sum is not a time, falls below the CFL step
+ * in fully wet cells as the mesh is refined and stalls convergence. */
+#define SWE_OPT_DRAIN_LENGTH 32
+
+/* Boundary tags of edge_elements[e][1] (include/Includes.h:21):
+ * SOLID_WALL (src/SpaceDisc.cpp:66-69) and FREE_FLOW, declared but
+ * unimplemented in the reference: a transmissive edge whose outside state is
+ * the inside edge state (the fluxer sees R = L).  PERIODIC and CUSTOM are
+ * rejected. */
+#define SWE_SOLID_WALL (-1)
+#define SWE_FREE_FLOW (-2)
 
 typedef struct swe_topology swe_topology;
 typedef struct swe_solver swe_solver;
@@ -56,7 +73,8 @@ const char* swe_last_error(void);
 /* Topology::create / Topology ctor (reference include/TriangMesh.h:16-40,
  * pybind/Topology.cpp:21-27).  Copies (owns) the tables: edge_nodes and
  * edge_elements are E x 2 (edge_elements[e][1] < 0 on the boundary: a tag,
- * of which swe_create accepts only SOLID_WALL = -1), element_* are T x 3.
+ * of which swe_create accepts SWE_SOLID_WALL and SWE_FREE_FLOW), element_*
+ * are T x 3.
  * Indices out of range give SWE_ERR_DOMAIN. */
 int swe_topology_create(int64_t num_nodes, int64_t num_edges, const int64_t* edge_nodes,
                         const int64_t* edge_elements, int64_t num_elements,

@@ -22,6 +22,8 @@ OPT_HEAD_GRADIENT = 1
 OPT_HEAD_PARTWET2 = 2
 OPT_SEQ_DRAINING = 4
 OPT_SEQ_NODEMAX = 8
+OPT_FRICTION = 16
+OPT_DRAIN_LENGTH = 32
 
 _lib = None
 
@@ -115,7 +117,7 @@ class OracleSolver:
                               self.T, _ptr(t["tp"]), _ptr(t["te"]), _ptr(t["tt"]),
                               FLUX[flux], WAVE[wave], cor, tau, opts)
         if not self.h:
-            raise ValueError("unsupported boundary tag (only SOLID_WALL = -1)")
+            raise ValueError("unsupported boundary tag (SOLID_WALL = -1 and FREE_FLOW = -2 only)")
 
     def __del__(self):
         if getattr(self, "h", None):

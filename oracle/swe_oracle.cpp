@@ -59,7 +59,14 @@ enum Opt : int {
   OPT_HEAD_PARTWET2 = 2,      // D4 literal
   OPT_SEQ_DRAINING = 4,       // D5 literal
   OPT_SEQ_NODEMAX = 8,        // D6 literal
+  OPT_FRICTION = 16,          // linear friction -tau (hu, hv) (tau is unused in the reference)
+  OPT_DRAIN_LENGTH = 32,      // D16 corrected: outflow F0 * L in the draining dt
 };
+// Boundaries (include/Includes.h:21): SOLID_WALL = -1 (src/SpaceDisc.cpp:66-69)
+// and FREE_FLOW = -2, declared but unimplemented in the reference: here the
+// outside state equals the inside edge state (a transmissive edge), so the
+// fluxer sees R = L and the CFL length uses the inside cell only.
+constexpr Idx SOLID_WALL = -1, FREE_FLOW = -2;
 
 enum Flux : int { FLUX_HLL = 0, FLUX_HLLC = 1 };
 enum Wave : int { WS_RUSANOV = 0, WS_DAVIS = 1, WS_EINFELDT = 2 };
@@ -612,7 +619,7 @@ struct Solver {
   void HLL(Idx e, Idx from, Idx to, double* rr, double F[3]) const {
     double n[2];
     Norm(e, from, n);
-    Idx sl = Slot(e, from, to), sr = Slot(e, to, from);
+    Idx sl = Slot(e, from, to), sr = to < 0 ? sl : Slot(e, to, from);
     double be = bedge(e);
     double ul = edg[3 * sl + 1] * n[0] + edg[3 * sl + 2] * n[1];
     double hl = edg[3 * sl] - be;
@@ -626,7 +633,7 @@ struct Solver {
     if (ar - al <= 1e-10) return;
     if (rr) {
       double le = L(e);
-      double dl = 2. * Area(from) / le, dr = 2. * Area(to) / le;
+      double dl = 2. * Area(from) / le, dr = to < 0 ? dl : 2. * Area(to) / le;
       double c = std::fabs(cor);
       double lw = dmin(dl, dr) / (c + dmax(-al, ar));
       *rr = dmin(*rr, lw);
@@ -643,7 +650,7 @@ struct Solver {
     double t[2], n[2];
     Tang(e, from, t);
     n[0] = t[1]; n[1] = -t[0];
-    Idx sl = Slot(e, from, to), sr = Slot(e, to, from);
+    Idx sl = Slot(e, from, to), sr = to < 0 ? sl : Slot(e, to, from);
     double be = bedge(e);
     double ul = edg[3 * sl + 1] * n[0] + edg[3 * sl + 2] * n[1];
     double vl = edg[3 * sl + 1] * t[0] + edg[3 * sl + 2] * t[1];
@@ -660,7 +667,7 @@ struct Solver {
     ustar /= (hr * (ar - ur) - hl * (al - ul));
     if (rr) {
       double le = L(e);
-      double dl = 2. * Area(from) / le, dr = 2. * Area(to) / le;
+      double dl = 2. * Area(from) / le, dr = to < 0 ? dl : 2. * Area(to) / le;
       double c = std::fabs(cor);
       double lw = dmin(dl, dr) / (c + dmax(tol, dmax(al, ar)));
       *rr = dmin(*rr, lw);
@@ -692,7 +699,7 @@ struct Solver {
     for (Idx e = 0; e < m.E; ++e) {
       Idx lf = m.et[2 * e], lt = m.et[2 * e + 1];
       double* F = &f[3 * e];
-      if (lt == -1) {
+      if (lt == SOLID_WALL) {
         double n[2];
         Norm(e, lf, n);
         double U[3] = {h(lf), 0., 0.};
@@ -714,7 +721,11 @@ struct Solver {
     for (int k = 0; k < 3; ++k) {
       Idx e = m.te[3 * i + k];
       double fe = f[3 * e];
-      sum += dmax(0., (i == m.et[2 * e]) ? fe : -fe);
+      double out = (i == m.et[2 * e]) ? fe : -fe;
+      // D16: the reference adds the flux per unit length; the outflow
+      // through the edge is F0 * L (OPT_DRAIN_LENGTH)
+      if (opts & OPT_DRAIN_LENGTH) out = out * L(e);
+      sum += dmax(0., out);
     }
     return sum > tol ? Area(i) * hi / sum : kInf;
   }
@@ -745,6 +756,13 @@ struct Solver {
       double q = 0.5 * h_ek * h_ek;
       res[1] += dtk * (n[0] * c_ek * q);
       res[2] += dtk * (n[1] * c_ek * q);
+    }
+    if ((opts & OPT_FRICTION) && tau != 0.) {
+      const double hh = h(i);
+      const bool dry = IsDryCell(i);
+      const double uu = dry ? 0. : u(i), vv = dry ? 0. : v(i);
+      res[1] -= dt * (tau * (hh * uu));
+      res[2] -= dt * (tau * (hh * vv));
     }
   }
 
@@ -831,7 +849,7 @@ Solver* orc_create(int64_t V, const double* xyz, int64_t E, const int64_t* ep,
                    const int64_t* et, int64_t T, const int64_t* tp, const int64_t* te,
                    const int64_t* tt, int flux, int ws, double cor, double tau, int opts) {
   for (int64_t e = 0; e < E; ++e)
-    if (et[2 * e + 1] < -1) return nullptr;  // D11
+    if (et[2 * e + 1] < FREE_FLOW) return nullptr;  // D11: walls and free flow only
   auto* s = new Solver;
   Mesh& m = s->m;
   m.V = V; m.E = E; m.T = T;

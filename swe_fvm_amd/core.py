@@ -23,6 +23,10 @@ SWE_ERR_DOMAIN = 4
 SCHEMES = {"Euler": 0, "SSPRK2": 1, "SSPRK3": 2}
 OPT_HEAD_GRADIENT = 1
 OPT_HEAD_PARTWET2 = 2
+OPT_FRICTION = 16  # linear friction -tau (hu, hv); the reference leaves tau unused
+OPT_DRAIN_LENGTH = 32  # draining dt from the outflow F0 * L (the reference omits L)
+SOLID_WALL = -1  # boundary tags, include/Includes.h:21
+FREE_FLOW = -2  # transmissive edge (declared but unimplemented in the reference)
 
 
 class DomainError(RuntimeError):
@@ -173,6 +177,16 @@ class Topology:
 
     def IsEdgeBoundary(self, e):
         return bool(self.tables()["et"][e, 1] < 0)
+
+    def with_boundary_tag(self, edges, tag):
+        """A copy whose boundary edges `edges` carry `tag` (SOLID_WALL or
+        FREE_FLOW) in EdgeTriangs[e][1] (include/Includes.h:21)."""
+        t = self.tables()
+        edges = np.asarray(edges, dtype=np.int64)
+        if np.any(t["et"][edges, 1] >= 0):
+            raise DomainError("only boundary edges carry a tag")
+        t["et"][edges, 1] = int(tag)
+        return Topology.create(self.NumNodes(), t["ep"], t["et"], t["tp"], t["te"], t["tt"])
 
 
 def _take_mesh(rc, xyz_p, nn, h):

@@ -53,6 +53,9 @@ int single_mesh(const swe_solver* s) {
   return SWE_OK;
 }
 
+// the linear friction coefficient of the update (0 unless SWE_OPT_FRICTION)
+double friction(const swe_solver* s) { return (s->opts & SWE_OPT_FRICTION) ? s->tau : 0.; }
+
 int dev_fail(hipError_t e, const char* what) {
   return fail(SWE_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
@@ -86,8 +89,8 @@ int run_stage(swe_solver* s, int in, int u0, int out, double a0, double a1, doub
   SWE_CHECK(launch_recon1(s->M, S, s->W, s->opts, s->stream), "recon1");
   SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
   SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + slot, s->flux_items, s->stream), "flux");
-  SWE_CHECK(launch_drain(s->M, S, s->W, s->stream), "drain");
-  StageArgs A{a0, a1, coef, dt_fixed, cfl, s->cor, s->rbits, dts_log};
+  SWE_CHECK(launch_drain(s->M, S, s->W, s->opts, s->stream), "drain");
+  StageArgs A{a0, a1, coef, dt_fixed, cfl, s->cor, s->rbits, dts_log, friction(s)};
   SWE_CHECK(launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->upd_items, s->stream), "update");
   s->last_r_slot = slot;
   return SWE_OK;
@@ -125,6 +128,8 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
   if (!xyz || !out) return fail(SWE_ERR_ARG, "null argument");
   if (flux < 0 || flux > 1 || wavespeed < 0 || wavespeed > 2)
     return fail(SWE_ERR_ARG, "unknown flux or wave-speed estimate");
+  if (options & ~(SWE_OPT_HEAD_GRADIENT | SWE_OPT_HEAD_PARTWET2 | SWE_OPT_FRICTION | SWE_OPT_DRAIN_LENGTH))
+    return fail(SWE_ERR_ARG, "unknown option bits");
   if (m.T > 0x7fffffff || m.E > 0x7fffffff || m.V > 0x7fffffff)
     return fail(SWE_ERR_ARG, "mesh too large for int32 device indices");
   const int64_t T = m.T, E = m.E, V = m.V;
@@ -136,8 +141,8 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
     int64_t a = m.ep[2 * e], b = m.ep[2 * e + 1], f = m.et[2 * e], t = m.et[2 * e + 1];
     if (a < 0 || a >= V || b < 0 || b >= V) return fail(SWE_ERR_ARG, "edge node out of range");
     if (f < 0 || f >= T || t >= T) return fail(SWE_ERR_ARG, "edge element out of range");
-    if (t < -1)
-      return fail(SWE_ERR_DOMAIN, "boundary tag other than SOLID_WALL (-1) is not supported");
+    if (t < -2)
+      return fail(SWE_ERR_DOMAIN, "boundary tag other than SOLID_WALL (-1) or FREE_FLOW (-2) is not supported");
     ep0[size_t(e)] = int(a); ep1[size_t(e)] = int(b); et0[size_t(e)] = int(f); et1[size_t(e)] = int(t);
   }
   std::vector<int> deg(size_t(V) + 1, 0);
@@ -461,8 +466,8 @@ int swe_rhs(swe_solver* s, int64_t cell, double dt, double* out3) {
   DevState S = state_of(s, s->cur);
   double* d = nullptr;
   SWE_CHECK(hipMalloc((void**)&d, 3 * sizeof(double)), "hipMalloc");
-  hipError_t e = launch_drain(s->M, S, s->W, s->stream);
-  if (e == hipSuccess) e = launch_rhs_one(s->M, S, s->W, int(cell), dt, s->cor, d, s->stream);
+  hipError_t e = launch_drain(s->M, S, s->W, s->opts, s->stream);
+  if (e == hipSuccess) e = launch_rhs_one(s->M, S, s->W, int(cell), dt, s->cor, friction(s), d, s->stream);
   int rc = (e != hipSuccess) ? dev_fail(e, "rhs") : download(s, out3, d, 3 * sizeof(double));
   hipStreamSynchronize(s->stream);
   hipFree(d);
@@ -558,8 +563,8 @@ static int batch_step(swe_solver* s, int scheme, double cfl, double t_end) {
     SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
     SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->B.rbits + q, s->flux_items, s->stream), "flux");
     if (q == 0) SWE_CHECK(launch_batch_dt(s->B, cfl, t_end, s->stream), "batch_dt");
-    SWE_CHECK(launch_drain(s->M, S, s->W, s->stream), "drain");
-    StageArgs A{st[q].a0, st[q].a1, st[q].coef, 0., cfl, s->cor, nullptr, nullptr};
+    SWE_CHECK(launch_drain(s->M, S, s->W, s->opts, s->stream), "drain");
+    StageArgs A{st[q].a0, st[q].a1, st[q].coef, 0., cfl, s->cor, nullptr, nullptr, friction(s)};
     SWE_CHECK(launch_update(s->M, S, state_of(s, st[q].u0), out_of(s, st[q].out), s->W, A, s->upd_items, s->stream),
               "update");
   }
@@ -753,10 +758,10 @@ int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nstep
         launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + st, s->flux_items, s->stream);
         hipEventRecord(ev[4], s->stream);
         hipEventRecord(ev[5], s->stream);
-        launch_drain(s->M, S, s->W, s->stream);
+        launch_drain(s->M, S, s->W, s->opts, s->stream);
         hipEventRecord(ev[6], s->stream);
         hipEventRecord(ev[7], s->stream);
-        StageArgs A{a0, a1, coef, dt, cfl, s->cor, s->rbits, st == 0 ? s->dts + k : nullptr};
+        StageArgs A{a0, a1, coef, dt, cfl, s->cor, s->rbits, st == 0 ? s->dts + k : nullptr, friction(s)};
         launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->upd_items, s->stream);
         hipEventRecord(ev[8], s->stream);
         hipError_t e = hipEventSynchronize(ev[8]);
@@ -846,11 +851,11 @@ int swe_autotune(swe_solver* s, int reps, int* flux_items, int* update_items, do
     rc = timed([&] { return launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, rslot, fi[q], s->stream); },
                t[q]);
   if (!rc) {
-    hipError_t e = launch_drain(s->M, S, s->W, s->stream);
+    hipError_t e = launch_drain(s->M, S, s->W, s->opts, s->stream);
     if (e != hipSuccess) rc = dev_fail(e, "drain");
   }
   // Euler-form stage with a small fixed step into the free buffer
-  StageArgs A{0., 0., 1., 1e-6, 0., s->cor, rslot, nullptr};
+  StageArgs A{0., 0., 1., 1e-6, 0., s->cor, rslot, nullptr, friction(s)};
   for (int q = 0; q < 2 && !rc; ++q)
     rc = timed([&] {
       return launch_update(s->M, S, S, out_of(s, scratch), s->W, A, ui[q], s->stream);

@@ -37,6 +37,10 @@
 namespace swe {
 
 #define CELL_BND (1u << 3)
+// boundary tags (include/Includes.h:21): walls (src/SpaceDisc.cpp:66-69) and
+// free flow, whose outside state is the inside edge state (transmissive)
+constexpr int kSolidWall = -1;
+constexpr int kOptDrainLength = 32;  // SWE_OPT_DRAIN_LENGTH
 
 __device__ __forceinline__ int logical_block() {
   // Workgroups are dispatched round-robin over the 8 XCDs; hand each XCD a
@@ -119,9 +123,9 @@ __global__ void __launch_bounds__(256) k_setup_edges(DevMesh M, DevGeomOut G) {
   if (e >= M.E) return;
   const int f = M.et0[e], t = M.et1[e];
   double dm = 0.;
-  if (t >= 0) {
+  if (t != kSolidWall) {  // interior, or free flow (the inside cell twice)
     const double le = edge_len(node(M, M.ep0[e]), node(M, M.ep1[e]));
-    const double dl = 2. * G.area[f] / le, dr = 2. * G.area[t] / le;
+    const double dl = 2. * G.area[f] / le, dr = t < 0 ? dl : 2. * G.area[t] / le;
     dm = dmin(dl, dr);
   }
   G.edmin[e] = dm;
@@ -386,7 +390,7 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, 
     q.Pb = node(M, M.ep1[es]);
     q.edmin = M.edmin[es];
     const int f = M.et0[es], t = M.et1[es];
-    const int tr = t < 0 ? f : t;  // wall edges read their own cell twice
+    const int tr = t < 0 ? f : t;  // boundary edges read their own cell twice: R = L
     q.t = t;
     q.mf = load_muscl(S, W, f);
     q.mt = load_muscl(S, W, tr);
@@ -395,7 +399,7 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, 
     q.cxt = M.cx[tr];
     q.cyt = M.cy[tr];
     // the cell average enters only the wall flux: walls alone load it
-    q.hf = t < 0 ? S.w[f] - M.bc[f] : 0.;
+    q.hf = t == kSolidWall ? S.w[f] - M.bc[f] : 0.;
     q.inst = M.cinst ? M.cinst[f] : 0;
     q.on = e < M.e1 && (!M.cinst || M.bactive[q.inst]);
   }
@@ -414,7 +418,7 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, 
     const double nx = ty, ny = -tx;
     double F[3];
     double lwi = 1.0;
-    if (q.t < 0) {
+    if (q.t == kSolidWall) {
       elem_flux(nx, ny, q.hf, 0., 0., F[0], F[1], F[2]);
     } else {
       Slot L = slot_value(q.mf, q.cxf, q.cyf, E);
@@ -456,7 +460,10 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, 
 
 // ----------------------------------------------------------- draining dt --
 // TimeDisc::ComputeDrainingDt (src/TimeDisc.cpp:43-66) on the stage-start
-// state.
+// state.  The reference sums the outflowing F0, a flux per unit length, so
+// its "time" A h / sum is not one (D16); with LEN the outflow through each
+// edge is F0 * L (the edge length from the cell's own vertices).
+template <bool LEN>
 __global__ void __launch_bounds__(256) k_drain(DevMesh M, DevState S, DevWork W) {
   int64_t i = M.c0 + global_index();
   if (i >= M.c1) return;
@@ -467,11 +474,20 @@ __global__ void __launch_bounds__(256) k_drain(DevMesh M, DevState S, DevWork W)
     d = 0.;
   } else {
     uint32_t fl = M.cflag[i];
+    V3 p0, p1, p2;
+    if (LEN) {
+      p0 = node(M, M.tp[i]);
+      p1 = node(M, M.tp[M.T + i]);
+      p2 = node(M, M.tp[2 * M.T + i]);
+    }
     double sum = 0.;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       double fe = W.f0[M.te[k * M.T + i]];
-      sum += dmax(0., sgn_pos(fl, k) ? fe : -fe);
+      double out = sgn_pos(fl, k) ? fe : -fe;
+      if (LEN)
+        out = out * edge_len(pick(p0, p1, p2, ep0_local(fl, k)), pick(p0, p1, p2, ep1_local(fl, k)));
+      sum += dmax(0., out);
     }
     d = sum > kTol ? M.area[i] * h / sum : INFINITY;
   }
@@ -485,6 +501,7 @@ __global__ void __launch_bounds__(256) k_drain(DevMesh M, DevState S, DevWork W)
 struct RhsIn {
   uint32_t fl;
   Muscl mu;
+  double wi;  // cell average level
   V3 p[3];  // the cell's vertices
   double dti;
   double F0[3], F1[3], F2[3], dtik[3];
@@ -493,6 +510,7 @@ __device__ __forceinline__ void rhs_load(const DevMesh& M, const DevState& S, co
                                          int i, RhsIn& q) {
   q.fl = M.cflag[i];
   q.mu = load_muscl(S, W, i);
+  q.wi = S.w[i];
   q.p[0] = node(M, M.tp[i]);
   q.p[1] = node(M, M.tp[M.T + i]);
   q.p[2] = node(M, M.tp[2 * M.T + i]);
@@ -508,8 +526,8 @@ __device__ __forceinline__ void rhs_load(const DevMesh& M, const DevState& S, co
     q.dtik[k] = nb < 0 ? INFINITY : d;
   }
 }
-__device__ __forceinline__ void rhs_compute(const RhsIn& q, double dts, double cor, double& r0,
-                                            double& r1, double& r2) {
+__device__ __forceinline__ void rhs_compute(const RhsIn& q, double dts, double cor, double fric,
+                                            double& r0, double& r1, double& r2) {
   r0 = 0.; r1 = 0.; r2 = 0.;
   // Domain::T / Area (src/Bathymetry.cpp:24-27,84-90)
   const V3 c = centroid(q.p[0], q.p[1], q.p[2]);
@@ -546,13 +564,20 @@ __device__ __forceinline__ void rhs_compute(const RhsIn& q, double dts, double c
     r1 += dtk * (nx * c_ek * q2);
     r2 += dtk * (ny * c_ek * q2);
   }
+  // linear friction -tau (hu, hv) of the cell average (opt-in; the
+  // reference stores tau unused, include/SpaceDisc.h:47)
+  if (fric != 0.) {
+    const double hh = q.wi - c.z;
+    r1 -= dts * (fric * (hh * q.mu.u0));
+    r2 -= dts * (fric * (hh * q.mu.v0));
+  }
 }
 __device__ __forceinline__ void rhs_cell(const DevMesh& M, const DevState& S, const DevWork& W,
-                                         int i, double dts, double cor, double& r0, double& r1,
-                                         double& r2) {
+                                         int i, double dts, double cor, double fric, double& r0,
+                                         double& r1, double& r2) {
   RhsIn q;
   rhs_load(M, S, W, i, q);
-  rhs_compute(q, dts, cor, r0, r1, r2);
+  rhs_compute(q, dts, cor, fric, r0, r1, r2);
 }
 
 __device__ __forceinline__ double stage_dt(const StageArgs& A) {
@@ -605,7 +630,7 @@ __global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, DevState 
       dts = A.coef * M.bdt[inst];
     }
     double R0, R1, R2;
-    rhs_compute(in[it], dts, A.cor, R0, R1, R2);
+    rhs_compute(in[it], dts, A.cor, A.fric, R0, R1, R2);
     const double h = w[it] - bc[it];
     const double Uc0 = h, Uc1 = h * u[it], Uc2 = h * v[it];
     double r0, r1, r2;
@@ -822,9 +847,9 @@ __global__ void __launch_bounds__(256) k_sum_partials(const double* partial, int
 }
 
 __global__ void k_rhs_one(DevMesh M, DevState S, DevWork W, int i, double dt, double cor,
-                          double* out) {
+                          double fric, double* out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  rhs_cell(M, S, W, i, dt, cor, out[0], out[1], out[2]);
+  rhs_cell(M, S, W, i, dt, cor, fric, out[0], out[1], out[2]);
 }
 
 // -------------------------------------------------------------- launch --
@@ -873,8 +898,12 @@ hipError_t launch_flux(const DevMesh& M, const DevState& S, const DevWork& W, in
 #undef SWE_FLUX_CASE
   return hipErrorInvalidValue;
 }
-hipError_t launch_drain(const DevMesh& M, const DevState& S, const DevWork& W, hipStream_t st) {
-  hipLaunchKernelGGL(k_drain, grid_for(M.c1 - M.c0), dim3(256), 0, st, M, S, W);
+hipError_t launch_drain(const DevMesh& M, const DevState& S, const DevWork& W, int opts,
+                        hipStream_t st) {
+  if (opts & kOptDrainLength)
+    hipLaunchKernelGGL(k_drain<true>, grid_for(M.c1 - M.c0), dim3(256), 0, st, M, S, W);
+  else
+    hipLaunchKernelGGL(k_drain<false>, grid_for(M.c1 - M.c0), dim3(256), 0, st, M, S, W);
   return hipGetLastError();
 }
 hipError_t launch_update(const DevMesh& M, const DevState& S, const DevState& U0, const DevOut& O,
@@ -916,8 +945,8 @@ hipError_t launch_integrals(const DevMesh& M, const DevState& S, const DevWork& 
   return hipGetLastError();
 }
 hipError_t launch_rhs_one(const DevMesh& M, const DevState& S, const DevWork& W, int i, double dt,
-                          double cor, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_rhs_one, dim3(1), dim3(64), 0, st, M, S, W, i, dt, cor, out);
+                          double cor, double fric, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_rhs_one, dim3(1), dim3(64), 0, st, M, S, W, i, dt, cor, fric, out);
   return hipGetLastError();
 }
 

@@ -14,7 +14,8 @@
 //     cells  cx, cy, bc (centroid x, y, z), area, and an AoS copy
 //            crec {cx, cy, bc, zmin, zmax, cflag} (48 B) for the
 //            reconstruction's neighbour gathers
-//     edges  edmin (min over the two cells of 2 area / length)
+//     edges  edmin (min over the two cells of 2 area / length; the inside
+//            cell alone on a free-flow edge, 0 on a wall)
 //   Edge lengths, midpoints and tangents are recomputed from px, py, pz.
 //   state  w, u, v                        double[T] per buffer
 //   work   MUSCL (mw, mgx, mgy): pass 1, overwritten in place by pass 2 for
@@ -118,6 +119,7 @@ struct StageArgs {
   double cor;
   const unsigned long long* rbits;  // r of the step's first flux pass (double bits)
   double* dts_log;     // receives the step's dt (first stage only), may be null
+  double fric;         // linear friction coefficient (0: none)
 };
 
 hipError_t launch_setup_geometry(const DevMesh& M, const DevGeomOut& G, hipStream_t st);
@@ -129,7 +131,9 @@ hipError_t launch_recon2(const DevMesh& M, const DevState& S, const DevWork& W, 
 // update (1 or 2); results are identical for every choice
 hipError_t launch_flux(const DevMesh& M, const DevState& S, const DevWork& W, int flux, int ws,
                        double cor, unsigned long long* rbits, int items, hipStream_t st);
-hipError_t launch_drain(const DevMesh& M, const DevState& S, const DevWork& W, hipStream_t st);
+// opts & 32 (SWE_OPT_DRAIN_LENGTH): outflow F0 * L in the draining dt
+hipError_t launch_drain(const DevMesh& M, const DevState& S, const DevWork& W, int opts,
+                        hipStream_t st);
 hipError_t launch_update(const DevMesh& M, const DevState& S, const DevState& U0, const DevOut& O,
                          const DevWork& W, const StageArgs& A, int items, hipStream_t st);
 hipError_t launch_batch_reset_r(const DevBatch& B, int slot, hipStream_t st);
@@ -145,6 +149,6 @@ int64_t integrals_blocks(const DevMesh& M);
 hipError_t launch_integrals(const DevMesh& M, const DevState& S, const DevWork& W, double* partial,
                             double* per_cell, double* out, hipStream_t st);
 hipError_t launch_rhs_one(const DevMesh& M, const DevState& S, const DevWork& W, int i, double dt,
-                          double cor, double* out, hipStream_t st);
+                          double cor, double fric, double* out, hipStream_t st);
 
 }  // namespace swe

@@ -143,12 +143,13 @@ def test_jittered_mesh_invariants():
 
 def test_solver_needs_a_device_or_rejects_bad_tables():
     """Unsupported boundary tags are rejected before any device work
-    (src/SpaceDisc.cpp:66-72 only implements SOLID_WALL); without a device
-    creation fails loudly instead of falling back to the CPU."""
+    (src/SpaceDisc.cpp:66-72 implements SOLID_WALL; FREE_FLOW is added here,
+    PERIODIC and CUSTOM are not); without a device creation fails loudly
+    instead of falling back to the CPU."""
     xyz, topo = swe.jittered_mesh(4, 0.0, 1)
     t = topo.tables()
     et = t["et"].copy()
-    et[np.where(et[:, 1] < 0)[0][0], 1] = -2  # FREE_FLOW
+    et[np.where(et[:, 1] < 0)[0][0], 1] = -3  # PERIODIC
     bad = swe.Topology.create(topo.NumNodes(), t["ep"], et, t["tp"], t["te"], t["tt"])
     fl = swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt]
     prim = np.zeros((topo.NumTriangles(), 3))

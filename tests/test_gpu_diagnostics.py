@@ -50,8 +50,8 @@ def test_mass_integral_conserved_on_device():
     assert abs(m1 - m0) <= 1e-13 * m0
 
 
-def _run_to(dom, prim, t_end, cfl):
-    ens = Ensemble(FL, [dom], [prim])
+def _run_to(dom, prim, t_end, cfl, options=0):
+    ens = Ensemble(FL, [dom], [prim], options=options)
     ens.run("SSPRK2", cfl=cfl, t_end=t_end, graph=True)
     t, steps = ens.times()
     assert t[0] == t_end
@@ -115,3 +115,34 @@ def test_thacker_one_period_against_analytic(cfl):
         assert abs(m1 - m0) <= 1e-12 * m0
         errs.append(nrm["L1"][0])
     assert max(errs) < 0.0375, errs
+
+
+def test_thacker_converges_with_the_draining_length_fix():
+    """With SWE_OPT_DRAIN_LENGTH (outflow F0 * L in the draining dt, D16)
+    the same configuration converges: the L1 error of w after one period
+    falls by more than half per mesh doubling (n = 25, 50, 100), and the GPU
+    still equals the oracle bit for bit at n = 25."""
+    errs = []
+    for n in (25, 50, 100):
+        dom, prim, th = W.thacker(n=n, quad=20)
+        st, steps = _run_to(dom, prim, th.period, 0.15, options=swe.OPT_DRAIN_LENGTH)
+        if n == 25:
+            orc = O.OracleSolver(dom.geometry, dom.topology.tables(), "HLLC", "Einfeldt",
+                                 opts=O.OPT_DRAIN_LENGTH)
+            orc.set_state(prim)
+            tk = 0.0
+            for _ in range(steps):
+                orc.compute_interface_values()
+                d = 0.15 * orc.compute_fluxes()
+                if d < th.period - tk:
+                    orc.run("SSPRK2", 1, dt=0.0, cfl=0.15)
+                    tk = tk + d
+                else:
+                    orc.run("SSPRK2", 1, dt=th.period - tk)
+                    tk = th.period
+            assert np.array_equal(st, orc.state())
+        sd = swe.SpaceDisc(FL, dom, st, options=swe.OPT_DRAIN_LENGTH)
+        nrm = A.error_norms(sd, th, th.period, n_partial=20)
+        print(f"thacker drain-length n={n}: {steps} steps, L1 {nrm['L1']}", flush=True)
+        errs.append(nrm["L1"][0])
+    assert errs[1] < 0.5 * errs[0] and errs[2] < 0.5 * errs[1], errs
