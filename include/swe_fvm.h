@@ -222,6 +222,12 @@ int swe_integrals(swe_solver* s, double* out3, double* per_cell);
  * for dry cells).  Any pointer may be null. */
 int swe_get_muscl(swe_solver* s, double* w0, double* gx, double* gy);
 
+/* Failure detection (no reference counterpart: the reference has no NaN or
+ * depth checks): out5 = (number of cells with a non-finite w, u or v,
+ * min h, max h, max |u|, max |u| + sqrt(h)), the speeds over wet cells.
+ * Exact (min / max / counts) and identical run to run. */
+int swe_state_stats(swe_solver* s, double* out5);
+
 /* ---------------------------------------------------------------- tuning --- */
 
 /* Work items per thread of the flux kernel (1, 2 or 4 edges) and of the

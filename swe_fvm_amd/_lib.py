@@ -61,6 +61,7 @@ SIGNATURES = [
     ("swe_batch_time", _int, [_p, _p, _p]),
     ("swe_integrals", _int, [_p, _p, _p]),
     ("swe_get_muscl", _int, [_p, _p, _p, _p]),
+    ("swe_state_stats", _int, [_p, _p]),
     ("swe_set_tuning", _int, [_p, _int, _int]),
     ("swe_get_tuning", _int, [_p, _p, _p]),
     ("swe_autotune", _int, [_p, _int, _p, _p, _p]),

@@ -354,6 +354,14 @@ class SpaceDisc:
         _check(_lib.load().swe_integrals(self._h, _ptr(out), _ptr(pc) if per_cell else None))
         return (out, pc) if per_cell else out
 
+    def Stats(self):
+        """Health of the current state (failure detection): dict with the
+        number of cells holding a non-finite value, min / max depth, max
+        speed and max |u| + sqrt(h) over wet cells."""
+        out = np.zeros(5, np.float64)
+        _check(_lib.load().swe_state_stats(self._h, _ptr(out)))
+        return {"nonfinite": int(out[0]), "hmin": out[1], "hmax": out[2], "umax": out[3], "cmax": out[4]}
+
     def GetCor(self):
         return self.cor
 

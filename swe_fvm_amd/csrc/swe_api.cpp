@@ -912,4 +912,22 @@ int swe_get_muscl(swe_solver* s, double* w0, double* gx, double* gy) {
   return SWE_OK;
 }
 
+// Health of the current state: out5 = (cells with a non-finite value,
+// min h, max h, max |u|, max |u| + sqrt(h)) over all cells (wet cells for
+// the speeds).  Exact and identical run to run.
+int swe_state_stats(swe_solver* s, double* out5) {
+  if (!s || !out5) return fail(SWE_ERR_ARG, "null argument");
+  if (int rc = set_device(s)) return rc;
+  const int64_t nb = integrals_blocks(s->M);
+  double* d = nullptr;
+  SWE_CHECK(hipMalloc((void**)&d, sizeof(double) * size_t(5 * nb + 5)), "hipMalloc");
+  hipError_t e = launch_stats(s->M, state_of(s, s->cur), d + 5, d, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out5, d, 5 * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+  hipError_t e2 = hipStreamSynchronize(s->stream);
+  hipFree(d);
+  if (e == hipSuccess) e = e2;
+  if (e != hipSuccess) return dev_fail(e, "state_stats");
+  return SWE_OK;
+}
+
 }  // extern "C"

@@ -846,6 +846,68 @@ __global__ void __launch_bounds__(256) k_sum_partials(const double* partial, int
     for (int q = 0; q < 3; ++q) out[q] = v[q];
 }
 
+// ------------------------------------------------------------ health --
+// State statistics for failure detection: non-finite cells, min and max of
+// h = w - b, max |u| and max |u| + sqrt(h) over wet cells.  Block partials,
+// then one block; min / max and integer counts are order-independent, so
+// the result is exact and identical run to run.
+struct Stats {
+  double nonfinite, hmin, hmax, umax, cmax;
+};
+__device__ inline void stats_combine(Stats& a, const Stats& b) {
+  a.nonfinite += b.nonfinite;
+  a.hmin = dmin(a.hmin, b.hmin);
+  a.hmax = dmax(a.hmax, b.hmax);
+  a.umax = dmax(a.umax, b.umax);
+  a.cmax = dmax(a.cmax, b.cmax);
+}
+__device__ inline Stats block_stats(Stats v) {
+  __shared__ Stats red[kBlock];
+  const int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (t < off) stats_combine(red[t], red[t + off]);
+    __syncthreads();
+  }
+  return red[0];
+}
+__global__ void __launch_bounds__(256) k_stats(DevMesh M, DevState S, double* partial) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  Stats v{0., INFINITY, -INFINITY, 0., 0.};
+  if (i < M.T) {
+    const double w = S.w[i], u = S.u[i], vv = S.v[i];
+    if (!isfinite(w) || !isfinite(u) || !isfinite(vv)) {
+      v.nonfinite = 1.;
+    } else {
+      const double h = w - M.bc[i];
+      v.hmin = h;
+      v.hmax = h;
+      if (is_wet(h)) {
+        const double sp = sqrt(u * u + vv * vv);
+        v.umax = sp;
+        v.cmax = sp + sqrt(h);
+      }
+    }
+  }
+  const Stats b = block_stats(v);
+  if (threadIdx.x == 0) {
+    double* p = partial + 5 * (int64_t)blockIdx.x;
+    p[0] = b.nonfinite; p[1] = b.hmin; p[2] = b.hmax; p[3] = b.umax; p[4] = b.cmax;
+  }
+}
+__global__ void __launch_bounds__(256) k_stats_final(const double* partial, int64_t nb, double* out) {
+  Stats v{0., INFINITY, -INFINITY, 0., 0.};
+  for (int64_t b = threadIdx.x; b < nb; b += kBlock) {
+    const double* p = partial + 5 * b;
+    stats_combine(v, Stats{p[0], p[1], p[2], p[3], p[4]});
+  }
+  const Stats r = block_stats(v);
+  if (threadIdx.x == 0) {
+    out[0] = r.nonfinite; out[1] = r.hmin; out[2] = r.hmax; out[3] = r.umax; out[4] = r.cmax;
+  }
+}
+
 __global__ void k_rhs_one(DevMesh M, DevState S, DevWork W, int i, double dt, double cor,
                           double fric, double* out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -936,6 +998,13 @@ hipError_t launch_max_wp(const DevMesh& M, const DevWork& W, double* out, hipStr
   return hipGetLastError();
 }
 int64_t integrals_blocks(const DevMesh& M) { return (M.T + kBlock - 1) / kBlock; }
+hipError_t launch_stats(const DevMesh& M, const DevState& S, double* partial, double* out,
+                        hipStream_t st) {
+  const int64_t nb = integrals_blocks(M);
+  if (nb > 0) hipLaunchKernelGGL(k_stats, dim3((unsigned)nb), dim3(256), 0, st, M, S, partial);
+  hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(256), 0, st, partial, nb, out);
+  return hipGetLastError();
+}
 hipError_t launch_integrals(const DevMesh& M, const DevState& S, const DevWork& W, double* partial,
                             double* per_cell, double* out, hipStream_t st) {
   const int64_t nb = integrals_blocks(M);

@@ -148,6 +148,10 @@ hipError_t launch_max_wp(const DevMesh& M, const DevWork& W, double* out, hipStr
 int64_t integrals_blocks(const DevMesh& M);
 hipError_t launch_integrals(const DevMesh& M, const DevState& S, const DevWork& W, double* partial,
                             double* per_cell, double* out, hipStream_t st);
+// state statistics (non-finite count, min h, max h, max |u|, max |u| +
+// sqrt(h)): partial needs 5 * integrals_blocks(M) doubles, out 5
+hipError_t launch_stats(const DevMesh& M, const DevState& S, double* partial, double* out,
+                        hipStream_t st);
 hipError_t launch_rhs_one(const DevMesh& M, const DevState& S, const DevWork& W, int i, double dt,
                           double cor, double fric, double* out, hipStream_t st);
 
