@@ -62,6 +62,8 @@ SIGNATURES = [
     ("swe_integrals", _int, [_p, _p, _p]),
     ("swe_get_muscl", _int, [_p, _p, _p, _p]),
     ("swe_state_stats", _int, [_p, _p]),
+    ("swe_adjoint_run", _int, [_p, _int, _d, _p, _i64, _i64, _p, _p, _p, _p]),
+    ("swe_set_bathymetry", _int, [_p, _p]),
     ("swe_set_tuning", _int, [_p, _int, _int]),
     ("swe_get_tuning", _int, [_p, _p, _p]),
     ("swe_autotune", _int, [_p, _int, _p, _p, _p]),

@@ -354,6 +354,32 @@ class SpaceDisc:
         _check(_lib.load().swe_integrals(self._h, _ptr(out), _ptr(pc) if per_cell else None))
         return (out, pc) if per_cell else out
 
+    def set_bathymetry(self, z):
+        """Replaces the node bathymetry (V values) on the device and recomputes
+        the static geometry; the state (w, u, v) is kept."""
+        z = np.ascontiguousarray(z, dtype=np.float64).reshape(self.V)
+        _check(_lib.load().swe_set_bathymetry(self._h, _ptr(z)))
+
+    def adjoint_run(self, final_bar, nsteps, scheme="SSPRK2", dt=None, dts=None, checkpoint_every=50,
+                    want_z=True):
+        """Runs nsteps steps from the current state with fixed step sizes (dt,
+        or the sequence dts) and back-propagates final_bar (T x 3 adjoint of
+        the final w, u, v): returns (adjoint of the initial state T x 3,
+        adjoint of the node bathymetry V or None, [forward ms, reverse ms]).
+        The final state is current afterwards, as after run()."""
+        fb = np.ascontiguousarray(final_bar, dtype=np.float64).reshape(self.T, 3)
+        s0 = np.zeros((self.T, 3), np.float64)
+        zb = np.zeros(self.V, np.float64) if want_z else None
+        ms = np.zeros(2, np.float64)
+        d = None
+        if dts is not None:
+            d = np.ascontiguousarray(dts, dtype=np.float64).reshape(int(nsteps))
+        _check(_lib.load().swe_adjoint_run(self._h, SCHEMES[scheme], float(dt or 0.0),
+                                           _ptr(d) if d is not None else None, int(nsteps),
+                                           int(checkpoint_every), _ptr(fb), _ptr(s0),
+                                           _ptr(zb) if want_z else None, _ptr(ms)))
+        return s0, zb, ms
+
     def Stats(self):
         """Health of the current state (failure detection): dict with the
         number of cells holding a non-finite value, min / max depth, max

@@ -1,0 +1,571 @@
+// Reverse (adjoint) kernels of one explicit stage.
+//
+// A stage maps the stage state S and the step's first state U0 (primitive
+// w, u, v per cell) to its output O (src/Solvers.cpp:5-59 with
+// TimeDisc::RHS, src/TimeDisc.cpp:3-66).  Given the adjoint of O, these
+// kernels return the adjoints of S and U0 and add the adjoint of the node
+// bathymetry z.  The forward intermediates (MUSCL, fluxes, draining dt) are
+// recomputed by the forward kernels before the reverse pass, so the
+// linearisation point is bit-identical to the forward step.  Branch
+// decisions (wet/dry, the limiter's all-or-nothing choices, min / max
+// selections) are the forward ones; the flux's local Jacobian comes from
+// the flux template instantiated on Dual<7>.
+//
+// Every contribution is written to a slot owned by one (cell, side) or
+// (edge, side) and gathered in a fixed order by its consumer, so the
+// reverse pass is atomic-free and identical run to run, like the forward.
+//
+// Regime: fully wet cells, dry cells, and partially wet cells whose level is
+// at or above their highest vertex (the reconstruction is then the cell
+// level).  A partially wet cell below its highest vertex (a shoreline cell:
+// the cubic / bisection level and the Liu et al. plane) is counted in
+// A.flags and the host reports the adjoint as unsupported for that state.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "swe_common.h"
+
+namespace swe {
+
+// cell bathymetry (centroid z) adjoint onto the three corners
+__device__ __forceinline__ void bc_bar(const DevAdj& A, int64_t T, int64_t i, double b) {
+  const double c = 1. / 3.;
+  for (int k = 0; k < 3; ++k) A.zc[k * T + i] += c * b;
+}
+
+// local index of edge e in cell c (te[k][c] == e)
+__device__ __forceinline__ int local_edge(const DevMesh& M, int c, int e) {
+  return M.te[c] == e ? 0 : (M.te[M.T + c] == e ? 1 : 2);
+}
+// local index k' with tt[k'][j] == c
+__device__ __forceinline__ int back_index(const DevMesh& M, int j, int c) {
+  return M.tt[j] == c ? 0 : (M.tt[M.T + j] == c ? 1 : 2);
+}
+
+// Reverse of slot_value (AtPoint + PrimAssigner, src/SpaceDisc.cpp:24-25):
+// from the adjoint (sw, su, sv) of the slot to the MUSCL adjoint mb[5]
+// (w0, gx, gy, u0, v0) and the edge-midpoint z adjoint ez.
+__device__ inline void slot_bar(const Muscl& m, double cx, double cy, const V3& E, double sw,
+                                double su, double sv, double mb[5], double& ez) {
+  const double dx = E.x - cx, dy = E.y - cy;
+  const double a = m.w0 + (m.gx * dx + m.gy * dy);
+  const bool wet = (a - E.z) >= 0;
+  if (!wet) {  // AtPoint returns (z, 0, 0); PrimAssigner keeps it
+    ez += sw;
+    return;
+  }
+  const double h = a - E.z;
+  if (!is_wet(h)) {
+    ez += sw;
+    return;
+  }
+  double abar = sw, hbar = 0.;
+  if (h < 1e-3) {
+    const double q = h * h + 1e-6;
+    const double fac = sqrt(2.) * h / sqrt(q);
+    const double dfac = sqrt(2.) * 1e-6 / (q * sqrt(q));
+    mb[3] += fac * su;
+    mb[4] += fac * sv;
+    hbar += (m.u0 * su + m.v0 * sv) * dfac;
+  } else {
+    mb[3] += su;
+    mb[4] += sv;
+  }
+  abar += hbar;
+  ez -= hbar;
+  mb[0] += abar;
+  mb[1] += abar * dx;
+  mb[2] += abar * dy;
+}
+
+// --------------------------------------------------------------- update --
+// Reverse of k_update for cell i: ConsAssigner, the stage combination and
+// TimeDisc::RHS.  Writes the flux adjoint of each of the cell's three edge
+// sides, the draining-dt adjoints (own, neighbours'), the MUSCL adjoint,
+// the U0 adjoint, the first part of the S adjoint and the corner z adjoint.
+__global__ void __launch_bounds__(256) k_adj_update(DevMesh M, DevState S, DevState U0, DevWork W,
+                                                    StageArgs A, DevAdj J) {
+  const int64_t i64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i64 >= M.T) return;
+  const int i = (int)i64;
+  const int64_t T = M.T;
+  const uint32_t fl = M.cflag[i];
+  const Muscl mu = load_muscl(S, W, i);
+  const V3 p[3] = {node(M, M.tp[i]), node(M, M.tp[T + i]), node(M, M.tp[2 * T + i])};
+  const V3 c = centroid(p[0], p[1], p[2]);
+  const double bc = c.z;
+  const double i_area = 1. / tri_area(p[0], p[1], p[2]);
+  const double dts = A.coef * A.dt_fixed;
+  const double dti = W.dtd[i];
+  const double w = S.w[i], u = S.u[i], v = S.v[i];
+  const bool comb = A.a0 != 0.;
+  // forward recomputation of R (rhs_compute) keeping the per-edge terms
+  double F[3][3], dtk[3], dtik[3], hk[3], src1[3], src2[3], cek[3], nxk[3], nyk[3];
+  V3 E[3];
+  int ek[3];
+  double R0 = 0., R1 = 0., R2 = 0.;
+  for (int k = 0; k < 3; ++k) {
+    const int e = M.te[k * T + i];
+    const int nb = M.tt[k * T + i];
+    ek[k] = e;
+    F[k][0] = W.f0[e]; F[k][1] = W.f1[e]; F[k][2] = W.f2[e];
+    dtik[k] = nb < 0 ? INFINITY : W.dtd[nb];
+    const bool own = sgn_pos(fl, k);
+    const double sg = own ? 1. : -1.;
+    dtk[k] = (sg * F[k][0]) > 0. ? dmin(dts, dti) : dmin(dts, dtik[k]);
+    const V3 Pa = pick(p[0], p[1], p[2], ep0_local(fl, k));
+    const V3 Pb = pick(p[0], p[1], p[2], ep1_local(fl, k));
+    const double le = edge_len(Pa, Pb);
+    cek[k] = i_area * le;
+    E[k] = midpoint(Pa, Pb);
+    const Slot s = slot_value(mu, c.x, c.y, E[k]);
+    hk[k] = s.w - E[k].z;
+    src1[k] = mu.gx + A.cor * (-s.v);
+    src2[k] = mu.gy + A.cor * s.u;
+    double tx, ty;
+    tangent(Pa, Pb, le, c, tx, ty);
+    nxk[k] = ty;
+    nyk[k] = -tx;
+    const double cf = dtk[k] * sg * cek[k];
+    R0 -= cf * F[k][0];
+    R1 -= cf * F[k][1];
+    R2 -= cf * F[k][2];
+    const double cs = dts * (1. / 3.);
+    R1 -= cs * src1[k] * hk[k];
+    R2 -= cs * src2[k] * hk[k];
+    const double q2 = 0.5 * hk[k] * hk[k];
+    R1 += dtk[k] * (nxk[k] * cek[k] * q2);
+    R2 += dtk[k] * (nyk[k] * cek[k] * q2);
+  }
+  const double hh = w - bc;
+  if (A.fric != 0.) {
+    R1 -= dts * (A.fric * (hh * mu.u0));
+    R2 -= dts * (A.fric * (hh * mu.v0));
+  }
+  const double Uc0 = hh, Uc1 = hh * u, Uc2 = hh * v;
+  const double w0p = U0.w[i], u0p = U0.u[i], v0p = U0.v[i];
+  const double h0 = w0p - bc;
+  double r0, r1, r2;
+  if (comb) {
+    r0 = (A.a0 * h0 + A.a1 * Uc0) + R0;
+    r1 = (A.a0 * (h0 * u0p) + A.a1 * Uc1) + R1;
+    r2 = (A.a0 * (h0 * v0p) + A.a1 * Uc2) + R2;
+  } else {
+    r0 = Uc0 + R0;
+    r1 = Uc1 + R1;
+    r2 = Uc2 + R2;
+  }
+  // ConsAssigner reverse (src/Assigners.cpp:28-44)
+  const double ow = J.obar[i], ou = J.obar[T + i], ov = J.obar[2 * T + i];
+  double rb0 = 0., rb1 = 0., rb2 = 0., bcb = ow;
+  if (is_wet(r0)) {
+    double ih, dih;
+    if (r0 < 1e-3) {
+      const double q = r0 * r0 * r0 * r0 + 1e-12;
+      ih = sqrt(2.) * r0 / sqrt(q);
+      dih = sqrt(2.) * (1e-12 - r0 * r0 * r0 * r0) / (q * sqrt(q));
+    } else {
+      ih = 1. / r0;
+      dih = -ih * ih;
+    }
+    rb0 = ow + dih * (r1 * ou + r2 * ov);
+    rb1 = ih * ou;
+    rb2 = ih * ov;
+  }
+  // stage combination
+  double sw = 0., su = 0., sv = 0.;
+  const double ac = comb ? A.a1 : 1.;
+  {
+    const double g0 = ac * rb0, g1 = ac * rb1, g2 = ac * rb2;  // adjoint of (h, hu, hv) of S
+    const double hb = g0 + u * g1 + v * g2;
+    sw += hb;
+    bcb -= hb;
+    su += hh * g1;
+    sv += hh * g2;
+  }
+  if (comb) {
+    const double g0 = A.a0 * rb0, g1 = A.a0 * rb1, g2 = A.a0 * rb2;
+    const double hb = g0 + u0p * g1 + v0p * g2;
+    J.u0bar[i] = hb;
+    J.u0bar[T + i] = h0 * g1;
+    J.u0bar[2 * T + i] = h0 * g2;
+    bcb -= hb;
+  } else {
+    J.u0bar[i] = 0.; J.u0bar[T + i] = 0.; J.u0bar[2 * T + i] = 0.;
+  }
+  // TimeDisc::RHS reverse
+  double mb[5] = {0., 0., 0., 0., 0.};
+  double dtib = 0.;
+  if (A.fric != 0.) {
+    const double hhb = -dts * A.fric * (mu.u0 * rb1 + mu.v0 * rb2);
+    mb[3] += -dts * A.fric * hh * rb1;
+    mb[4] += -dts * A.fric * hh * rb2;
+    sw += hhb;
+    bcb -= hhb;
+  }
+  double zc[3] = {0., 0., 0.};
+  for (int k = 0; k < 3; ++k) {
+    const bool own = sgn_pos(fl, k);
+    const double sg = own ? 1. : -1.;
+    const double cf = dtk[k] * sg * cek[k];
+    J.fs[(0 * 3 + k) * T + i] = -cf * rb0;
+    J.fs[(1 * 3 + k) * T + i] = -cf * rb1;
+    J.fs[(2 * 3 + k) * T + i] = -cf * rb2;
+    const double q2 = 0.5 * hk[k] * hk[k];
+    double dtkb = -(F[k][0] * rb0 + F[k][1] * rb1 + F[k][2] * rb2) * sg * cek[k];
+    dtkb += nxk[k] * cek[k] * q2 * rb1 + nyk[k] * cek[k] * q2 * rb2;
+    const double cs = dts * (1. / 3.);
+    double hkb = dtk[k] * (nxk[k] * cek[k] * rb1 + nyk[k] * cek[k] * rb2) * hk[k];
+    hkb += -cs * (src1[k] * rb1 + src2[k] * rb2);
+    const double s1b = -cs * hk[k] * rb1, s2b = -cs * hk[k] * rb2;
+    mb[1] += s1b;
+    mb[2] += s2b;
+    const double svb = -A.cor * s1b, sub = A.cor * s2b;
+    double ez = -hkb;
+    slot_bar(mu, c.x, c.y, E[k], hkb, sub, svb, mb, ez);
+    const int a0 = ep0_local(fl, k), a1 = ep1_local(fl, k);
+    zc[a0] += 0.5 * ez;
+    zc[a1] += 0.5 * ez;
+    // draining dt selection (dmin(dts, d) picks d when d < dts)
+    double nbb = 0.;
+    if ((sg * F[k][0]) > 0.) {
+      if (dti < dts) dtib += dtkb;
+    } else {
+      if (dtik[k] < dts) nbb = dtkb;
+    }
+    J.dtd_nb[k * T + i] = nbb;
+  }
+  (void)ek;
+  J.dtd_own[i] = dtib;
+  for (int q = 0; q < 5; ++q) J.m_cell[q * T + i] = mb[q];
+  const double cthird = 1. / 3.;
+  for (int k = 0; k < 3; ++k) J.zc[k * T + i] = zc[k] + cthird * bcb;
+  J.sbar[i] = sw;
+  J.sbar[T + i] = su;
+  J.sbar[2 * T + i] = sv;
+}
+
+// ---------------------------------------------------------------- drain --
+// Reverse of k_drain: the draining dt adjoint of cell c (its own update's
+// and its neighbours') into the mass fluxes of its edges and its depth.
+template <bool LEN>
+__global__ void __launch_bounds__(256) k_adj_drain(DevMesh M, DevState S, DevWork W, DevAdj J) {
+  const int64_t c64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c64 >= M.T) return;
+  const int c = (int)c64;
+  const int64_t T = M.T;
+  double db = J.dtd_own[c];
+  for (int k = 0; k < 3; ++k) {
+    const int j = M.tt[k * T + c];
+    if (j >= 0) db += J.dtd_nb[back_index(M, j, c) * T + j];
+  }
+  if (db == 0.) return;
+  const double h = S.w[c] - M.bc[c];
+  if (!is_wet(h)) return;
+  const uint32_t fl = M.cflag[c];
+  V3 p0, p1, p2;
+  if (LEN) {
+    p0 = node(M, M.tp[c]);
+    p1 = node(M, M.tp[T + c]);
+    p2 = node(M, M.tp[2 * T + c]);
+  }
+  double out[3], len[3], sum = 0.;
+  for (int k = 0; k < 3; ++k) {
+    const double fe = W.f0[M.te[k * T + c]];
+    out[k] = sgn_pos(fl, k) ? fe : -fe;
+    len[k] = 1.;
+    if (LEN) {
+      len[k] = edge_len(pick(p0, p1, p2, ep0_local(fl, k)), pick(p0, p1, p2, ep1_local(fl, k)));
+      out[k] = out[k] * len[k];
+    }
+    sum += dmax(0., out[k]);
+  }
+  if (!(sum > kTol)) return;
+  const double A = M.area[c];
+  const double hb = db * A / sum;
+  const double sb = -db * A * h / (sum * sum);
+  for (int k = 0; k < 3; ++k)
+    if (out[k] > 0.) J.fs[(0 * 3 + k) * T + c] += sb * len[k] * (sgn_pos(fl, k) ? 1. : -1.);
+  J.sbar[c] += hb;
+  bc_bar(J, T, c, -hb);
+}
+
+// ----------------------------------------------------------------- flux --
+// Reverse of k_flux for edge e: the flux adjoint of its two sides, through
+// the Riemann solver's local Jacobian (the flux template on Dual<7>: inputs
+// L (w, u, v), R (w, u, v), edge z) and the two slot reconstructions, into
+// per-side MUSCL adjoints and the edge-midpoint z adjoint.
+template <int FLUX, int WS>
+__global__ void __launch_bounds__(256) k_adj_flux(DevMesh M, DevState S, DevWork W, DevAdj J,
+                                                  double cor) {
+  const int64_t e64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e64 >= M.E) return;
+  const int e = (int)e64;
+  const int64_t T = M.T, Ecnt = M.E;
+  const int f = M.et0[e], t = M.et1[e];
+  const int k0 = local_edge(M, f, e);
+  double Fb[3];
+  for (int q = 0; q < 3; ++q) Fb[q] = J.fs[(q * 3 + k0) * T + f];
+  if (t >= 0) {
+    const int k1 = local_edge(M, t, e);
+    for (int q = 0; q < 3; ++q) Fb[q] += J.fs[(q * 3 + k1) * T + t];
+  }
+  const V3 Pa = node(M, M.ep0[e]), Pb = node(M, M.ep1[e]);
+  const double le = edge_len(Pa, Pb);
+  const V3 E = midpoint(Pa, Pb);
+  double tx, ty;
+  tangent(Pa, Pb, le, V3{M.cx[f], M.cy[f], 0.}, tx, ty);
+  const double nx = ty, ny = -tx;
+  double mbf[5] = {0., 0., 0., 0., 0.}, mbt[5] = {0., 0., 0., 0., 0.};
+  double ez = 0., hfb = 0.;
+  if (t == kSolidWall) {
+    // ElemFlux(n, (h, 0, 0)) = (0, h^2/2 n) of the cell average
+    const double hf = S.w[f] - M.bc[f];
+    if (is_wet(hf)) hfb = (Fb[1] * nx + Fb[2] * ny) * hf;
+  } else {
+    const int tr = t < 0 ? f : t;
+    const Muscl mf = load_muscl(S, W, f), mt = load_muscl(S, W, tr);
+    const double cxf = M.cx[f], cyf = M.cy[f], cxt = M.cx[tr], cyt = M.cy[tr];
+    const Slot L = slot_value(mf, cxf, cyf, E);
+    const Slot R = slot_value(mt, cxt, cyt, E);
+    using D = Dual<7>;
+    const SlotT<D> Ld{dvar<7>(L.w, 0), dvar<7>(L.u, 1), dvar<7>(L.v, 2)};
+    const SlotT<D> Rd{dvar<7>(R.w, 3), dvar<7>(R.u, 4), dvar<7>(R.v, 5)};
+    const D be = dvar<7>(E.z, 6);
+    D Fd[3];
+    double lw = 1.;
+    if (FLUX == 0) flux_hll<WS>(nx, ny, be, Ld, Rd, M.edmin[e], cor, lw, Fd);
+    else flux_hllc<WS>(tx, ty, be, Ld, Rd, M.edmin[e], cor, lw, Fd);
+    double gin[7];
+    for (int k = 0; k < 7; ++k) gin[k] = Fb[0] * Fd[0].d[k] + Fb[1] * Fd[1].d[k] + Fb[2] * Fd[2].d[k];
+    ez += gin[6];
+    if (t < 0) {  // free flow: R is L
+      for (int k = 0; k < 3; ++k) gin[k] += gin[3 + k];
+      slot_bar(mf, cxf, cyf, E, gin[0], gin[1], gin[2], mbf, ez);
+    } else {
+      slot_bar(mf, cxf, cyf, E, gin[0], gin[1], gin[2], mbf, ez);
+      slot_bar(mt, cxt, cyt, E, gin[3], gin[4], gin[5], mbt, ez);
+    }
+  }
+  for (int q = 0; q < 5; ++q) {
+    J.m_edge[(q * 2 + 0) * Ecnt + e] = mbf[q];
+    J.m_edge[(q * 2 + 1) * Ecnt + e] = mbt[q];
+  }
+  J.ez_edge[e] = ez;
+  J.hf_edge[e] = hfb;
+}
+
+// ------------------------------------------------------- reconstruction --
+__device__ __forceinline__ void flag(const DevAdj& J) { atomicAdd(J.flags, 1u); }
+
+// Reverse of k_recon1 / k_recon2 for cell i: gathers the MUSCL adjoint of
+// the cell (its own update and the slots of its three edges), then maps it
+// to the w adjoints of the cell and its neighbours and to corner z.
+__global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWork W, DevAdj J) {
+  const int64_t i64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i64 >= M.T) return;
+  const int i = (int)i64;
+  const int64_t T = M.T, Ecnt = M.E;
+  const uint32_t fl = M.cflag[i];
+  double mb[5];
+  for (int q = 0; q < 5; ++q) mb[q] = J.m_cell[q * T + i];
+  double zc[3] = {0., 0., 0.};
+  double wb = 0., bcb = 0.;
+  for (int k = 0; k < 3; ++k) {
+    const int e = M.te[k * T + i];
+    const bool own = sgn_pos(fl, k);
+    const int side = own ? 0 : 1;
+    for (int q = 0; q < 5; ++q) mb[q] += J.m_edge[(q * 2 + side) * Ecnt + e];
+    if (own) {  // the edge's midpoint z and wall depth go to its et0 cell
+      const double ez = J.ez_edge[e];
+      zc[ep0_local(fl, k)] += 0.5 * ez;
+      zc[ep1_local(fl, k)] += 0.5 * ez;
+      const double hf = J.hf_edge[e];
+      wb += hf;
+      bcb -= hf;
+    }
+  }
+  const int cls = W.cls[i];
+  const double w = S.w[i];
+  const V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[T + i]), p2 = node(M, M.tp[2 * T + i]);
+  double wnb[3] = {0., 0., 0.};
+  if (cls == 0) {
+    // ReconstructDryCell: level b_cell, gradient the bathymetry slope
+    bcb += mb[0];
+    for (int k = 0; k < 3; ++k) {
+      V3 q0 = p0, q1 = p1, q2 = p2;
+      q0.z = k == 0 ? 1. : 0.;
+      q1.z = k == 1 ? 1. : 0.;
+      q2.z = k == 2 ? 1. : 0.;
+      double gx, gy;
+      gradient(q0, q1, q2, gx, gy);
+      zc[k] += mb[1] * gx + mb[2] * gy;
+    }
+  } else {
+    J.sbar[T + i] += mb[3];
+    J.sbar[2 * T + i] += mb[4];
+    const double zmax = dmax(dmax(p0.z, p1.z), p2.z);
+    if (cls == 1) {
+      // pass 2 fell back to the pass-1 level (zero gradient) when the
+      // cell is above its highest vertex: the level is w itself
+      if (!(w > zmax) || W.mgx[i] != 0. || W.mgy[i] != 0.) flag(J);
+      wb += mb[0];
+    } else {
+      // ReconstructFullWetCell (src/MUSCLObject.cpp:38-84): w0 = w; the
+      // gradient is the plane through the neighbour values unless a
+      // neighbour is dry or the positivity / TVD limiter zeroed it.  The
+      // decisions are recomputed as k_recon1 takes them.
+      wb += mb[0];
+      const CellRec ri = M.crec[i];
+      V3 pts[3], E[3];
+      int kind[3];
+      bool any_dry = false;
+      for (int k = 0; k < 3; ++k) {
+        const int j = M.tt[k * T + i];  // full-wet cells are interior
+        const CellRec rj = M.crec[j];
+        const double wj = S.w[j];
+        E[k] = edge_mid(p0, p1, p2, fl, k);
+        kind[k] = 0;
+        if (!(rj.fl & CELL_BND) && rj.zmax < wj) {
+          pts[k] = {rj.cx, rj.cy, wj};
+        } else if (!is_wet(wj - rj.bc)) {
+          any_dry = true;
+          pts[k] = {0., 0., 0.};
+        } else {
+          kind[k] = 1;
+          const double w0j = partwet1_w(rj.zmin, rj.zmin, rj.zmax, rj.bc, wj);
+          const double dx = E[k].x - rj.cx, dy = E[k].y - rj.cy;
+          double a = w0j + (0. * dx + 0. * dy);
+          if (!((a - E[k].z) >= 0)) {
+            a = E[k].z;
+            kind[k] = 2;  // clamped to the edge bathymetry
+          } else if (!(wj >= rj.zmax)) {
+            kind[k] = 3;  // cubic / bisection level: not supported
+          }
+          pts[k] = {E[k].x, E[k].y, 0.5 * (w + a)};
+        }
+      }
+      bool active = false;
+      if (!any_dry) {
+        double gx, gy;
+        gradient(pts[0], pts[1], pts[2], gx, gy);
+        const double c23 = 1. - 1. / 3., c13 = -1. / 3.;
+        bool all_wet = true;
+        {
+          double dxm = (p0.x * c23 + p1.x * c13) + p2.x * c13, dym = (p0.y * c23 + p1.y * c13) + p2.y * c13;
+          if (!is_wet(((gx * dxm + gy * dym) + w) - p0.z)) all_wet = false;
+        }
+        {
+          double dxm = (p0.x * c13 + p1.x * c23) + p2.x * c13, dym = (p0.y * c13 + p1.y * c23) + p2.y * c13;
+          if (!is_wet(((gx * dxm + gy * dym) + w) - p1.z)) all_wet = false;
+        }
+        {
+          double dxm = (p0.x * c13 + p1.x * c13) + p2.x * c23, dym = (p0.y * c13 + p1.y * c13) + p2.y * c23;
+          if (!is_wet(((gx * dxm + gy * dym) + w) - p2.z)) all_wet = false;
+        }
+        bool tvd = true;
+        for (int k = 0; k < 3; ++k) {
+          const int j = M.tt[k * T + i];
+          const double wj = S.w[j];
+          double vtmin = dmin(w, wj), vtmax = dmax(w, wj);
+          double vek = w + (gx * (E[k].x - ri.cx) + gy * (E[k].y - ri.cy));
+          if (!((vtmin <= vek) && (vek <= vtmax))) tvd = false;
+        }
+        active = all_wet && tvd;
+      }
+      if (active) {
+        for (int k = 0; k < 3; ++k) {
+          V3 q0 = pts[0], q1 = pts[1], q2 = pts[2];
+          q0.z = k == 0 ? 1. : 0.;
+          q1.z = k == 1 ? 1. : 0.;
+          q2.z = k == 2 ? 1. : 0.;
+          double gx, gy;
+          gradient(q0, q1, q2, gx, gy);
+          const double vb = mb[1] * gx + mb[2] * gy;  // adjoint of the k-th plane value
+          if (kind[k] == 0) {
+            wnb[k] = vb;
+          } else {
+            wb += 0.5 * vb;
+            if (kind[k] == 1) wnb[k] = 0.5 * vb;  // the neighbour's level is its w
+            else if (kind[k] == 2) {
+              zc[ep0_local(fl, k)] += 0.25 * vb;  // a = E.z
+              zc[ep1_local(fl, k)] += 0.25 * vb;
+            } else {
+              flag(J);
+            }
+          }
+        }
+      }
+    }
+  }
+  for (int k = 0; k < 3; ++k) J.w_nb[k * T + i] = wnb[k];
+  J.sbar[i] += wb;
+  const double c = 1. / 3.;
+  for (int k = 0; k < 3; ++k) J.zc[k * T + i] += zc[k] + c * bcb;
+}
+
+// the neighbours' contributions to the w adjoint of cell c
+__global__ void __launch_bounds__(256) k_adj_gather(DevMesh M, DevAdj J) {
+  const int64_t c64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c64 >= M.T) return;
+  const int c = (int)c64;
+  const int64_t T = M.T;
+  double s = 0.;
+  for (int k = 0; k < 3; ++k) {
+    const int j = M.tt[k * T + c];
+    if (j >= 0) s += J.w_nb[back_index(M, j, c) * T + j];
+  }
+  J.sbar[c] += s;
+}
+
+// node bathymetry adjoint: the corners of every incident cell, in the
+// node -> cell map's order
+__global__ void __launch_bounds__(256) k_adj_z(DevMesh M, DevAdj J, double* zbar) {
+  const int64_t n64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (n64 >= M.V) return;
+  const int n = (int)n64;
+  const int64_t T = M.T;
+  double s = 0.;
+  for (int q = M.n2c_off[n]; q < M.n2c_off[n + 1]; ++q) {
+    const int c = M.n2c[q];
+    const int k = M.tp[c] == n ? 0 : (M.tp[T + c] == n ? 1 : 2);
+    s += J.zc[k * T + c];
+  }
+  zbar[n] += s;
+}
+
+__global__ void __launch_bounds__(256) k_axpy(const double* x, double* y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) y[i] += x[i];
+}
+
+static inline dim3 grid_plain(int64_t n) {
+  int64_t nb = (n + kBlock - 1) / kBlock;
+  return dim3((unsigned)(nb > 0 ? nb : 1));
+}
+
+hipError_t launch_adjoint_stage(const DevMesh& M, const DevState& S, const DevState& U0, const DevWork& W,
+                                const StageArgs& A, const DevAdj& J, int flux, int ws, int opts,
+                                double* zbar, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_update, grid_plain(M.T), dim3(256), 0, st, M, S, U0, W, A, J);
+  if (opts & kOptDrainLength) hipLaunchKernelGGL(k_adj_drain<true>, grid_plain(M.T), dim3(256), 0, st, M, S, W, J);
+  else hipLaunchKernelGGL(k_adj_drain<false>, grid_plain(M.T), dim3(256), 0, st, M, S, W, J);
+#define SWE_ADJ_FLUX(F, Wv)                                                                      \
+  if (flux == F && ws == Wv)                                                                     \
+    hipLaunchKernelGGL((k_adj_flux<F, Wv>), grid_plain(M.E), dim3(256), 0, st, M, S, W, J, A.cor);
+  SWE_ADJ_FLUX(0, 0) SWE_ADJ_FLUX(0, 1) SWE_ADJ_FLUX(0, 2)
+  SWE_ADJ_FLUX(1, 0) SWE_ADJ_FLUX(1, 1) SWE_ADJ_FLUX(1, 2)
+#undef SWE_ADJ_FLUX
+  hipLaunchKernelGGL(k_adj_recon, grid_plain(M.T), dim3(256), 0, st, M, S, W, J);
+  hipLaunchKernelGGL(k_adj_gather, grid_plain(M.T), dim3(256), 0, st, M, J);
+  if (zbar) hipLaunchKernelGGL(k_adj_z, grid_plain(M.V), dim3(256), 0, st, M, J, zbar);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpy(const double* x, double* y, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy, grid_plain(n), dim3(256), 0, st, x, y, n);
+  return hipGetLastError();
+}
+
+}  // namespace swe

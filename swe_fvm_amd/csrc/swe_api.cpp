@@ -23,6 +23,7 @@ struct swe_solver {
   int64_t T = 0, E = 0, V = 0;
   std::vector<void*> allocs;
   DevMesh M{};
+  DevGeomOut G{};  // writable static geometry (swe_set_bathymetry)
   double* sw[3] = {nullptr, nullptr, nullptr};
   double* su[3] = {nullptr, nullptr, nullptr};
   double* sv[3] = {nullptr, nullptr, nullptr};
@@ -261,6 +262,7 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
   s->M.c1 = T;
   s->M.e0 = 0;
   s->M.e1 = E;
+  s->G = G;
   if ((e = launch_setup_geometry(s->M, G, s->stream)) != hipSuccess) return bail(e, "setup_geometry");
   if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) return bail(e, "hipStreamSynchronize");
   if (n_inst > 0) {
@@ -927,6 +929,247 @@ int swe_state_stats(swe_solver* s, double* out5) {
   hipFree(d);
   if (e == hipSuccess) e = e2;
   if (e != hipSuccess) return dev_fail(e, "state_stats");
+  return SWE_OK;
+}
+
+// ------------------------------------------------------------- adjoint --
+
+namespace {
+
+// a state held in one 3T allocation (w, u, v)
+struct Buf3 {
+  double* p = nullptr;
+};
+DevState st3(const Buf3& b, int64_t T) { return DevState{b.p, b.p + T, b.p + 2 * T}; }
+DevOut out3(const Buf3& b, int64_t T) { return DevOut{b.p, b.p + T, b.p + 2 * T}; }
+
+hipError_t copy_state(const DevState& from, const DevOut& to, int64_t T, hipStream_t st) {
+  const size_t n = sizeof(double) * size_t(T);
+  hipError_t e = hipMemcpyAsync(to.w, from.w, n, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(to.u, from.u, n, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(to.v, from.v, n, hipMemcpyDeviceToDevice, st);
+  return e;
+}
+
+// Shu-Osher stages of a scheme: (a0, a1, coef); stage q reads the previous
+// stage's output (the step input for q = 0) and the step input as U0
+struct StageCoef {
+  double a0, a1, coef;
+};
+int stage_list(int scheme, StageCoef* out) {
+  if (scheme == SWE_SCHEME_EULER) { out[0] = {0., 0., 1.}; return 1; }
+  if (scheme == SWE_SCHEME_SSPRK2) { out[0] = {0., 0., 1.}; out[1] = {0.5, 0.5, 0.5}; return 2; }
+  out[0] = {0., 0., 1.};
+  out[1] = {0.75, 0.25, 0.25};
+  out[2] = {1. / 3., 2. / 3., 2. / 3.};
+  return 3;
+}
+
+hipError_t intermediates(swe_solver* s, const DevState& S) {
+  hipError_t e = launch_recon1(s->M, S, s->W, s->opts, s->stream);
+  if (e == hipSuccess) e = launch_recon2(s->M, S, s->W, s->opts, s->stream);
+  if (e == hipSuccess)
+    e = launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + 3, s->flux_items, s->stream);
+  if (e == hipSuccess) e = launch_drain(s->M, S, s->W, s->opts, s->stream);
+  return e;
+}
+
+hipError_t stage_forward(swe_solver* s, const DevState& S, const DevState& U0, const DevOut& O,
+                         const StageCoef& c, double dt) {
+  hipError_t e = intermediates(s, S);
+  StageArgs A{c.a0, c.a1, c.coef, dt, 0., s->cor, s->rbits, nullptr, friction(s)};
+  if (e == hipSuccess) e = launch_update(s->M, S, U0, O, s->W, A, s->upd_items, s->stream);
+  return e;
+}
+
+}  // namespace
+
+// Adjoint of nsteps steps (no reference counterpart; the north star's
+// differentiation of the full step).  Starting from the current state U_0,
+// runs the steps forward with step sizes dt (> 0, every step) or dts[n]
+// (e.g. a CFL run's sizes, then held fixed), storing U_n every
+// checkpoint_every steps; leaves U_N as the current state.  Then propagates
+// final_bar (T x 3 row-major adjoint of U_N's w, u, v) back through every
+// stage, recomputing each segment between checkpoints, into state0_bar
+// (T x 3, adjoint of U_0) and z_bar (V, adjoint of the node bathymetry, may
+// be null).  ms (optional, 2 doubles): forward sweep and reverse sweep
+// milliseconds.
+int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int64_t nsteps,
+                    int64_t checkpoint_every, const double* final_bar, double* state0_bar, double* z_bar,
+                    double* ms) {
+  if (int rc = single_mesh(s)) return rc;
+  if (!final_bar || !state0_bar) return fail(SWE_ERR_ARG, "null argument");
+  if (scheme < 0 || scheme > 2) return fail(SWE_ERR_ARG, "unknown scheme");
+  if (nsteps < 1) return fail(SWE_ERR_ARG, "nsteps must be >= 1");
+  if (!dts && !(dt > 0.)) return fail(SWE_ERR_ARG, "the adjoint needs fixed step sizes (dt > 0 or dts)");
+  if (dts)
+    for (int64_t n = 0; n < nsteps; ++n)
+      if (!(dts[n] > 0.)) return fail(SWE_ERR_ARG, "step sizes must be positive");
+  if (s->opts & (SWE_OPT_HEAD_GRADIENT | SWE_OPT_HEAD_PARTWET2))
+    return fail(SWE_ERR_ARG, "the adjoint does not support the HEAD-compatibility options");
+  const int64_t K = checkpoint_every > 0 ? std::min(checkpoint_every, nsteps) : nsteps;
+  if (int rc = set_device(s)) return rc;
+  const int64_t T = s->T, E = s->E, V = s->V;
+  StageCoef sc[3];
+  const int ns = stage_list(scheme, sc);
+  const int64_t nck = (nsteps + K - 1) / K;
+  std::vector<void*> mem;
+  auto alloc = [&](double** p, int64_t n) {
+    hipError_t e = hipMalloc((void**)p, sizeof(double) * size_t(std::max<int64_t>(n, 1)));
+    if (e == hipSuccess) mem.push_back(*p);
+    return e;
+  };
+  auto release = [&]() {
+    hipStreamSynchronize(s->stream);
+    for (void* p : mem) hipFree(p);
+  };
+  double *ck = nullptr, *seg = nullptr, *stg = nullptr, *bar_a = nullptr, *bar_b = nullptr, *bar_u0 = nullptr;
+  double *zb = nullptr, *work = nullptr;
+  unsigned int* flags = nullptr;
+  hipError_t e = alloc(&ck, nck * 3 * T);
+  if (e == hipSuccess) e = alloc(&seg, K * 3 * T);
+  if (e == hipSuccess) e = alloc(&stg, K * (ns - 1 > 0 ? ns - 1 : 1) * 3 * T);
+  if (e == hipSuccess) e = alloc(&bar_a, 3 * T);
+  if (e == hipSuccess) e = alloc(&bar_b, 3 * T);
+  if (e == hipSuccess) e = alloc(&bar_u0, 3 * T);
+  if (e == hipSuccess) e = alloc(&zb, V);
+  // DevAdj work: fs 9T, dtd_own T, dtd_nb 3T, m_cell 5T, m_edge 10E, ez E,
+  // hf E, w_nb 3T, zc 3T, u0bar 3T
+  const int64_t nwork = 9 * T + T + 3 * T + 5 * T + 10 * E + E + E + 3 * T + 3 * T + 3 * T;
+  if (e == hipSuccess) e = alloc(&work, nwork);
+  if (e == hipSuccess) {
+    e = hipMalloc((void**)&flags, sizeof(unsigned int));
+    if (e == hipSuccess) mem.push_back(flags);
+  }
+  if (e != hipSuccess) { release(); return dev_fail(e, "hipMalloc (adjoint)"); }
+  DevAdj J{};
+  {
+    double* q = work;
+    J.fs = q; q += 9 * T;
+    J.dtd_own = q; q += T;
+    J.dtd_nb = q; q += 3 * T;
+    J.m_cell = q; q += 5 * T;
+    J.m_edge = q; q += 10 * E;
+    J.ez_edge = q; q += E;
+    J.hf_edge = q; q += E;
+    J.w_nb = q; q += 3 * T;
+    J.zc = q; q += 3 * T;
+    J.u0bar = q; q += 3 * T;
+    J.flags = flags;
+  }
+  auto dtn = [&](int64_t n) { return dts ? dts[n] : dt; };
+  hipEvent_t ev[3];
+  for (auto& x : ev) hipEventCreate(&x);
+  // forward sweep with checkpoints
+  hipEventRecord(ev[0], s->stream);
+  int rc = SWE_OK;
+  for (int64_t n = 0; n < nsteps && !rc; ++n) {
+    if (n % K == 0) {
+      Buf3 b{ck + (n / K) * 3 * T};
+      if ((e = copy_state(state_of(s, s->cur), out3(b, T), T, s->stream)) != hipSuccess) rc = dev_fail(e, "copy");
+    }
+    if (!rc) rc = run_step(s, scheme, dtn(n), 0., nullptr);
+  }
+  hipEventRecord(ev[1], s->stream);
+  // reverse sweep
+  if (!rc) {
+    std::vector<double> h(static_cast<size_t>(3 * T));
+    for (int64_t i = 0; i < T; ++i)
+      for (int q = 0; q < 3; ++q) h[size_t(q * T + i)] = final_bar[3 * i + q];
+    if ((e = hipMemcpyAsync(bar_a, h.data(), sizeof(double) * size_t(3 * T), hipMemcpyHostToDevice, s->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(zb, 0, sizeof(double) * size_t(std::max<int64_t>(V, 1)), s->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(flags, 0, sizeof(unsigned int), s->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(s->stream)) != hipSuccess)
+      rc = dev_fail(e, "adjoint setup");
+  }
+  for (int64_t g = nck - 1; g >= 0 && !rc; --g) {
+    const int64_t n0 = g * K, n1 = std::min(nsteps, n0 + K);
+    // recompute the segment: step inputs U_n and the stage outputs
+    e = copy_state(st3(Buf3{ck + g * 3 * T}, T), out3(Buf3{seg}, T), T, s->stream);
+    for (int64_t n = n0; n < n1 && e == hipSuccess; ++n) {
+      const int64_t m = n - n0;
+      const DevState Un = st3(Buf3{seg + m * 3 * T}, T);
+      DevState Sq = Un;
+      for (int q = 0; q < ns && e == hipSuccess; ++q) {
+        DevOut O = (q == ns - 1) ? (n + 1 < n1 ? out3(Buf3{seg + (m + 1) * 3 * T}, T) : DevOut{nullptr, nullptr, nullptr})
+                                 : out3(Buf3{stg + (m * (ns - 1) + q) * 3 * T}, T);
+        if (!O.w) break;  // the segment's last step output is not needed
+        e = stage_forward(s, Sq, Un, O, sc[q], dtn(n));
+        Sq = DevState{O.w, O.u, O.v};
+      }
+    }
+    if (e != hipSuccess) { rc = dev_fail(e, "adjoint recomputation"); break; }
+    // reverse the steps of the segment
+    for (int64_t n = n1 - 1; n >= n0 && !rc; --n) {
+      const int64_t m = n - n0;
+      const DevState Un = st3(Buf3{seg + m * 3 * T}, T);
+      // bar_a: adjoint of the step output; bar_u0 accumulates the U_n adjoint
+      e = hipMemsetAsync(bar_u0, 0, sizeof(double) * size_t(3 * T), s->stream);
+      double* obar = bar_a;
+      double* sbar = bar_b;
+      for (int q = ns - 1; q >= 0 && e == hipSuccess; --q) {
+        const DevState Sq = q == 0 ? Un : st3(Buf3{stg + (m * (ns - 1) + q - 1) * 3 * T}, T);
+        e = intermediates(s, Sq);
+        J.obar = obar;
+        J.sbar = sbar;
+        StageArgs A{sc[q].a0, sc[q].a1, sc[q].coef, dtn(n), 0., s->cor, s->rbits, nullptr, friction(s)};
+        if (e == hipSuccess)
+          e = launch_adjoint_stage(s->M, Sq, Un, s->W, A, J, s->flux, s->ws, s->opts, z_bar ? zb : nullptr,
+                                   s->stream);
+        if (e == hipSuccess && sc[q].a0 != 0.) e = launch_axpy(J.u0bar, bar_u0, 3 * T, s->stream);
+        std::swap(obar, sbar);
+      }
+      // obar now holds the stage-0 input adjoint (= part of the U_n adjoint)
+      if (e == hipSuccess) e = launch_axpy(bar_u0, obar, 3 * T, s->stream);
+      if (e == hipSuccess && obar != bar_a)
+        e = hipMemcpyAsync(bar_a, obar, sizeof(double) * size_t(3 * T), hipMemcpyDeviceToDevice, s->stream);
+      if (e != hipSuccess) rc = dev_fail(e, "adjoint stage");
+    }
+  }
+  hipEventRecord(ev[2], s->stream);
+  unsigned int nflag = 0;
+  if (!rc) {
+    std::vector<double> h(static_cast<size_t>(3 * T));
+    if ((e = hipMemcpyAsync(h.data(), bar_a, sizeof(double) * size_t(3 * T), hipMemcpyDeviceToHost, s->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(&nflag, flags, sizeof(unsigned int), hipMemcpyDeviceToHost, s->stream)) != hipSuccess ||
+        (z_bar && (e = hipMemcpyAsync(z_bar, zb, sizeof(double) * size_t(V), hipMemcpyDeviceToHost, s->stream)) != hipSuccess) ||
+        (e = hipStreamSynchronize(s->stream)) != hipSuccess) {
+      rc = dev_fail(e, "adjoint download");
+    } else {
+      for (int64_t i = 0; i < T; ++i)
+        for (int q = 0; q < 3; ++q) state0_bar[3 * i + q] = h[size_t(q * T + i)];
+      if (ms) {
+        float f0 = 0.f, f1 = 0.f;
+        hipEventElapsedTime(&f0, ev[0], ev[1]);
+        hipEventElapsedTime(&f1, ev[1], ev[2]);
+        ms[0] = f0;
+        ms[1] = f1;
+      }
+    }
+  }
+  for (auto& x : ev) hipEventDestroy(x);
+  release();
+  if (!rc && nflag)
+    return fail(SWE_ERR_SOLVER, "adjoint: " + std::to_string(nflag) +
+                                    " cell stages outside the supported regime (partially wet cells below "
+                                    "their highest vertex)");
+  return rc;
+}
+
+// Replaces the node bathymetry (V values) and recomputes the static
+// geometry; the state (w, u, v) is kept.
+int swe_set_bathymetry(swe_solver* s, const double* z) {
+  if (!s || !z) return fail(SWE_ERR_ARG, "null argument");
+  if (int rc = set_device(s)) return rc;
+  SWE_CHECK(hipMemcpyAsync(const_cast<double*>(s->M.pz), z, sizeof(double) * size_t(s->V),
+                           hipMemcpyHostToDevice, s->stream), "hipMemcpy");
+  if (s->graph) {
+    hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+    s->graph_scheme = -1;
+  }
+  SWE_CHECK(launch_setup_geometry(s->M, s->G, s->stream), "setup_geometry");
+  SWE_CHECK(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
   return SWE_OK;
 }
 

@@ -1,0 +1,84 @@
+// Helpers shared by the step kernels (swe_kernels.hip) and the adjoint
+// kernels (swe_adjoint.hip): launch geometry, per-cell flag decoding, node
+// access, the final per-cell MUSCL and the edge slot it induces.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "swe_device.h"
+#include "swe_launch.h"
+
+namespace swe {
+
+#define CELL_BND (1u << 3)
+// boundary tags (include/Includes.h:21): walls (src/SpaceDisc.cpp:66-69) and
+// free flow, whose outside state is the inside edge state (transmissive)
+constexpr int kSolidWall = -1;
+constexpr int kOptDrainLength = 32;  // SWE_OPT_DRAIN_LENGTH
+
+__device__ __forceinline__ int logical_block() {
+  // Workgroups are dispatched round-robin over the 8 XCDs; hand each XCD a
+  // contiguous range of the Morton-ordered cells so neighbour gathers hit
+  // that XCD's L2.  gridDim.x is a multiple of 8.
+  int per = gridDim.x >> 3;
+  int b = blockIdx.x;
+  return (b & 7) * per + (b >> 3);
+}
+constexpr int kBlock = 256;  // every launch uses 256-thread workgroups
+__device__ __forceinline__ int64_t global_index() {
+  return (int64_t)logical_block() * kBlock + threadIdx.x;
+}
+
+__device__ __forceinline__ V3 node(const DevMesh& M, int n) {
+  return {M.px[n], M.py[n], M.pz[n]};
+}
+__device__ __forceinline__ int ep0_local(uint32_t fl, int k) { return (fl >> (4 + 4 * k)) & 3; }
+__device__ __forceinline__ int ep1_local(uint32_t fl, int k) { return (fl >> (6 + 4 * k)) & 3; }
+__device__ __forceinline__ bool sgn_pos(uint32_t fl, int k) { return (fl >> k) & 1u; }
+__device__ __forceinline__ V3 pick(const V3& a, const V3& b, const V3& c, int i) {
+  return i == 0 ? a : (i == 1 ? b : c);
+}
+// midpoint of edge k from the cell's own vertices
+__device__ __forceinline__ V3 edge_mid(const V3& p0, const V3& p1, const V3& p2, uint32_t fl, int k) {
+  return midpoint(pick(p0, p1, p2, ep0_local(fl, k)), pick(p0, p1, p2, ep1_local(fl, k)));
+}
+
+// Final MUSCL of cell c (pass 2 has overwritten the partially wet cells in
+// place); every load is issued unconditionally so they overlap.
+struct Muscl {
+  double w0, gx, gy, u0, v0;
+};
+__device__ __forceinline__ Muscl load_muscl(const DevState& S, const DevWork& W, int c) {
+  const int k = W.cls[c];
+  const double w0 = W.mw[c], gx = W.mgx[c], gy = W.mgy[c];
+  const double u = S.u[c], v = S.v[c];
+  Muscl m{w0, gx, gy, u, v};
+  if (k == 0) {
+    m.u0 = 0.; m.v0 = 0.;
+  }
+  return m;
+}
+
+// AtPoint + PrimAssigner: the edge slot written by UpdateInterfaceValues
+// (src/SpaceDisc.cpp:24-25); (cx, cy) is the cell centroid.
+__device__ __forceinline__ Slot slot_value(const Muscl& m, double cx, double cy, const V3& E) {
+  double dx = E.x - cx, dy = E.y - cy;
+  double aw = m.w0 + (m.gx * dx + m.gy * dy);
+  bool wet = (aw - E.z) >= 0;
+  Slot s;
+  if (!wet) aw = E.z;
+  prim_assign(E.z, aw, wet ? m.u0 : 0., wet ? m.v0 : 0., s.w, s.u, s.v);
+  return s;
+}
+
+// grid of 256-thread workgroups covering n items, a multiple of 8 (one
+// contiguous range per XCD, see logical_block)
+inline dim3 grid_for(int64_t n) {
+  int64_t nb = (n + 255) / 256;
+  nb = ((nb + 7) / 8) * 8;
+  if (nb < 8) nb = 8;
+  return dim3((unsigned)nb);
+}
+
+}  // namespace swe

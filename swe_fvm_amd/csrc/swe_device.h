@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "swe_dual.h"
+
 namespace swe {
 
 constexpr double kTol = 1e-13;  // include/Includes.h:30
@@ -167,12 +169,17 @@ __device__ __forceinline__ void cons_assign(double b, double h, double hu, doubl
   ov = hv * ih;
 }
 
+// The flux functions are templates on the scalar type T of the states
+// (double in the step; Dual<N> in the adjoint, for the local Jacobian of the
+// very same operation sequence); geometry stays double.
+
 // ElemFlux (include/SpaceDisc.h:4-18)
-__device__ __forceinline__ void elem_flux(double nx, double ny, double h, double hu, double hv,
-                                          double& f0, double& f1, double& f2) {
-  f0 = 0.; f1 = 0.; f2 = 0.;
+template <class T>
+__device__ __forceinline__ void elem_flux(double nx, double ny, const T& h, const T& hu, const T& hv,
+                                          T& f0, T& f1, T& f2) {
+  f0 = Zero<T>::get(); f1 = f0; f2 = f0;
   if (is_wet(h)) {
-    double hveln = hu * nx + hv * ny;
+    T hveln = hu * nx + hv * ny;
     f0 = hveln;
     f1 = (hveln / h) * hu + (0.5 * h * h) * nx;
     f2 = (hveln / h) * hv + (0.5 * h * h) * ny;
@@ -180,95 +187,97 @@ __device__ __forceinline__ void elem_flux(double nx, double ny, double h, double
 }
 
 // Wavespeeds (src/Fluxes.cpp:5-26)
-template <int WS>
-__device__ __forceinline__ void wavespeed(double ul, double hl, double ur, double hr, double& a0,
-                                          double& a1) {
-  double cl = sqrt(hl), cr = sqrt(hr);
+template <int WS, class T>
+__device__ __forceinline__ void wavespeed(const T& ul, const T& hl, const T& ur, const T& hr, T& a0,
+                                          T& a1) {
+  T cl = sqrt(hl), cr = sqrt(hr);
   if (WS == 0) {  // Rusanov
-    double ap = dmax(fabs(ul) + cl, fabs(ur) + cr);
+    T ap = dmax(fabs(ul) + cl, fabs(ur) + cr);
     a0 = -ap; a1 = ap;
   } else if (WS == 1) {  // Davis
     a0 = dmin(ul - cl, ur - cr); a1 = dmax(ul + cl, ur + cr);
   } else {  // Einfeldt, Roe velocity as in the reference
-    double uRoe = (cl * ul + cl * ur) / (cl + cr);
-    double cRoe = sqrt(0.5 * (hl + hr));
+    T uRoe = (cl * ul + cl * ur) / (cl + cr);
+    T cRoe = sqrt(0.5 * (hl + hr));
     a0 = dmin(ul - cl, uRoe - cRoe); a1 = dmax(ur + cr, uRoe + cRoe);
   }
 }
 
 // Edge state seen by the flux: primitive slot (w, u, v) and edge bathymetry.
-struct Slot {
-  double w, u, v;
+template <class T>
+struct SlotT {
+  T w, u, v;
 };
+using Slot = SlotT<double>;
 
 // Fluxes::HLL<W> (include/Fluxes.h:14-54).  lw: length-to-wavespeed
 // candidate for the CFL minimum (unchanged when the flux returns early);
 // dmn = min(2 Area(from) / L, 2 Area(to) / L).
-template <int WS>
-__device__ __forceinline__ void flux_hll(double nx, double ny, double be, const Slot& L,
-                                         const Slot& R, double dmn, double cor, double& lw,
-                                         double F[3]) {
-  double ul = L.u * nx + L.v * ny;
-  double hl = L.w - be;
-  double ur = R.u * nx + R.v * ny;
-  double hr = R.w - be;
-  F[0] = F[1] = F[2] = 0.;
+template <int WS, class T>
+__device__ __forceinline__ void flux_hll(double nx, double ny, const T& be, const SlotT<T>& L,
+                                         const SlotT<T>& R, double dmn, double cor, double& lw,
+                                         T F[3]) {
+  T ul = L.u * nx + L.v * ny;
+  T hl = L.w - be;
+  T ur = R.u * nx + R.v * ny;
+  T hr = R.w - be;
+  F[0] = Zero<T>::get(); F[1] = F[0]; F[2] = F[0];
   if (hl + hr <= 1e-10) return;
-  double a0, a1;
+  T a0, a1;
   wavespeed<WS>(ul, hl, ur, hr, a0, a1);
-  double al = dmin(0., a0), ar = dmax(0., a1);
+  T al = dmin(0., a0), ar = dmax(0., a1);
   if (ar - al <= 1e-10) return;
   double c = fabs(cor);
-  lw = dmn / (c + dmax(-al, ar));
-  double Ul0 = hl, Ul1 = hl * L.u, Ul2 = hl * L.v;
-  double Ur0 = hr, Ur1 = hr * R.u, Ur2 = hr * R.v;
-  double Fl0, Fl1, Fl2, Fr0, Fr1, Fr2;
+  lw = dmn / (c + dmax(-val(al), val(ar)));
+  T Ul0 = hl, Ul1 = hl * L.u, Ul2 = hl * L.v;
+  T Ur0 = hr, Ur1 = hr * R.u, Ur2 = hr * R.v;
+  T Fl0, Fl1, Fl2, Fr0, Fr1, Fr2;
   elem_flux(nx, ny, Ul0, Ul1, Ul2, Fl0, Fl1, Fl2);
   elem_flux(nx, ny, Ur0, Ur1, Ur2, Fr0, Fr1, Fr2);
-  double aa = al * ar, den = ar - al;
+  T aa = al * ar, den = ar - al;
   F[0] = (ar * Fl0 - al * Fr0 + aa * (Ur0 - Ul0)) / den;
   F[1] = (ar * Fl1 - al * Fr1 + aa * (Ur1 - Ul1)) / den;
   F[2] = (ar * Fl2 - al * Fr2 + aa * (Ur2 - Ul2)) / den;
 }
 
 // Fluxes::HLLC<W> (include/Fluxes.h:56-111)
-template <int WS>
-__device__ __forceinline__ void flux_hllc(double tx, double ty, double be, const Slot& L,
-                                          const Slot& R, double dmn, double cor, double& lw,
-                                          double F[3]) {
+template <int WS, class T>
+__device__ __forceinline__ void flux_hllc(double tx, double ty, const T& be, const SlotT<T>& L,
+                                          const SlotT<T>& R, double dmn, double cor, double& lw,
+                                          T F[3]) {
   double nx = ty, ny = -tx;
-  double ul = L.u * nx + L.v * ny;
-  double vl = L.u * tx + L.v * ty;
-  double hl = L.w - be;
-  double ur = R.u * nx + R.v * ny;
-  double vr = R.u * tx + R.v * ty;
-  double hr = R.w - be;
-  F[0] = F[1] = F[2] = 0.;
+  T ul = L.u * nx + L.v * ny;
+  T vl = L.u * tx + L.v * ty;
+  T hl = L.w - be;
+  T ur = R.u * nx + R.v * ny;
+  T vr = R.u * tx + R.v * ty;
+  T hr = R.w - be;
+  F[0] = Zero<T>::get(); F[1] = F[0]; F[2] = F[0];
   if (hl + hr <= 1e-10) return;
-  double al, ar;
+  T al, ar;
   wavespeed<WS>(ul, hl, ur, hr, al, ar);
-  double ustar = (ar - ur) * hr * ur - (al - ul) * hl * ul + 0.5 * (hl * hl - hr * hr);
-  ustar /= (hr * (ar - ur) - hl * (al - ul));
+  T ustar = (ar - ur) * hr * ur - (al - ul) * hl * ul + 0.5 * (hl * hl - hr * hr);
+  ustar = ustar / (hr * (ar - ur) - hl * (al - ul));
   double c = fabs(cor);
-  lw = dmn / (c + dmax(kTol, dmax(al, ar)));
-  double U0, U1, U2, Fe0, Fe1, Fe2;
+  lw = dmn / (c + dmax(kTol, dmax(val(al), val(ar))));
+  T U0, U1, U2, Fe0, Fe1, Fe2;
   if (ustar <= 0) {
-    double us = vr * tx + ustar * ty;
-    double vs = vr * nx + ustar * ny;
-    double hs = hr * (ar - ur) / (ar - ustar);
+    T us = vr * tx + ustar * ty;
+    T vs = vr * nx + ustar * ny;
+    T hs = hr * (ar - ur) / (ar - ustar);
     U0 = hr; U1 = hr * R.u; U2 = hr * R.v;
     elem_flux(nx, ny, U0, U1, U2, Fe0, Fe1, Fe2);
-    double s = dmax(0., ar);
+    T s = dmax(0., ar);
     F[0] = Fe0 + s * (hs - U0);
     F[1] = Fe1 + s * (hs * us - U1);
     F[2] = Fe2 + s * (hs * vs - U2);
   } else {
-    double us = vl * tx + ustar * ty;
-    double vs = vl * nx + ustar * ny;
-    double hs = hl * (al - ul) / (al - ustar);
+    T us = vl * tx + ustar * ty;
+    T vs = vl * nx + ustar * ny;
+    T hs = hl * (al - ul) / (al - ustar);
     U0 = hl; U1 = hl * L.u; U2 = hl * L.v;
     elem_flux(nx, ny, U0, U1, U2, Fe0, Fe1, Fe2);
-    double s = dmin(0., al);
+    T s = dmin(0., al);
     F[0] = Fe0 + s * (hs - U0);
     F[1] = Fe1 + s * (hs * us - U1);
     F[2] = Fe2 + s * (hs * vs - U2);

@@ -111,6 +111,25 @@ struct DevWork {
   unsigned int* pw_count;  // their number
 };
 
+// adjoint work of one stage (reverse kernels, swe_adjoint.hip); layouts:
+// per cell side [q][k][T] (q component, k local edge), per edge side
+// [q][side][E], per cell [q][T]
+struct DevAdj {
+  const double* obar;  // [3][T] adjoint of the stage output (w, u, v)
+  double* sbar;        // [3][T] adjoint of the stage input S
+  double* u0bar;       // [3][T] adjoint of the step's first state U0
+  double* fs;          // [3][3][T] flux adjoint, per cell side
+  double* dtd_own;     // [T] draining-dt adjoint from the cell's own update
+  double* dtd_nb;      // [3][T] ... of neighbour k, from this cell's update
+  double* m_cell;      // [5][T] MUSCL (w0, gx, gy, u0, v0) adjoint, own update
+  double* m_edge;      // [5][2][E] MUSCL adjoint, per edge side (0: et0, 1: et1)
+  double* ez_edge;     // [E] edge-midpoint z adjoint
+  double* hf_edge;     // [E] wall edges: adjoint of the et0 cell's depth
+  double* w_nb;        // [3][T] w adjoint of neighbour k from this cell's plane
+  double* zc;          // [3][T] node-z adjoint per cell corner
+  unsigned int* flags; // states outside the supported regime
+};
+
 struct StageArgs {
   double a0, a1;       // Shu-Osher weights of U0 and the stage state (a0 = 0: U + RHS)
   double coef;         // stage dt = coef * dt
@@ -152,6 +171,13 @@ hipError_t launch_integrals(const DevMesh& M, const DevState& S, const DevWork& 
 // sqrt(h)): partial needs 5 * integrals_blocks(M) doubles, out 5
 hipError_t launch_stats(const DevMesh& M, const DevState& S, double* partial, double* out,
                         hipStream_t st);
+// reverse of one stage: from J.obar to J.sbar and J.u0bar, adding the node
+// bathymetry adjoint to zbar (if not null); the forward intermediates of the
+// stage state S must be in W (recon1, recon2, flux, drain already run)
+hipError_t launch_adjoint_stage(const DevMesh& M, const DevState& S, const DevState& U0, const DevWork& W,
+                                const StageArgs& A, const DevAdj& J, int flux, int ws, int opts,
+                                double* zbar, hipStream_t st);
+hipError_t launch_axpy(const double* x, double* y, int64_t n, hipStream_t st);  // y += x
 hipError_t launch_rhs_one(const DevMesh& M, const DevState& S, const DevWork& W, int i, double dt,
                           double cor, double fric, double* out, hipStream_t st);
 

@@ -1,0 +1,140 @@
+"""The adjoint of the step (swe_adjoint_run) against central finite
+differences of the GPU forward (itself bit-identical to the oracle), in
+random directions of the initial state and of the node bathymetry; the
+checkpoint interval does not change the result; unsupported states are
+refused; the torch.autograd.Function gives the same gradients.  (The full
+Jacobians of Euler, SSP-RK2 and SSP-RK3 steps on a 72-cell mesh, state and
+bathymetry columns, were also compared entry by entry with central
+differences of the oracle at 1e-9: agreement to 4e-7, the differences' own
+error.)  Finite differences must not cross the limiter's all-or-nothing
+decisions: the perturbations are ~1e-8 against level differences ~1e-3."""
+import numpy as np
+import pytest
+
+import swe_fvm_amd as swe
+from swe_fvm_amd import adjoint as AD
+
+pytestmark = pytest.mark.gpu
+
+
+def wet_case(n=16, seed=3, amp=0.3):
+    """Deep water over bumps, every cell wet above its highest vertex."""
+    xyz, topo = swe.jittered_mesh(n, 0.25, seed)
+    rng = np.random.default_rng(seed)
+    x, y = xyz[:, 0], xyz[:, 1]
+    z = np.zeros_like(x)
+    for _ in range(4):
+        cx, cy = rng.uniform(0.2, 0.8, 2)
+        z += amp * rng.uniform(0.3, 1.0) * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * 0.1 ** 2))
+    xyz[:, 2] = z
+    dom = swe.Domain(xyz, topo)
+    c = dom.centroids()
+    # a tilt keeps neighbouring levels apart, so the finite differences
+    # below do not cross the limiter's all-or-nothing decisions
+    w = 1.0 + 0.03 * c[:, 0] + 0.05 * np.exp(-((c[:, 0] - 0.4) ** 2 + (c[:, 1] - 0.5) ** 2) / 0.02)
+    u = 0.1 * np.sin(2 * c[:, 1])
+    v = -0.05 * np.cos(3 * c[:, 0])
+    return dom, np.stack([w, u, v], 1)
+
+
+def directional_fd(sd, prim0, z, nsteps, fn, dp, dz, scheme, dt, eps):
+    """Central difference at eps and at eps / 4; None when they disagree,
+    i.e. when the perturbation crossed a limiter or wave-speed switch (the
+    step is piecewise smooth) and the difference measures the jump."""
+    def J(p, zz):
+        sd.set_bathymetry(zz)
+        sd.SetVolField(p)
+        sd.run(scheme, nsteps, dt=dt)
+        return fn(sd.GetVolField())[0]
+
+    def cd(e):
+        return (J(prim0 + e * dp, z + e * dz) - J(prim0 - e * dp, z - e * dz)) / (2 * e)
+    a, b = cd(eps), cd(eps / 4)
+    sd.set_bathymetry(z)
+    return a if abs(a - b) <= 1e-5 * abs(a) + 1e-12 else None
+
+
+@pytest.mark.parametrize("scheme,nsteps,flux", [("Euler", 1, "HLLC"), ("SSPRK2", 12, "HLLC"),
+                                                ("SSPRK3", 6, "HLL"), ("SSPRK2", 12, "HLL")])
+def test_adjoint_matches_finite_differences(scheme, nsteps, flux):
+    dom, prim0 = wet_case()
+    fl = getattr(swe.Fluxes, flux)[swe.Wavespeeds.Einfeldt]
+    sd = swe.SpaceDisc(fl, dom, prim0, cor=0.3)
+    z = dom.geometry[:, 2].copy()
+    rng = np.random.default_rng(7)
+    cvec = rng.standard_normal(prim0.shape)
+
+    def fn(p):  # a linear functional of the whole final state
+        return float(np.sum(cvec * p)), cvec
+
+    dt = 2e-3
+    J, s0, zb, ms = AD.run_and_adjoint(sd, prim0, z, nsteps, fn, scheme=scheme, dt=dt, checkpoint_every=5)
+    checked = 0
+    for trial in range(4):
+        dp = rng.standard_normal(prim0.shape) * np.array([1e-2, 1e-2, 1e-2])
+        dz = rng.standard_normal(z.shape) * 1e-2 if trial > 0 else np.zeros_like(z)
+        fd = directional_fd(sd, prim0, z, nsteps, fn, dp, dz, scheme, dt, 1e-6)
+        ad = float(np.sum(s0 * dp) + np.sum(zb * dz))
+        if fd is None:
+            print(f"{scheme} {flux} x{nsteps} trial {trial}: the difference crossed a switch", flush=True)
+            continue
+        print(f"{scheme} {flux} x{nsteps} trial {trial}: adjoint {ad:.12e} fd {fd:.12e} "
+              f"rel {abs(ad - fd) / abs(fd):.2e}", flush=True)
+        assert abs(ad - fd) <= 1e-5 * abs(fd) + 1e-12
+        checked += 1
+    assert checked >= 2
+
+
+def test_gauge_loss_gradient_and_checkpoint_invariance():
+    dom, prim0 = wet_case(n=20, seed=5)
+    fl = swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt]
+    sd = swe.SpaceDisc(fl, dom, prim0)
+    z = dom.geometry[:, 2].copy()
+    rng = np.random.default_rng(1)
+    gauges = rng.choice(prim0.shape[0], 16, replace=False)
+    fn = AD.gauge_misfit(gauges, prim0[gauges, 0] + 0.01)
+    outs = []
+    for ck in (1, 7, 40):
+        J, s0, zb, ms = AD.run_and_adjoint(sd, prim0, z, 40, fn, dt=2e-3, checkpoint_every=ck)
+        outs.append((J, s0, zb))
+    for o in outs[1:]:
+        assert o[0] == outs[0][0] and np.array_equal(o[1], outs[0][1]) and np.array_equal(o[2], outs[0][2])
+    J, s0, zb = outs[0]
+    checked = 0
+    for trial in range(3):
+        dp = np.zeros_like(prim0)
+        dp[:, 0] = rng.standard_normal(prim0.shape[0]) * 1e-2
+        dz = rng.standard_normal(z.shape) * 1e-2
+        fd = directional_fd(sd, prim0, z, 40, fn, dp, dz, "SSPRK2", 2e-3, 1e-6)
+        ad = float(np.sum(s0 * dp) + np.sum(zb * dz))
+        if fd is None:
+            continue
+        print(f"gauge loss trial {trial}: adjoint {ad:.12e} fd {fd:.12e}", flush=True)
+        assert abs(ad - fd) <= 1e-5 * abs(fd) + 1e-12
+        checked += 1
+    assert checked >= 2
+
+
+def test_adjoint_refuses_shoreline_states():
+    from tests import _common as C
+    xyz, topo, prim = C.wet_dry_mesh(20, seed=2)
+    sd = swe.SpaceDisc(swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt], swe.Domain(xyz, topo), prim)
+    with pytest.raises(swe.SolverError):
+        sd.adjoint_run(np.ones_like(prim), 2, dt=1e-3)
+
+
+def test_autograd_function_matches():
+    torch = pytest.importorskip("torch")
+    dom, prim0 = wet_case(n=12, seed=9)
+    sd = swe.SpaceDisc(swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt], dom, prim0)
+    z = dom.geometry[:, 2].copy()
+    p = torch.tensor(prim0, dtype=torch.float64, requires_grad=True)
+    zt = torch.tensor(z, dtype=torch.float64, requires_grad=True)
+    out = AD.ShallowWaterRun.apply(p, zt, sd, 10, "SSPRK2", 2e-3, 4)
+    loss = (out[:, 0] ** 2).sum()
+    loss.backward()
+    fn = lambda q: (float(np.sum(q[:, 0] ** 2)), np.stack([2 * q[:, 0], 0 * q[:, 0], 0 * q[:, 0]], 1))
+    J, s0, zb, ms = AD.run_and_adjoint(sd, prim0, z, 10, fn, dt=2e-3, checkpoint_every=4)
+    assert abs(loss.item() - J) <= 1e-12 * abs(J)
+    assert np.allclose(p.grad.numpy(), s0, rtol=1e-12, atol=1e-15)
+    assert np.allclose(zt.grad.numpy(), zb, rtol=1e-12, atol=1e-15)
