@@ -162,6 +162,81 @@ def bench_ensemble(args):
         "dtype": "f64", "data": "synthetic", "n_gpus": 1}))
 
 
+def bench_adjoint(args):
+    """Bathymetry inversion (250,632 cells, HLLC<Einfeldt>, SSP-RK2, fixed dt,
+    args.adj_steps steps): the gradient of the gauge misfit with respect to
+    the initial state and the node bathymetry by the adjoint.  Reports the
+    plain forward time, the adjoint's forward sweep (checkpoints) and reverse
+    sweep (segment recomputation with stored intermediates, then the reverse
+    stages), and value = (forward sweep + reverse sweep) / forward.  The
+    gradient is checked in two random bathymetry directions by central
+    differences over the first args.adj_check_steps steps, at eps and eps/4:
+    a pair that disagrees crossed one of the step's switches (limiter,
+    wave-speed min/max) and is reported as inconclusive."""
+    import swe_fvm_amd as swe
+    from swe_fvm_amd import adjoint as AD
+    from swe_fvm_amd import workloads as W
+    t0 = time.time()
+    dom, prim, z_true, gauges = W.adjoint_inversion(n=args.adj_n)
+    T = prim.shape[0]
+    fl = swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt]
+    sd = swe.SpaceDisc(fl, dom, prim)
+    z = dom.geometry[:, 2].copy()
+    dt = args.adj_dt
+    ck = args.adj_checkpoint if args.adj_checkpoint > 0 else args.adj_steps
+
+    def targets(nsteps):  # the run over the "true" bathymetry
+        sd.set_bathymetry(z_true)
+        sd.SetVolField(prim)
+        sd.run("SSPRK2", nsteps, dt=dt)
+        return sd.GetVolField()[gauges, 0].copy()
+
+    fn = AD.gauge_misfit(gauges, targets(args.adj_steps))
+    print(f"[bench] adjoint workload {T} cells, {args.adj_steps} steps, setup {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    sd.set_bathymetry(z)
+    sd.SetVolField(prim)
+    ms_fwd = sd.time_run("SSPRK2", args.adj_steps, dt=dt)
+    J, s0, zb, ms = AD.run_and_adjoint(sd, prim, z, args.adj_steps, fn, dt=dt, checkpoint_every=ck)
+    print(f"[bench] forward {ms_fwd:.1f} ms, adjoint sweeps {ms[0]:.1f} + {ms[1]:.1f} ms",
+          file=sys.stderr, flush=True)
+    # gradient checks on a short horizon
+    nc = args.adj_check_steps
+    fc = AD.gauge_misfit(gauges, targets(nc))
+    _, _, zbc, _ = AD.run_and_adjoint(sd, prim, z, nc, fc, dt=dt, checkpoint_every=nc)
+    rng = np.random.default_rng(3)
+    taper = W.wall_taper(dom.geometry[:, 0]) * W.wall_taper(dom.geometry[:, 1])
+
+    def Jz(zz):
+        sd.set_bathymetry(zz)
+        sd.SetVolField(prim)
+        sd.run("SSPRK2", nc, dt=dt)
+        return fc(sd.GetVolField())[0]
+
+    checks = []
+    for _ in range(2):
+        dz = rng.standard_normal(z.shape) * 1e-2 * taper
+        cd = lambda e: (Jz(z + e * dz) - Jz(z - e * dz)) / (2 * e)
+        a, b = cd(1e-6), cd(2.5e-7)
+        ad = float(np.sum(zbc * dz))
+        ok = abs(a - b) <= 1e-4 * abs(a) + 1e-300
+        checks.append({"steps": nc, "adjoint": ad, "fd": a, "fd_quarter_eps": b,
+                       "rel": abs(ad - a) / max(abs(a), 1e-300) if ok else None,
+                       "conclusive": bool(ok)})
+    sd.set_bathymetry(z)
+    out = {
+        "metric": "adjoint/forward time ratio (gauge-misfit gradient wrt initial state and bathymetry)",
+        "value": (ms[0] + ms[1]) / ms_fwd, "unit": "ratio", "higher_is_better": False,
+        "forward_ms": ms_fwd, "adjoint_forward_sweep_ms": ms[0], "adjoint_reverse_sweep_ms": ms[1],
+        "loss": J, "gradient_checks": checks,
+        "config": {"workload": f"bathymetry_inversion_T{T}", "steps": args.adj_steps, "dt": dt,
+                   "gauges": int(len(gauges)), "checkpoint_every": ck, "scheme": "SSPRK2",
+                   "flux": "HLLC<Einfeldt>"},
+        "dtype": "f64", "data": "synthetic", "n_gpus": 1,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,11 +254,23 @@ def main():
     ap.add_argument("--ensemble-cells", type=int, default=20_000_000)
     ap.add_argument("--t-end", type=float, default=1.0)
     ap.add_argument("--seq-steps", type=int, default=40)
+    ap.add_argument("--adjoint", action="store_true",
+                    help="bathymetry-inversion adjoint workload instead of the dam break")
+    ap.add_argument("--adj-n", type=int, default=354, help="adjoint mesh size (T = 2 n^2)")
+    ap.add_argument("--adj-steps", type=int, default=2000)
+    ap.add_argument("--adj-dt", type=float, default=2e-4)
+    ap.add_argument("--adj-checkpoint", type=int, default=0,
+                    help="adjoint checkpoint interval (0: every step's intermediates in one segment)")
+    ap.add_argument("--adj-check-steps", type=int, default=100,
+                    help="horizon of the finite-difference gradient checks")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default work items per thread (profiling runs)")
     args = ap.parse_args()
     if args.ensemble:
         bench_ensemble(args)
+        return
+    if args.adjoint:
+        bench_adjoint(args)
         return
     if args.copy_calibration:
         import torch

@@ -5,8 +5,8 @@
 // TimeDisc::RHS, src/TimeDisc.cpp:3-66).  Given the adjoint of O, these
 // kernels return the adjoints of S and U0 and add the adjoint of the node
 // bathymetry z.  The forward intermediates (MUSCL, fluxes, draining dt) are
-// recomputed by the forward kernels before the reverse pass, so the
-// linearisation point is bit-identical to the forward step.  Branch
+// those the forward kernels wrote for this stage, so the linearisation point
+// is bit-identical to the forward step.  Branch
 // decisions (wet/dry, the limiter's all-or-nothing choices, min / max
 // selections) are the forward ones; the flux's local Jacobian comes from
 // the flux template instantiated on Dual<7>.
@@ -28,12 +28,6 @@
 
 namespace swe {
 
-// cell bathymetry (centroid z) adjoint onto the three corners
-__device__ __forceinline__ void bc_bar(const DevAdj& A, int64_t T, int64_t i, double b) {
-  const double c = 1. / 3.;
-  for (int k = 0; k < 3; ++k) A.zc[k * T + i] += c * b;
-}
-
 // local index of edge e in cell c (te[k][c] == e)
 __device__ __forceinline__ int local_edge(const DevMesh& M, int c, int e) {
   return M.te[c] == e ? 0 : (M.te[M.T + c] == e ? 1 : 2);
@@ -41,6 +35,17 @@ __device__ __forceinline__ int local_edge(const DevMesh& M, int c, int e) {
 // local index k' with tt[k'][j] == c
 __device__ __forceinline__ int back_index(const DevMesh& M, int j, int c) {
   return M.tt[j] == c ? 0 : (M.tt[M.T + j] == c ? 1 : 2);
+}
+
+// sum over the neighbours j of cell c of their per-side value toward c
+__device__ __forceinline__ double gather_nb(const DevMesh& M, const int8_t* back, const double* wnb, int c) {
+  const int64_t T = M.T;
+  double s = 0.;
+  for (int k = 0; k < 3; ++k) {
+    const int j = M.tt[k * T + c];
+    if (j >= 0) s += wnb[back[k * T + c] * T + j];
+  }
+  return s;
 }
 
 // Reverse of slot_value (AtPoint + PrimAssigner, src/SpaceDisc.cpp:24-25):
@@ -81,11 +86,13 @@ __device__ inline void slot_bar(const Muscl& m, double cx, double cy, const V3& 
 
 // --------------------------------------------------------------- update --
 // Reverse of k_update for cell i: ConsAssigner, the stage combination and
-// TimeDisc::RHS.  Writes the flux adjoint of each of the cell's three edge
-// sides, the draining-dt adjoints (own, neighbours'), the MUSCL adjoint,
-// the U0 adjoint, the first part of the S adjoint and the corner z adjoint.
+// TimeDisc::RHS.  Writes the adjoint of the conserved update (from which
+// k_adj_flux weighs each edge side), the draining-dt adjoints (own,
+// neighbours'), the MUSCL adjoint, the U0 adjoint, the first part of the S
+// adjoint and the corner z adjoint.
 __global__ void __launch_bounds__(256) k_adj_update(DevMesh M, DevState S, DevState U0, DevWork W,
-                                                    StageArgs A, DevAdj J) {
+                                                    StageArgs A, DevAdj J, const double* wnb_in,
+                                                    int u0mode) {
   const int64_t i64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i64 >= M.T) return;
   const int i = (int)i64;
@@ -157,7 +164,10 @@ __global__ void __launch_bounds__(256) k_adj_update(DevMesh M, DevState S, DevSt
     r2 = Uc2 + R2;
   }
   // ConsAssigner reverse (src/Assigners.cpp:28-44)
-  const double ow = J.obar[i], ou = J.obar[T + i], ov = J.obar[2 * T + i];
+  // the w adjoint still owes the previous reverse stage's neighbour terms
+  double ow = J.obar[i];
+  if (wnb_in) ow += gather_nb(M, J.back, wnb_in, i);
+  const double ou = J.obar[T + i], ov = J.obar[2 * T + i];
   double rb0 = 0., rb1 = 0., rb2 = 0., bcb = ow;
   if (is_wet(r0)) {
     double ih, dih;
@@ -184,15 +194,24 @@ __global__ void __launch_bounds__(256) k_adj_update(DevMesh M, DevState S, DevSt
     su += hh * g1;
     sv += hh * g2;
   }
-  if (comb) {
+  if (comb) {  // the U0 adjoint, summed over the stages that combine U0
     const double g0 = A.a0 * rb0, g1 = A.a0 * rb1, g2 = A.a0 * rb2;
     const double hb = g0 + u0p * g1 + v0p * g2;
-    J.u0bar[i] = hb;
-    J.u0bar[T + i] = h0 * g1;
-    J.u0bar[2 * T + i] = h0 * g2;
+    if (u0mode & kU0Accumulate) {
+      J.u0bar[i] += hb;
+      J.u0bar[T + i] += h0 * g1;
+      J.u0bar[2 * T + i] += h0 * g2;
+    } else {
+      J.u0bar[i] = hb;
+      J.u0bar[T + i] = h0 * g1;
+      J.u0bar[2 * T + i] = h0 * g2;
+    }
     bcb -= hb;
-  } else {
-    J.u0bar[i] = 0.; J.u0bar[T + i] = 0.; J.u0bar[2 * T + i] = 0.;
+  }
+  if (u0mode & kU0Into) {  // first stage: its input is U0 itself
+    sw += J.u0bar[i];
+    su += J.u0bar[T + i];
+    sv += J.u0bar[2 * T + i];
   }
   // TimeDisc::RHS reverse
   double mb[5] = {0., 0., 0., 0., 0.};
@@ -205,13 +224,12 @@ __global__ void __launch_bounds__(256) k_adj_update(DevMesh M, DevState S, DevSt
     bcb -= hhb;
   }
   double zc[3] = {0., 0., 0.};
+  J.rb[i] = rb0;  // the flux adjoint of side k is -dtk sg cek rb (k_adj_flux)
+  J.rb[T + i] = rb1;
+  J.rb[2 * T + i] = rb2;
   for (int k = 0; k < 3; ++k) {
     const bool own = sgn_pos(fl, k);
     const double sg = own ? 1. : -1.;
-    const double cf = dtk[k] * sg * cek[k];
-    J.fs[(0 * 3 + k) * T + i] = -cf * rb0;
-    J.fs[(1 * 3 + k) * T + i] = -cf * rb1;
-    J.fs[(2 * 3 + k) * T + i] = -cf * rb2;
     const double q2 = 0.5 * hk[k] * hk[k];
     double dtkb = -(F[k][0] * rb0 + F[k][1] * rb1 + F[k][2] * rb2) * sg * cek[k];
     dtkb += nxk[k] * cek[k] * q2 * rb1 + nyk[k] * cek[k] * q2 * rb2;
@@ -228,41 +246,43 @@ __global__ void __launch_bounds__(256) k_adj_update(DevMesh M, DevState S, DevSt
     zc[a0] += 0.5 * ez;
     zc[a1] += 0.5 * ez;
     // draining dt selection (dmin(dts, d) picks d when d < dts)
+    // (written only where drain_bar reads them: draining dt below dts)
     double nbb = 0.;
     if ((sg * F[k][0]) > 0.) {
       if (dti < dts) dtib += dtkb;
     } else {
       if (dtik[k] < dts) nbb = dtkb;
     }
-    J.dtd_nb[k * T + i] = nbb;
+    if (dtik[k] < dts) J.dtd_nb[k * T + i] = nbb;
   }
   (void)ek;
-  J.dtd_own[i] = dtib;
+  if (dti < dts) J.dtd_own[i] = dtib;
   for (int q = 0; q < 5; ++q) J.m_cell[q * T + i] = mb[q];
   const double cthird = 1. / 3.;
-  for (int k = 0; k < 3; ++k) J.zc[k * T + i] = zc[k] + cthird * bcb;
+  for (int k = 0; k < 3; ++k) J.zc[k * T + i] += zc[k] + cthird * bcb;  // over every stage
   J.sbar[i] = sw;
   J.sbar[T + i] = su;
   J.sbar[2 * T + i] = sv;
 }
 
 // ---------------------------------------------------------------- drain --
-// Reverse of k_drain: the draining dt adjoint of cell c (its own update's
-// and its neighbours') into the mass fluxes of its edges and its depth.
+// Reverse of k_drain for cell c: the draining dt adjoint of the cell (its
+// own update's and its neighbours') into the mass fluxes of its edges (sb per
+// unit outflow, on the edges with out[k] > 0) and its depth (hb).  The dt
+// adjoint is nonzero only where the draining dt was below the stage dt, so
+// the consumers call this for those cells only; false when nothing flows.
 template <bool LEN>
-__global__ void __launch_bounds__(256) k_adj_drain(DevMesh M, DevState S, DevWork W, DevAdj J) {
-  const int64_t c64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (c64 >= M.T) return;
-  const int c = (int)c64;
+__device__ inline bool drain_bar(const DevMesh& M, const DevState& S, const DevWork& W, const DevAdj& J,
+                                 int c, double& sb, double& hb, double out[3], double len[3]) {
   const int64_t T = M.T;
   double db = J.dtd_own[c];
   for (int k = 0; k < 3; ++k) {
     const int j = M.tt[k * T + c];
-    if (j >= 0) db += J.dtd_nb[back_index(M, j, c) * T + j];
+    if (j >= 0) db += J.dtd_nb[J.back[k * T + c] * T + j];
   }
-  if (db == 0.) return;
+  if (db == 0.) return false;
   const double h = S.w[c] - M.bc[c];
-  if (!is_wet(h)) return;
+  if (!is_wet(h)) return false;
   const uint32_t fl = M.cflag[c];
   V3 p0, p1, p2;
   if (LEN) {
@@ -270,7 +290,7 @@ __global__ void __launch_bounds__(256) k_adj_drain(DevMesh M, DevState S, DevWor
     p1 = node(M, M.tp[T + c]);
     p2 = node(M, M.tp[2 * T + c]);
   }
-  double out[3], len[3], sum = 0.;
+  double sum = 0.;
   for (int k = 0; k < 3; ++k) {
     const double fe = W.f0[M.te[k * T + c]];
     out[k] = sgn_pos(fl, k) ? fe : -fe;
@@ -281,54 +301,77 @@ __global__ void __launch_bounds__(256) k_adj_drain(DevMesh M, DevState S, DevWor
     }
     sum += dmax(0., out[k]);
   }
-  if (!(sum > kTol)) return;
+  if (!(sum > kTol)) return false;
   const double A = M.area[c];
-  const double hb = db * A / sum;
-  const double sb = -db * A * h / (sum * sum);
-  for (int k = 0; k < 3; ++k)
-    if (out[k] > 0.) J.fs[(0 * 3 + k) * T + c] += sb * len[k] * (sgn_pos(fl, k) ? 1. : -1.);
-  J.sbar[c] += hb;
-  bc_bar(J, T, c, -hb);
+  hb = db * A / sum;
+  sb = -db * A * h / (sum * sum);
+  return true;
+}
+
+// The flux adjoint of cell c's side of edge e (local index k, sign sg, the
+// cell across o or < 0): -dtk sg cek rb, as TimeDisc::RHS weighs the edge's
+// flux in the cell's update, plus the draining term on the mass flux when
+// the cell's draining dt was active.
+template <bool LEN>
+__device__ inline void side_flux_bar(const DevMesh& M, const DevState& S, const DevWork& W, const DevAdj& J,
+                                     int c, int o, int k, double sg, double f0, double le, double dts,
+                                     double fb[3]) {
+  const int64_t T = M.T;
+  const double dtc = W.dtd[c];
+  const double dto = o < 0 ? INFINITY : W.dtd[o];
+  const double dtk = (sg * f0) > 0. ? dmin(dts, dtc) : dmin(dts, dto);
+  const double cek = (1. / M.area[c]) * le;
+  const double cf = dtk * sg * cek;
+  fb[0] = -cf * J.rb[c];
+  fb[1] = -cf * J.rb[T + c];
+  fb[2] = -cf * J.rb[2 * T + c];
+  if (dtc < dts) {
+    double sb, hb, out[3], len[3];
+    if (drain_bar<LEN>(M, S, W, J, c, sb, hb, out, len) && out[k] > 0.) fb[0] += sb * len[k] * sg;
+  }
 }
 
 // ----------------------------------------------------------------- flux --
-// Reverse of k_flux for edge e: the flux adjoint of its two sides, through
-// the Riemann solver's local Jacobian (the flux template on Dual<7>: inputs
-// L (w, u, v), R (w, u, v), edge z) and the two slot reconstructions, into
-// per-side MUSCL adjoints and the edge-midpoint z adjoint.
-template <int FLUX, int WS>
+// Reverse of k_flux for edge e: the flux adjoint of its two sides (from the
+// cells' update adjoints), through the Riemann solver's local Jacobian (the
+// flux template on Dual<7>: inputs L (w, u, v), R (w, u, v), edge z), into
+// the adjoints of the two slot values; k_adj_recon takes them through the
+// slot reconstruction.
+template <int FLUX, int WS, bool LEN>
 __global__ void __launch_bounds__(256) k_adj_flux(DevMesh M, DevState S, DevWork W, DevAdj J,
-                                                  double cor) {
+                                                  double cor, double dts) {
   const int64_t e64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (e64 >= M.E) return;
   const int e = (int)e64;
-  const int64_t T = M.T, Ecnt = M.E;
+  const int64_t Ecnt = M.E;
   const int f = M.et0[e], t = M.et1[e];
-  const int k0 = local_edge(M, f, e);
-  double Fb[3];
-  for (int q = 0; q < 3; ++q) Fb[q] = J.fs[(q * 3 + k0) * T + f];
-  if (t >= 0) {
-    const int k1 = local_edge(M, t, e);
-    for (int q = 0; q < 3; ++q) Fb[q] += J.fs[(q * 3 + k1) * T + t];
-  }
   const V3 Pa = node(M, M.ep0[e]), Pb = node(M, M.ep1[e]);
   const double le = edge_len(Pa, Pb);
-  const V3 E = midpoint(Pa, Pb);
-  double tx, ty;
-  tangent(Pa, Pb, le, V3{M.cx[f], M.cy[f], 0.}, tx, ty);
-  const double nx = ty, ny = -tx;
-  double mbf[5] = {0., 0., 0., 0., 0.}, mbt[5] = {0., 0., 0., 0., 0.};
-  double ez = 0., hfb = 0.;
+  const double f0 = W.f0[e];
+  double Fb[3], fb[3];
+  side_flux_bar<LEN>(M, S, W, J, f, t, J.eloc[e], 1., f0, le, dts, Fb);
+  if (t >= 0) {
+    side_flux_bar<LEN>(M, S, W, J, t, f, J.eloc[Ecnt + e], -1., f0, le, dts, fb);
+    for (int q = 0; q < 3; ++q) Fb[q] += fb[q];
+  }
+  double g[7] = {0., 0., 0., 0., 0., 0., 0.};
   if (t == kSolidWall) {
-    // ElemFlux(n, (h, 0, 0)) = (0, h^2/2 n) of the cell average
+    // ElemFlux(n, (h, 0, 0)) = (0, h^2/2 n) of the cell average: the depth
+    // adjoint travels in side 0's w slot
+    double tx, ty;
+    tangent(Pa, Pb, le, V3{M.cx[f], M.cy[f], 0.}, tx, ty);
+    const double nx = ty, ny = -tx;
     const double hf = S.w[f] - M.bc[f];
-    if (is_wet(hf)) hfb = (Fb[1] * nx + Fb[2] * ny) * hf;
+    if (is_wet(hf)) g[0] = (Fb[1] * nx + Fb[2] * ny) * hf;
   } else {
     const int tr = t < 0 ? f : t;
     const Muscl mf = load_muscl(S, W, f), mt = load_muscl(S, W, tr);
-    const double cxf = M.cx[f], cyf = M.cy[f], cxt = M.cx[tr], cyt = M.cy[tr];
-    const Slot L = slot_value(mf, cxf, cyf, E);
-    const Slot R = slot_value(mt, cxt, cyt, E);
+    const V3 E = midpoint(Pa, Pb);
+    double tx, ty;
+    tangent(Pa, Pb, le, V3{M.cx[f], M.cy[f], 0.}, tx, ty);
+    const double nx = ty, ny = -tx;
+    const Slot L = slot_value(mf, M.cx[f], M.cy[f], E);
+    const Slot R = slot_value(mt, M.cx[tr], M.cy[tr], E);
     using D = Dual<7>;
     const SlotT<D> Ld{dvar<7>(L.w, 0), dvar<7>(L.u, 1), dvar<7>(L.v, 2)};
     const SlotT<D> Rd{dvar<7>(R.w, 3), dvar<7>(R.u, 4), dvar<7>(R.v, 5)};
@@ -337,23 +380,19 @@ __global__ void __launch_bounds__(256) k_adj_flux(DevMesh M, DevState S, DevWork
     double lw = 1.;
     if (FLUX == 0) flux_hll<WS>(nx, ny, be, Ld, Rd, M.edmin[e], cor, lw, Fd);
     else flux_hllc<WS>(tx, ty, be, Ld, Rd, M.edmin[e], cor, lw, Fd);
-    double gin[7];
-    for (int k = 0; k < 7; ++k) gin[k] = Fb[0] * Fd[0].d[k] + Fb[1] * Fd[1].d[k] + Fb[2] * Fd[2].d[k];
-    ez += gin[6];
+    for (int k = 0; k < 7; ++k) g[k] = Fb[0] * Fd[0].d[k] + Fb[1] * Fd[1].d[k] + Fb[2] * Fd[2].d[k];
     if (t < 0) {  // free flow: R is L
-      for (int k = 0; k < 3; ++k) gin[k] += gin[3 + k];
-      slot_bar(mf, cxf, cyf, E, gin[0], gin[1], gin[2], mbf, ez);
-    } else {
-      slot_bar(mf, cxf, cyf, E, gin[0], gin[1], gin[2], mbf, ez);
-      slot_bar(mt, cxt, cyt, E, gin[3], gin[4], gin[5], mbt, ez);
+      for (int k = 0; k < 3; ++k) {
+        g[k] += g[3 + k];
+        g[3 + k] = 0.;
+      }
     }
   }
-  for (int q = 0; q < 5; ++q) {
-    J.m_edge[(q * 2 + 0) * Ecnt + e] = mbf[q];
-    J.m_edge[(q * 2 + 1) * Ecnt + e] = mbt[q];
+  for (int q = 0; q < 3; ++q) {
+    J.sl_edge[(q * 2 + 0) * Ecnt + e] = g[q];
+    J.sl_edge[(q * 2 + 1) * Ecnt + e] = g[3 + q];
   }
-  J.ez_edge[e] = ez;
-  J.hf_edge[e] = hfb;
+  J.ez_edge[e] = g[6];
 }
 
 // ------------------------------------------------------- reconstruction --
@@ -362,7 +401,8 @@ __device__ __forceinline__ void flag(const DevAdj& J) { atomicAdd(J.flags, 1u); 
 // Reverse of k_recon1 / k_recon2 for cell i: gathers the MUSCL adjoint of
 // the cell (its own update and the slots of its three edges), then maps it
 // to the w adjoints of the cell and its neighbours and to corner z.
-__global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWork W, DevAdj J) {
+template <bool LEN>
+__global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWork W, DevAdj J, double dts) {
   const int64_t i64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i64 >= M.T) return;
   const int i = (int)i64;
@@ -372,23 +412,37 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
   for (int q = 0; q < 5; ++q) mb[q] = J.m_cell[q * T + i];
   double zc[3] = {0., 0., 0.};
   double wb = 0., bcb = 0.;
-  for (int k = 0; k < 3; ++k) {
-    const int e = M.te[k * T + i];
-    const bool own = sgn_pos(fl, k);
-    const int side = own ? 0 : 1;
-    for (int q = 0; q < 5; ++q) mb[q] += J.m_edge[(q * 2 + side) * Ecnt + e];
-    if (own) {  // the edge's midpoint z and wall depth go to its et0 cell
-      const double ez = J.ez_edge[e];
+  if (W.dtd[i] < dts) {  // the depth term of the draining dt
+    double sb, hb, out[3], len[3];
+    if (drain_bar<LEN>(M, S, W, J, i, sb, hb, out, len)) {
+      wb += hb;
+      bcb -= hb;
+    }
+  }
+  const V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[T + i]), p2 = node(M, M.tp[2 * T + i]);
+  {
+    // the slot values of the cell's three edges (AtPoint at the midpoints)
+    const Muscl mu = load_muscl(S, W, i);
+    const double cxi = M.cx[i], cyi = M.cy[i];
+    for (int k = 0; k < 3; ++k) {
+      const int e = M.te[k * T + i];
+      const bool own = sgn_pos(fl, k);
+      const int side = own ? 0 : 1;
+      if (M.tt[k * T + i] == kSolidWall) {
+        const double hf = J.sl_edge[e];  // side 0 w: the depth adjoint
+        wb += hf;
+        bcb -= hf;
+        continue;
+      }
+      double ez = own ? J.ez_edge[e] : 0.;  // the flux's own z term, once
+      slot_bar(mu, cxi, cyi, edge_mid(p0, p1, p2, fl, k), J.sl_edge[(0 * 2 + side) * Ecnt + e],
+               J.sl_edge[(1 * 2 + side) * Ecnt + e], J.sl_edge[(2 * 2 + side) * Ecnt + e], mb, ez);
       zc[ep0_local(fl, k)] += 0.5 * ez;
       zc[ep1_local(fl, k)] += 0.5 * ez;
-      const double hf = J.hf_edge[e];
-      wb += hf;
-      bcb -= hf;
     }
   }
   const int cls = W.cls[i];
   const double w = S.w[i];
-  const V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[T + i]), p2 = node(M, M.tp[2 * T + i]);
   double wnb[3] = {0., 0., 0.};
   if (cls == 0) {
     // ReconstructDryCell: level b_cell, gradient the bathymetry slope
@@ -505,18 +559,13 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
   for (int k = 0; k < 3; ++k) J.zc[k * T + i] += zc[k] + c * bcb;
 }
 
-// the neighbours' contributions to the w adjoint of cell c
+// the neighbours' contributions to the w adjoint of cell c (after the last
+// reverse stage; earlier ones are gathered by the next k_adj_update)
 __global__ void __launch_bounds__(256) k_adj_gather(DevMesh M, DevAdj J) {
   const int64_t c64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (c64 >= M.T) return;
   const int c = (int)c64;
-  const int64_t T = M.T;
-  double s = 0.;
-  for (int k = 0; k < 3; ++k) {
-    const int j = M.tt[k * T + c];
-    if (j >= 0) s += J.w_nb[back_index(M, j, c) * T + j];
-  }
-  J.sbar[c] += s;
+  J.sbar[c] += gather_nb(M, J.back, J.w_nb, c);
 }
 
 // node bathymetry adjoint: the corners of every incident cell, in the
@@ -535,9 +584,23 @@ __global__ void __launch_bounds__(256) k_adj_z(DevMesh M, DevAdj J, double* zbar
   zbar[n] += s;
 }
 
-__global__ void __launch_bounds__(256) k_axpy(const double* x, double* y, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) y[i] += x[i];
+// the back-index and edge-local-index tables of the reverse kernels
+__global__ void __launch_bounds__(256) k_adj_tables(DevMesh M, DevAdj J) {
+  const int64_t x = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t T = M.T;
+  if (x < T) {
+    const int c = (int)x;
+    for (int k = 0; k < 3; ++k) {
+      const int j = M.tt[k * T + c];
+      J.back[k * T + c] = (int8_t)(j >= 0 ? back_index(M, j, c) : 0);
+    }
+  }
+  if (x < M.E) {
+    const int e = (int)x;
+    const int t = M.et1[e];
+    J.eloc[e] = (int8_t)local_edge(M, M.et0[e], e);
+    J.eloc[M.E + e] = (int8_t)(t >= 0 ? local_edge(M, t, e) : 0);
+  }
 }
 
 static inline dim3 grid_plain(int64_t n) {
@@ -547,24 +610,34 @@ static inline dim3 grid_plain(int64_t n) {
 
 hipError_t launch_adjoint_stage(const DevMesh& M, const DevState& S, const DevState& U0, const DevWork& W,
                                 const StageArgs& A, const DevAdj& J, int flux, int ws, int opts,
-                                double* zbar, hipStream_t st) {
-  hipLaunchKernelGGL(k_adj_update, grid_plain(M.T), dim3(256), 0, st, M, S, U0, W, A, J);
-  if (opts & kOptDrainLength) hipLaunchKernelGGL(k_adj_drain<true>, grid_plain(M.T), dim3(256), 0, st, M, S, W, J);
-  else hipLaunchKernelGGL(k_adj_drain<false>, grid_plain(M.T), dim3(256), 0, st, M, S, W, J);
-#define SWE_ADJ_FLUX(F, Wv)                                                                      \
-  if (flux == F && ws == Wv)                                                                     \
-    hipLaunchKernelGGL((k_adj_flux<F, Wv>), grid_plain(M.E), dim3(256), 0, st, M, S, W, J, A.cor);
-  SWE_ADJ_FLUX(0, 0) SWE_ADJ_FLUX(0, 1) SWE_ADJ_FLUX(0, 2)
-  SWE_ADJ_FLUX(1, 0) SWE_ADJ_FLUX(1, 1) SWE_ADJ_FLUX(1, 2)
+                                const double* wnb_in, int u0mode, hipStream_t st) {
+  const double dts = A.coef * A.dt_fixed;
+  hipLaunchKernelGGL(k_adj_update, grid_plain(M.T), dim3(256), 0, st, M, S, U0, W, A, J, wnb_in, u0mode);
+#define SWE_ADJ_FLUX(F, Wv, L)                                                                   \
+  if (flux == F && ws == Wv && len == L)                                                         \
+    hipLaunchKernelGGL((k_adj_flux<F, Wv, L>), grid_plain(M.E), dim3(256), 0, st, M, S, W, J, A.cor, dts);
+  for (int li = 0; li < 2; ++li) {
+    const bool len = li == 1;
+    if (len != ((opts & kOptDrainLength) != 0)) continue;
+    SWE_ADJ_FLUX(0, 0, false) SWE_ADJ_FLUX(0, 1, false) SWE_ADJ_FLUX(0, 2, false)
+    SWE_ADJ_FLUX(1, 0, false) SWE_ADJ_FLUX(1, 1, false) SWE_ADJ_FLUX(1, 2, false)
+    SWE_ADJ_FLUX(0, 0, true) SWE_ADJ_FLUX(0, 1, true) SWE_ADJ_FLUX(0, 2, true)
+    SWE_ADJ_FLUX(1, 0, true) SWE_ADJ_FLUX(1, 1, true) SWE_ADJ_FLUX(1, 2, true)
+  }
 #undef SWE_ADJ_FLUX
-  hipLaunchKernelGGL(k_adj_recon, grid_plain(M.T), dim3(256), 0, st, M, S, W, J);
-  hipLaunchKernelGGL(k_adj_gather, grid_plain(M.T), dim3(256), 0, st, M, J);
-  if (zbar) hipLaunchKernelGGL(k_adj_z, grid_plain(M.V), dim3(256), 0, st, M, J, zbar);
+  if (opts & kOptDrainLength) hipLaunchKernelGGL(k_adj_recon<true>, grid_plain(M.T), dim3(256), 0, st, M, S, W, J, dts);
+  else hipLaunchKernelGGL(k_adj_recon<false>, grid_plain(M.T), dim3(256), 0, st, M, S, W, J, dts);
   return hipGetLastError();
 }
 
-hipError_t launch_axpy(const double* x, double* y, int64_t n, hipStream_t st) {
-  hipLaunchKernelGGL(k_axpy, grid_plain(n), dim3(256), 0, st, x, y, n);
+hipError_t launch_adjoint_tables(const DevMesh& M, const DevAdj& J, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_tables, grid_plain(M.T > M.E ? M.T : M.E), dim3(256), 0, st, M, J);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_finish(const DevMesh& M, const DevAdj& J, double* zbar, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_gather, grid_plain(M.T), dim3(256), 0, st, M, J);
+  if (zbar) hipLaunchKernelGGL(k_adj_z, grid_plain(M.V), dim3(256), 0, st, M, J, zbar);
   return hipGetLastError();
 }
 

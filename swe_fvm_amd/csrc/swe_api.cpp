@@ -965,34 +965,20 @@ int stage_list(int scheme, StageCoef* out) {
   return 3;
 }
 
-hipError_t intermediates(swe_solver* s, const DevState& S) {
-  hipError_t e = launch_recon1(s->M, S, s->W, s->opts, s->stream);
-  if (e == hipSuccess) e = launch_recon2(s->M, S, s->W, s->opts, s->stream);
-  if (e == hipSuccess)
-    e = launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + 3, s->flux_items, s->stream);
-  if (e == hipSuccess) e = launch_drain(s->M, S, s->W, s->opts, s->stream);
-  return e;
-}
-
-hipError_t stage_forward(swe_solver* s, const DevState& S, const DevState& U0, const DevOut& O,
-                         const StageCoef& c, double dt) {
-  hipError_t e = intermediates(s, S);
-  StageArgs A{c.a0, c.a1, c.coef, dt, 0., s->cor, s->rbits, nullptr, friction(s)};
-  if (e == hipSuccess) e = launch_update(s->M, S, U0, O, s->W, A, s->upd_items, s->stream);
-  return e;
-}
-
 }  // namespace
 
-// Adjoint of nsteps steps (no reference counterpart; the north star's
-// differentiation of the full step).  Starting from the current state U_0,
-// runs the steps forward with step sizes dt (> 0, every step) or dts[n]
-// (e.g. a CFL run's sizes, then held fixed), storing U_n every
-// checkpoint_every steps; leaves U_N as the current state.  Then propagates
-// final_bar (T x 3 row-major adjoint of U_N's w, u, v) back through every
-// stage, recomputing each segment between checkpoints, into state0_bar
-// (T x 3, adjoint of U_0) and z_bar (V, adjoint of the node bathymetry, may
-// be null).  ms (optional, 2 doubles): forward sweep and reverse sweep
+// Adjoint of nsteps steps (no reference counterpart: the differentiation of
+// the full step).  Starting from the current state U_0, runs the steps with
+// step sizes dt (> 0, every step) or dts[n] (e.g. a CFL run's sizes, then
+// held fixed), storing U_n every checkpoint_every steps; leaves U_N as the
+// current state.  Each segment between checkpoints is then recomputed from
+// its checkpoint with every stage's intermediates (MUSCL, fluxes, draining
+// dt) kept in device memory, and final_bar (T x 3 row-major adjoint of U_N's
+// w, u, v) is propagated back through its stages into state0_bar (T x 3,
+// adjoint of U_0) and z_bar (V, adjoint of the node bathymetry; may be
+// null).  The last segment's recomputation is the forward run itself, so
+// checkpoint_every >= nsteps runs the steps once.  ms (optional, 2
+// doubles): forward sweep and reverse sweep (recomputation included)
 // milliseconds.
 int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int64_t nsteps,
                     int64_t checkpoint_every, const double* final_bar, double* state0_bar, double* z_bar,
@@ -1014,123 +1000,155 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
   const int ns = stage_list(scheme, sc);
   const int64_t nck = (nsteps + K - 1) / K;
   std::vector<void*> mem;
-  auto alloc = [&](double** p, int64_t n) {
-    hipError_t e = hipMalloc((void**)p, sizeof(double) * size_t(std::max<int64_t>(n, 1)));
-    if (e == hipSuccess) mem.push_back(*p);
+  auto alloc = [&](auto** p, int64_t bytes) {
+    hipError_t e = hipMalloc((void**)p, size_t(std::max<int64_t>(bytes, 8)));
+    if (e == hipSuccess) mem.push_back((void*)*p);
     return e;
   };
   auto release = [&]() {
     hipStreamSynchronize(s->stream);
     for (void* p : mem) hipFree(p);
   };
-  double *ck = nullptr, *seg = nullptr, *stg = nullptr, *bar_a = nullptr, *bar_b = nullptr, *bar_u0 = nullptr;
+  const int64_t S3 = 3 * T * int64_t(sizeof(double));
+  // per-stage work slot: mw, mgx, mgy, wrec1, dtd (T doubles each), f0, f1,
+  // f2 (E each), pw_list (T ints), cls (T bytes), pw_count
+  const int64_t slot_bytes = ((5 * T + 3 * E) * 8 + 4 * T + T + 64 + 255) / 256 * 256;
+  double *ck = nullptr, *seg = nullptr, *stg = nullptr, *bar_a = nullptr, *bar_b = nullptr;
   double *zb = nullptr, *work = nullptr;
+  char* slots = nullptr;
   unsigned int* flags = nullptr;
-  hipError_t e = alloc(&ck, nck * 3 * T);
-  if (e == hipSuccess) e = alloc(&seg, K * 3 * T);
-  if (e == hipSuccess) e = alloc(&stg, K * (ns - 1 > 0 ? ns - 1 : 1) * 3 * T);
-  if (e == hipSuccess) e = alloc(&bar_a, 3 * T);
-  if (e == hipSuccess) e = alloc(&bar_b, 3 * T);
-  if (e == hipSuccess) e = alloc(&bar_u0, 3 * T);
-  if (e == hipSuccess) e = alloc(&zb, V);
-  // DevAdj work: fs 9T, dtd_own T, dtd_nb 3T, m_cell 5T, m_edge 10E, ez E,
-  // hf E, w_nb 3T, zc 3T, u0bar 3T
-  const int64_t nwork = 9 * T + T + 3 * T + 5 * T + 10 * E + E + E + 3 * T + 3 * T + 3 * T;
-  if (e == hipSuccess) e = alloc(&work, nwork);
-  if (e == hipSuccess) {
-    e = hipMalloc((void**)&flags, sizeof(unsigned int));
-    if (e == hipSuccess) mem.push_back(flags);
-  }
+  hipError_t e = alloc(&ck, nck * S3);
+  if (e == hipSuccess) e = alloc(&seg, K * S3);
+  if (e == hipSuccess) e = alloc(&stg, K * std::max(ns - 1, 1) * S3);
+  if (e == hipSuccess) e = alloc(&slots, K * ns * slot_bytes);
+  if (e == hipSuccess) e = alloc(&bar_a, S3);
+  if (e == hipSuccess) e = alloc(&bar_b, S3);
+  if (e == hipSuccess) e = alloc(&zb, V * 8);
+  // DevAdj work: rb 3T, dtd_own T, dtd_nb 3T, m_cell 5T, sl_edge 6E, ez E,
+  // w_nb 3T, zc 3T, u0bar 3T; back 3T and eloc 2E bytes
+  const int64_t nwork = 3 * T + T + 3 * T + 5 * T + 6 * E + E + 3 * T + 3 * T + 3 * T;
+  if (e == hipSuccess) e = alloc(&work, nwork * 8 + ((3 * T + 2 * E + 255) / 256) * 256);
+  if (e == hipSuccess) e = alloc(&flags, int64_t(sizeof(unsigned int)));
   if (e != hipSuccess) { release(); return dev_fail(e, "hipMalloc (adjoint)"); }
+  auto slot_work = [&](int64_t idx) {
+    char* b = slots + idx * slot_bytes;
+    DevWork W{};
+    double* d = (double*)b;
+    W.mw = d; d += T;
+    W.mgx = d; d += T;
+    W.mgy = d; d += T;
+    W.wrec1 = d; d += T;
+    W.dtd = d; d += T;
+    W.f0 = d; d += E;
+    W.f1 = d; d += E;
+    W.f2 = d; d += E;
+    int* ip = (int*)d;
+    W.pw_list = ip; ip += T;
+    W.cls = (int8_t*)ip;
+    W.pw_count = (unsigned int*)((((uintptr_t)(W.cls + T)) + 63) / 64 * 64);
+    return W;
+  };
   DevAdj J{};
   {
     double* q = work;
-    J.fs = q; q += 9 * T;
+    J.rb = q; q += 3 * T;
     J.dtd_own = q; q += T;
     J.dtd_nb = q; q += 3 * T;
     J.m_cell = q; q += 5 * T;
-    J.m_edge = q; q += 10 * E;
+    J.sl_edge = q; q += 6 * E;
     J.ez_edge = q; q += E;
-    J.hf_edge = q; q += E;
     J.w_nb = q; q += 3 * T;
     J.zc = q; q += 3 * T;
     J.u0bar = q; q += 3 * T;
+    J.back = (int8_t*)q;
+    J.eloc = J.back + 3 * T;
     J.flags = flags;
   }
   auto dtn = [&](int64_t n) { return dts ? dts[n] : dt; };
   hipEvent_t ev[3];
   for (auto& x : ev) hipEventCreate(&x);
-  // forward sweep with checkpoints
+  // forward sweep: checkpoints up to the start of the last segment
   hipEventRecord(ev[0], s->stream);
   int rc = SWE_OK;
-  for (int64_t n = 0; n < nsteps && !rc; ++n) {
-    if (n % K == 0) {
-      Buf3 b{ck + (n / K) * 3 * T};
-      if ((e = copy_state(state_of(s, s->cur), out3(b, T), T, s->stream)) != hipSuccess) rc = dev_fail(e, "copy");
-    }
+  const int64_t n_last = (nck - 1) * K;
+  for (int64_t n = 0; n < n_last && !rc; ++n) {
+    if (n % K == 0)
+      if ((e = copy_state(state_of(s, s->cur), out3(Buf3{ck + (n / K) * 3 * T}, T), T, s->stream)) != hipSuccess)
+        rc = dev_fail(e, "copy");
     if (!rc) rc = run_step(s, scheme, dtn(n), 0., nullptr);
   }
+  if (!rc && (e = copy_state(state_of(s, s->cur), out3(Buf3{ck + (nck - 1) * 3 * T}, T), T, s->stream)) != hipSuccess)
+    rc = dev_fail(e, "copy");
   hipEventRecord(ev[1], s->stream);
-  // reverse sweep
   if (!rc) {
     std::vector<double> h(static_cast<size_t>(3 * T));
     for (int64_t i = 0; i < T; ++i)
       for (int q = 0; q < 3; ++q) h[size_t(q * T + i)] = final_bar[3 * i + q];
-    if ((e = hipMemcpyAsync(bar_a, h.data(), sizeof(double) * size_t(3 * T), hipMemcpyHostToDevice, s->stream)) != hipSuccess ||
+    if ((e = hipMemcpyAsync(bar_a, h.data(), size_t(S3), hipMemcpyHostToDevice, s->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(J.zc, 0, sizeof(double) * size_t(3 * T), s->stream)) != hipSuccess ||
         (e = hipMemsetAsync(zb, 0, sizeof(double) * size_t(std::max<int64_t>(V, 1)), s->stream)) != hipSuccess ||
         (e = hipMemsetAsync(flags, 0, sizeof(unsigned int), s->stream)) != hipSuccess ||
+        (e = launch_adjoint_tables(s->M, J, s->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(s->stream)) != hipSuccess)
       rc = dev_fail(e, "adjoint setup");
   }
+  double* ob = bar_a;
+  double* sb = bar_b;
+  const double* wpend = nullptr;
   for (int64_t g = nck - 1; g >= 0 && !rc; --g) {
     const int64_t n0 = g * K, n1 = std::min(nsteps, n0 + K);
-    // recompute the segment: step inputs U_n and the stage outputs
+    // recompute the segment keeping step inputs, stage outputs and every
+    // stage's intermediates; the last segment also produces U_N
     e = copy_state(st3(Buf3{ck + g * 3 * T}, T), out3(Buf3{seg}, T), T, s->stream);
     for (int64_t n = n0; n < n1 && e == hipSuccess; ++n) {
       const int64_t m = n - n0;
       const DevState Un = st3(Buf3{seg + m * 3 * T}, T);
       DevState Sq = Un;
       for (int q = 0; q < ns && e == hipSuccess; ++q) {
-        DevOut O = (q == ns - 1) ? (n + 1 < n1 ? out3(Buf3{seg + (m + 1) * 3 * T}, T) : DevOut{nullptr, nullptr, nullptr})
-                                 : out3(Buf3{stg + (m * (ns - 1) + q) * 3 * T}, T);
-        if (!O.w) break;  // the segment's last step output is not needed
-        e = stage_forward(s, Sq, Un, O, sc[q], dtn(n));
+        DevOut O;
+        if (q < ns - 1) O = out3(Buf3{stg + (m * (ns - 1) + q) * 3 * T}, T);
+        else if (n + 1 < n1) O = out3(Buf3{seg + (m + 1) * 3 * T}, T);
+        else O = g == nck - 1 ? out_of(s, (s->cur + 1) % 3) : out3(Buf3{sb}, T);  // U_N, or unused
+        const DevWork Wq = slot_work(m * ns + q);
+        e = launch_recon1(s->M, Sq, Wq, s->opts, s->stream);
+        if (e == hipSuccess) e = launch_recon2(s->M, Sq, Wq, s->opts, s->stream);
+        if (e == hipSuccess)
+          e = launch_flux(s->M, Sq, Wq, s->flux, s->ws, s->cor, s->rbits + 3, s->flux_items, s->stream);
+        if (e == hipSuccess) e = launch_drain(s->M, Sq, Wq, s->opts, s->stream);
+        StageArgs A{sc[q].a0, sc[q].a1, sc[q].coef, dtn(n), 0., s->cor, s->rbits, nullptr, friction(s)};
+        if (e == hipSuccess) e = launch_update(s->M, Sq, Un, O, Wq, A, s->upd_items, s->stream);
         Sq = DevState{O.w, O.u, O.v};
       }
     }
     if (e != hipSuccess) { rc = dev_fail(e, "adjoint recomputation"); break; }
-    // reverse the steps of the segment
+    if (g == nck - 1) s->cur = (s->cur + 1) % 3;  // U_N is the current state
+    // reverse the steps of the segment: ob holds the adjoint of the stage
+    // output, its w part owing the neighbour terms wpend of the stage after
     for (int64_t n = n1 - 1; n >= n0 && !rc; --n) {
       const int64_t m = n - n0;
       const DevState Un = st3(Buf3{seg + m * 3 * T}, T);
-      // bar_a: adjoint of the step output; bar_u0 accumulates the U_n adjoint
-      e = hipMemsetAsync(bar_u0, 0, sizeof(double) * size_t(3 * T), s->stream);
-      double* obar = bar_a;
-      double* sbar = bar_b;
       for (int q = ns - 1; q >= 0 && e == hipSuccess; --q) {
         const DevState Sq = q == 0 ? Un : st3(Buf3{stg + (m * (ns - 1) + q - 1) * 3 * T}, T);
-        e = intermediates(s, Sq);
-        J.obar = obar;
-        J.sbar = sbar;
+        J.obar = ob;
+        J.sbar = sb;
         StageArgs A{sc[q].a0, sc[q].a1, sc[q].coef, dtn(n), 0., s->cor, s->rbits, nullptr, friction(s)};
-        if (e == hipSuccess)
-          e = launch_adjoint_stage(s->M, Sq, Un, s->W, A, J, s->flux, s->ws, s->opts, z_bar ? zb : nullptr,
-                                   s->stream);
-        if (e == hipSuccess && sc[q].a0 != 0.) e = launch_axpy(J.u0bar, bar_u0, 3 * T, s->stream);
-        std::swap(obar, sbar);
+        const int u0mode = (q < ns - 1 ? kU0Accumulate : 0) | (q == 0 && ns > 1 ? kU0Into : 0);
+        e = launch_adjoint_stage(s->M, Sq, Un, slot_work(m * ns + q), A, J, s->flux, s->ws, s->opts, wpend,
+                                 u0mode, s->stream);
+        wpend = J.w_nb;
+        std::swap(ob, sb);
       }
-      // obar now holds the stage-0 input adjoint (= part of the U_n adjoint)
-      if (e == hipSuccess) e = launch_axpy(bar_u0, obar, 3 * T, s->stream);
-      if (e == hipSuccess && obar != bar_a)
-        e = hipMemcpyAsync(bar_a, obar, sizeof(double) * size_t(3 * T), hipMemcpyDeviceToDevice, s->stream);
       if (e != hipSuccess) rc = dev_fail(e, "adjoint stage");
     }
   }
+  J.sbar = ob;  // the U_0 adjoint, its neighbour terms gathered here
+  if (!rc && (e = launch_adjoint_finish(s->M, J, z_bar ? zb : nullptr, s->stream)) != hipSuccess)
+    rc = dev_fail(e, "adjoint finish");
   hipEventRecord(ev[2], s->stream);
   unsigned int nflag = 0;
   if (!rc) {
     std::vector<double> h(static_cast<size_t>(3 * T));
-    if ((e = hipMemcpyAsync(h.data(), bar_a, sizeof(double) * size_t(3 * T), hipMemcpyDeviceToHost, s->stream)) != hipSuccess ||
+    if ((e = hipMemcpyAsync(h.data(), ob, size_t(S3), hipMemcpyDeviceToHost, s->stream)) != hipSuccess ||
         (e = hipMemcpyAsync(&nflag, flags, sizeof(unsigned int), hipMemcpyDeviceToHost, s->stream)) != hipSuccess ||
         (z_bar && (e = hipMemcpyAsync(z_bar, zb, sizeof(double) * size_t(V), hipMemcpyDeviceToHost, s->stream)) != hipSuccess) ||
         (e = hipStreamSynchronize(s->stream)) != hipSuccess) {

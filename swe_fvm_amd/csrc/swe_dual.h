@@ -67,10 +67,13 @@ SWE_DUAL_BINOP(*, a.v * b.v, a.d[k] * b.v + a.v * b.d[k])
 
 template <int N>
 __host__ __device__ __forceinline__ Dual<N> operator/(const Dual<N>& a, const Dual<N>& b) {
+  // the value is the exact quotient (branch decisions downstream compare
+  // it); the derivatives use one reciprocal
   Dual<N> r;
   r.v = a.v / b.v;
+  const double ib = 1. / b.v;
 #pragma unroll
-  for (int k = 0; k < N; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) / b.v;
+  for (int k = 0; k < N; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * ib;
   return r;
 }
 template <int N>
@@ -128,8 +131,9 @@ template <int N>
 __host__ __device__ __forceinline__ Dual<N> operator/(const Dual<N>& a, double b) {
   Dual<N> r;
   r.v = a.v / b;
+  const double ib = 1. / b;
 #pragma unroll
-  for (int k = 0; k < N; ++k) r.d[k] = a.d[k] / b;
+  for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * ib;
   return r;
 }
 template <int N>

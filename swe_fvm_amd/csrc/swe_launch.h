@@ -112,21 +112,23 @@ struct DevWork {
 };
 
 // adjoint work of one stage (reverse kernels, swe_adjoint.hip); layouts:
-// per cell side [q][k][T] (q component, k local edge), per edge side
-// [q][side][E], per cell [q][T]
+// per cell [q][T] or [k][T] (q component, k local edge), per edge side
+// [q][side][E] (side 0: et0, 1: et1)
 struct DevAdj {
   const double* obar;  // [3][T] adjoint of the stage output (w, u, v)
   double* sbar;        // [3][T] adjoint of the stage input S
   double* u0bar;       // [3][T] adjoint of the step's first state U0
-  double* fs;          // [3][3][T] flux adjoint, per cell side
+  double* rb;          // [3][T] adjoint of the cell's conserved update r
   double* dtd_own;     // [T] draining-dt adjoint from the cell's own update
   double* dtd_nb;      // [3][T] ... of neighbour k, from this cell's update
   double* m_cell;      // [5][T] MUSCL (w0, gx, gy, u0, v0) adjoint, own update
-  double* m_edge;      // [5][2][E] MUSCL adjoint, per edge side (0: et0, 1: et1)
-  double* ez_edge;     // [E] edge-midpoint z adjoint
-  double* hf_edge;     // [E] wall edges: adjoint of the et0 cell's depth
+  double* sl_edge;     // [3][2][E] adjoint of the edge slot (w, u, v) per side;
+                       // wall edges: side 0 w holds the et0 cell's depth adjoint
+  double* ez_edge;     // [E] edge-midpoint z adjoint of the flux itself
   double* w_nb;        // [3][T] w adjoint of neighbour k from this cell's plane
-  double* zc;          // [3][T] node-z adjoint per cell corner
+  double* zc;          // [3][T] node-z adjoint per cell corner, summed over stages
+  int8_t* back;        // [3][T] local index of this cell in neighbour k
+  int8_t* eloc;        // [2][E] local index of edge e in its et0 / et1 cell
   unsigned int* flags; // states outside the supported regime
 };
 
@@ -171,13 +173,22 @@ hipError_t launch_integrals(const DevMesh& M, const DevState& S, const DevWork& 
 // sqrt(h)): partial needs 5 * integrals_blocks(M) doubles, out 5
 hipError_t launch_stats(const DevMesh& M, const DevState& S, double* partial, double* out,
                         hipStream_t st);
-// reverse of one stage: from J.obar to J.sbar and J.u0bar, adding the node
-// bathymetry adjoint to zbar (if not null); the forward intermediates of the
-// stage state S must be in W (recon1, recon2, flux, drain already run)
+// reverse of one stage: from J.obar (whose w part still owes the previous
+// reverse stage's neighbour terms wnb_in, when not null) to J.sbar (whose w
+// part owes J.w_nb), accumulating the node-z adjoint per cell corner in
+// J.zc.  u0mode: kU0Accumulate adds this stage's U0 adjoint to J.u0bar
+// (else it is set, for stages that combine U0), kU0Into adds J.u0bar into
+// J.sbar (the first stage, whose input is U0).  The forward intermediates of
+// the stage state S must be in W (recon1, recon2, flux, drain of S).
+constexpr int kU0Accumulate = 1, kU0Into = 2;
 hipError_t launch_adjoint_stage(const DevMesh& M, const DevState& S, const DevState& U0, const DevWork& W,
                                 const StageArgs& A, const DevAdj& J, int flux, int ws, int opts,
-                                double* zbar, hipStream_t st);
-hipError_t launch_axpy(const double* x, double* y, int64_t n, hipStream_t st);  // y += x
+                                const double* wnb_in, int u0mode, hipStream_t st);
+// the mesh tables J.back and J.eloc (once per mesh)
+hipError_t launch_adjoint_tables(const DevMesh& M, const DevAdj& J, hipStream_t st);
+// after the last reverse stage: J.sbar's w part gathers J.w_nb; zbar (if not
+// null) += the corners of J.zc at every node
+hipError_t launch_adjoint_finish(const DevMesh& M, const DevAdj& J, double* zbar, hipStream_t st);
 hipError_t launch_rhs_one(const DevMesh& M, const DevState& S, const DevWork& W, int i, double dt,
                           double cor, double fric, double* out, hipStream_t st);
 

@@ -198,3 +198,39 @@ def ensemble(n_inst=256, total=20_000_000, seed=11, sizes=None):
     contiguous block of cells)."""
     ns = np.sort(ensemble_sizes(n_inst, total, seed=seed)) if sizes is None else sizes
     return [ensemble_member(n, seed * 7919 + k) for k, n in enumerate(ns)]
+
+
+def _smooth_noise(x, y, rng, sigma, modes=24, kmax=4):
+    """Smooth random field: a sum of `modes` Fourier modes with wave numbers
+    up to kmax, scaled to standard deviation sigma over the given points."""
+    f = np.zeros_like(x)
+    for _ in range(modes):
+        m, n = rng.integers(-kmax, kmax + 1, 2)
+        f += rng.standard_normal() * np.cos(2 * np.pi * (m * x + n * y) + rng.uniform(0, 2 * np.pi))
+    sd = f.std()
+    return f * (sigma / sd) if sd > 0 else f
+
+
+def adjoint_inversion(n=354, seed=5, n_gauges=64, noise=0.02, depth=1.0):
+    """Bathymetry inversion: a seeded Delaunay mesh (T = 2 n^2; n = 354 gives
+    250,632 cells), deep water over Gaussian bumps (every cell wet above its
+    highest vertex), a surface hump released at rest, `n_gauges` seeded gauge
+    cells, and a "true" bathymetry z + N(0, noise)-smoothed noise whose run
+    gives the gauge targets.  Returns (domain, initial state, true node
+    bathymetry, gauge cell ids)."""
+    xyz, topo = jittered_mesh(n, 0.25, seed)
+    rng = np.random.default_rng(seed)
+    x, y = xyz[:, 0], xyz[:, 1]
+    z = np.zeros_like(x)
+    for _ in range(6):
+        cx, cy = rng.uniform(0.15, 0.85, 2)
+        z += rng.uniform(0.1, 0.4) * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * rng.uniform(0.05, 0.15) ** 2))
+    z = z * wall_taper(x) * wall_taper(y)
+    xyz[:, 2] = z
+    z_true = z + _smooth_noise(x, y, rng, noise) * wall_taper(x) * wall_taper(y)
+    dom = Domain(xyz, topo)
+    c = dom.centroids()
+    w = depth + 0.1 * np.exp(-((c[:, 0] - 0.35) ** 2 + (c[:, 1] - 0.6) ** 2) / (2 * 0.08 ** 2))
+    prim = np.stack([w, np.zeros_like(w), np.zeros_like(w)], axis=1)
+    gauges = np.sort(rng.choice(prim.shape[0], n_gauges, replace=False))
+    return dom, prim, z_true, gauges

@@ -115,6 +115,71 @@ def test_gauge_loss_gradient_and_checkpoint_invariance():
     assert checked >= 2
 
 
+def draining_dt(sd, dom, prim):
+    """Per-cell draining dt h A / sum of outflows F0 L (with OPT_DRAIN_LENGTH)
+    from the GPU's fluxes of the state prim."""
+    sd.SetVolField(prim)
+    sd.ComputeInterfaceValues()
+    sd.ComputeFluxes()
+    F0 = sd.GetFluxes()[:, 0]
+    t = dom.topology.tables()
+    xyz = dom.geometry
+    L = np.hypot(*(xyz[t["ep"][:, 1], :2] - xyz[t["ep"][:, 0], :2]).T)
+    out = np.zeros(prim.shape[0])
+    for side, sg in ((0, 1.0), (1, -1.0)):
+        c = t["et"][:, side]
+        m = c >= 0
+        np.add.at(out, c[m], np.maximum(0.0, sg * F0[m] * L[m]))
+    h = prim[:, 0] - xyz[t["tp"], 2].mean(1)
+    with np.errstate(divide="ignore"):
+        return np.where(out > 1e-12, h * dom.areas() / out, np.inf)
+
+
+@pytest.mark.parametrize("scheme", ["Euler", "SSPRK2"])
+def test_adjoint_through_active_draining_dt(scheme):
+    """Shallow water with a rough velocity field at CFL 0.8: the draining dt
+    of some cells is below the stage dt, so the flux and depth terms of its
+    adjoint are exercised (OPT_DRAIN_LENGTH)."""
+    xyz, topo = swe.jittered_mesh(16, 0.25, 4)
+    rng = np.random.default_rng(4)
+    xyz[:, 2] = 0.01 * rng.random(xyz.shape[0])
+    dom = swe.Domain(xyz, topo)
+    c = dom.centroids()
+    T = c.shape[0]
+    prim0 = np.stack([0.06 + 0.01 * c[:, 0], rng.uniform(-0.4, 0.4, T), rng.uniform(-0.4, 0.4, T)], 1)
+    fl = swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt]
+    sd = swe.SpaceDisc(fl, dom, prim0, options=swe.OPT_DRAIN_LENGTH)
+    sd.SetVolField(prim0)
+    sd.ComputeInterfaceValues()
+    sd.ComputeFluxes()
+    dt = 0.8 * sd.GetMinLenToWavespeed()
+    active = int(np.sum(draining_dt(sd, dom, prim0) < dt))
+    print(f"{scheme}: dt {dt:.3e}, {active} of {T} cells with an active draining dt", flush=True)
+    assert active > 0
+    z = dom.geometry[:, 2].copy()
+    wvec = rng.standard_normal(prim0.shape)
+
+    def fn(p):
+        return float(np.sum(wvec * p)), wvec
+
+    nsteps = 2
+    J, s0, zb, ms = AD.run_and_adjoint(sd, prim0, z, nsteps, fn, scheme=scheme, dt=dt, checkpoint_every=1)
+    checked = 0
+    for trial in range(4):
+        dp = rng.standard_normal(prim0.shape) * np.array([1e-3, 1e-2, 1e-2])
+        dz = rng.standard_normal(z.shape) * 1e-3 if trial > 0 else np.zeros_like(z)
+        fd = directional_fd(sd, prim0, z, nsteps, fn, dp, dz, scheme, dt, 1e-6)
+        ad = float(np.sum(s0 * dp) + np.sum(zb * dz))
+        if fd is None:
+            print(f"draining {scheme} trial {trial}: the difference crossed a switch", flush=True)
+            continue
+        print(f"draining {scheme} trial {trial}: adjoint {ad:.12e} fd {fd:.12e} "
+              f"rel {abs(ad - fd) / abs(fd):.2e}", flush=True)
+        assert abs(ad - fd) <= 1e-5 * abs(fd) + 1e-12
+        checked += 1
+    assert checked >= 2
+
+
 def test_adjoint_refuses_shoreline_states():
     from tests import _common as C
     xyz, topo, prim = C.wet_dry_mesh(20, seed=2)
