@@ -230,21 +230,23 @@ int swe_state_stats(swe_solver* s, double* out5);
 
 /* ------------------------------------------------------------- adjoint --- */
 
-/* Adjoint of nsteps steps (no reference counterpart: the differentiation
- * of the full step asked of this port).  From the current state U_0, runs
- * the steps with fixed sizes (dt > 0 for every step, or dts[nsteps], e.g.
- * the sizes a CFL run logged, then held fixed), storing U_n every
- * checkpoint_every steps (<= 0: only U_0); U_N is the current state on
- * return.  Then propagates final_bar (T x 3 row-major adjoint of U_N's w, u,
- * v) back through every stage, recomputing each segment between
- * checkpoints from its checkpoint, into state0_bar (T x 3, adjoint of U_0)
- * and z_bar (V, adjoint of the node bathymetry; may be NULL).  Branch
- * decisions (wet/dry, limiter, wave-speed selections) are the forward
- * ones; the Riemann solver's local Jacobian is exact (dual numbers).
- * Supported: every cell fully wet, dry, or partially wet at or above its
- * highest vertex; shoreline cells below it give SWE_ERR_SOLVER (the state
- * is still advanced).  ms (optional, 2 doubles): forward and reverse sweep
- * milliseconds.  Single-mesh handles only. */
+/* Adjoint of nsteps steps (no reference counterpart).  From the current
+ * state U_0, runs the steps with fixed sizes (dt > 0 for every step, or
+ * dts[nsteps], e.g. the sizes a CFL run logged, then held fixed), storing
+ * U_n every checkpoint_every steps (<= 0 or >= nsteps: one segment); U_N is
+ * the current state on return.  Then propagates final_bar (T x 3 row-major
+ * adjoint of U_N's w, u, v) back through every stage into state0_bar (T x
+ * 3, adjoint of U_0) and z_bar (V, adjoint of the node bathymetry; may be
+ * NULL).  Each segment, last first, is recomputed from its checkpoint with
+ * every stage's intermediates kept on the device (about 81 bytes per cell
+ * and stage plus 24 per cell and stage state), then reversed; the last
+ * segment's recomputation is the forward run.  Branch decisions (wet/dry,
+ * limiter, wave-speed selections) are the forward ones; the Riemann
+ * solver's local Jacobian is exact (dual numbers).  Supported: every cell
+ * fully wet, dry, or partially wet at or above its highest vertex;
+ * shoreline cells below it give SWE_ERR_SOLVER (the state is still
+ * advanced).  ms (optional, 2 doubles): forward sweep and reverse sweep
+ * (recomputation included) milliseconds.  Single-mesh handles only. */
 int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int64_t nsteps,
                     int64_t checkpoint_every, const double* final_bar, double* state0_bar,
                     double* z_bar, double* ms);
