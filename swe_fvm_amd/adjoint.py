@@ -35,6 +35,25 @@ def run_and_adjoint(sd, prim0, z, nsteps, final_bar_fn, scheme="SSPRK2", dt=None
     return J, s0, zb, ms
 
 
+def batch_gauge_misfit(members):
+    """Sum over batch members of their gauge misfits: members is a list of
+    (gauge cell ids in the stacked domain, targets); returns fn like
+    gauge_misfit and the per-member values of the last call in fn.parts."""
+    fns = [gauge_misfit(g, t) for g, t in members]
+
+    def fn(prim):
+        g = np.zeros_like(prim)
+        parts = []
+        for f in fns:
+            j, gk = f(prim)
+            parts.append(j)
+            g += gk
+        fn.parts = parts
+        return float(np.sum(parts)), g
+    fn.parts = []
+    return fn
+
+
 def gauge_misfit(gauges, target):
     """J = sum over gauge cells of (w - target)^2 and its adjoint."""
     gauges = np.asarray(gauges)
@@ -74,5 +93,5 @@ if torch is not None:
             sd.SetVolField(ctx.p0)
             s0, zb, ms = sd.adjoint_run(gout.detach().cpu().numpy().astype(np.float64), nsteps, scheme=scheme,
                                         dt=dt, checkpoint_every=ck)
-            ctx.ms = ms
+            sd.last_adjoint_ms = ms  # forward-sweep and reverse-sweep milliseconds
             return torch.from_numpy(s0), torch.from_numpy(zb), None, None, None, None, None

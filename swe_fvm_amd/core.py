@@ -189,6 +189,30 @@ class Topology:
         return Topology.create(self.NumNodes(), t["ep"], t["et"], t["tp"], t["te"], t["tt"])
 
 
+def stack_domains(domains):
+    """One Domain holding the given domains side by side (a disjoint union:
+    node, edge and cell indices of member k offset by the sizes of members
+    0..k-1; boundary tags kept).  Every stepping kernel then advances all
+    members in one launch, and with a fixed dt each member's cells evolve
+    bit for bit as in a run of that member alone.  Returns (Domain, offsets)
+    with offsets = dict(nodes=, edges=, cells=) arrays of length n + 1."""
+    tabs = [d.topology.tables() for d in domains]
+    nv = np.cumsum([0] + [d.geometry.shape[0] for d in domains])
+    ne = np.cumsum([0] + [t["ep"].shape[0] for t in tabs])
+    nt = np.cumsum([0] + [t["tp"].shape[0] for t in tabs])
+
+    def cells(a, k):  # cell ids >= 0 shift, boundary tags stay
+        return np.where(a >= 0, a + nt[k], a)
+    ep = np.concatenate([t["ep"] + nv[k] for k, t in enumerate(tabs)])
+    et = np.concatenate([cells(t["et"], k) for k, t in enumerate(tabs)])
+    tp = np.concatenate([t["tp"] + nv[k] for k, t in enumerate(tabs)])
+    te = np.concatenate([t["te"] + ne[k] for k, t in enumerate(tabs)])
+    tt = np.concatenate([cells(t["tt"], k) for k, t in enumerate(tabs)])
+    topo = Topology.create(int(nv[-1]), ep, et, tp, te, tt)
+    geo = np.concatenate([d.geometry for d in domains])
+    return Domain(geo, topo), dict(nodes=nv, edges=ne, cells=nt)
+
+
 def _take_mesh(rc, xyz_p, nn, h):
     _check(rc)
     L = _lib.load()

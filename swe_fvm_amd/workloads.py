@@ -200,7 +200,7 @@ def ensemble(n_inst=256, total=20_000_000, seed=11, sizes=None):
     return [ensemble_member(n, seed * 7919 + k) for k, n in enumerate(ns)]
 
 
-def _smooth_noise(x, y, rng, sigma, modes=24, kmax=4):
+def smooth_field(x, y, rng, sigma, modes=24, kmax=4):
     """Smooth random field: a sum of `modes` Fourier modes with wave numbers
     up to kmax, scaled to standard deviation sigma over the given points."""
     f = np.zeros_like(x)
@@ -227,10 +227,32 @@ def adjoint_inversion(n=354, seed=5, n_gauges=64, noise=0.02, depth=1.0):
         z += rng.uniform(0.1, 0.4) * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * rng.uniform(0.05, 0.15) ** 2))
     z = z * wall_taper(x) * wall_taper(y)
     xyz[:, 2] = z
-    z_true = z + _smooth_noise(x, y, rng, noise) * wall_taper(x) * wall_taper(y)
+    z_true = z + smooth_field(x, y, rng, noise) * wall_taper(x) * wall_taper(y)
     dom = Domain(xyz, topo)
     c = dom.centroids()
+    # the hump over a smooth seeded swell (amplitude 2e-3): no region of the
+    # surface is exactly flat, where the limiter's decisions would make the
+    # step non-differentiable
     w = depth + 0.1 * np.exp(-((c[:, 0] - 0.35) ** 2 + (c[:, 1] - 0.6) ** 2) / (2 * 0.08 ** 2))
+    w = w + smooth_field(c[:, 0], c[:, 1], rng, 2e-3)
     prim = np.stack([w, np.zeros_like(w), np.zeros_like(w)], axis=1)
     gauges = np.sort(rng.choice(prim.shape[0], n_gauges, replace=False))
     return dom, prim, z_true, gauges
+
+
+def adjoint_inversion_batch(members=32, n=354, seed=5, n_gauges=64, noise=0.02, depth=1.0):
+    """A batch of bathymetry inversions on the mesh of adjoint_inversion:
+    member k has its own "true" bathymetry (z + N(0, noise)-smoothed noise,
+    seeded by (seed, k)) and its own seeded gauge cells; the initial state is
+    shared.  Returns dict(member=(domain, initial state), z=node bathymetry,
+    z_true=[V] * members, gauges=[local cell ids] * members)."""
+    dom, prim, z_true0, gauges0 = adjoint_inversion(n=n, seed=seed, n_gauges=n_gauges, noise=noise, depth=depth)
+    x, y = dom.geometry[:, 0], dom.geometry[:, 1]
+    taper = wall_taper(x) * wall_taper(y)
+    z = dom.geometry[:, 2].copy()
+    zt, gs = [z_true0], [gauges0]
+    for k in range(1, members):
+        rng = np.random.default_rng([seed, k])
+        zt.append(z + smooth_field(x, y, rng, noise) * taper)
+        gs.append(np.sort(rng.choice(prim.shape[0], n_gauges, replace=False)))
+    return dict(member=(dom, prim), z=z, z_true=zt, gauges=gs)

@@ -164,3 +164,25 @@ def test_classic_thacker_initial_state_error_is_small_and_converges():
         nrm = A.error_norms(_FakeSD(dom, orc, prim), th, 0.0, n_partial=10)
         errs.append(nrm["L1"][0])
     assert errs[0] < 2e-2 and errs[1] < 0.35 * errs[0], errs
+
+
+def test_stack_domains_tables():
+    """The stacked domain of a batch: sizes add up, member k's tables are
+    its own shifted by the offsets, boundary tags are kept."""
+    import swe_fvm_amd as swe
+    a = swe.Domain(*swe.jittered_mesh(6, 0.2, 1))
+    xyz, topo = swe.jittered_mesh(5, 0.2, 2)
+    b = swe.Domain(xyz, topo.with_boundary_tag(np.nonzero(topo.tables()["et"][:, 1] < 0)[0][:3], swe.FREE_FLOW))
+    big, off = swe.stack_domains([a, b, a])
+    t, ta, tb = big.topology.tables(), a.topology.tables(), b.topology.tables()
+    assert list(off["cells"]) == [0, 72, 122, 194] and list(off["nodes"]) == [0, 49, 85, 134]
+    assert big.topology.NumEdges() == 2 * a.topology.NumEdges() + b.topology.NumEdges()
+    e0, e1 = off["edges"][1], off["edges"][2]
+    assert np.array_equal(t["tp"][72:122], tb["tp"] + 49)
+    assert np.array_equal(t["te"][72:122], tb["te"] + e0)
+    et = t["et"][e0:e1]
+    assert np.array_equal(et[:, 0], tb["et"][:, 0] + 72)
+    assert np.array_equal(et[:, 1], np.where(tb["et"][:, 1] >= 0, tb["et"][:, 1] + 72, tb["et"][:, 1]))
+    assert np.sum(et[:, 1] == swe.FREE_FLOW) == 3
+    assert np.array_equal(t["tt"][122:], np.where(ta["tt"] >= 0, ta["tt"] + 122, ta["tt"]))
+    assert np.array_equal(big.geometry[85:], a.geometry)
