@@ -175,10 +175,13 @@ def bench_adjoint(args):
     adjoint step per forward step, recomputation included;
     gradient_cost_ratio = (forward sweep + reverse sweep) / forward.
     Checks: member 0 run alone reproduces its slice of the batch bit for
-    bit, and its gradient against central differences in args.adj_fd random
-    directions (bathymetry and initial surface together), over the full
-    horizon and over args.adj_check_steps steps, at eps and eps/4 (a pair
-    that disagrees crossed one of the step's switches: inconclusive)."""
+    bit; in args.adj_fd random directions (bathymetry and initial surface
+    together) its gradient satisfies the duality with the tangent of the
+    dual-number build of the forward over the full horizon, and is compared
+    with central differences over the full horizon and over
+    args.adj_check_steps steps, at eps and eps/4 (a pair that disagrees
+    crossed one of the step's switches, the limiter's all-or-nothing
+    decisions: inconclusive)."""
     import torch
 
     import swe_fvm_amd as swe
@@ -252,6 +255,21 @@ def bench_adjoint(args):
         return {"steps": nsteps, "directions": len(res), "conclusive": len(rels),
                 "max_rel": max(rels) if rels else None, "results": res}
     f0 = AD.gauge_misfit(g0, t0v)
+    # duality with the dual-number build of the forward, full horizon:
+    # <dJ/dU_N, U_N'> = <dJ/dU_0, U_0'> + <dJ/dz, z'> for every direction
+    ybar = f0(fin_fwd[:T1])[1]
+    dual = []
+    tl_ms = 0.0
+    for dz, dp in dirs:
+        sd1.set_bathymetry(z1)
+        sd1.SetVolField(prim1)
+        tdot, tms = sd1.tangent_run(dp, N, z_dot=dz, scheme="SSPRK2", dt=dt)
+        tl_ms += tms
+        lhs = float(np.sum(ybar * tdot))
+        rhs = float(np.sum(gp[:T1] * dp) + np.sum(gz[:V1] * dz))
+        dual.append({"tangent": lhs, "adjoint": rhs, "rel": abs(lhs - rhs) / max(abs(lhs), 1e-300)})
+    tangent = {"steps": N, "directions": len(dual), "max_rel": max(r["rel"] for r in dual),
+               "tangent_run_ms": tl_ms / max(len(dual), 1), "results": dual}
     fd_full = fd_checks(N, f0, gz[:V1], gp[:T1])
     nc = args.adj_check_steps
     fc = AD.gauge_misfit(g0, run1(d["z_true"][0], prim1, nc)[g0, 0].copy())
@@ -264,7 +282,7 @@ def bench_adjoint(args):
         "forward_ms": ms_fwd, "adjoint_forward_sweep_ms": ms[0], "adjoint_reverse_sweep_ms": ms[1],
         "forward_cell_updates_per_s": B * T1 * N / (ms_fwd * 1e-3),
         "loss": float(loss.item()), "member0_slice_bitwise": slice_same,
-        "fd_full_horizon": fd_full, "fd_short_horizon": fd_short,
+        "tangent_duality": tangent, "fd_full_horizon": fd_full, "fd_short_horizon": fd_short,
         "config": {"workload": f"bathymetry_inversion_batch{B}_T{T1}", "members": B, "cells": B * T1,
                    "steps": N, "dt": dt, "gauges_per_member": int(len(g0)), "checkpoint_every": ck,
                    "scheme": "SSPRK2", "flux": "HLLC<Einfeldt>", "gradient": "torch.autograd.Function"},

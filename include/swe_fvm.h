@@ -250,6 +250,18 @@ int swe_state_stats(swe_solver* s, double* out5);
 int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int64_t nsteps,
                     int64_t checkpoint_every, const double* final_bar, double* state0_bar,
                     double* z_bar, double* ms);
+/* Tangent-linear run (no reference counterpart): the directional
+ * derivative of nsteps steps of fixed sizes (dt > 0, or dts[nsteps]) along
+ * state0_dot (T x 3 row-major tangent of the initial w, u, v) and z_dot (V
+ * tangent of the node bathymetry; NULL: zero), computed by the stage kernels
+ * built on dual numbers.  Their value parts are the forward step's, bit for
+ * bit: U_N is the current state on return.  final_dot (T x 3) receives the
+ * tangent of U_N.  Same regime as swe_adjoint_run (SWE_ERR_SOLVER outside
+ * it); with the adjoint it satisfies <final_bar, final_dot> =
+ * <state0_bar, state0_dot> + <z_bar, z_dot>.  ms (optional, 1 double):
+ * milliseconds.  Single-mesh handles only. */
+int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int64_t nsteps,
+                    const double* state0_dot, const double* z_dot, double* final_dot, double* ms);
 /* Replaces the node bathymetry z (V values) and recomputes the static
  * geometry; the state is kept (Domain's z, include/Bathymetry.h:43). */
 int swe_set_bathymetry(swe_solver* s, const double* z);

@@ -404,6 +404,24 @@ class SpaceDisc:
                                            _ptr(zb) if want_z else None, _ptr(ms)))
         return s0, zb, ms
 
+    def tangent_run(self, state0_dot, nsteps, z_dot=None, scheme="SSPRK2", dt=None, dts=None):
+        """Runs nsteps steps from the current state with fixed step sizes,
+        carrying the tangent (state0_dot: T x 3, z_dot: V or None) through
+        the dual-number build of the stage kernels: returns (tangent of the
+        final state T x 3, ms).  The final state is current afterwards and
+        equals run()'s bit for bit."""
+        sd0 = np.ascontiguousarray(state0_dot, dtype=np.float64).reshape(self.T, 3)
+        zd = None if z_dot is None else np.ascontiguousarray(z_dot, dtype=np.float64).reshape(self.V)
+        out = np.zeros((self.T, 3), np.float64)
+        ms = np.zeros(1, np.float64)
+        d = None
+        if dts is not None:
+            d = np.ascontiguousarray(dts, dtype=np.float64).reshape(int(nsteps))
+        _check(_lib.load().swe_tangent_run(self._h, SCHEMES[scheme], float(dt or 0.0),
+                                           _ptr(d) if d is not None else None, int(nsteps), _ptr(sd0),
+                                           _ptr(zd) if zd is not None else None, _ptr(out), _ptr(ms)))
+        return out, float(ms[0])
+
     def Stats(self):
         """Health of the current state (failure detection): dict with the
         number of cells holding a non-finite value, min / max depth, max

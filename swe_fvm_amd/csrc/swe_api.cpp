@@ -1174,6 +1174,108 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
   return rc;
 }
 
+// Tangent-linear run (no reference counterpart): the directional
+// derivative of nsteps steps (fixed sizes dt or dts[n]) along the initial
+// state tangent state0_dot (T x 3 row-major) and the node bathymetry tangent
+// z_dot (V, may be null: zero), by the stage kernels built on dual numbers.
+// The values advance as the forward step's, bit for bit, and U_N is the
+// current state on return; final_dot receives the tangent of U_N (T x 3).
+// ms (optional): milliseconds.
+int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int64_t nsteps,
+                    const double* state0_dot, const double* z_dot, double* final_dot, double* ms) {
+  if (int rc = single_mesh(s)) return rc;
+  if (!state0_dot || !final_dot) return fail(SWE_ERR_ARG, "null argument");
+  if (scheme < 0 || scheme > 2) return fail(SWE_ERR_ARG, "unknown scheme");
+  if (nsteps < 1) return fail(SWE_ERR_ARG, "nsteps must be >= 1");
+  if (!dts && !(dt > 0.)) return fail(SWE_ERR_ARG, "the tangent run needs fixed step sizes (dt > 0 or dts)");
+  if (dts)
+    for (int64_t n = 0; n < nsteps; ++n)
+      if (!(dts[n] > 0.)) return fail(SWE_ERR_ARG, "step sizes must be positive");
+  if (s->opts & (SWE_OPT_HEAD_GRADIENT | SWE_OPT_HEAD_PARTWET2))
+    return fail(SWE_ERR_ARG, "the tangent run does not support the HEAD-compatibility options");
+  if (int rc = set_device(s)) return rc;
+  const int64_t T = s->T, E = s->E, V = s->V;
+  // tangent states (three, rotating like the values), node z tangent,
+  // intermediates, flag counter
+  const int64_t n8 = 9 * T + V + 4 * T + 3 * E;
+  double* buf = nullptr;
+  unsigned int* flags = nullptr;
+  SWE_CHECK(hipMalloc((void**)&buf, sizeof(double) * size_t(n8)), "hipMalloc (tangent)");
+  if (hipMalloc((void**)&flags, sizeof(unsigned int)) != hipSuccess) {
+    hipFree(buf);
+    return fail(SWE_ERR_DEVICE, "hipMalloc (tangent)");
+  }
+  double* q = buf;
+  double* td[3];
+  for (int k = 0; k < 3; ++k) { td[k] = q; q += 3 * T; }
+  double* zd = q; q += V;
+  DevTan Q{};
+  Q.zd = zd;
+  Q.mw = q; q += T;
+  Q.mgx = q; q += T;
+  Q.mgy = q; q += T;
+  Q.dtd = q; q += T;
+  Q.f0 = q; q += E;
+  Q.f1 = q; q += E;
+  Q.f2 = q; q += E;
+  Q.flags = flags;
+  auto tst = [&](int k) { return DevState{td[k], td[k] + T, td[k] + 2 * T}; };
+  auto tout = [&](int k) { return DevOut{td[k], td[k] + T, td[k] + 2 * T}; };
+  std::vector<double> h(static_cast<size_t>(3 * T));
+  for (int64_t i = 0; i < T; ++i)
+    for (int c = 0; c < 3; ++c) h[size_t(c * T + i)] = state0_dot[3 * i + c];
+  std::vector<double> hz(static_cast<size_t>(std::max<int64_t>(V, 1)), 0.);
+  if (z_dot) std::copy(z_dot, z_dot + V, hz.begin());
+  int rc = SWE_OK;
+  hipError_t e = hipMemcpyAsync(td[s->cur], h.data(), sizeof(double) * size_t(3 * T), hipMemcpyHostToDevice,
+                                s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(zd, hz.data(), sizeof(double) * size_t(V), hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(flags, 0, sizeof(unsigned int), s->stream);
+  StageCoef sc[3];
+  const int ns = stage_list(scheme, sc);
+  hipEvent_t ev[2];
+  for (auto& x : ev) hipEventCreate(&x);
+  hipEventRecord(ev[0], s->stream);
+  for (int64_t n = 0; n < nsteps && e == hipSuccess; ++n) {
+    // the slots of run_step: stage k reads the previous stage's output (the
+    // step input a for k = 0) and writes outs[k]; U0 is slot a
+    const int a = s->cur, b = (a + 1) % 3, c = (a + 2) % 3;
+    const int outs[3] = {b, ns == 2 ? a : c, a};
+    for (int k = 0; k < ns && e == hipSuccess; ++k) {
+      const int in = k == 0 ? a : outs[k - 1];
+      StageArgs A{sc[k].a0, sc[k].a1, sc[k].coef, dts ? dts[n] : dt, 0., s->cor, s->rbits, nullptr, friction(s)};
+      e = launch_tangent_stage(s->M, state_of(s, in), tst(in), state_of(s, a), tst(a), out_of(s, outs[k]),
+                               tout(outs[k]), s->W, Q, A, s->flux, s->ws, s->opts, s->stream);
+    }
+    s->cur = ns == 1 ? b : a;
+  }
+  hipEventRecord(ev[1], s->stream);
+  unsigned int nflag = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), td[s->cur], sizeof(double) * size_t(3 * T), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&nflag, flags, sizeof(unsigned int), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) {
+    rc = dev_fail(e, "tangent run");
+  } else {
+    for (int64_t i = 0; i < T; ++i)
+      for (int c = 0; c < 3; ++c) final_dot[3 * i + c] = h[size_t(c * T + i)];
+    if (ms) {
+      float f = 0.f;
+      hipEventElapsedTime(&f, ev[0], ev[1]);
+      *ms = f;
+    }
+  }
+  for (auto& x : ev) hipEventDestroy(x);
+  hipStreamSynchronize(s->stream);
+  hipFree(buf);
+  hipFree(flags);
+  if (!rc && nflag)
+    return fail(SWE_ERR_SOLVER, "tangent: " + std::to_string(nflag) +
+                                    " cell stages outside the supported regime (partially wet cells below "
+                                    "their highest vertex)");
+  return rc;
+}
+
 // Replaces the node bathymetry (V values) and recomputes the static
 // geometry; the state (w, u, v) is kept.
 int swe_set_bathymetry(swe_solver* s, const double* z) {

@@ -132,6 +132,20 @@ struct DevAdj {
   unsigned int* flags; // states outside the supported regime
 };
 
+// tangents of the tangent-linear step (swe_tangent.hip): the node
+// bathymetry's (input) and the stage intermediates' (written per stage)
+struct DevTan {
+  const double* zd;    // [V] node z tangent
+  double* mw;          // [T] MUSCL level, gradient tangents
+  double* mgx;
+  double* mgy;
+  double* f0;          // [E] flux tangents
+  double* f1;
+  double* f2;
+  double* dtd;         // [T] draining-dt tangent
+  unsigned int* flags; // states outside the supported regime
+};
+
 struct StageArgs {
   double a0, a1;       // Shu-Osher weights of U0 and the stage state (a0 = 0: U + RHS)
   double coef;         // stage dt = coef * dt
@@ -184,6 +198,13 @@ constexpr int kU0Accumulate = 1, kU0Into = 2;
 hipError_t launch_adjoint_stage(const DevMesh& M, const DevState& S, const DevState& U0, const DevWork& W,
                                 const StageArgs& A, const DevAdj& J, int flux, int ws, int opts,
                                 const double* wnb_in, int u0mode, hipStream_t st);
+// one stage of the tangent-linear step: values (S, U0 -> O, intermediates
+// in W) exactly as the forward stage, tangents (Sd, U0d, Q.zd -> Od,
+// intermediates in Q); fixed dt (A.dt_fixed)
+hipError_t launch_tangent_stage(const DevMesh& M, const DevState& S, const DevState& Sd, const DevState& U0,
+                                const DevState& U0d, const DevOut& O, const DevOut& Od, const DevWork& W,
+                                const DevTan& Q, const StageArgs& A, int flux, int ws, int opts,
+                                hipStream_t st);
 // the mesh tables J.back and J.eloc (once per mesh)
 hipError_t launch_adjoint_tables(const DevMesh& M, const DevAdj& J, hipStream_t st);
 // after the last reverse stage: J.sbar's w part gathers J.w_nb; zbar (if not

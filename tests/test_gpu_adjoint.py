@@ -180,6 +180,46 @@ def test_adjoint_through_active_draining_dt(scheme):
     assert checked >= 2
 
 
+@pytest.mark.parametrize("scheme,flux,opts", [("Euler", "HLLC", 0), ("SSPRK2", "HLLC", 0), ("SSPRK3", "HLL", 0),
+                                               ("SSPRK2", "HLL", swe.OPT_FRICTION),
+                                               ("SSPRK2", "HLLC", swe.OPT_DRAIN_LENGTH)])
+def test_tangent_linear_duality(scheme, flux, opts):
+    """The dual-number build of the stage kernels (swe_tangent_run): its
+    values are the forward's bit for bit, its tangent agrees with central
+    differences, and with the adjoint it satisfies the duality
+    <ybar, J xdot> = <J^T ybar, xdot> to round-off."""
+    dom, prim0 = wet_case(n=16, seed=3)
+    fl = getattr(swe.Fluxes, flux)[swe.Wavespeeds.Einfeldt]
+    sd = swe.SpaceDisc(fl, dom, prim0, cor=0.3, tau=0.5, options=opts)
+    z = dom.geometry[:, 2].copy()
+    rng = np.random.default_rng(11)
+    n, dt = 8, 2e-3
+    sd.SetVolField(prim0)
+    sd.run(scheme, n, dt=dt)
+    ref = sd.GetVolField()
+    for trial in range(3):
+        pdot = rng.standard_normal(prim0.shape) * np.array([1e-2, 1e-2, 1e-2])
+        zdot = rng.standard_normal(z.shape) * 1e-2 if trial > 0 else None
+        sd.SetVolField(prim0)
+        tdot, _ = sd.tangent_run(pdot, n, z_dot=zdot, scheme=scheme, dt=dt)
+        assert np.array_equal(sd.GetVolField(), ref)
+        ybar = rng.standard_normal(prim0.shape)
+        sd.SetVolField(prim0)
+        s0, zb, _ = sd.adjoint_run(ybar, n, scheme=scheme, dt=dt, checkpoint_every=3)
+        lhs = float(np.sum(ybar * tdot))
+        rhs = float(np.sum(s0 * pdot) + (np.sum(zb * zdot) if zdot is not None else 0.0))
+        scale = float(np.sum(np.abs(ybar * tdot)))
+        print(f"{scheme} {flux} opts {opts} trial {trial}: <ybar, J xdot> {lhs:.15e} <J^T ybar, xdot> {rhs:.15e} "
+              f"diff/scale {abs(lhs - rhs) / scale:.1e}", flush=True)
+        assert abs(lhs - rhs) <= 1e-12 * scale
+        # the tangent against central differences of the forward
+        fn = lambda p, c=ybar: (float(np.sum(c * p)), c)
+        fd = directional_fd(sd, prim0, z, n, fn, pdot, zdot if zdot is not None else np.zeros_like(z),
+                            scheme, dt, 1e-6)
+        if fd is not None:
+            assert abs(fd - lhs) <= 1e-5 * abs(fd) + 1e-12
+
+
 def test_stacked_batch_equals_members():
     """A batch (stack_domains) gives each member's forward and adjoint bit
     for bit: the kernels never mix members."""
