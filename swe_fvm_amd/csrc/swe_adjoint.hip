@@ -9,7 +9,7 @@
 // is bit-identical to the forward step.  Branch
 // decisions (wet/dry, the limiter's all-or-nothing choices, min / max
 // selections) are the forward ones; the flux's local Jacobian comes from
-// the flux template instantiated on Dual<7>.
+// the flux template instantiated on Dual<6>.
 //
 // Every contribution is written to a slot owned by one (cell, side) or
 // (edge, side) and gathered in a fixed order by its consumer, so the
@@ -334,7 +334,8 @@ __device__ inline void side_flux_bar(const DevMesh& M, const DevState& S, const 
 // ----------------------------------------------------------------- flux --
 // Reverse of k_flux for edge e: the flux adjoint of its two sides (from the
 // cells' update adjoints), through the Riemann solver's local Jacobian (the
-// flux template on Dual<7>: inputs L (w, u, v), R (w, u, v), edge z), into
+// flux template on Dual<6>: inputs L (w, u, v), R (w, u, v); edge z by the
+// chain rule), into
 // the adjoints of the two slot values; k_adj_recon takes them through the
 // slot reconstruction.
 template <int FLUX, int WS, bool LEN>
@@ -372,15 +373,19 @@ __global__ void __launch_bounds__(256) k_adj_flux(DevMesh M, DevState S, DevWork
     const double nx = ty, ny = -tx;
     const Slot L = slot_value(mf, M.cx[f], M.cy[f], E);
     const Slot R = slot_value(mt, M.cx[tr], M.cy[tr], E);
-    using D = Dual<7>;
-    const SlotT<D> Ld{dvar<7>(L.w, 0), dvar<7>(L.u, 1), dvar<7>(L.v, 2)};
-    const SlotT<D> Rd{dvar<7>(R.w, 3), dvar<7>(R.u, 4), dvar<7>(R.v, 5)};
-    const D be = dvar<7>(E.z, 6);
+    // the edge bathymetry enters the fluxer only through hl = L.w - z and
+    // hr = R.w - z (include/Fluxes.h:18-21,63-66): dF/dz = -(dF/dL.w +
+    // dF/dR.w), so six directions carry the Jacobian
+    using D = Dual<6>;
+    const SlotT<D> Ld{dvar<6>(L.w, 0), dvar<6>(L.u, 1), dvar<6>(L.v, 2)};
+    const SlotT<D> Rd{dvar<6>(R.w, 3), dvar<6>(R.u, 4), dvar<6>(R.v, 5)};
+    const D be = dconst<6>(E.z);
     D Fd[3];
     double lw = 1.;
     if (FLUX == 0) flux_hll<WS>(nx, ny, be, Ld, Rd, M.edmin[e], cor, lw, Fd);
     else flux_hllc<WS>(tx, ty, be, Ld, Rd, M.edmin[e], cor, lw, Fd);
-    for (int k = 0; k < 7; ++k) g[k] = Fb[0] * Fd[0].d[k] + Fb[1] * Fd[1].d[k] + Fb[2] * Fd[2].d[k];
+    for (int k = 0; k < 6; ++k) g[k] = Fb[0] * Fd[0].d[k] + Fb[1] * Fd[1].d[k] + Fb[2] * Fd[2].d[k];
+    g[6] = -(g[0] + g[3]);
     if (t < 0) {  // free flow: R is L
       for (int k = 0; k < 3; ++k) {
         g[k] += g[3 + k];
