@@ -53,6 +53,7 @@ __global__ void __launch_bounds__(256) k_setup_cells(DevMesh M, DevGeomOut G) {
   r.fl = M.cflag[i];
   r.pad = 0;
   G.crec[i] = r;
+  G.nrec[i] = NRec{c.x, c.y, (r.fl & CELL_BND) ? INFINITY : r.zmax, c.z};
 }
 
 // min over the edge's two cells of 2 area / length (the flux's CFL length)
@@ -75,27 +76,40 @@ __global__ void __launch_bounds__(256) k_setup_edges(DevMesh M, DevGeomOut G) {
 // (src/MUSCLObject.cpp:31-112).
 template <bool HEAD_GRAD>
 __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W) {
-  int64_t i = M.c0 + global_index();
-  if (i >= M.c1) return;
+  // The workgroup's 256 Morton-consecutive cells stage their level and
+  // neighbour record in LDS; most neighbours lie in the same workgroup and
+  // are read from there, the others from global memory.
+  __shared__ double s_w[kBlock];
+  __shared__ NRec s_r[kBlock];
+  const int64_t blk0 = M.c0 + (int64_t)logical_block() * kBlock;
+  const int64_t i = blk0 + threadIdx.x;
+  const bool live = i < M.c1;
+  if (live) {
+    s_w[threadIdx.x] = S.w[i];
+    s_r[threadIdx.x] = M.nrec[i];
+  }
+  __syncthreads();
+  if (!live) return;
   if (M.cinst && !M.bactive[M.cinst[i]]) return;  // finished ensemble member
-  // every input of every branch is loaded up front, so the gathers overlap
-  const double w = S.w[i];
+  const double w = s_w[threadIdx.x];
   const uint32_t fl = M.cflag[i];
   const V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[M.T + i]), p2 = node(M, M.tp[2 * M.T + i]);
-  double wn[3], bcn[3], zmxn[3], zmnn[3], cxn[3], cyn[3];
-  uint32_t fln[3];
+  double wn[3];
+  NRec rn[3];
+  int jn[3];
 #pragma unroll
   for (int kk = 0; kk < 3; ++kk) {
     int j = M.tt[kk * M.T + i];
     j = j < 0 ? (int)i : j;  // boundary cells never take the full-wet branch
-    wn[kk] = S.w[j];
-    const CellRec rj = M.crec[j];
-    fln[kk] = rj.fl;
-    bcn[kk] = rj.bc;
-    zmxn[kk] = rj.zmax;
-    zmnn[kk] = rj.zmin;
-    cxn[kk] = rj.cx;
-    cyn[kk] = rj.cy;
+    jn[kk] = j;
+    const int64_t lj = j - blk0;
+    if (lj >= 0 && lj < kBlock && j < M.c1) {
+      wn[kk] = s_w[lj];
+      rn[kk] = s_r[lj];
+    } else {
+      wn[kk] = S.w[j];
+      rn[kk] = M.nrec[j];
+    }
   }
   const CellRec ri = M.crec[i];
   const double bc = ri.bc;
@@ -120,15 +134,16 @@ __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W
 #pragma unroll
     for (int kk = 0; kk < 3; ++kk) {
       E[kk] = edge_mid(p0, p1, p2, fl, kk);
-      const double wj = wn[kk], bcj = bcn[kk];
-      if (!(fln[kk] & CELL_BND) && zmxn[kk] < wj) {
-        pts[kk] = {cxn[kk], cyn[kk], HEAD_GRAD ? bcj : wj};
+      const double wj = wn[kk], bcj = rn[kk].bc;
+      if (rn[kk].zmx < wj) {  // interior full-wet neighbour (zmx = +inf on boundary cells)
+        pts[kk] = {rn[kk].cx, rn[kk].cy, HEAD_GRAD ? bcj : wj};
       } else if (!is_wet(wj - bcj)) {
         any_dry = true;
         pts[kk] = {0., 0., 0.};
       } else {
-        double w0j = partwet1_w(zmnn[kk], zmnn[kk], zmxn[kk], bcj, wj);
-        double dx = E[kk].x - cxn[kk], dy = E[kk].y - cyn[kk];
+        const CellRec rj = M.crec[jn[kk]];  // partially wet neighbour: the full record
+        double w0j = partwet1_w(rj.zmin, rj.zmin, rj.zmax, bcj, wj);
+        double dx = E[kk].x - rn[kk].cx, dy = E[kk].y - rn[kk].cy;
         double a = w0j + (0. * dx + 0. * dy);
         if (!((a - E[kk].z) >= 0)) a = E[kk].z;
         double val = 0.5 * (w + a);
