@@ -15,7 +15,7 @@ def read_gz_text(name):
 
 
 def parse_msh_text(text):
-    """Minimal Gmsh 4.1 ASCII parser (test-side, independent of the library)."""
+    """Minimal Gmsh 4.1 / 4.2 ASCII parser (test-side, independent of the library)."""
     lines = text.split("\n")
     i = 0
     nodes, bnd, tris = {}, [], []
@@ -69,12 +69,23 @@ class Golden:
 
     def bowl_path(self):
         """bowl.msh decompressed into a temporary file (for the library reader)."""
-        if "bowl_path" not in self._cache:
+        return self.msh_path("bowl.msh.gz")
+
+    def msh_path(self, name):
+        """A gzipped mesh fixture decompressed into a temporary file."""
+        key = "path:" + name
+        if key not in self._cache:
             fd, path = tempfile.mkstemp(suffix=".msh")
             with os.fdopen(fd, "w") as f:
-                f.write(read_gz_text("bowl.msh.gz"))
-            self._cache["bowl_path"] = path
-        return self._cache["bowl_path"]
+                f.write(read_gz_text(name))
+            self._cache[key] = path
+        return self._cache[key]
+
+    def basic(self):
+        """notebooks/basic.msh: (xyz, triangles, boundary lines) by the test-side parser."""
+        if "basic" not in self._cache:
+            self._cache["basic"] = parse_msh_text(read_gz_text("basic.msh.gz"))
+        return self._cache["basic"]
 
     def topology(self):
         if "topo" not in self._cache:

@@ -2,6 +2,7 @@
 
 Reads (never imports or runs) these data files of the reference checkout:
   examples/bowl.msh        Gmsh 4.1 mesh, 7555 nodes, 14788 triangles
+  notebooks/basic.msh      Gmsh 4.2 mesh, 160 nodes, four named walls
   notebooks/topology.dat   tables written from bowl.msh by examples/Main.cpp:146-165
   notebooks/geometry.dat   node x y z written by examples/Main.cpp:136-144
   notebooks/out0.dat       w hu hv before one Euler step (examples/Main.cpp:192)
@@ -19,7 +20,7 @@ import sys
 REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
 OUT = os.path.dirname(os.path.abspath(__file__))
 
-FILES = ["examples/bowl.msh", "notebooks/topology.dat", "notebooks/geometry.dat",
+FILES = ["examples/bowl.msh", "notebooks/basic.msh", "notebooks/topology.dat", "notebooks/geometry.dat",
          "notebooks/out0.dat", "notebooks/out1.dat"]
 
 for rel in FILES:

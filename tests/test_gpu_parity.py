@@ -56,6 +56,23 @@ def test_bowl_gauss_wave_euler(golden):
         assert_state_close(sd, orc, xyz, tp, what=f"bowl Euler opts={opts}")
 
 
+def test_basic_msh_matches_oracle(golden):
+    """notebooks/basic.msh (four named walls) read by the library: a hump
+    over a sloping bottom with a dry corner, SSP-RK2 in CFL mode, both
+    fluxes, against the oracle."""
+    xyz, topo = swe.read_msh(golden.msh_path("basic.msh.gz"))
+    xyz[:, 2] = 0.6 * (xyz[:, 0] + xyz[:, 1] + 2.0) / 4.0
+    tp = topo.tables()["tp"]
+    c = xyz[tp].mean(1)
+    w = 0.8 + 0.2 * np.exp(-8.0 * (c[:, 0] ** 2 + c[:, 1] ** 2))
+    prim = np.stack([np.maximum(w, C.cell_bath(xyz, tp)), 0.05 * np.ones(len(c)), np.zeros(len(c))], 1)
+    for flux in ("HLL", "HLLC"):
+        sd, orc = make_pair(xyz, topo, prim, flux, "Einfeldt")
+        sd.run("SSPRK2", 40, dt=0.0, cfl=0.3)
+        orc.run("SSPRK2", 40, dt=0.0, cfl=0.3)
+        assert_state_close(sd, orc, xyz, tp, what=f"basic.msh {flux}")
+
+
 @pytest.mark.parametrize("flux,wave", FLUXES)
 def test_interface_values_and_fluxes(flux, wave):
     """ComputeInterfaceValues + ComputeFluxes getters on a wet/dry mesh."""
