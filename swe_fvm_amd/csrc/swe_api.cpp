@@ -234,7 +234,6 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
   ALLOC(G.area, T);
   ALLOC(G.edmin, E);
   ALLOC(G.crec, T);
-  ALLOC(G.nrec, T);
 #undef ALLOC
 #define H2D(dst, src, n)                                                                     \
   if ((e = hipMemcpy(dst, src, sizeof(*(src)) * size_t(n), hipMemcpyHostToDevice)) != hipSuccess) \
@@ -258,7 +257,7 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
     if ((e = hipMemset(s->sv[b], 0, sizeof(double) * size_t(std::max<int64_t>(T, 1)))) != hipSuccess) return bail(e, "hipMemset");
   }
   s->M = DevMesh{dpx, dpy, dpz, dtp, dte, dtt, dfl, dep0, dep1, det0, det1, doff, dn2c, T, E, V,
-                 G.cx, G.cy, G.bc, G.area, G.edmin, G.crec, G.nrec};
+                 G.cx, G.cy, G.bc, G.area, G.edmin, G.crec};
   s->M.c0 = 0;
   s->M.c1 = T;
   s->M.e0 = 0;

@@ -53,7 +53,6 @@ __global__ void __launch_bounds__(256) k_setup_cells(DevMesh M, DevGeomOut G) {
   r.fl = M.cflag[i];
   r.pad = 0;
   G.crec[i] = r;
-  G.nrec[i] = NRec{c.x, c.y, (r.fl & CELL_BND) ? INFINITY : r.zmax, c.z};
 }
 
 // min over the edge's two cells of 2 area / length (the flux's CFL length)
@@ -76,22 +75,26 @@ __global__ void __launch_bounds__(256) k_setup_edges(DevMesh M, DevGeomOut G) {
 // (src/MUSCLObject.cpp:31-112).
 template <bool HEAD_GRAD>
 __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W) {
-  // The workgroup's 256 Morton-consecutive cells stage their level and
-  // neighbour record in LDS; most neighbours lie in the same workgroup and
-  // are read from there, the others from global memory.
+  // The workgroup's 256 Morton-consecutive cells stage their level and the
+  // part of their record a neighbour reads (NRec) in LDS; most neighbours
+  // lie in the same workgroup and are read from there, the others from
+  // global memory.
   __shared__ double s_w[kBlock];
   __shared__ NRec s_r[kBlock];
   const int64_t blk0 = M.c0 + (int64_t)logical_block() * kBlock;
   const int64_t i = blk0 + threadIdx.x;
   const bool live = i < M.c1;
-  if (live) {
-    s_w[threadIdx.x] = S.w[i];
-    s_r[threadIdx.x] = M.nrec[i];
+  CellRec ri{};
+  double w = 0.;
+  if (live) {  // the cell's own record fills its LDS entry (no second read)
+    ri = M.crec[i];
+    w = S.w[i];
+    s_w[threadIdx.x] = w;
+    s_r[threadIdx.x] = NRec{ri.cx, ri.cy, (ri.fl & CELL_BND) ? INFINITY : ri.zmax, ri.bc};
   }
   __syncthreads();
   if (!live) return;
   if (M.cinst && !M.bactive[M.cinst[i]]) return;  // finished ensemble member
-  const double w = s_w[threadIdx.x];
   const uint32_t fl = M.cflag[i];
   const V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[M.T + i]), p2 = node(M, M.tp[2 * M.T + i]);
   double wn[3];
@@ -106,12 +109,12 @@ __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W
     if (lj >= 0 && lj < kBlock && j < M.c1) {
       wn[kk] = s_w[lj];
       rn[kk] = s_r[lj];
-    } else {
+    } else {  // the owner's workgroup reads the same record: an L2 hit mostly
       wn[kk] = S.w[j];
-      rn[kk] = M.nrec[j];
+      const CellRec r = M.crec[j];
+      rn[kk] = NRec{r.cx, r.cy, (r.fl & CELL_BND) ? INFINITY : r.zmax, r.bc};
     }
   }
-  const CellRec ri = M.crec[i];
   const double bc = ri.bc;
   const double zmx = ri.zmax, zmn = ri.zmin;
   const double cx = ri.cx, cy = ri.cy;

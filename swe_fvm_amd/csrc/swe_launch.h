@@ -12,8 +12,7 @@
 //   static geometry (computed once by the setup kernels with the step's own
 //   formulas, so every use is bit-identical to recomputing it):
 //     cells  cx, cy, bc (centroid x, y, z), area, and an AoS copy
-//            crec {cx, cy, bc, zmin, zmax, cflag} (48 B) and nrec {cx,
-//            cy, zmax or +inf on boundary cells, bc} (32 B) for the
+//            crec {cx, cy, bc, zmin, zmax, cflag} (48 B) for the
 //            reconstruction's neighbour gathers
 //     edges  edmin (min over the two cells of 2 area / length; the inside
 //            cell alone on a free-flow edge, 0 on a wall)
@@ -39,7 +38,8 @@ struct __attribute__((aligned(16))) CellRec {
 
 // the part of a neighbour's record the full-wet reconstruction reads in its
 // common branch: centroid, highest vertex (+inf on boundary cells, which
-// never count as full-wet neighbours) and centroid bathymetry; 32 bytes
+// never count as full-wet neighbours) and centroid bathymetry; 32 bytes,
+// staged in LDS by k_recon1
 struct __attribute__((aligned(32))) NRec {
   double cx, cy, zmx, bc;
 };
@@ -66,7 +66,6 @@ struct DevMesh {
   const double* area;
   const double* edmin;
   const CellRec* crec;
-  const NRec* nrec;
   // batched ensembles (null for a single mesh): instance of every cell and
   // the per-instance step size and activity flag
   const int* cinst;
@@ -92,7 +91,6 @@ struct DevBatch {
 struct DevGeomOut {
   double *cx, *cy, *bc, *area, *edmin;
   CellRec* crec;
-  NRec* nrec;
 };
 
 struct DevState {
