@@ -115,16 +115,16 @@ def test_gauge_loss_gradient_and_checkpoint_invariance():
     assert checked >= 2
 
 
-def draining_dt(sd, dom, prim):
-    """Per-cell draining dt h A / sum of outflows F0 L (with OPT_DRAIN_LENGTH)
-    from the GPU's fluxes of the state prim."""
+def draining_dt(sd, dom, prim, length=True):
+    """Per-cell draining dt h A / sum of outflows F0 L (with OPT_DRAIN_LENGTH;
+    F0 alone otherwise, as the reference) from the GPU's fluxes of prim."""
     sd.SetVolField(prim)
     sd.ComputeInterfaceValues()
     sd.ComputeFluxes()
     F0 = sd.GetFluxes()[:, 0]
     t = dom.topology.tables()
     xyz = dom.geometry
-    L = np.hypot(*(xyz[t["ep"][:, 1], :2] - xyz[t["ep"][:, 0], :2]).T)
+    L = np.hypot(*(xyz[t["ep"][:, 1], :2] - xyz[t["ep"][:, 0], :2]).T) if length else np.ones(len(F0))
     out = np.zeros(prim.shape[0])
     for side, sg in ((0, 1.0), (1, -1.0)):
         c = t["et"][:, side]
@@ -218,6 +218,45 @@ def test_tangent_linear_duality(scheme, flux, opts):
                             scheme, dt, 1e-6)
         if fd is not None:
             assert abs(fd - lhs) <= 1e-5 * abs(fd) + 1e-12
+
+
+def test_duality_on_the_inversion_workload():
+    """The bathymetry-inversion workload at 1,800 cells with a swirling
+    initial flow, 100 SSP-RK2 steps at CFL ~0.2: with the reference's
+    draining dt (outflow F0 without edge length) the draining dt is below the
+    stage dt in many moving cells (as in the 250k-cell inversion, whose cells
+    are 140 times smaller), so its adjoint is exercised broadly; duality with
+    the tangent-linear build holds."""
+    from swe_fvm_amd import workloads as W
+    dom, prim0, z_true, gauges = W.adjoint_inversion(n=30, n_gauges=16)
+    c0 = dom.centroids()
+    prim0[:, 1] = 0.3 * np.sin(2 * np.pi * c0[:, 1])
+    prim0[:, 2] = 0.3 * np.cos(2 * np.pi * c0[:, 0])
+    sd = swe.SpaceDisc(swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt], dom, prim0)
+    z = dom.geometry[:, 2].copy()
+    n, dt = 100, 5e-3
+    sd.SetVolField(prim0)
+    sd.run("SSPRK2", n // 2, dt=dt)
+    mid = sd.GetVolField()
+    active = int(np.sum(draining_dt(sd, dom, mid, length=False) < dt))
+    print(f"inversion workload: {active} of {mid.shape[0]} cells with an active draining dt", flush=True)
+    assert active > mid.shape[0] // 10
+    rng = np.random.default_rng(5)
+    c = dom.centroids()
+    fn = AD.gauge_misfit(gauges, prim0[gauges, 0] + 1e-3)
+    J, s0, zb, _ = AD.run_and_adjoint(sd, prim0, z, n, fn, dt=dt, checkpoint_every=40)
+    ybar = fn(sd.GetVolField())[1]
+    for _ in range(3):
+        pdot = np.zeros_like(prim0)
+        pdot[:, 0] = W.smooth_field(c[:, 0], c[:, 1], rng, 1e-2)
+        zdot = W.smooth_field(dom.geometry[:, 0], dom.geometry[:, 1], rng, 1e-2)
+        sd.SetVolField(prim0)
+        tdot, _ = sd.tangent_run(pdot, n, z_dot=zdot, dt=dt)
+        lhs = float(np.sum(ybar * tdot))
+        rhs = float(np.sum(s0 * pdot) + np.sum(zb * zdot))
+        scale = float(np.sum(np.abs(ybar * tdot)))
+        print(f"inversion duality: {lhs:.15e} {rhs:.15e} diff/scale {abs(lhs - rhs) / scale:.1e}", flush=True)
+        assert abs(lhs - rhs) <= 1e-12 * scale
 
 
 def test_stacked_batch_equals_members():
