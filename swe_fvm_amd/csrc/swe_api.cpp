@@ -331,14 +331,14 @@ int swe_batch_create(int64_t n, const swe_topology* const* topos, const double* 
   return rc;
 }
 
-void swe_destroy(swe_solver* s) {
+void swe_destroy(swe_solver* s) {  // teardown: failures cannot be reported
   if (!s) return;
-  hipSetDevice(s->device);
-  if (s->stream) hipStreamSynchronize(s->stream);
-  if (s->graph) hipGraphExecDestroy(s->graph);
-  for (void* p : s->allocs) hipFree(p);
-  if (s->dts) hipFree(s->dts);
-  if (s->stream) hipStreamDestroy(s->stream);
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->graph) (void)hipGraphExecDestroy(s->graph);
+  for (void* p : s->allocs) (void)hipFree(p);
+  if (s->dts) (void)hipFree(s->dts);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
 
@@ -407,17 +407,17 @@ static int edge_and_src(swe_solver* s, double* edg, double* src) {
   size_t n = size_t(6 * std::max<int64_t>(s->E, 1));
   SWE_CHECK(hipMalloc((void**)&de, sizeof(double) * n), "hipMalloc");
   hipError_t e = hipMalloc((void**)&ds, sizeof(double) * n);
-  if (e != hipSuccess) { hipFree(de); return dev_fail(e, "hipMalloc"); }
-  hipMemsetAsync(de, 0, sizeof(double) * n, s->stream);
-  hipMemsetAsync(ds, 0, sizeof(double) * n, s->stream);
-  e = launch_edge_field(s->M, state_of(s, s->cur), s->W, s->cor, de, ds, s->stream);
+  if (e != hipSuccess) { (void)hipFree(de); return dev_fail(e, "hipMalloc"); }
+  e = hipMemsetAsync(de, 0, sizeof(double) * n, s->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(ds, 0, sizeof(double) * n, s->stream);
+  if (e == hipSuccess) e = launch_edge_field(s->M, state_of(s, s->cur), s->W, s->cor, de, ds, s->stream);
   int rc = SWE_OK;
   if (e != hipSuccess) rc = dev_fail(e, "edge_field");
   if (!rc && edg) rc = download(s, edg, de, sizeof(double) * size_t(6 * s->E));
   if (!rc && src) rc = download(s, src, ds, sizeof(double) * size_t(6 * s->E));
-  hipStreamSynchronize(s->stream);
-  hipFree(de);
-  hipFree(ds);
+  (void)hipStreamSynchronize(s->stream);  // cleanup: errors were reported above
+  (void)hipFree(de);
+  (void)hipFree(ds);
   return rc;
 }
 
@@ -450,8 +450,8 @@ int swe_get_max_wp(swe_solver* s, double* max_wp) {
   SWE_CHECK(hipMalloc((void**)&d, sizeof(double) * size_t(std::max<int64_t>(s->V, 1))), "hipMalloc");
   hipError_t e = launch_max_wp(s->M, s->W, d, s->stream);
   int rc = (e != hipSuccess) ? dev_fail(e, "max_wp") : download(s, max_wp, d, sizeof(double) * size_t(s->V));
-  hipStreamSynchronize(s->stream);
-  hipFree(d);
+  (void)hipStreamSynchronize(s->stream);
+  (void)hipFree(d);
   return rc;
 }
 
@@ -471,8 +471,8 @@ int swe_rhs(swe_solver* s, int64_t cell, double dt, double* out3) {
   hipError_t e = launch_drain(s->M, S, s->W, s->opts, s->stream);
   if (e == hipSuccess) e = launch_rhs_one(s->M, S, s->W, int(cell), dt, s->cor, friction(s), d, s->stream);
   int rc = (e != hipSuccess) ? dev_fail(e, "rhs") : download(s, out3, d, 3 * sizeof(double));
-  hipStreamSynchronize(s->stream);
-  hipFree(d);
+  (void)hipStreamSynchronize(s->stream);
+  (void)hipFree(d);
   return rc;
 }
 
@@ -483,7 +483,7 @@ int swe_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nsteps) {
   if (!(dt > 0.) && !(cfl > 0.)) return fail(SWE_ERR_ARG, "CFL mode needs cfl > 0");
   if (int rc = set_device(s)) return rc;
   if (nsteps > s->dts_cap) {
-    if (s->dts) { hipStreamSynchronize(s->stream); hipFree(s->dts); s->dts = nullptr; }
+    if (s->dts) { (void)hipStreamSynchronize(s->stream); (void)hipFree(s->dts); s->dts = nullptr; }
     SWE_CHECK(hipMalloc((void**)&s->dts, sizeof(double) * size_t(nsteps)), "hipMalloc");
     s->dts_cap = nsteps;
   }
@@ -608,7 +608,7 @@ static void batch_window(swe_solver* s, const std::vector<int>& act) {
 }
 
 static int batch_capture(swe_solver* s, int scheme, double cfl, double t_end, int per_graph) {
-  if (s->graph) { hipGraphExecDestroy(s->graph); s->graph = nullptr; }
+  if (s->graph) { (void)hipGraphExecDestroy(s->graph); s->graph = nullptr; }
   const int cur0 = s->cur;
   hipGraph_t g;
   SWE_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
@@ -619,7 +619,7 @@ static int batch_capture(swe_solver* s, int scheme, double cfl, double t_end, in
   if (rc) return rc;
   if (ce != hipSuccess) return dev_fail(ce, "hipStreamEndCapture");
   hipError_t ie = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
-  hipGraphDestroy(g);
+  (void)hipGraphDestroy(g);
   if (ie != hipSuccess) return dev_fail(ie, "hipGraphInstantiate");
   s->graph_scheme = scheme;
   s->graph_cfl = cfl;
@@ -717,27 +717,34 @@ int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nstep
   if (int rc = set_device(s)) return rc;
   hipEvent_t e0, e1;
   SWE_CHECK(hipEventCreate(&e0), "hipEventCreate");
-  SWE_CHECK(hipEventCreate(&e1), "hipEventCreate");
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(SWE_ERR_DEVICE, "hipEventCreate");
+  }
   int rc = SWE_OK;
   if (!kernel_ms) {
-    hipEventRecord(e0, s->stream);
-    rc = swe_run(s, scheme, dt, cfl, nsteps);
-    hipEventRecord(e1, s->stream);
-    hipEventSynchronize(e1);
+    hipError_t e = hipEventRecord(e0, s->stream);
+    if (e == hipSuccess) {
+      rc = swe_run(s, scheme, dt, cfl, nsteps);
+      e = hipEventRecord(e1, s->stream);
+    }
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
     float f = 0.f;
-    hipEventElapsedTime(&f, e0, e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
+    if (e != hipSuccess && !rc) rc = dev_fail(e, "timing events");
     *ms = f;
   } else {
     // bracket each kernel of each stage with its own event pair
     const int nk = 5;
     double acc[nk] = {0, 0, 0, 0, 0};
     int64_t cnt = 0;
-    std::vector<hipEvent_t> ev(2 * nk);
-    for (auto& x : ev) hipEventCreate(&x);
+    std::vector<hipEvent_t> ev(2 * nk, nullptr);
+    for (auto& x : ev)
+      if (!rc && hipEventCreate(&x) != hipSuccess) rc = fail(SWE_ERR_DEVICE, "hipEventCreate");
     float total = 0.f;
     int nst = scheme == SWE_SCHEME_EULER ? 1 : (scheme == SWE_SCHEME_SSPRK2 ? 2 : 3);
     if (nsteps > s->dts_cap) {
-      if (s->dts) { hipStreamSynchronize(s->stream); hipFree(s->dts); s->dts = nullptr; }
+      if (s->dts) { (void)hipStreamSynchronize(s->stream); (void)hipFree(s->dts); s->dts = nullptr; }
       SWE_CHECK(hipMalloc((void**)&s->dts, sizeof(double) * size_t(nsteps)), "hipMalloc");
       s->dts_cap = nsteps;
     }
@@ -751,29 +758,31 @@ int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nstep
         else if (st == 1) { in = b; u0 = a; out = c; a0 = .75; a1 = .25; coef = .25; }
         else { in = c; u0 = a; out = a; a0 = 1. / 3.; a1 = 2. / 3.; coef = 2. / 3.; }
         DevState S = state_of(s, in);
-        hipEventRecord(ev[0], s->stream);
-        launch_recon1(s->M, S, s->W, s->opts, s->stream);
-        hipEventRecord(ev[1], s->stream);
-        launch_recon2(s->M, S, s->W, s->opts, s->stream);
-        hipEventRecord(ev[2], s->stream);
-        hipEventRecord(ev[3], s->stream);
-        launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + st, s->flux_items, s->stream);
-        hipEventRecord(ev[4], s->stream);
-        hipEventRecord(ev[5], s->stream);
-        launch_drain(s->M, S, s->W, s->opts, s->stream);
-        hipEventRecord(ev[6], s->stream);
-        hipEventRecord(ev[7], s->stream);
         StageArgs A{a0, a1, coef, dt, cfl, s->cor, s->rbits, st == 0 ? s->dts + k : nullptr, friction(s)};
-        launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->upd_items, s->stream);
-        hipEventRecord(ev[8], s->stream);
-        hipError_t e = hipEventSynchronize(ev[8]);
+        hipError_t e = hipEventRecord(ev[0], s->stream);
+        if (e == hipSuccess) e = launch_recon1(s->M, S, s->W, s->opts, s->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[1], s->stream);
+        if (e == hipSuccess) e = launch_recon2(s->M, S, s->W, s->opts, s->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[2], s->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[3], s->stream);
+        if (e == hipSuccess)
+          e = launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + st, s->flux_items, s->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[4], s->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[5], s->stream);
+        if (e == hipSuccess) e = launch_drain(s->M, S, s->W, s->opts, s->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[6], s->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[7], s->stream);
+        if (e == hipSuccess) e = launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->upd_items, s->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[8], s->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[8]);
         if (e != hipSuccess) { rc = dev_fail(e, "kernel"); break; }
-        float f;
-        hipEventElapsedTime(&f, ev[0], ev[1]); acc[0] += f; total += f;
-        hipEventElapsedTime(&f, ev[1], ev[2]); acc[1] += f; total += f;
-        hipEventElapsedTime(&f, ev[3], ev[4]); acc[2] += f; total += f;
-        hipEventElapsedTime(&f, ev[5], ev[6]); acc[3] += f; total += f;
-        hipEventElapsedTime(&f, ev[7], ev[8]); acc[4] += f; total += f;
+        const int pairs[5][2] = {{0, 1}, {1, 2}, {3, 4}, {5, 6}, {7, 8}};
+        for (int q = 0; q < 5; ++q) {
+          float f = 0.f;
+          if (hipEventElapsedTime(&f, ev[size_t(pairs[q][0])], ev[size_t(pairs[q][1])]) != hipSuccess) f = 0.f;
+          acc[q] += f;
+          total += f;
+        }
         ++cnt;
         s->last_r_slot = st;
       }
@@ -782,10 +791,11 @@ int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nstep
     s->last_nsteps = nsteps;
     for (int q = 0; q < nk; ++q) kernel_ms[q] = cnt ? acc[q] / double(cnt) : 0.;
     *ms = total;
-    for (auto& x : ev) hipEventDestroy(x);
+    for (auto& x : ev)
+      if (x) (void)hipEventDestroy(x);
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (!rc) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = dev_fail(e, "run");
@@ -802,7 +812,7 @@ int swe_set_tuning(swe_solver* s, int flux_items, int update_items) {
     s->flux_items = flux_items;
     s->upd_items = update_items;
     if (s->graph) {  // a captured ensemble step holds the old launches
-      hipGraphExecDestroy(s->graph);
+      (void)hipGraphExecDestroy(s->graph);
       s->graph = nullptr;
       s->graph_scheme = -1;
     }
@@ -862,8 +872,8 @@ int swe_autotune(swe_solver* s, int reps, int* flux_items, int* update_items, do
     rc = timed([&] {
       return launch_update(s->M, S, S, out_of(s, scratch), s->W, A, ui[q], s->stream);
     }, t[3 + q]);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (rc) return rc;
   int bf = 0, bu = 0;
   for (int q = 1; q < 3; ++q) if (t[q] < t[bf]) bf = q;
@@ -895,8 +905,8 @@ int swe_integrals(swe_solver* s, double* out3, double* per_cell) {
   if (e == hipSuccess && per_cell)
     e = hipMemcpyAsync(per_cell, pc, sizeof(double) * size_t(3 * s->T), hipMemcpyDeviceToHost, s->stream);
   hipError_t e2 = hipStreamSynchronize(s->stream);
-  hipFree(d);
-  if (pc) hipFree(pc);
+  (void)hipFree(d);
+  if (pc) (void)hipFree(pc);
   if (e == hipSuccess) e = e2;
   if (e != hipSuccess) return dev_fail(e, "integrals");
   return SWE_OK;
@@ -926,7 +936,7 @@ int swe_state_stats(swe_solver* s, double* out5) {
   hipError_t e = launch_stats(s->M, state_of(s, s->cur), d + 5, d, s->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(out5, d, 5 * sizeof(double), hipMemcpyDeviceToHost, s->stream);
   hipError_t e2 = hipStreamSynchronize(s->stream);
-  hipFree(d);
+  (void)hipFree(d);
   if (e == hipSuccess) e = e2;
   if (e != hipSuccess) return dev_fail(e, "state_stats");
   return SWE_OK;
@@ -935,6 +945,32 @@ int swe_state_stats(swe_solver* s, double* out5) {
 // ------------------------------------------------------------- adjoint --
 
 namespace {
+
+// optional timing of the adjoint and tangent sweeps: events that cannot be
+// created or recorded leave the reported times at zero and do not affect
+// the computation
+struct SweepTimer {
+  hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+  SweepTimer() {
+    for (auto& x : e)
+      if (hipEventCreate(&x) != hipSuccess) x = nullptr;
+  }
+  ~SweepTimer() {
+    for (auto& x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+  void mark(int k, hipStream_t st) {
+    if (e[k] && hipEventRecord(e[k], st) != hipSuccess) {
+      (void)hipEventDestroy(e[k]);
+      e[k] = nullptr;
+    }
+  }
+  float ms(int a, int b) const {  // after a synchronisation of the stream
+    float f = 0.f;
+    if (!e[a] || !e[b] || hipEventElapsedTime(&f, e[a], e[b]) != hipSuccess) f = 0.f;
+    return f;
+  }
+};
 
 // a state held in one 3T allocation (w, u, v)
 struct Buf3 {
@@ -1005,9 +1041,9 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
     if (e == hipSuccess) mem.push_back((void*)*p);
     return e;
   };
-  auto release = [&]() {
-    hipStreamSynchronize(s->stream);
-    for (void* p : mem) hipFree(p);
+  auto release = [&]() {  // cleanup: errors are reported where they occur
+    (void)hipStreamSynchronize(s->stream);
+    for (void* p : mem) (void)hipFree(p);
   };
   const int64_t S3 = 3 * T * int64_t(sizeof(double));
   // per-stage work slot: mw, mgx, mgy, wrec1, dtd (T doubles each), f0, f1,
@@ -1065,10 +1101,9 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
     J.flags = flags;
   }
   auto dtn = [&](int64_t n) { return dts ? dts[n] : dt; };
-  hipEvent_t ev[3];
-  for (auto& x : ev) hipEventCreate(&x);
+  SweepTimer tm;
   // forward sweep: checkpoints up to the start of the last segment
-  hipEventRecord(ev[0], s->stream);
+  tm.mark(0, s->stream);
   int rc = SWE_OK;
   const int64_t n_last = (nck - 1) * K;
   for (int64_t n = 0; n < n_last && !rc; ++n) {
@@ -1079,7 +1114,7 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
   }
   if (!rc && (e = copy_state(state_of(s, s->cur), out3(Buf3{ck + (nck - 1) * 3 * T}, T), T, s->stream)) != hipSuccess)
     rc = dev_fail(e, "copy");
-  hipEventRecord(ev[1], s->stream);
+  tm.mark(1, s->stream);
   if (!rc) {
     std::vector<double> h(static_cast<size_t>(3 * T));
     for (int64_t i = 0; i < T; ++i)
@@ -1144,7 +1179,7 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
   J.sbar = ob;  // the U_0 adjoint, its neighbour terms gathered here
   if (!rc && (e = launch_adjoint_finish(s->M, J, z_bar ? zb : nullptr, s->stream)) != hipSuccess)
     rc = dev_fail(e, "adjoint finish");
-  hipEventRecord(ev[2], s->stream);
+  tm.mark(2, s->stream);
   unsigned int nflag = 0;
   if (!rc) {
     std::vector<double> h(static_cast<size_t>(3 * T));
@@ -1157,15 +1192,11 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
       for (int64_t i = 0; i < T; ++i)
         for (int q = 0; q < 3; ++q) state0_bar[3 * i + q] = h[size_t(q * T + i)];
       if (ms) {
-        float f0 = 0.f, f1 = 0.f;
-        hipEventElapsedTime(&f0, ev[0], ev[1]);
-        hipEventElapsedTime(&f1, ev[1], ev[2]);
-        ms[0] = f0;
-        ms[1] = f1;
+        ms[0] = tm.ms(0, 1);
+        ms[1] = tm.ms(1, 2);
       }
     }
   }
-  for (auto& x : ev) hipEventDestroy(x);
   release();
   if (!rc && nflag)
     return fail(SWE_ERR_SOLVER, "adjoint: " + std::to_string(nflag) +
@@ -1202,7 +1233,7 @@ int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int
   unsigned int* flags = nullptr;
   SWE_CHECK(hipMalloc((void**)&buf, sizeof(double) * size_t(n8)), "hipMalloc (tangent)");
   if (hipMalloc((void**)&flags, sizeof(unsigned int)) != hipSuccess) {
-    hipFree(buf);
+    (void)hipFree(buf);
     return fail(SWE_ERR_DEVICE, "hipMalloc (tangent)");
   }
   double* q = buf;
@@ -1233,9 +1264,8 @@ int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int
   if (e == hipSuccess) e = hipMemsetAsync(flags, 0, sizeof(unsigned int), s->stream);
   StageCoef sc[3];
   const int ns = stage_list(scheme, sc);
-  hipEvent_t ev[2];
-  for (auto& x : ev) hipEventCreate(&x);
-  hipEventRecord(ev[0], s->stream);
+  SweepTimer tm;
+  tm.mark(0, s->stream);
   for (int64_t n = 0; n < nsteps && e == hipSuccess; ++n) {
     // the slots of run_step: stage k reads the previous stage's output (the
     // step input a for k = 0) and writes outs[k]; U0 is slot a
@@ -1249,7 +1279,7 @@ int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int
     }
     s->cur = ns == 1 ? b : a;
   }
-  hipEventRecord(ev[1], s->stream);
+  tm.mark(1, s->stream);
   unsigned int nflag = 0;
   if (e == hipSuccess) e = hipMemcpyAsync(h.data(), td[s->cur], sizeof(double) * size_t(3 * T), hipMemcpyDeviceToHost, s->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(&nflag, flags, sizeof(unsigned int), hipMemcpyDeviceToHost, s->stream);
@@ -1259,16 +1289,11 @@ int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int
   } else {
     for (int64_t i = 0; i < T; ++i)
       for (int c = 0; c < 3; ++c) final_dot[3 * i + c] = h[size_t(c * T + i)];
-    if (ms) {
-      float f = 0.f;
-      hipEventElapsedTime(&f, ev[0], ev[1]);
-      *ms = f;
-    }
+    if (ms) *ms = tm.ms(0, 1);
   }
-  for (auto& x : ev) hipEventDestroy(x);
-  hipStreamSynchronize(s->stream);
-  hipFree(buf);
-  hipFree(flags);
+  (void)hipStreamSynchronize(s->stream);  // cleanup: errors were reported above
+  (void)hipFree(buf);
+  (void)hipFree(flags);
   if (!rc && nflag)
     return fail(SWE_ERR_SOLVER, "tangent: " + std::to_string(nflag) +
                                     " cell stages outside the supported regime (partially wet cells below "
@@ -1284,7 +1309,7 @@ int swe_set_bathymetry(swe_solver* s, const double* z) {
   SWE_CHECK(hipMemcpyAsync(const_cast<double*>(s->M.pz), z, sizeof(double) * size_t(s->V),
                            hipMemcpyHostToDevice, s->stream), "hipMemcpy");
   if (s->graph) {
-    hipGraphExecDestroy(s->graph);
+    (void)hipGraphExecDestroy(s->graph);
     s->graph = nullptr;
     s->graph_scheme = -1;
   }
