@@ -213,6 +213,57 @@ def stack_domains(domains):
     return Domain(geo, topo), dict(nodes=nv, edges=ne, cells=nt)
 
 
+def _spread_bits(x):
+    """The bits of x (< 2^32) moved to the even bit positions of a 64-bit word."""
+    x = x.astype(np.uint64)
+    for shift, mask in ((16, 0x0000FFFF0000FFFF), (8, 0x00FF00FF00FF00FF), (4, 0x0F0F0F0F0F0F0F0F),
+                        (2, 0x3333333333333333), (1, 0x5555555555555555)):
+        x = (x | (x << np.uint64(shift))) & np.uint64(mask)
+    return x
+
+
+def reorder_cells(domain, perm):
+    """The domain with its cells in the order perm (new cell k = old cell
+    perm[k], vertex order kept), nodes renumbered by first appearance and the
+    tables rebuilt by the reference reader's conventions; boundary edges keep
+    their order and tags.  Returns (Domain, perm, node_perm) with new node m =
+    old node node_perm[m]."""
+    t = domain.topology.tables()
+    perm = np.asarray(perm, dtype=np.int64)
+    V = domain.geometry.shape[0]
+    tri = t["tp"][perm]
+    flat = tri.ravel()
+    _, first = np.unique(flat, return_index=True)
+    used = flat[np.sort(first)]
+    rest = np.setdiff1d(np.arange(V), used, assume_unique=True)
+    node_perm = np.concatenate([used, rest]).astype(np.int64)
+    new_id = np.empty(V, np.int64)
+    new_id[node_perm] = np.arange(V)
+    b = np.nonzero(t["et"][:, 1] < 0)[0]
+    topo = Topology.build(V, new_id[tri], new_id[t["ep"][b]])
+    tags = t["et"][b, 1]
+    for tag in np.unique(tags[tags != SOLID_WALL]):
+        topo = topo.with_boundary_tag(np.nonzero(tags == tag)[0], int(tag))
+    return Domain(domain.geometry[node_perm], topo), perm, node_perm
+
+
+def morton_order(domain):
+    """Cells along the Morton curve of their centroids (21 bits per axis),
+    nodes by first appearance: the layout the kernels are designed for
+    (neighbours of a workgroup's cells share its L2 lines and LDS tile).
+    Returns reorder_cells' (Domain, perm, node_perm).  The reference's
+    results depend on the numbering only through which cell of an edge is
+    et0 (the flux orientation), i.e. at round-off for symmetric wave speeds
+    and through the Einfeldt estimate's asymmetry otherwise."""
+    c = domain.centroids()[:, :2]
+    lo = c.min(axis=0)
+    span = c.max(axis=0) - lo
+    span[span == 0] = 1.0
+    q = np.clip((c - lo) / span * 2097151.0, 0, 2097151).astype(np.uint64)
+    key = _spread_bits(q[:, 0]) | (_spread_bits(q[:, 1]) << np.uint64(1))
+    return reorder_cells(domain, np.argsort(key, kind="stable"))
+
+
 def _take_mesh(rc, xyz_p, nn, h):
     _check(rc)
     L = _lib.load()

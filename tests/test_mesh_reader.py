@@ -50,3 +50,48 @@ def test_basic_msh_step_conserves_mass(golden):
     v0, v1 = np.sum(a * prim[:, 0]), np.sum(a * o.state()[:, 0])
     assert abs(v1 - v0) <= 1e-13 * v0
     assert np.max(np.abs(o.state()[:, 0] - prim[:, 0])) > 1e-3
+
+
+def test_morton_order_of_the_bowl(golden):
+    """morton_order on bowl.msh: the same cells (vertex order kept) in
+    Morton order, tables by the builder's conventions, neighbours closer in
+    index; the oracle's SSP-RK2 run on the reordered mesh equals the run on
+    the original, cell by cell, with the symmetric Davis speeds (the
+    reference's results depend on the numbering only through which cell of
+    an edge is et0)."""
+    xyz, topo = swe.read_msh(golden.bowl_path())
+    dom = swe.Domain(xyz, topo)
+    dom2, perm, nperm = swe.morton_order(dom)
+    t, t2 = topo.tables(), dom2.topology.tables()
+    assert np.array_equal(np.sort(perm), np.arange(len(perm)))
+    assert np.array_equal(dom2.geometry[t2["tp"]], xyz[t["tp"][perm]])
+    assert np.array_equal(t2["et"][:, 1] < 0, np.arange(len(t2["et"])) < np.sum(t["et"][:, 1] < 0))
+    rebuilt = swe.Topology.build(xyz.shape[0], t2["tp"], t2["ep"][t2["et"][:, 1] < 0]).tables()
+    assert all(np.array_equal(rebuilt[k], t2[k]) for k in t2)
+
+    def spread(tt):
+        i = np.repeat(np.arange(tt.shape[0]), 3)
+        j = tt.ravel()
+        return np.mean(np.abs(j[j >= 0] - i[j >= 0]))
+    assert spread(t2["tt"]) < 0.5 * spread(t["tt"])
+    c = dom.centroids()
+    z = 0.3 * np.exp(-((c[:, 0] - 4) ** 2 + (c[:, 1] - 4) ** 2))
+    prim = np.stack([1.0 + z, 0.1 * np.ones(len(c)), np.zeros(len(c))], 1)
+    a = O.OracleSolver(xyz, t, "HLL", "Davis")
+    a.set_state(prim)
+    a.run("SSPRK2", 10, dt=0.0, cfl=0.3)
+    b = O.OracleSolver(dom2.geometry, t2, "HLL", "Davis")
+    b.set_state(prim[perm])
+    b.run("SSPRK2", 10, dt=0.0, cfl=0.3)
+    assert np.max(np.abs(b.state() - a.state()[perm])) <= 1e-13
+
+
+def test_reorder_keeps_boundary_tags():
+    xyz, topo = swe.jittered_mesh(8, 0.2, 3)
+    t = topo.tables()
+    b = np.nonzero(t["et"][:, 1] < 0)[0]
+    tagged = topo.with_boundary_tag(b[::3], swe.FREE_FLOW)
+    rng = np.random.default_rng(0)
+    dom2, perm, nperm = swe.reorder_cells(swe.Domain(xyz, tagged), rng.permutation(topo.NumTriangles()))
+    et2 = dom2.topology.tables()["et"]
+    assert np.array_equal(et2[: len(b), 1], tagged.tables()["et"][b, 1])
