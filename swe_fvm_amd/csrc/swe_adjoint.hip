@@ -433,7 +433,7 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
       const int e = M.te[k * T + i];
       const bool own = sgn_pos(fl, k);
       const int side = own ? 0 : 1;
-      if (M.tt[k * T + i] == kSolidWall) {
+      if (M.et1[e] == kSolidWall) {  // the edge's tag (tt is -1 on every boundary edge)
         const double hf = J.sl_edge[e];  // side 0 w: the depth adjoint
         wb += hf;
         bcb -= hf;

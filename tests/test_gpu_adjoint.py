@@ -259,6 +259,36 @@ def test_duality_on_the_inversion_workload():
         assert abs(lhs - rhs) <= 1e-12 * scale
 
 
+def test_duality_with_free_flow_edges():
+    """Half the boundary edges FREE_FLOW (R = L across them; the adjoint adds
+    the outside slot's adjoint to the inside one): duality with the
+    tangent-linear build, state and bathymetry directions, a flow leaving
+    through the open edges."""
+    dom0, prim0 = wet_case(n=14, seed=8)
+    t = dom0.topology.tables()
+    b = np.nonzero(t["et"][:, 1] < 0)[0]
+    dom = swe.Domain(dom0.geometry, dom0.topology.with_boundary_tag(b[::2], swe.FREE_FLOW))
+    prim0 = prim0.copy()
+    prim0[:, 1] += 0.2
+    sd = swe.SpaceDisc(swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt], dom, prim0, cor=0.2)
+    z = dom.geometry[:, 2].copy()
+    rng = np.random.default_rng(4)
+    n, dt = 10, 2e-3
+    ybar = rng.standard_normal(prim0.shape)
+    sd.SetVolField(prim0)
+    s0, zb, _ = sd.adjoint_run(ybar, n, dt=dt, checkpoint_every=4)
+    for _ in range(2):
+        pdot = rng.standard_normal(prim0.shape) * 1e-2
+        zdot = rng.standard_normal(z.shape) * 1e-2
+        sd.SetVolField(prim0)
+        tdot, _ = sd.tangent_run(pdot, n, z_dot=zdot, dt=dt)
+        lhs = float(np.sum(ybar * tdot))
+        rhs = float(np.sum(s0 * pdot) + np.sum(zb * zdot))
+        scale = float(np.sum(np.abs(ybar * tdot)))
+        print(f"free flow duality: {lhs:.15e} {rhs:.15e} diff/scale {abs(lhs - rhs) / scale:.1e}", flush=True)
+        assert abs(lhs - rhs) <= 1e-12 * scale
+
+
 def test_stacked_batch_equals_members():
     """A batch (stack_domains) gives each member's forward and adjoint bit
     for bit: the kernels never mix members."""
