@@ -15,16 +15,16 @@
 // (edge, side) and gathered in a fixed order by its consumer, so the
 // reverse pass is atomic-free and identical run to run, like the forward.
 //
-// Regime: fully wet cells, dry cells, and partially wet cells whose level is
-// at or above their highest vertex (the reconstruction is then the cell
-// level).  A partially wet cell below its highest vertex (a shoreline cell:
-// the cubic / bisection level and the Liu et al. plane) is counted in
-// A.flags and the host reports the adjoint as unsupported for that state.
+// Partially wet cells: the local Jacobians of PartWet1 and PartWet2 come
+// from their generic restatements (swe_pw.h) on dual numbers; the adjoint of
+// PartWet2's node maximum goes to the cell attaining it (k_adj_pw,
+// k_adj_pwnode), so the whole reconstruction is differentiated.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
 
 #include "swe_common.h"
+#include "swe_pw.h"
 
 namespace swe {
 
@@ -46,6 +46,88 @@ __device__ __forceinline__ double gather_nb(const DevMesh& M, const int8_t* back
     if (j >= 0) s += wnb[back[k * T + c] * T + j];
   }
   return s;
+}
+
+// value and attaining cell (-1: the node's own height) of the pass-1 node
+// maximum at node n (node_max_pass1)
+__device__ inline double node_max_arg(const DevMesh& M, const DevWork& W, int n, const V3& P, int& arg) {
+  double mx = P.z;
+  arg = -1;
+  for (int q = M.n2c_off[n]; q < M.n2c_off[n + 1]; ++q) {
+    const int c = M.n2c[q];
+    double w0 = W.mw[c], gx = W.mgx[c], gy = W.mgy[c];
+    if (W.cls[c] == 1) { w0 = W.wrec1[c]; gx = 0.; gy = 0.; }
+    const double dx = P.x - M.cx[c], dy = P.y - M.cy[c];
+    double a = w0 + (gx * dx + gy * dy);
+    if (!((a - P.z) >= 0)) a = P.z;
+    if (mx < a) {
+      mx = a;
+      arg = c;
+    }
+  }
+  return mx;
+}
+
+// lowest vertex of cell c by PartWet2's sort (recon2_cell)
+__device__ inline int lowest_node(const DevMesh& M, int c) {
+  const int64_t T = M.T;
+  int i0 = M.tp[c], i1 = M.tp[T + c], i2 = M.tp[2 * T + c];
+  double z0 = M.pz[i0], z1 = M.pz[i1], z2 = M.pz[i2];
+  double tz;
+  int ti;
+  if (z0 > z1) { tz = z0; z0 = z1; z1 = tz; ti = i0; i0 = i1; i1 = ti; }
+  if (z1 > z2) { tz = z1; z1 = z2; z2 = tz; ti = i1; i1 = i2; i2 = ti; }
+  if (z0 > z1) { ti = i0; i0 = i1; i1 = ti; }
+  return i0;
+}
+
+// PartWet1 level of cell c: d level / d (w, z of local vertices 0..2)
+__device__ inline void pw1_jac(const DevMesh& M, const DevState& S, int c, double jac[4]) {
+  using D = Dual<4>;
+  const int64_t T = M.T;
+  const D z0 = dvar<4>(M.pz[M.tp[c]], 1), z1 = dvar<4>(M.pz[M.tp[T + c]], 2), z2 = dvar<4>(M.pz[M.tp[2 * T + c]], 3);
+  const double t = 1. / 3.;
+  const D bc = (z0 * t + z1 * t) + z2 * t;
+  const D zmn = dmin(dmin(z0, z1), z2), zmx = dmax(dmax(z0, z1), z2);
+  const D lv = partwet1_g(zmn, zmn, zmx, bc, dvar<4>(S.w[c], 0));
+  for (int k = 0; k < 4; ++k) jac[k] = lv.d[k];
+}
+
+// Final MUSCL (w0, gx, gy) of partially wet cell c: PartWet2, or the
+// PartWet1 level where pass 2 falls back, as functions of (w, z of local
+// vertices 0..2, pass-1 node maximum at the lowest vertex); true when the
+// node maximum enters
+__device__ inline bool pw2_jac(const DevMesh& M, const DevState& S, const DevWork& W, int c, double jac[3][5]) {
+  using D = Dual<5>;
+  const int64_t T = M.T;
+  int i0 = M.tp[c], i1 = M.tp[T + c], i2 = M.tp[2 * T + c];
+  PT<D> Q0{dconst<5>(M.px[i0]), dconst<5>(M.py[i0]), dvar<5>(M.pz[i0], 1)};
+  PT<D> Q1{dconst<5>(M.px[i1]), dconst<5>(M.py[i1]), dvar<5>(M.pz[i1], 2)};
+  PT<D> Q2{dconst<5>(M.px[i2]), dconst<5>(M.py[i2]), dvar<5>(M.pz[i2], 3)};
+  const double t = 1. / 3.;
+  const D bc = (Q0.z * t + Q1.z * t) + Q2.z * t;
+  const D w = dvar<5>(S.w[c], 0);
+  const D zmn = dmin(dmin(Q0.z, Q1.z), Q2.z), zmx = dmax(dmax(Q0.z, Q1.z), Q2.z);
+  D w0 = partwet1_g(zmn, zmn, zmx, bc, w), gx = dconst<5>(0.), gy = dconst<5>(0.);
+  bool used = false;
+  sort3_g(Q0, Q1, Q2, i0, i1, i2);
+  if (!((w.v > Q2.z.v) || (Q2.z.v - Q0.z.v < kTol))) {
+    int arg;
+    const double w23 = node_max_arg(M, W, i0, V3{Q0.x.v, Q0.y.v, Q0.z.v}, arg);
+    D a, bx, by;
+    if (partwet2_g(Q0, Q1, Q2, M.cx[c], M.cy[c], bc, w, dvar<5>(w23, 4), a, bx, by)) {
+      w0 = a;
+      gx = bx;
+      gy = by;
+      used = true;
+    }
+  }
+  for (int k = 0; k < 5; ++k) {
+    jac[0][k] = w0.d[k];
+    jac[1][k] = gx.d[k];
+    jac[2][k] = gy.d[k];
+  }
+  return used;
 }
 
 // Reverse of slot_value (AtPoint + PrimAssigner, src/SpaceDisc.cpp:24-25):
@@ -400,8 +482,69 @@ __global__ void __launch_bounds__(256) k_adj_flux(DevMesh M, DevState S, DevWork
   J.ez_edge[e] = g[6];
 }
 
+// ------------------------------------------------------ partially wet --
+// For each partially wet cell (pass 2's list): its final MUSCL adjoint (its
+// own update's and its edge slots', gathered as k_adj_recon gathers it)
+// through PartWet2 / PartWet1: the cell's own w and corner-z adjoints, and
+// the adjoint of the node maximum at its lowest vertex.
+__global__ void __launch_bounds__(256) k_adj_pw(DevMesh M, DevState S, DevWork W, DevAdj J) {
+  const unsigned npw = *W.pw_count;
+  const int64_t T = M.T, Ecnt = M.E;
+  for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < npw; q += gridDim.x * blockDim.x) {
+    const int i = W.pw_list[q];
+    const uint32_t fl = M.cflag[i];
+    double mb[5];
+    for (int c = 0; c < 5; ++c) mb[c] = J.m_cell[c * T + i];
+    const Muscl mu = load_muscl(S, W, i);
+    const V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[T + i]), p2 = node(M, M.tp[2 * T + i]);
+    for (int k = 0; k < 3; ++k) {
+      const int e = M.te[k * T + i];
+      if (M.et1[e] == kSolidWall) continue;
+      const int side = sgn_pos(fl, k) ? 0 : 1;
+      double ez = 0.;
+      slot_bar(mu, M.cx[i], M.cy[i], edge_mid(p0, p1, p2, fl, k), J.sl_edge[(0 * 2 + side) * Ecnt + e],
+               J.sl_edge[(1 * 2 + side) * Ecnt + e], J.sl_edge[(2 * 2 + side) * Ecnt + e], mb, ez);
+    }
+    double jac[3][5];
+    const bool used = pw2_jac(M, S, W, i, jac);
+    for (int c = 0; c < 4; ++c) J.pw_own[c * T + i] = mb[0] * jac[0][c] + mb[1] * jac[1][c] + mb[2] * jac[2][c];
+    J.pw_w23bar[i] = used ? mb[0] * jac[0][4] + mb[1] * jac[1][4] + mb[2] * jac[2][4] : 0.;
+  }
+}
+
+// The node maxima's adjoints, one owner per node (the first partially wet
+// cell around it, in the node-to-cell map's order, whose lowest vertex it
+// is): summed over those cells and routed to the cell attaining the
+// maximum (its corner slot, read and cleared by k_adj_recon) or to the
+// node's own height.
+__global__ void __launch_bounds__(256) k_adj_pwnode(DevMesh M, DevWork W, DevAdj J) {
+  const unsigned npw = *W.pw_count;
+  const int64_t T = M.T;
+  for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < npw; q += gridDim.x * blockDim.x) {
+    const int i = W.pw_list[q];
+    const int n = lowest_node(M, i);
+    int owner = -1;
+    double sum = 0.;
+    for (int r = M.n2c_off[n]; r < M.n2c_off[n + 1]; ++r) {
+      const int c = M.n2c[r];
+      if (W.cls[c] == 1 && lowest_node(M, c) == n) {
+        if (owner < 0) owner = c;
+        sum += J.pw_w23bar[c];
+      }
+    }
+    if (owner != i || sum == 0.) continue;
+    int arg;
+    node_max_arg(M, W, n, node(M, n), arg);
+    if (arg >= 0) {
+      const int k = M.tp[arg] == n ? 0 : (M.tp[T + arg] == n ? 1 : 2);
+      J.p1bar[k * T + arg] = sum;
+    } else {
+      J.zn_acc[n] += sum;
+    }
+  }
+}
+
 // ------------------------------------------------------- reconstruction --
-__device__ __forceinline__ void flag(const DevAdj& J) { atomicAdd(J.flags, 1u); }
 
 // Reverse of k_recon1 / k_recon2 for cell i: gathers the MUSCL adjoint of
 // the cell (its own update and the slots of its three edges), then maps it
@@ -449,6 +592,23 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
   const int cls = W.cls[i];
   const double w = S.w[i];
   double wnb[3] = {0., 0., 0.};
+  // the node maxima this cell attains (k_adj_pwnode): adjoint of its pass-1
+  // reconstruction's value at those vertices
+  double mp[3] = {0., 0., 0.};
+  if (*W.pw_count) {
+    const V3 pv[3] = {p0, p1, p2};
+    for (int k = 0; k < 3; ++k) {
+      const double pk = J.p1bar[k * T + i];
+      if (pk != 0.) {
+        J.p1bar[k * T + i] = 0.;
+        mp[0] += pk;
+        mp[1] += pk * (pv[k].x - M.cx[i]);
+        mp[2] += pk * (pv[k].y - M.cy[i]);
+      }
+    }
+  }
+  if (cls != 1)  // pass 1 is the final reconstruction
+    for (int q = 0; q < 3; ++q) mb[q] += mp[q];
   if (cls == 0) {
     // ReconstructDryCell: level b_cell, gradient the bathymetry slope
     bcb += mb[0];
@@ -466,10 +626,17 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
     J.sbar[2 * T + i] += mb[4];
     const double zmax = dmax(dmax(p0.z, p1.z), p2.z);
     if (cls == 1) {
-      // pass 2 fell back to the pass-1 level (zero gradient) when the
-      // cell is above its highest vertex: the level is w itself
-      if (!(w > zmax) || W.mgx[i] != 0. || W.mgy[i] != 0.) flag(J);
-      wb += mb[0];
+      // PartWet2 / PartWet1 (k_adj_pw), and the PartWet1 level as the
+      // pass-1 value of the node maxima this cell attains
+      (void)zmax;
+      wb += J.pw_own[i];
+      for (int k = 0; k < 3; ++k) zc[k] += J.pw_own[(1 + k) * T + i];
+      if (mp[0] != 0.) {
+        double jac[4];
+        pw1_jac(M, S, i, jac);
+        wb += mp[0] * jac[0];
+        for (int k = 0; k < 3; ++k) zc[k] += mp[0] * jac[1 + k];
+      }
     } else {
       // ReconstructFullWetCell (src/MUSCLObject.cpp:38-84): w0 = w; the
       // gradient is the plane through the neighbour values unless a
@@ -500,7 +667,7 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
             a = E[k].z;
             kind[k] = 2;  // clamped to the edge bathymetry
           } else if (!(wj >= rj.zmax)) {
-            kind[k] = 3;  // cubic / bisection level: not supported
+            kind[k] = 3;  // cube-root / bisection level of the neighbour
           }
           pts[k] = {E[k].x, E[k].y, 0.5 * (w + a)};
         }
@@ -551,7 +718,22 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
               zc[ep0_local(fl, k)] += 0.25 * vb;  // a = E.z
               zc[ep1_local(fl, k)] += 0.25 * vb;
             } else {
-              flag(J);
+              // the neighbour's PartWet1 level: its w (gathered by it), the
+              // two vertices it shares with this cell (own corners) and the
+              // opposite one (per edge side, folded in k_adj_z)
+              const int j = M.tt[k * T + i];
+              double jac[4];
+              pw1_jac(M, S, j, jac);
+              const double lb = 0.5 * vb;
+              wnb[k] = lb * jac[0];
+              const int e = M.te[k * T + i];
+              const int sj = M.et0[e] == j ? 0 : 1;
+              for (int m = 0; m < 3; ++m) {
+                const int nj = M.tp[m * T + j];
+                const int mi = M.tp[i] == nj ? 0 : (M.tp[T + i] == nj ? 1 : (M.tp[2 * T + i] == nj ? 2 : -1));
+                if (mi >= 0) zc[mi] += lb * jac[1 + m];
+                else J.zopp_acc[sj * Ecnt + e] += lb * jac[1 + m];
+              }
             }
           }
         }
@@ -585,7 +767,13 @@ __global__ void __launch_bounds__(256) k_adj_z(DevMesh M, DevAdj J, double* zbar
     const int c = M.n2c[q];
     const int k = M.tp[c] == n ? 0 : (M.tp[T + c] == n ? 1 : 2);
     s += J.zc[k * T + c];
+    // the side of c on its edge opposite this vertex
+    const uint32_t fl = M.cflag[c];
+    for (int kk = 0; kk < 3; ++kk)
+      if (ep0_local(fl, kk) != k && ep1_local(fl, kk) != k)
+        s += J.zopp_acc[(sgn_pos(fl, kk) ? 0 : 1) * M.E + M.te[kk * T + c]];
   }
+  s += J.zn_acc[n];
   zbar[n] += s;
 }
 
@@ -630,6 +818,13 @@ hipError_t launch_adjoint_stage(const DevMesh& M, const DevState& S, const DevSt
     SWE_ADJ_FLUX(1, 0, true) SWE_ADJ_FLUX(1, 1, true) SWE_ADJ_FLUX(1, 2, true)
   }
 #undef SWE_ADJ_FLUX
+  {
+    // partially wet cells: grid-stride over pass 2's list
+    dim3 g = grid_plain(M.T);
+    if (g.x > 1024) g.x = 1024;
+    hipLaunchKernelGGL(k_adj_pw, g, dim3(256), 0, st, M, S, W, J);
+    hipLaunchKernelGGL(k_adj_pwnode, g, dim3(256), 0, st, M, W, J);
+  }
   if (opts & kOptDrainLength) hipLaunchKernelGGL(k_adj_recon<true>, grid_plain(M.T), dim3(256), 0, st, M, S, W, J, dts);
   else hipLaunchKernelGGL(k_adj_recon<false>, grid_plain(M.T), dim3(256), 0, st, M, S, W, J, dts);
   return hipGetLastError();

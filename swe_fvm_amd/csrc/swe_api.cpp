@@ -1061,8 +1061,9 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
   if (e == hipSuccess) e = alloc(&bar_b, S3);
   if (e == hipSuccess) e = alloc(&zb, V * 8);
   // DevAdj work: rb 3T, dtd_own T, dtd_nb 3T, m_cell 5T, sl_edge 6E, ez E,
-  // w_nb 3T, zc 3T, u0bar 3T; back 3T and eloc 2E bytes
-  const int64_t nwork = 3 * T + T + 3 * T + 5 * T + 6 * E + E + 3 * T + 3 * T + 3 * T;
+  // w_nb 3T, zc 3T, u0bar 3T, pw_own 4T, pw_w23bar T, p1bar 3T, zn_acc V,
+  // zopp_acc 2E; back 3T and eloc 2E bytes
+  const int64_t nwork = 3 * T + T + 3 * T + 5 * T + 6 * E + E + 3 * T + 3 * T + 3 * T + 4 * T + T + 3 * T + V + 2 * E;
   if (e == hipSuccess) e = alloc(&work, nwork * 8 + ((3 * T + 2 * E + 255) / 256) * 256);
   if (e == hipSuccess) e = alloc(&flags, int64_t(sizeof(unsigned int)));
   if (e != hipSuccess) { release(); return dev_fail(e, "hipMalloc (adjoint)"); }
@@ -1096,6 +1097,11 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
     J.w_nb = q; q += 3 * T;
     J.zc = q; q += 3 * T;
     J.u0bar = q; q += 3 * T;
+    J.pw_own = q; q += 4 * T;
+    J.pw_w23bar = q; q += T;
+    J.p1bar = q; q += 3 * T;
+    J.zn_acc = q; q += V;
+    J.zopp_acc = q; q += 2 * E;
     J.back = (int8_t*)q;
     J.eloc = J.back + 3 * T;
     J.flags = flags;
@@ -1121,6 +1127,9 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
       for (int q = 0; q < 3; ++q) h[size_t(q * T + i)] = final_bar[3 * i + q];
     if ((e = hipMemcpyAsync(bar_a, h.data(), size_t(S3), hipMemcpyHostToDevice, s->stream)) != hipSuccess ||
         (e = hipMemsetAsync(J.zc, 0, sizeof(double) * size_t(3 * T), s->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(J.p1bar, 0, sizeof(double) * size_t(3 * T), s->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(J.zn_acc, 0, sizeof(double) * size_t(std::max<int64_t>(V, 1)), s->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(J.zopp_acc, 0, sizeof(double) * size_t(std::max<int64_t>(2 * E, 1)), s->stream)) != hipSuccess ||
         (e = hipMemsetAsync(zb, 0, sizeof(double) * size_t(std::max<int64_t>(V, 1)), s->stream)) != hipSuccess ||
         (e = hipMemsetAsync(flags, 0, sizeof(unsigned int), s->stream)) != hipSuccess ||
         (e = launch_adjoint_tables(s->M, J, s->stream)) != hipSuccess ||
@@ -1228,7 +1237,7 @@ int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int
   const int64_t T = s->T, E = s->E, V = s->V;
   // tangent states (three, rotating like the values), node z tangent,
   // intermediates, flag counter
-  const int64_t n8 = 9 * T + V + 4 * T + 3 * E;
+  const int64_t n8 = 9 * T + V + 5 * T + 3 * E;
   double* buf = nullptr;
   unsigned int* flags = nullptr;
   SWE_CHECK(hipMalloc((void**)&buf, sizeof(double) * size_t(n8)), "hipMalloc (tangent)");
@@ -1245,6 +1254,7 @@ int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int
   Q.mw = q; q += T;
   Q.mgx = q; q += T;
   Q.mgy = q; q += T;
+  Q.wrec1 = q; q += T;
   Q.dtd = q; q += T;
   Q.f0 = q; q += E;
   Q.f1 = q; q += E;

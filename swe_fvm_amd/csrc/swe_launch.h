@@ -137,6 +137,17 @@ struct DevAdj {
   double* zc;          // [3][T] node-z adjoint per cell corner, summed over stages
   int8_t* back;        // [3][T] local index of this cell in neighbour k
   int8_t* eloc;        // [2][E] local index of edge e in its et0 / et1 cell
+  // partially wet cells (k_adj_pw, k_adj_pwnode): their own w and corner-z
+  // adjoints, the adjoint of the node maximum at their lowest vertex, the
+  // latter routed to the cell attaining the maximum (per corner of that
+  // cell, cleared by its consumer) or, summed over stages, to the node's
+  // own height; the height adjoint of a neighbour's vertex opposite an edge
+  // (partially wet neighbour levels), per edge side, summed over stages
+  double* pw_own;      // [4][T] w, corner z 0..2
+  double* pw_w23bar;   // [T]
+  double* p1bar;       // [3][T] routed pass-1 level adjoint per corner
+  double* zn_acc;      // [V]
+  double* zopp_acc;    // [2][E]
   unsigned int* flags; // states outside the supported regime
 };
 
@@ -147,6 +158,7 @@ struct DevTan {
   double* mw;          // [T] MUSCL level, gradient tangents
   double* mgx;
   double* mgy;
+  double* wrec1;       // [T] pass-1 level tangent of partially wet cells
   double* f0;          // [E] flux tangents
   double* f1;
   double* f2;

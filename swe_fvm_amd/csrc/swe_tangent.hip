@@ -1,7 +1,7 @@
 // Tangent-linear step: the forward stage kernels built on dual numbers.
 //
 // Each kernel repeats the operation sequence of its forward counterpart
-// (k_recon1 + the pass-2 fallback, k_flux, k_drain, k_update with
+// (k_recon1, k_recon2, k_flux, k_drain, k_update with
 // TimeDisc::RHS, src/SpaceDisc.cpp:15-74, src/TimeDisc.cpp:3-66,
 // src/Assigners.cpp:8-44) with the state, the node bathymetry and every
 // quantity derived from them carried as Dual<1>: the value parts are the
@@ -10,15 +10,16 @@
 // decisions are taken on the values, as the forward takes them.  Geometry in
 // the plane (x, y) has no tangent.
 //
-// Regime, as the adjoint's: fully wet cells, dry cells, and partially wet
-// cells at or above their highest vertex; a partially wet cell below it (or
-// a partially wet neighbour whose cubic level enters an active gradient)
-// is counted in DevTan::flags.
+// Partially wet cells: PartWet1's level (cube root or bisection root, with
+// their implicit derivatives) in pass 1, PartWet2's plane in pass 2
+// (k_tl_recon2) through the pass-1 node maximum, whose tangent is that of
+// the cell attaining it (swe_pw.h).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
 
 #include "swe_common.h"
+#include "swe_pw.h"
 
 namespace swe {
 
@@ -115,13 +116,11 @@ __device__ __forceinline__ SlotT<D> slot_value_t(const MusclT& m, double cx, dou
   return s;
 }
 
-__device__ __forceinline__ void flag(const DevTan& Q) { atomicAdd(Q.flags, 1u); }
-
 }  // namespace
 
 // ------------------------------------------------------ reconstruction --
-// k_recon1 with the partially wet cells' pass-2 fallback (level w, zero
-// gradient: the only partially wet reconstruction in the regime)
+// pass 1 (k_recon1): dry, fully wet and the partially wet cells' PartWet1
+// level (kept in wrec1 for the node maxima of pass 2)
 __global__ void __launch_bounds__(256) k_tl_recon(DevMesh M, DevState S, DevState Sd, DevWork W, DevTan Q) {
   const int64_t i64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i64 >= M.T) return;
@@ -140,11 +139,14 @@ __global__ void __launch_bounds__(256) k_tl_recon(DevMesh M, DevState S, DevStat
     gradient_t(p0, p1, p2, gx, gy);
   } else if ((fl & CELL_BND) || !(ri.zmax < w.v)) {
     k = 1;
-    if (!(w.v > ri.zmax)) flag(Q);  // below the highest vertex: PartWet1/2 proper
+    const D zmn = dmin(dmin(p0.z, p1.z), p2.z), zmx = dmax(dmax(p0.z, p1.z), p2.z);
+    w0 = partwet1_g(zmn, zmn, zmx, bc, w);
+    W.wrec1[i] = w0.v;
+    Q.wrec1[i] = w0.d[0];
   } else {
     k = 2;
     P3 pts[3], E[3];
-    bool any_dry = false, cubic_nb = false;
+    bool any_dry = false;
 #pragma unroll
     for (int kk = 0; kk < 3; ++kk) {
       E[kk] = midpoint_t(pick_t(p0, p1, p2, ep0_local(fl, kk)), pick_t(p0, p1, p2, ep1_local(fl, kk)));
@@ -157,15 +159,13 @@ __global__ void __launch_bounds__(256) k_tl_recon(DevMesh M, DevState S, DevStat
         any_dry = true;
         pts[kk] = {0., 0., dzero()};
       } else {
-        // PartWet1 level of the neighbour: its w at or above its highest
-        // vertex; otherwise the cubic / bisection value (no tangent)
-        const double w0jv = partwet1_w(rj.zmin, rj.zmin, rj.zmax, rj.bc, wj.v);
-        const bool above = wj.v >= rj.zmax;
-        const D w0j = dd(w0jv, above ? wj.d[0] : 0.);
+        // PartWet1 level of the partially wet neighbour, from its vertices
+        const P3 q0 = node_t(M, Q, M.tp[j]), q1 = node_t(M, Q, M.tp[T + j]), q2 = node_t(M, Q, M.tp[2 * T + j]);
+        const D zmnj = dmin(dmin(q0.z, q1.z), q2.z), zmxj = dmax(dmax(q0.z, q1.z), q2.z);
+        const D w0j = partwet1_g(zmnj, zmnj, zmxj, centroid_t(q0, q1, q2).z, wj);
         const double dx = E[kk].x - rj.cx, dy = E[kk].y - rj.cy;
         D a = w0j + (0. * dx + 0. * dy);
         if (!((a - E[kk].z) >= 0)) a = E[kk].z;
-        else if (!above) cubic_nb = true;
         pts[kk] = {E[kk].x, E[kk].y, 0.5 * (w + a)};
       }
     }
@@ -196,13 +196,64 @@ __global__ void __launch_bounds__(256) k_tl_recon(DevMesh M, DevState S, DevStat
       }
       gx = tvd * gx;
       gy = tvd * gy;
-      if (all_wet && tvd != 0. && cubic_nb) flag(Q);
     }
   }
   W.mw[i] = w0.v;
   W.mgx[i] = gx.v;
   W.mgy[i] = gy.v;
   W.cls[i] = k;
+  Q.mw[i] = w0.d[0];
+  Q.mgx[i] = gx.d[0];
+  Q.mgy[i] = gy.d[0];
+}
+
+// pass 2 (recon2_cell): PartWet2 of the partially wet cells through the
+// pass-1 node maximum at their lowest vertex (node_max_pass1), whose tangent
+// is the attaining cell's pass-1 tangent (or the node's own height)
+__device__ inline D node_max_t(const DevMesh& M, const DevWork& W, const DevTan& Q, int n, const P3& P) {
+  D mx = P.z;
+  for (int q = M.n2c_off[n]; q < M.n2c_off[n + 1]; ++q) {
+    const int c = M.n2c[q];
+    D w0 = dd(W.mw[c], Q.mw[c]), gx = dd(W.mgx[c], Q.mgx[c]), gy = dd(W.mgy[c], Q.mgy[c]);
+    if (W.cls[c] == 1) {
+      w0 = dd(W.wrec1[c], Q.wrec1[c]);
+      gx = dzero();
+      gy = dzero();
+    }
+    const double dx = P.x - M.cx[c], dy = P.y - M.cy[c];
+    D a = w0 + (gx * dx + gy * dy);
+    if (!((a - P.z) >= 0)) a = P.z;
+    mx = dmax(mx, a);
+  }
+  return mx;
+}
+
+__global__ void __launch_bounds__(256) k_tl_recon2(DevMesh M, DevState S, DevState Sd, DevWork W, DevTan Q) {
+  const int64_t i64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i64 >= M.T) return;
+  const int i = (int)i64;
+  if (W.cls[i] != 1) return;
+  const int64_t T = M.T;
+  int i0 = M.tp[i], i1 = M.tp[T + i], i2 = M.tp[2 * T + i];
+  const P3 v0 = node_t(M, Q, i0), v1 = node_t(M, Q, i1), v2 = node_t(M, Q, i2);
+  const D bc = centroid_t(v0, v1, v2).z;
+  PT<D> Q0{dd(v0.x, 0.), dd(v0.y, 0.), v0.z}, Q1{dd(v1.x, 0.), dd(v1.y, 0.), v1.z}, Q2{dd(v2.x, 0.), dd(v2.y, 0.), v2.z};
+  sort3_g(Q0, Q1, Q2, i0, i1, i2);
+  const D w = dd(S.w[i], Sd.w[i]);
+  D w0 = dd(W.wrec1[i], Q.wrec1[i]), gx = dzero(), gy = dzero();  // fallback: the pass-1 level
+  const bool fallback = (w.v > Q2.z.v) || (Q2.z.v - Q0.z.v < kTol);
+  if (!fallback) {
+    const D w23 = node_max_t(M, W, Q, i0, P3{Q0.x.v, Q0.y.v, Q0.z});
+    D a, bx, by;
+    if (partwet2_g(Q0, Q1, Q2, M.cx[i], M.cy[i], bc, w, w23, a, bx, by)) {
+      w0 = a;
+      gx = bx;
+      gy = by;
+    }
+  }
+  W.mw[i] = w0.v;
+  W.mgx[i] = gx.v;
+  W.mgy[i] = gy.v;
   Q.mw[i] = w0.d[0];
   Q.mgx[i] = gx.d[0];
   Q.mgy[i] = gy.d[0];
@@ -372,6 +423,7 @@ hipError_t launch_tangent_stage(const DevMesh& M, const DevState& S, const DevSt
                                 const DevTan& Q, const StageArgs& A, int flux, int ws, int opts,
                                 hipStream_t st) {
   hipLaunchKernelGGL(k_tl_recon, grid_plain(M.T), dim3(256), 0, st, M, S, Sd, W, Q);
+  hipLaunchKernelGGL(k_tl_recon2, grid_plain(M.T), dim3(256), 0, st, M, S, Sd, W, Q);
 #define SWE_TL_FLUX(F, Wv)                                                                         \
   if (flux == F && ws == Wv)                                                                       \
     hipLaunchKernelGGL((k_tl_flux<F, Wv>), grid_plain(M.E), dim3(256), 0, st, M, S, Sd, W, Q, A.cor);

@@ -356,11 +356,106 @@ def test_stacked_batch_equals_members():
         assert np.array_equal(a0, s0[cs]) and np.array_equal(az, zb[vs])
 
 
-def test_adjoint_refuses_shoreline_states():
-    from tests import _common as C
-    xyz, topo, prim = C.wet_dry_mesh(20, seed=2)
-    sd = swe.SpaceDisc(swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt], swe.Domain(xyz, topo), prim)
-    with pytest.raises(swe.SolverError):
+def shoreline_case(n=20, seed=2):
+    """Emerging bumps under a sloping, non-flat surface: dry, partially wet
+    (below their highest vertex: PartWet1 cube-root and bisection levels,
+    PartWet2 planes) and fully wet cells.  The surface is nowhere flat (on a
+    flat surface every gradient is zero and the TVD check sits on its
+    boundary, a kink that finite differences average over).  Returns (domain,
+    prim, mask of dry cells, mask of their nodes): perturbing a dry cell's
+    level or its bathymetry moves it off the wet/dry kink h = 0."""
+    xyz, topo, _ = C.wet_dry_mesh(n, seed=seed)
+    dom = swe.Domain(xyz, topo)
+    tp = topo.tables()["tp"]
+    c = dom.centroids()
+    bc = C.cell_bath(xyz, tp)
+    level = 0.8 + 0.03 * np.sin(4 * c[:, 0] + 0.3) * np.cos(3 * c[:, 1]) + 0.1 * c[:, 0]
+    h = np.maximum(0.0, level - bc)
+    rng = np.random.default_rng(seed + 1)
+    prim = np.stack([bc + h, rng.normal(0, 0.1, len(h)) * (h > 0), rng.normal(0, 0.1, len(h)) * (h > 0)], 1)
+    dry = h <= 1e-12
+    dry_nodes = np.zeros(xyz.shape[0], bool)
+    dry_nodes[tp[dry].ravel()] = True
+    return dom, prim, dry, dry_nodes
+
+
+@pytest.mark.parametrize("scheme", ["Euler", "SSPRK2"])
+def test_tangent_through_shorelines(scheme):
+    """Through shorelines (partially wet cells below their highest vertex:
+    PartWet1's cube-root and bisection levels, PartWet2's planes through the
+    node maxima): the tangent-linear values equal the forward's bit for bit
+    and the tangent agrees with central differences of the forward."""
+    dom, prim0, dry, dry_nodes = shoreline_case()
+    xyz, tp = dom.geometry, dom.topology.tables()["tp"]
+    sd = swe.SpaceDisc(swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt], dom, prim0)
+    sd.ComputeInterfaceValues()
+    cls = sd.CellClass()
+    below = int(np.sum((cls == 1) & (prim0[:, 0] < xyz[tp, 2].max(1)) & ~dry))
+    print(f"{below} partially wet cells below their highest vertex, {int(dry.sum())} dry", flush=True)
+    assert below > 10
+    z = dom.geometry[:, 2].copy()
+    n, dt = 4, 1e-3
+    sd.SetVolField(prim0)
+    sd.run(scheme, n, dt=dt)
+    ref = sd.GetVolField()
+    rng = np.random.default_rng(12)
+    ybar = rng.standard_normal(prim0.shape)
+    fn = lambda p, c=ybar: (float(np.sum(c * p)), c)
+    checked = 0
+    for trial in range(4):
+        pdot = rng.standard_normal(prim0.shape) * 1e-2
+        pdot[dry] = 0.0
+        zdot = rng.standard_normal(z.shape) * 1e-2 if trial % 2 else np.zeros_like(z)
+        zdot[dry_nodes] = 0.0
+        sd.SetVolField(prim0)
+        tdot, _ = sd.tangent_run(pdot, n, z_dot=zdot, scheme=scheme, dt=dt)
+        assert np.array_equal(sd.GetVolField(), ref)
+        lhs = float(np.sum(ybar * tdot))
+        fd = directional_fd(sd, prim0, z, n, fn, pdot, zdot, scheme, dt, 1e-6)
+        if fd is None:
+            print(f"shoreline {scheme} trial {trial}: the difference crossed a switch", flush=True)
+            continue
+        print(f"shoreline {scheme} trial {trial}: tangent {lhs:.12e} fd {fd:.12e} "
+              f"rel {abs(lhs - fd) / abs(fd):.2e}", flush=True)
+        assert abs(lhs - fd) <= 1e-5 * abs(fd) + 1e-12
+        checked += 1
+    assert checked >= 2
+
+
+@pytest.mark.parametrize("scheme,flux", [("Euler", "HLLC"), ("SSPRK2", "HLLC"), ("SSPRK3", "HLL")])
+def test_adjoint_through_shorelines(scheme, flux):
+    """The adjoint through partially wet cells below their highest vertex
+    (PartWet1/PartWet2 local Jacobians, the node maxima routed to the cells
+    attaining them, neighbours' cube-root and bisection levels): duality with
+    the tangent-linear build (itself checked against central differences in
+    test_tangent_through_shorelines), state and bathymetry directions."""
+    dom, prim0, dry, dry_nodes = shoreline_case()
+    fl = getattr(swe.Fluxes, flux)[swe.Wavespeeds.Einfeldt]
+    sd = swe.SpaceDisc(fl, dom, prim0, cor=0.2)
+    z = dom.geometry[:, 2].copy()
+    n, dt = 6, 1e-3
+    rng = np.random.default_rng(21)
+    ybar = rng.standard_normal(prim0.shape)
+    sd.SetVolField(prim0)
+    s0, zb, _ = sd.adjoint_run(ybar, n, scheme=scheme, dt=dt, checkpoint_every=4)
+    for _ in range(3):
+        pdot = rng.standard_normal(prim0.shape) * 1e-2
+        zdot = rng.standard_normal(z.shape) * 1e-2
+        sd.SetVolField(prim0)
+        tdot, _ = sd.tangent_run(pdot, n, z_dot=zdot, scheme=scheme, dt=dt)
+        lhs = float(np.sum(ybar * tdot))
+        rhs = float(np.sum(s0 * pdot) + np.sum(zb * zdot))
+        scale = float(np.sum(np.abs(ybar * tdot)))
+        print(f"shoreline {scheme} {flux} duality: {lhs:.15e} {rhs:.15e} diff/scale {abs(lhs - rhs) / scale:.1e}",
+              flush=True)
+        assert abs(lhs - rhs) <= 1e-12 * scale
+
+
+def test_adjoint_refuses_head_options():
+    xyz, topo, prim = C.wet_dry_mesh(12, seed=2)
+    sd = swe.SpaceDisc(swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt], swe.Domain(xyz, topo), prim,
+                       options=swe.OPT_HEAD_GRADIENT)
+    with pytest.raises(ValueError):
         sd.adjoint_run(np.ones_like(prim), 2, dt=1e-3)
 
 
