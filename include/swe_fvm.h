@@ -242,11 +242,12 @@ int swe_state_stats(swe_solver* s, double* out5);
  * and stage plus 24 per cell and stage state), then reversed; the last
  * segment's recomputation is the forward run.  Branch decisions (wet/dry,
  * limiter, wave-speed selections) are the forward ones; the Riemann
- * solver's local Jacobian is exact (dual numbers).  Supported: every cell
- * fully wet, dry, or partially wet at or above its highest vertex;
- * shoreline cells below it give SWE_ERR_SOLVER (the state is still
- * advanced).  ms (optional, 2 doubles): forward sweep and reverse sweep
- * (recomputation included) milliseconds.  Single-mesh handles only. */
+ * solver's and the partially wet reconstructions' local Jacobians are
+ * exact (dual numbers; implicit derivatives of the cube-root and bisection
+ * levels), the node maxima's adjoints go to the cells attaining them.  The
+ * HEAD-compatibility options give SWE_ERR_ARG.  ms (optional, 2 doubles):
+ * forward sweep and reverse sweep (recomputation included) milliseconds.
+ * Single-mesh handles only. */
 int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int64_t nsteps,
                     int64_t checkpoint_every, const double* final_bar, double* state0_bar,
                     double* z_bar, double* ms);
@@ -256,8 +257,7 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
  * tangent of the node bathymetry; NULL: zero), computed by the stage kernels
  * built on dual numbers.  Their value parts are the forward step's, bit for
  * bit: U_N is the current state on return.  final_dot (T x 3) receives the
- * tangent of U_N.  Same regime as swe_adjoint_run (SWE_ERR_SOLVER outside
- * it); with the adjoint it satisfies <final_bar, final_dot> =
+ * tangent of U_N.  With the adjoint it satisfies <final_bar, final_dot> =
  * <state0_bar, state0_dot> + <z_bar, z_dot>.  ms (optional, 1 double):
  * milliseconds.  Single-mesh handles only. */
 int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int64_t nsteps,
