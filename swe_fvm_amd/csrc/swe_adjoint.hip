@@ -68,8 +68,9 @@ __device__ inline double node_max_arg(const DevMesh& M, const DevWork& W, int n,
   return mx;
 }
 
-// lowest vertex of cell c by PartWet2's sort (recon2_cell)
-__device__ inline int lowest_node(const DevMesh& M, int c) {
+// lowest vertex of cell c by PartWet2's sort (recon2_cell), and the highest
+// vertex height
+__device__ inline int lowest_node(const DevMesh& M, int c, double& zmx) {
   const int64_t T = M.T;
   int i0 = M.tp[c], i1 = M.tp[T + c], i2 = M.tp[2 * T + c];
   double z0 = M.pz[i0], z1 = M.pz[i1], z2 = M.pz[i2];
@@ -78,26 +79,18 @@ __device__ inline int lowest_node(const DevMesh& M, int c) {
   if (z0 > z1) { tz = z0; z0 = z1; z1 = tz; ti = i0; i0 = i1; i1 = ti; }
   if (z1 > z2) { tz = z1; z1 = z2; z2 = tz; ti = i1; i1 = i2; i2 = ti; }
   if (z0 > z1) { ti = i0; i0 = i1; i1 = ti; }
+  zmx = z2;
   return i0;
-}
-
-// PartWet1 level of cell c: d level / d (w, z of local vertices 0..2)
-__device__ inline void pw1_jac(const DevMesh& M, const DevState& S, int c, double jac[4]) {
-  using D = Dual<4>;
-  const int64_t T = M.T;
-  const D z0 = dvar<4>(M.pz[M.tp[c]], 1), z1 = dvar<4>(M.pz[M.tp[T + c]], 2), z2 = dvar<4>(M.pz[M.tp[2 * T + c]], 3);
-  const double t = 1. / 3.;
-  const D bc = (z0 * t + z1 * t) + z2 * t;
-  const D zmn = dmin(dmin(z0, z1), z2), zmx = dmax(dmax(z0, z1), z2);
-  const D lv = partwet1_g(zmn, zmn, zmx, bc, dvar<4>(S.w[c], 0));
-  for (int k = 0; k < 4; ++k) jac[k] = lv.d[k];
 }
 
 // Final MUSCL (w0, gx, gy) of partially wet cell c: PartWet2, or the
 // PartWet1 level where pass 2 falls back, as functions of (w, z of local
 // vertices 0..2, pass-1 node maximum at the lowest vertex); true when the
-// node maximum enters
-__device__ inline bool pw2_jac(const DevMesh& M, const DevState& S, const DevWork& W, int c, double jac[3][5]) {
+// node maximum enters.  p1[4]: the PartWet1 level's derivatives in (w, z of
+// local vertices 0..2), the cell's pass-1 value where a node maximum or a
+// full-wet neighbour's plane takes it.
+__device__ inline bool pw2_jac(const DevMesh& M, const DevState& S, const DevWork& W, int c, double jac[3][5],
+                               double p1[4]) {
   using D = Dual<5>;
   const int64_t T = M.T;
   int i0 = M.tp[c], i1 = M.tp[T + c], i2 = M.tp[2 * T + c];
@@ -109,6 +102,7 @@ __device__ inline bool pw2_jac(const DevMesh& M, const DevState& S, const DevWor
   const D w = dvar<5>(S.w[c], 0);
   const D zmn = dmin(dmin(Q0.z, Q1.z), Q2.z), zmx = dmax(dmax(Q0.z, Q1.z), Q2.z);
   D w0 = partwet1_g(zmn, zmn, zmx, bc, w), gx = dconst<5>(0.), gy = dconst<5>(0.);
+  for (int k = 0; k < 4; ++k) p1[k] = w0.d[k];
   bool used = false;
   sort3_g(Q0, Q1, Q2, i0, i1, i2);
   if (!((w.v > Q2.z.v) || (Q2.z.v - Q0.z.v < kTol))) {
@@ -486,7 +480,9 @@ __global__ void __launch_bounds__(256) k_adj_flux(DevMesh M, DevState S, DevWork
 // For each partially wet cell (pass 2's list): its final MUSCL adjoint (its
 // own update's and its edge slots', gathered as k_adj_recon gathers it)
 // through PartWet2 / PartWet1: the cell's own w and corner-z adjoints, and
-// the adjoint of the node maximum at its lowest vertex.
+// the adjoint of the node maximum at its lowest vertex; and the Jacobian of
+// its PartWet1 level, which k_adj_recon applies where that level enters a
+// node maximum or a full-wet neighbour's plane (no dual numbers there).
 __global__ void __launch_bounds__(256) k_adj_pw(DevMesh M, DevState S, DevWork W, DevAdj J) {
   const unsigned npw = *W.pw_count;
   const int64_t T = M.T, Ecnt = M.E;
@@ -505,34 +501,42 @@ __global__ void __launch_bounds__(256) k_adj_pw(DevMesh M, DevState S, DevWork W
       slot_bar(mu, M.cx[i], M.cy[i], edge_mid(p0, p1, p2, fl, k), J.sl_edge[(0 * 2 + side) * Ecnt + e],
                J.sl_edge[(1 * 2 + side) * Ecnt + e], J.sl_edge[(2 * 2 + side) * Ecnt + e], mb, ez);
     }
-    double jac[3][5];
-    const bool used = pw2_jac(M, S, W, i, jac);
-    for (int c = 0; c < 4; ++c) J.pw_own[c * T + i] = mb[0] * jac[0][c] + mb[1] * jac[1][c] + mb[2] * jac[2][c];
+    double jac[3][5], lj[4];
+    const bool used = pw2_jac(M, S, W, i, jac, lj);
+    for (int c = 0; c < 4; ++c) {
+      J.pw_own[c * T + i] = mb[0] * jac[0][c] + mb[1] * jac[1][c] + mb[2] * jac[2][c];
+      J.pw1_jac[c * T + i] = lj[c];
+    }
     J.pw_w23bar[i] = used ? mb[0] * jac[0][4] + mb[1] * jac[1][4] + mb[2] * jac[2][4] : 0.;
   }
 }
 
-// The node maxima's adjoints, one owner per node (the first partially wet
-// cell around it, in the node-to-cell map's order, whose lowest vertex it
-// is): summed over those cells and routed to the cell attaining the
-// maximum (its corner slot, read and cleared by k_adj_recon) or to the
-// node's own height.
-__global__ void __launch_bounds__(256) k_adj_pwnode(DevMesh M, DevWork W, DevAdj J) {
+// The node maxima's adjoints, one owner per node (the first cell of pass
+// 2's list around it, in the node-to-cell map's order, whose lowest vertex
+// it is and whose node-maximum adjoint is nonzero): summed over those cells
+// and routed to the cell attaining the maximum (its corner slot, read and
+// cleared by k_adj_recon) or to the node's own height.  A cell is in the
+// list when it is partially wet with its level not above every vertex.
+__global__ void __launch_bounds__(256) k_adj_pwnode(DevMesh M, DevState S, DevWork W, DevAdj J) {
   const unsigned npw = *W.pw_count;
   const int64_t T = M.T;
   for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < npw; q += gridDim.x * blockDim.x) {
     const int i = W.pw_list[q];
-    const int n = lowest_node(M, i);
+    if (J.pw_w23bar[i] == 0.) continue;
+    double zmx;
+    const int n = lowest_node(M, i, zmx);
     int owner = -1;
     double sum = 0.;
     for (int r = M.n2c_off[n]; r < M.n2c_off[n + 1]; ++r) {
       const int c = M.n2c[r];
-      if (W.cls[c] == 1 && lowest_node(M, c) == n) {
-        if (owner < 0) owner = c;
-        sum += J.pw_w23bar[c];
-      }
+      if (W.cls[c] != 1) continue;
+      if (lowest_node(M, c, zmx) != n || S.w[c] > zmx) continue;
+      const double g = J.pw_w23bar[c];
+      if (g == 0.) continue;
+      if (owner < 0) owner = c;
+      sum += g;
     }
-    if (owner != i || sum == 0.) continue;
+    if (owner != i) continue;
     int arg;
     node_max_arg(M, W, n, node(M, n), arg);
     if (arg >= 0) {
@@ -625,17 +629,18 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
     J.sbar[T + i] += mb[3];
     J.sbar[2 * T + i] += mb[4];
     const double zmax = dmax(dmax(p0.z, p1.z), p2.z);
-    if (cls == 1) {
+    if (cls == 1 && w > zmax) {
+      // not in pass 2's list: the level is w with a zero gradient, also as
+      // the pass-1 value of the node maxima this cell attains
+      wb += mb[0] + mp[0];
+    } else if (cls == 1) {
       // PartWet2 / PartWet1 (k_adj_pw), and the PartWet1 level as the
       // pass-1 value of the node maxima this cell attains
-      (void)zmax;
       wb += J.pw_own[i];
       for (int k = 0; k < 3; ++k) zc[k] += J.pw_own[(1 + k) * T + i];
       if (mp[0] != 0.) {
-        double jac[4];
-        pw1_jac(M, S, i, jac);
-        wb += mp[0] * jac[0];
-        for (int k = 0; k < 3; ++k) zc[k] += mp[0] * jac[1 + k];
+        wb += mp[0] * J.pw1_jac[i];
+        for (int k = 0; k < 3; ++k) zc[k] += mp[0] * J.pw1_jac[(1 + k) * T + i];
       }
     } else {
       // ReconstructFullWetCell (src/MUSCLObject.cpp:38-84): w0 = w; the
@@ -721,18 +726,17 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
               // the neighbour's PartWet1 level: its w (gathered by it), the
               // two vertices it shares with this cell (own corners) and the
               // opposite one (per edge side, folded in k_adj_z)
-              const int j = M.tt[k * T + i];
-              double jac[4];
-              pw1_jac(M, S, j, jac);
+              const int j = M.tt[k * T + i];  // partially wet: in pass 2's list
               const double lb = 0.5 * vb;
-              wnb[k] = lb * jac[0];
+              wnb[k] = lb * J.pw1_jac[j];
               const int e = M.te[k * T + i];
               const int sj = M.et0[e] == j ? 0 : 1;
               for (int m = 0; m < 3; ++m) {
                 const int nj = M.tp[m * T + j];
                 const int mi = M.tp[i] == nj ? 0 : (M.tp[T + i] == nj ? 1 : (M.tp[2 * T + i] == nj ? 2 : -1));
-                if (mi >= 0) zc[mi] += lb * jac[1 + m];
-                else J.zopp_acc[sj * Ecnt + e] += lb * jac[1 + m];
+                const double g = lb * J.pw1_jac[(1 + m) * T + j];
+                if (mi >= 0) zc[mi] += g;
+                else J.zopp_acc[sj * Ecnt + e] += g;
               }
             }
           }
@@ -823,7 +827,7 @@ hipError_t launch_adjoint_stage(const DevMesh& M, const DevState& S, const DevSt
     dim3 g = grid_plain(M.T);
     if (g.x > 1024) g.x = 1024;
     hipLaunchKernelGGL(k_adj_pw, g, dim3(256), 0, st, M, S, W, J);
-    hipLaunchKernelGGL(k_adj_pwnode, g, dim3(256), 0, st, M, W, J);
+    hipLaunchKernelGGL(k_adj_pwnode, g, dim3(256), 0, st, M, S, W, J);
   }
   if (opts & kOptDrainLength) hipLaunchKernelGGL(k_adj_recon<true>, grid_plain(M.T), dim3(256), 0, st, M, S, W, J, dts);
   else hipLaunchKernelGGL(k_adj_recon<false>, grid_plain(M.T), dim3(256), 0, st, M, S, W, J, dts);

@@ -188,8 +188,10 @@ __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W
   W.mgy[i] = gy;
   W.cls[i] = k;
   // append partially wet cells to pass 2's work list, one atomic per wave
-  // (the order of the list is irrelevant: pass 2 is independent per cell)
-  const bool pw = k == 1;
+  // (the order of the list is irrelevant: pass 2 is independent per cell).
+  // A level above every vertex (wet boundary cells) is pass 2's fallback to
+  // the pass-1 level already written: such cells are left out.
+  const bool pw = k == 1 && !(w > zmx);
   const unsigned long long m = __ballot(pw);
   if (m) {
     const int lane = threadIdx.x & 63;

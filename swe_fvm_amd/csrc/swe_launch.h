@@ -144,6 +144,7 @@ struct DevAdj {
   // own height; the height adjoint of a neighbour's vertex opposite an edge
   // (partially wet neighbour levels), per edge side, summed over stages
   double* pw_own;      // [4][T] w, corner z 0..2
+  double* pw1_jac;     // [4][T] d PartWet1 level / d (w, corner z 0..2)
   double* pw_w23bar;   // [T]
   double* p1bar;       // [3][T] routed pass-1 level adjoint per corner
   double* zn_acc;      // [V]
