@@ -8,8 +8,9 @@
 // those the forward kernels wrote for this stage, so the linearisation point
 // is bit-identical to the forward step.  Branch
 // decisions (wet/dry, the limiter's all-or-nothing choices, min / max
-// selections) are the forward ones; the flux's local Jacobian comes from
-// the flux template instantiated on Dual<6>.
+// selections) are the forward ones; the Riemann flux is differentiated by
+// its reverse mode (swe_flux_adj.h), the transpose of the flux template's
+// dual-number derivative.
 //
 // Every contribution is written to a slot owned by one (cell, side) or
 // (edge, side) and gathered in a fixed order by its consumer, so the
@@ -24,6 +25,7 @@
 #include <stdint.h>
 
 #include "swe_common.h"
+#include "swe_flux_adj.h"
 #include "swe_pw.h"
 
 namespace swe {
@@ -409,14 +411,12 @@ __device__ inline void side_flux_bar(const DevMesh& M, const DevState& S, const 
 
 // ----------------------------------------------------------------- flux --
 // Reverse of k_flux for edge e: the flux adjoint of its two sides (from the
-// cells' update adjoints), through the Riemann solver's local Jacobian (the
-// flux template on Dual<6>: inputs L (w, u, v), R (w, u, v); edge z by the
-// chain rule), into
-// the adjoints of the two slot values; k_adj_recon takes them through the
-// slot reconstruction.
+// cells' update adjoints), through the reverse mode of the Riemann solver
+// (inputs L (w, u, v), R (w, u, v); edge z by the chain rule), into the
+// adjoints of the two slot values; k_adj_recon takes them through the slot
+// reconstruction.
 template <int FLUX, int WS, bool LEN>
-__global__ void __launch_bounds__(256) k_adj_flux(DevMesh M, DevState S, DevWork W, DevAdj J,
-                                                  double cor, double dts) {
+__global__ void __launch_bounds__(256) k_adj_flux(DevMesh M, DevState S, DevWork W, DevAdj J, double dts) {
   const int64_t e64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (e64 >= M.E) return;
   const int e = (int)e64;
@@ -449,18 +449,11 @@ __global__ void __launch_bounds__(256) k_adj_flux(DevMesh M, DevState S, DevWork
     const double nx = ty, ny = -tx;
     const Slot L = slot_value(mf, M.cx[f], M.cy[f], E);
     const Slot R = slot_value(mt, M.cx[tr], M.cy[tr], E);
-    // the edge bathymetry enters the fluxer only through hl = L.w - z and
-    // hr = R.w - z (include/Fluxes.h:18-21,63-66): dF/dz = -(dF/dL.w +
-    // dF/dR.w), so six directions carry the Jacobian
-    using D = Dual<6>;
-    const SlotT<D> Ld{dvar<6>(L.w, 0), dvar<6>(L.u, 1), dvar<6>(L.v, 2)};
-    const SlotT<D> Rd{dvar<6>(R.w, 3), dvar<6>(R.u, 4), dvar<6>(R.v, 5)};
-    const D be = dconst<6>(E.z);
-    D Fd[3];
-    double lw = 1.;
-    if (FLUX == 0) flux_hll<WS>(nx, ny, be, Ld, Rd, M.edmin[e], cor, lw, Fd);
-    else flux_hllc<WS>(tx, ty, be, Ld, Rd, M.edmin[e], cor, lw, Fd);
-    for (int k = 0; k < 6; ++k) g[k] = Fb[0] * Fd[0].d[k] + Fb[1] * Fd[1].d[k] + Fb[2] * Fd[2].d[k];
+    // reverse mode of the Riemann solver (swe_flux_adj.h); the edge
+    // bathymetry enters it only through hl = L.w - z and hr = R.w - z
+    // (include/Fluxes.h:18-21,63-66): dF/dz = -(dF/dL.w + dF/dR.w)
+    if (FLUX == 0) flux_hll_vjp<WS>(nx, ny, E.z, L, R, Fb, g);
+    else flux_hllc_vjp<WS>(tx, ty, E.z, L, R, Fb, g);
     g[6] = -(g[0] + g[3]);
     if (t < 0) {  // free flow: R is L
       for (int k = 0; k < 3; ++k) {
@@ -812,7 +805,7 @@ hipError_t launch_adjoint_stage(const DevMesh& M, const DevState& S, const DevSt
   hipLaunchKernelGGL(k_adj_update, grid_plain(M.T), dim3(256), 0, st, M, S, U0, W, A, J, wnb_in, u0mode);
 #define SWE_ADJ_FLUX(F, Wv, L)                                                                   \
   if (flux == F && ws == Wv && len == L)                                                         \
-    hipLaunchKernelGGL((k_adj_flux<F, Wv, L>), grid_plain(M.E), dim3(256), 0, st, M, S, W, J, A.cor, dts);
+    hipLaunchKernelGGL((k_adj_flux<F, Wv, L>), grid_plain(M.E), dim3(256), 0, st, M, S, W, J, dts);
   for (int li = 0; li < 2; ++li) {
     const bool len = li == 1;
     if (len != ((opts & kOptDrainLength) != 0)) continue;

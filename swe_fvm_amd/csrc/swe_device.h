@@ -20,11 +20,11 @@ struct V3 {
 };
 
 // std::min / std::max semantics.
-__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+__host__ __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
+__host__ __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
 
 // IsWet: include/Bathymetry.h:5-8
-__device__ __forceinline__ bool is_wet(double h) { return h > 1e-12; }
+__host__ __device__ __forceinline__ bool is_wet(double h) { return h > 1e-12; }
 
 // Cube root by a fixed sequence of IEEE operations: x = m 2^(3q), m in
 // [0.5, 4), linear seed, five Newton steps y -= (y^3 - m) / (3 y^2).
@@ -170,12 +170,14 @@ __device__ __forceinline__ void cons_assign(double b, double h, double hu, doubl
 }
 
 // The flux functions are templates on the scalar type T of the states
-// (double in the step; Dual<N> in the adjoint, for the local Jacobian of the
-// very same operation sequence); geometry stays double.
+// (double in the step; Dual<N> in the tangent-linear kernels, for the
+// derivative of the very same operation sequence); geometry stays double.
+// They are host-callable as well, for the host check of their reverse mode
+// (swe_flux_adj.h) against the dual-number derivative.
 
 // ElemFlux (include/SpaceDisc.h:4-18)
 template <class T>
-__device__ __forceinline__ void elem_flux(double nx, double ny, const T& h, const T& hu, const T& hv,
+__host__ __device__ __forceinline__ void elem_flux(double nx, double ny, const T& h, const T& hu, const T& hv,
                                           T& f0, T& f1, T& f2) {
   f0 = Zero<T>::get(); f1 = f0; f2 = f0;
   if (is_wet(h)) {
@@ -188,7 +190,7 @@ __device__ __forceinline__ void elem_flux(double nx, double ny, const T& h, cons
 
 // Wavespeeds (src/Fluxes.cpp:5-26)
 template <int WS, class T>
-__device__ __forceinline__ void wavespeed(const T& ul, const T& hl, const T& ur, const T& hr, T& a0,
+__host__ __device__ __forceinline__ void wavespeed(const T& ul, const T& hl, const T& ur, const T& hr, T& a0,
                                           T& a1) {
   T cl = sqrt(hl), cr = sqrt(hr);
   if (WS == 0) {  // Rusanov
@@ -214,7 +216,7 @@ using Slot = SlotT<double>;
 // candidate for the CFL minimum (unchanged when the flux returns early);
 // dmn = min(2 Area(from) / L, 2 Area(to) / L).
 template <int WS, class T>
-__device__ __forceinline__ void flux_hll(double nx, double ny, const T& be, const SlotT<T>& L,
+__host__ __device__ __forceinline__ void flux_hll(double nx, double ny, const T& be, const SlotT<T>& L,
                                          const SlotT<T>& R, double dmn, double cor, double& lw,
                                          T F[3]) {
   T ul = L.u * nx + L.v * ny;
@@ -242,7 +244,7 @@ __device__ __forceinline__ void flux_hll(double nx, double ny, const T& be, cons
 
 // Fluxes::HLLC<W> (include/Fluxes.h:56-111)
 template <int WS, class T>
-__device__ __forceinline__ void flux_hllc(double tx, double ty, const T& be, const SlotT<T>& L,
+__host__ __device__ __forceinline__ void flux_hllc(double tx, double ty, const T& be, const SlotT<T>& L,
                                           const SlotT<T>& R, double dmn, double cor, double& lw,
                                           T F[3]) {
   double nx = ty, ny = -tx;
