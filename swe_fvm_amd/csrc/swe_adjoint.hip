@@ -168,7 +168,7 @@ __device__ inline void slot_bar(const Muscl& m, double cx, double cy, const V3& 
 // k_adj_flux weighs each edge side), the draining-dt adjoints (own,
 // neighbours'), the MUSCL adjoint, the U0 adjoint, the first part of the S
 // adjoint and the corner z adjoint.
-__global__ void __launch_bounds__(256) k_adj_update(DevMesh M, DevState S, DevState U0, DevWork W,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_adj_update(DevMesh M, DevState S, DevState U0, DevWork W,
                                                     StageArgs A, DevAdj J, const double* wnb_in,
                                                     int u0mode) {
   const int64_t i64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -335,7 +335,26 @@ __global__ void __launch_bounds__(256) k_adj_update(DevMesh M, DevState S, DevSt
   }
   (void)ek;
   if (dti < dts) J.dtd_own[i] = dtib;
-  for (int q = 0; q < 5; ++q) J.m_cell[q * T + i] = mb[q];
+  // the update's own MUSCL adjoint, applied here where the reconstruction's
+  // reverse is a fixed map: the level to the cell's w (full-wet cells, and
+  // partially wet ones above every vertex, which pass 2 leaves at w) or to
+  // the corners through b_cell (dry cells); u0, v0 to the cell's u, v (zero
+  // for dry cells).  What stays for k_adj_recon / k_adj_pw: the gradient
+  // part of dry and full-wet cells, all of it for cells in pass 2's list.
+  const int cls = W.cls[i];
+  const bool listed = cls == 1 && !(w > dmax(dmax(p[0].z, p[1].z), p[2].z));
+  if (cls == 0) {
+    bcb += mb[0];
+  } else {
+    if (!listed) sw += mb[0];
+    su += mb[3];
+    sv += mb[4];
+  }
+  if (listed) J.m_cell[i] = mb[0];
+  if (cls != 1 || listed) {
+    J.m_cell[T + i] = mb[1];
+    J.m_cell[2 * T + i] = mb[2];
+  }
   const double cthird = 1. / 3.;
   for (int k = 0; k < 3; ++k) J.zc[k * T + i] += zc[k] + cthird * bcb;  // over every stage
   J.sbar[i] = sw;
@@ -389,7 +408,8 @@ __device__ inline bool drain_bar(const DevMesh& M, const DevState& S, const DevW
 // The flux adjoint of cell c's side of edge e (local index k, sign sg, the
 // cell across o or < 0): -dtk sg cek rb, as TimeDisc::RHS weighs the edge's
 // flux in the cell's update, plus the draining term on the mass flux when
-// the cell's draining dt was active.
+// the cell's draining dt was active; the side of the cell's local edge 0
+// also stores the draining depth term for k_adj_recon.
 template <bool LEN>
 __device__ inline void side_flux_bar(const DevMesh& M, const DevState& S, const DevWork& W, const DevAdj& J,
                                      int c, int o, int k, double sg, double f0, double le, double dts,
@@ -405,7 +425,9 @@ __device__ inline void side_flux_bar(const DevMesh& M, const DevState& S, const 
   fb[2] = -cf * J.rb[2 * T + c];
   if (dtc < dts) {
     double sb, hb, out[3], len[3];
-    if (drain_bar<LEN>(M, S, W, J, c, sb, hb, out, len) && out[k] > 0.) fb[0] += sb * len[k] * sg;
+    const bool act = drain_bar<LEN>(M, S, W, J, c, sb, hb, out, len);
+    if (k == 0) J.drain_h[c] = act ? hb : 0.;  // once per cell, from its local edge 0
+    if (act && out[k] > 0.) fb[0] += sb * len[k] * sg;
   }
 }
 
@@ -462,11 +484,12 @@ __global__ void __launch_bounds__(256) k_adj_flux(DevMesh M, DevState S, DevWork
       }
     }
   }
+  // the edge z term g[6] = -(g[0] + g[3]) is formed by k_adj_recon from the
+  // two w slots (free flow: g[3] is merged into g[0], the sum unchanged)
   for (int q = 0; q < 3; ++q) {
     J.sl_edge[(q * 2 + 0) * Ecnt + e] = g[q];
     J.sl_edge[(q * 2 + 1) * Ecnt + e] = g[3 + q];
   }
-  J.ez_edge[e] = g[6];
 }
 
 // ------------------------------------------------------ partially wet --
@@ -482,8 +505,7 @@ __global__ void __launch_bounds__(256) k_adj_pw(DevMesh M, DevState S, DevWork W
   for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < npw; q += gridDim.x * blockDim.x) {
     const int i = W.pw_list[q];
     const uint32_t fl = M.cflag[i];
-    double mb[5];
-    for (int c = 0; c < 5; ++c) mb[c] = J.m_cell[c * T + i];
+    double mb[5] = {J.m_cell[i], J.m_cell[T + i], J.m_cell[2 * T + i], 0., 0.};  // u0, v0: unused here
     const Muscl mu = load_muscl(S, W, i);
     const V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[T + i]), p2 = node(M, M.tp[2 * T + i]);
     for (int k = 0; k < 3; ++k) {
@@ -546,29 +568,32 @@ __global__ void __launch_bounds__(256) k_adj_pwnode(DevMesh M, DevState S, DevWo
 // Reverse of k_recon1 / k_recon2 for cell i: gathers the MUSCL adjoint of
 // the cell (its own update and the slots of its three edges), then maps it
 // to the w adjoints of the cell and its neighbours and to corner z.
-template <bool LEN>
 __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWork W, DevAdj J, double dts) {
   const int64_t i64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i64 >= M.T) return;
   const int i = (int)i64;
   const int64_t T = M.T, Ecnt = M.E;
   const uint32_t fl = M.cflag[i];
-  double mb[5];
-  for (int q = 0; q < 5; ++q) mb[q] = J.m_cell[q * T + i];
+  const int cls = W.cls[i];
+  // the update's gradient adjoint (its level and velocity parts it applied
+  // itself); partially wet cells go through k_adj_pw's result
+  double mb[5] = {0., 0., 0., 0., 0.};
+  if (cls != 1) {
+    mb[1] = J.m_cell[T + i];
+    mb[2] = J.m_cell[2 * T + i];
+  }
   double zc[3] = {0., 0., 0.};
   double wb = 0., bcb = 0.;
-  if (W.dtd[i] < dts) {  // the depth term of the draining dt
-    double sb, hb, out[3], len[3];
-    if (drain_bar<LEN>(M, S, W, J, i, sb, hb, out, len)) {
-      wb += hb;
-      bcb -= hb;
-    }
+  if (W.dtd[i] < dts) {  // the depth term of the draining dt (k_adj_flux)
+    const double hb = J.drain_h[i];
+    wb += hb;
+    bcb -= hb;
   }
   const V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[T + i]), p2 = node(M, M.tp[2 * T + i]);
+  const double cxi = M.cx[i], cyi = M.cy[i];
   {
     // the slot values of the cell's three edges (AtPoint at the midpoints)
     const Muscl mu = load_muscl(S, W, i);
-    const double cxi = M.cx[i], cyi = M.cy[i];
     for (int k = 0; k < 3; ++k) {
       const int e = M.te[k * T + i];
       const bool own = sgn_pos(fl, k);
@@ -579,14 +604,14 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
         bcb -= hf;
         continue;
       }
-      double ez = own ? J.ez_edge[e] : 0.;  // the flux's own z term, once
+      // the flux's own z term, once: -(w adjoints of both sides)
+      double ez = own ? -(J.sl_edge[e] + J.sl_edge[Ecnt + e]) : 0.;
       slot_bar(mu, cxi, cyi, edge_mid(p0, p1, p2, fl, k), J.sl_edge[(0 * 2 + side) * Ecnt + e],
                J.sl_edge[(1 * 2 + side) * Ecnt + e], J.sl_edge[(2 * 2 + side) * Ecnt + e], mb, ez);
       zc[ep0_local(fl, k)] += 0.5 * ez;
       zc[ep1_local(fl, k)] += 0.5 * ez;
     }
   }
-  const int cls = W.cls[i];
   const double w = S.w[i];
   double wnb[3] = {0., 0., 0.};
   // the node maxima this cell attains (k_adj_pwnode): adjoint of its pass-1
@@ -599,8 +624,8 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
       if (pk != 0.) {
         J.p1bar[k * T + i] = 0.;
         mp[0] += pk;
-        mp[1] += pk * (pv[k].x - M.cx[i]);
-        mp[2] += pk * (pv[k].y - M.cy[i]);
+        mp[1] += pk * (pv[k].x - cxi);
+        mp[2] += pk * (pv[k].y - cyi);
       }
     }
   }
@@ -641,7 +666,6 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
       // neighbour is dry or the positivity / TVD limiter zeroed it.  The
       // decisions are recomputed as k_recon1 takes them.
       wb += mb[0];
-      const CellRec ri = M.crec[i];
       V3 pts[3], E[3];
       int kind[3];
       bool any_dry = false;
@@ -693,7 +717,7 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
           const int j = M.tt[k * T + i];
           const double wj = S.w[j];
           double vtmin = dmin(w, wj), vtmax = dmax(w, wj);
-          double vek = w + (gx * (E[k].x - ri.cx) + gy * (E[k].y - ri.cy));
+          double vek = w + (gx * (E[k].x - cxi) + gy * (E[k].y - cyi));
           if (!((vtmin <= vek) && (vek <= vtmax))) tvd = false;
         }
         active = all_wet && tvd;
@@ -822,8 +846,7 @@ hipError_t launch_adjoint_stage(const DevMesh& M, const DevState& S, const DevSt
     hipLaunchKernelGGL(k_adj_pw, g, dim3(256), 0, st, M, S, W, J);
     hipLaunchKernelGGL(k_adj_pwnode, g, dim3(256), 0, st, M, S, W, J);
   }
-  if (opts & kOptDrainLength) hipLaunchKernelGGL(k_adj_recon<true>, grid_plain(M.T), dim3(256), 0, st, M, S, W, J, dts);
-  else hipLaunchKernelGGL(k_adj_recon<false>, grid_plain(M.T), dim3(256), 0, st, M, S, W, J, dts);
+  hipLaunchKernelGGL(k_adj_recon, grid_plain(M.T), dim3(256), 0, st, M, S, W, J, dts);
   return hipGetLastError();
 }
 

@@ -1060,10 +1060,10 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
   if (e == hipSuccess) e = alloc(&bar_a, S3);
   if (e == hipSuccess) e = alloc(&bar_b, S3);
   if (e == hipSuccess) e = alloc(&zb, V * 8);
-  // DevAdj work: rb 3T, dtd_own T, dtd_nb 3T, m_cell 5T, sl_edge 6E, ez E,
+  // DevAdj work: rb 3T, dtd_own T, dtd_nb 3T, m_cell 3T, sl_edge 6E, drain_h T,
   // w_nb 3T, zc 3T, u0bar 3T, pw_own 4T, pw1_jac 4T, pw_w23bar T, p1bar 3T, zn_acc V,
   // zopp_acc 2E; back 3T and eloc 2E bytes
-  const int64_t nwork = 3 * T + T + 3 * T + 5 * T + 6 * E + E + 3 * T + 3 * T + 3 * T + 4 * T + 4 * T + T + 3 * T + V + 2 * E;
+  const int64_t nwork = 3 * T + T + 3 * T + 3 * T + 6 * E + T + 3 * T + 3 * T + 3 * T + 4 * T + 4 * T + T + 3 * T + V + 2 * E;
   if (e == hipSuccess) e = alloc(&work, nwork * 8 + ((3 * T + 2 * E + 255) / 256) * 256);
   if (e == hipSuccess) e = alloc(&flags, int64_t(sizeof(unsigned int)));
   if (e != hipSuccess) { release(); return dev_fail(e, "hipMalloc (adjoint)"); }
@@ -1091,9 +1091,9 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
     J.rb = q; q += 3 * T;
     J.dtd_own = q; q += T;
     J.dtd_nb = q; q += 3 * T;
-    J.m_cell = q; q += 5 * T;
+    J.m_cell = q; q += 3 * T;
     J.sl_edge = q; q += 6 * E;
-    J.ez_edge = q; q += E;
+    J.drain_h = q; q += T;
     J.w_nb = q; q += 3 * T;
     J.zc = q; q += 3 * T;
     J.u0bar = q; q += 3 * T;

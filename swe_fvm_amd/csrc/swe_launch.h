@@ -129,10 +129,10 @@ struct DevAdj {
   double* rb;          // [3][T] adjoint of the cell's conserved update r
   double* dtd_own;     // [T] draining-dt adjoint from the cell's own update
   double* dtd_nb;      // [3][T] ... of neighbour k, from this cell's update
-  double* m_cell;      // [5][T] MUSCL (w0, gx, gy, u0, v0) adjoint, own update
+  double* m_cell;      // [5][T] MUSCL (w0, gx, gy) adjoint the own update leaves
   double* sl_edge;     // [3][2][E] adjoint of the edge slot (w, u, v) per side;
                        // wall edges: side 0 w holds the et0 cell's depth adjoint
-  double* ez_edge;     // [E] edge-midpoint z adjoint of the flux itself
+  double* drain_h;     // [T] depth adjoint of the draining dt (k_adj_flux)
   double* w_nb;        // [3][T] w adjoint of neighbour k from this cell's plane
   double* zc;          // [3][T] node-z adjoint per cell corner, summed over stages
   int8_t* back;        // [3][T] local index of this cell in neighbour k
