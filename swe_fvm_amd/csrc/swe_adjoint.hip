@@ -824,7 +824,7 @@ static inline dim3 grid_plain(int64_t n) {
 
 hipError_t launch_adjoint_stage(const DevMesh& M, const DevState& S, const DevState& U0, const DevWork& W,
                                 const StageArgs& A, const DevAdj& J, int flux, int ws, int opts,
-                                const double* wnb_in, int u0mode, hipStream_t st) {
+                                const double* wnb_in, int u0mode, unsigned int npw, hipStream_t st) {
   const double dts = A.coef * A.dt_fixed;
   hipLaunchKernelGGL(k_adj_update, grid_plain(M.T), dim3(256), 0, st, M, S, U0, W, A, J, wnb_in, u0mode);
 #define SWE_ADJ_FLUX(F, Wv, L)                                                                   \
@@ -839,14 +839,29 @@ hipError_t launch_adjoint_stage(const DevMesh& M, const DevState& S, const DevSt
     SWE_ADJ_FLUX(1, 0, true) SWE_ADJ_FLUX(1, 1, true) SWE_ADJ_FLUX(1, 2, true)
   }
 #undef SWE_ADJ_FLUX
-  {
+  if (npw != 0u) {
     // partially wet cells: grid-stride over pass 2's list
-    dim3 g = grid_plain(M.T);
+    dim3 g = grid_plain(npw == kPwUnknown ? M.T : (int64_t)npw);
     if (g.x > 1024) g.x = 1024;
     hipLaunchKernelGGL(k_adj_pw, g, dim3(256), 0, st, M, S, W, J);
     hipLaunchKernelGGL(k_adj_pwnode, g, dim3(256), 0, st, M, S, W, J);
   }
   hipLaunchKernelGGL(k_adj_recon, grid_plain(M.T), dim3(256), 0, st, M, S, W, J, dts);
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) k_slot_counts(unsigned int* first, int64_t stride, int n,
+                                                     unsigned int* out) {
+  const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (x >= n) return;
+  unsigned int* c = (unsigned int*)((char*)first + (int64_t)x * stride);
+  if (out) out[x] = *c;
+  else *c = 0u;
+}
+
+hipError_t launch_slot_counts(unsigned int* first, int64_t stride, int n, unsigned int* out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_slot_counts, grid_plain(n), dim3(256), 0, st, first, stride, n, out);
   return hipGetLastError();
 }
 

@@ -1052,7 +1052,8 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
   double *ck = nullptr, *seg = nullptr, *stg = nullptr, *bar_a = nullptr, *bar_b = nullptr;
   double *zb = nullptr, *work = nullptr;
   char* slots = nullptr;
-  unsigned int* flags = nullptr;
+  unsigned int* cnt = nullptr;
+  std::vector<unsigned int> hcnt(static_cast<size_t>(K * ns), kPwUnknown);
   hipError_t e = alloc(&ck, nck * S3);
   if (e == hipSuccess) e = alloc(&seg, K * S3);
   if (e == hipSuccess) e = alloc(&stg, K * std::max(ns - 1, 1) * S3);
@@ -1065,7 +1066,7 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
   // zopp_acc 2E; back 3T and eloc 2E bytes
   const int64_t nwork = 3 * T + T + 3 * T + 3 * T + 6 * E + T + 3 * T + 3 * T + 3 * T + 4 * T + 4 * T + T + 3 * T + V + 2 * E;
   if (e == hipSuccess) e = alloc(&work, nwork * 8 + ((3 * T + 2 * E + 255) / 256) * 256);
-  if (e == hipSuccess) e = alloc(&flags, int64_t(sizeof(unsigned int)));
+  if (e == hipSuccess) e = alloc(&cnt, K * ns * int64_t(sizeof(unsigned int)));
   if (e != hipSuccess) { release(); return dev_fail(e, "hipMalloc (adjoint)"); }
   auto slot_work = [&](int64_t idx) {
     char* b = slots + idx * slot_bytes;
@@ -1105,7 +1106,6 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
     J.zopp_acc = q; q += 2 * E;
     J.back = (int8_t*)q;
     J.eloc = J.back + 3 * T;
-    J.flags = flags;
   }
   auto dtn = [&](int64_t n) { return dts ? dts[n] : dt; };
   SweepTimer tm;
@@ -1132,7 +1132,6 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
         (e = hipMemsetAsync(J.zn_acc, 0, sizeof(double) * size_t(std::max<int64_t>(V, 1)), s->stream)) != hipSuccess ||
         (e = hipMemsetAsync(J.zopp_acc, 0, sizeof(double) * size_t(std::max<int64_t>(2 * E, 1)), s->stream)) != hipSuccess ||
         (e = hipMemsetAsync(zb, 0, sizeof(double) * size_t(std::max<int64_t>(V, 1)), s->stream)) != hipSuccess ||
-        (e = hipMemsetAsync(flags, 0, sizeof(unsigned int), s->stream)) != hipSuccess ||
         (e = launch_adjoint_tables(s->M, J, s->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(s->stream)) != hipSuccess)
       rc = dev_fail(e, "adjoint setup");
@@ -1145,6 +1144,9 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
     // recompute the segment keeping step inputs, stage outputs and every
     // stage's intermediates; the last segment also produces U_N
     e = copy_state(st3(Buf3{ck + g * 3 * T}, T), out3(Buf3{seg}, T), T, s->stream);
+    // the slots' pass-2 counters, zeroed in one launch
+    const int nslot = int((n1 - n0) * ns);
+    if (e == hipSuccess) e = launch_slot_counts(slot_work(0).pw_count, slot_bytes, nslot, nullptr, s->stream);
     for (int64_t n = n0; n < n1 && e == hipSuccess; ++n) {
       const int64_t m = n - n0;
       const DevState Un = st3(Buf3{seg + m * 3 * T}, T);
@@ -1155,16 +1157,22 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
         else if (n + 1 < n1) O = out3(Buf3{seg + (m + 1) * 3 * T}, T);
         else O = g == nck - 1 ? out_of(s, (s->cur + 1) % 3) : out3(Buf3{sb}, T);  // U_N, or unused
         const DevWork Wq = slot_work(m * ns + q);
-        e = launch_recon1(s->M, Sq, Wq, s->opts, s->stream);
+        e = launch_recon1(s->M, Sq, Wq, s->opts, s->stream, false);
         if (e == hipSuccess) e = launch_recon2(s->M, Sq, Wq, s->opts, s->stream);
-        if (e == hipSuccess)
-          e = launch_flux(s->M, Sq, Wq, s->flux, s->ws, s->cor, s->rbits + 3, s->flux_items, s->stream);
+        if (e == hipSuccess)  // fixed dt: the CFL reduction is not read
+          e = launch_flux(s->M, Sq, Wq, s->flux, s->ws, s->cor, s->rbits + 3, s->flux_items, s->stream, false);
         if (e == hipSuccess) e = launch_drain(s->M, Sq, Wq, s->opts, s->stream);
         StageArgs A{sc[q].a0, sc[q].a1, sc[q].coef, dtn(n), 0., s->cor, s->rbits, nullptr, friction(s)};
         if (e == hipSuccess) e = launch_update(s->M, Sq, Un, O, Wq, A, s->upd_items, s->stream);
         Sq = DevState{O.w, O.u, O.v};
       }
     }
+    // the lengths of the segment's pass-2 lists, for the partially wet
+    // kernels of the reverse stages (one synchronisation per segment)
+    if (e == hipSuccess) e = launch_slot_counts(slot_work(0).pw_count, slot_bytes, nslot, cnt, s->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(hcnt.data(), cnt, sizeof(unsigned int) * size_t(nslot), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) { rc = dev_fail(e, "adjoint recomputation"); break; }
     if (g == nck - 1) s->cur = (s->cur + 1) % 3;  // U_N is the current state
     // reverse the steps of the segment: ob holds the adjoint of the stage
@@ -1179,7 +1187,7 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
         StageArgs A{sc[q].a0, sc[q].a1, sc[q].coef, dtn(n), 0., s->cor, s->rbits, nullptr, friction(s)};
         const int u0mode = (q < ns - 1 ? kU0Accumulate : 0) | (q == 0 && ns > 1 ? kU0Into : 0);
         e = launch_adjoint_stage(s->M, Sq, Un, slot_work(m * ns + q), A, J, s->flux, s->ws, s->opts, wpend,
-                                 u0mode, s->stream);
+                                 u0mode, hcnt[size_t(m * ns + q)], s->stream);
         wpend = J.w_nb;
         std::swap(ob, sb);
       }
@@ -1190,11 +1198,9 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
   if (!rc && (e = launch_adjoint_finish(s->M, J, z_bar ? zb : nullptr, s->stream)) != hipSuccess)
     rc = dev_fail(e, "adjoint finish");
   tm.mark(2, s->stream);
-  unsigned int nflag = 0;
   if (!rc) {
     std::vector<double> h(static_cast<size_t>(3 * T));
     if ((e = hipMemcpyAsync(h.data(), ob, size_t(S3), hipMemcpyDeviceToHost, s->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(&nflag, flags, sizeof(unsigned int), hipMemcpyDeviceToHost, s->stream)) != hipSuccess ||
         (z_bar && (e = hipMemcpyAsync(z_bar, zb, sizeof(double) * size_t(V), hipMemcpyDeviceToHost, s->stream)) != hipSuccess) ||
         (e = hipStreamSynchronize(s->stream)) != hipSuccess) {
       rc = dev_fail(e, "adjoint download");
@@ -1208,10 +1214,6 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
     }
   }
   release();
-  if (!rc && nflag)
-    return fail(SWE_ERR_SOLVER, "adjoint: " + std::to_string(nflag) +
-                                    " cell stages outside the supported regime (partially wet cells below "
-                                    "their highest vertex)");
   return rc;
 }
 
@@ -1237,15 +1239,10 @@ int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int
   if (int rc = set_device(s)) return rc;
   const int64_t T = s->T, E = s->E, V = s->V;
   // tangent states (three, rotating like the values), node z tangent,
-  // intermediates, flag counter
+  // intermediates
   const int64_t n8 = 9 * T + V + 5 * T + 3 * E;
   double* buf = nullptr;
-  unsigned int* flags = nullptr;
   SWE_CHECK(hipMalloc((void**)&buf, sizeof(double) * size_t(n8)), "hipMalloc (tangent)");
-  if (hipMalloc((void**)&flags, sizeof(unsigned int)) != hipSuccess) {
-    (void)hipFree(buf);
-    return fail(SWE_ERR_DEVICE, "hipMalloc (tangent)");
-  }
   double* q = buf;
   double* td[3];
   for (int k = 0; k < 3; ++k) { td[k] = q; q += 3 * T; }
@@ -1260,7 +1257,6 @@ int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int
   Q.f0 = q; q += E;
   Q.f1 = q; q += E;
   Q.f2 = q; q += E;
-  Q.flags = flags;
   auto tst = [&](int k) { return DevState{td[k], td[k] + T, td[k] + 2 * T}; };
   auto tout = [&](int k) { return DevOut{td[k], td[k] + T, td[k] + 2 * T}; };
   std::vector<double> h(static_cast<size_t>(3 * T));
@@ -1272,7 +1268,6 @@ int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int
   hipError_t e = hipMemcpyAsync(td[s->cur], h.data(), sizeof(double) * size_t(3 * T), hipMemcpyHostToDevice,
                                 s->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(zd, hz.data(), sizeof(double) * size_t(V), hipMemcpyHostToDevice, s->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(flags, 0, sizeof(unsigned int), s->stream);
   StageCoef sc[3];
   const int ns = stage_list(scheme, sc);
   SweepTimer tm;
@@ -1291,9 +1286,7 @@ int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int
     s->cur = ns == 1 ? b : a;
   }
   tm.mark(1, s->stream);
-  unsigned int nflag = 0;
   if (e == hipSuccess) e = hipMemcpyAsync(h.data(), td[s->cur], sizeof(double) * size_t(3 * T), hipMemcpyDeviceToHost, s->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(&nflag, flags, sizeof(unsigned int), hipMemcpyDeviceToHost, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   if (e != hipSuccess) {
     rc = dev_fail(e, "tangent run");
@@ -1304,11 +1297,6 @@ int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int
   }
   (void)hipStreamSynchronize(s->stream);  // cleanup: errors were reported above
   (void)hipFree(buf);
-  (void)hipFree(flags);
-  if (!rc && nflag)
-    return fail(SWE_ERR_SOLVER, "tangent: " + std::to_string(nflag) +
-                                    " cell stages outside the supported regime (partially wet cells below "
-                                    "their highest vertex)");
   return rc;
 }
 

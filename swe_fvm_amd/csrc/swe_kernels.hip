@@ -880,8 +880,8 @@ hipError_t launch_setup_geometry(const DevMesh& M, const DevGeomOut& G, hipStrea
   return hipGetLastError();
 }
 hipError_t launch_recon1(const DevMesh& M, const DevState& S, const DevWork& W, int opts,
-                         hipStream_t st) {
-  hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, st, W.pw_count);
+                         hipStream_t st, bool zero_count) {
+  if (zero_count) hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, st, W.pw_count);
   if (opts & 1) hipLaunchKernelGGL((k_recon1<true>), grid_for(M.c1 - M.c0), dim3(256), 0, st, M, S, W);
   else hipLaunchKernelGGL((k_recon1<false>), grid_for(M.c1 - M.c0), dim3(256), 0, st, M, S, W);
   return hipGetLastError();
@@ -895,8 +895,8 @@ hipError_t launch_recon2(const DevMesh& M, const DevState& S, const DevWork& W, 
   return hipGetLastError();
 }
 hipError_t launch_flux(const DevMesh& M, const DevState& S, const DevWork& W, int flux, int ws,
-                       double cor, unsigned long long* rbits, int items, hipStream_t st) {
-  if (!M.cinst) hipLaunchKernelGGL(k_reset_r, dim3(1), dim3(1), 0, st, rbits);
+                       double cor, unsigned long long* rbits, int items, hipStream_t st, bool reset) {
+  if (reset && !M.cinst) hipLaunchKernelGGL(k_reset_r, dim3(1), dim3(1), 0, st, rbits);
   if (M.e1 <= M.e0) return hipGetLastError();
   dim3 g = grid_for((M.e1 - M.e0 + items - 1) / items), b(256);
 #define SWE_FLUX_CASE(F, Wv, N)                                                        \

@@ -149,7 +149,6 @@ struct DevAdj {
   double* p1bar;       // [3][T] routed pass-1 level adjoint per corner
   double* zn_acc;      // [V]
   double* zopp_acc;    // [2][E]
-  unsigned int* flags; // states outside the supported regime
 };
 
 // tangents of the tangent-linear step (swe_tangent.hip): the node
@@ -164,7 +163,6 @@ struct DevTan {
   double* f1;
   double* f2;
   double* dtd;         // [T] draining-dt tangent
-  unsigned int* flags; // states outside the supported regime
 };
 
 struct StageArgs {
@@ -179,14 +177,17 @@ struct StageArgs {
 };
 
 hipError_t launch_setup_geometry(const DevMesh& M, const DevGeomOut& G, hipStream_t st);
+// zero_count false: the caller has zeroed W.pw_count (launch_slot_counts)
 hipError_t launch_recon1(const DevMesh& M, const DevState& S, const DevWork& W, int opts,
-                         hipStream_t st);
+                         hipStream_t st, bool zero_count = true);
 hipError_t launch_recon2(const DevMesh& M, const DevState& S, const DevWork& W, int opts,
                          hipStream_t st);
 // items: edges per thread of the flux (1, 2 or 4), cells per thread of the
 // update (1 or 2); results are identical for every choice
+// reset false: the reduction slot is not reset first (a fixed-dt pass that
+// never reads it)
 hipError_t launch_flux(const DevMesh& M, const DevState& S, const DevWork& W, int flux, int ws,
-                       double cor, unsigned long long* rbits, int items, hipStream_t st);
+                       double cor, unsigned long long* rbits, int items, hipStream_t st, bool reset = true);
 // opts & 32 (SWE_OPT_DRAIN_LENGTH): outflow F0 * L in the draining dt
 hipError_t launch_drain(const DevMesh& M, const DevState& S, const DevWork& W, int opts,
                         hipStream_t st);
@@ -215,10 +216,17 @@ hipError_t launch_stats(const DevMesh& M, const DevState& S, double* partial, do
 // (else it is set, for stages that combine U0), kU0Into adds J.u0bar into
 // J.sbar (the first stage, whose input is U0).  The forward intermediates of
 // the stage state S must be in W (recon1, recon2, flux, drain of S).
+// npw: the length of W's pass-2 list when the caller knows it (the
+// partially wet kernels are then skipped for an empty list and sized to it
+// otherwise), kPwUnknown when not.
 constexpr int kU0Accumulate = 1, kU0Into = 2;
+constexpr unsigned int kPwUnknown = 0xffffffffu;
 hipError_t launch_adjoint_stage(const DevMesh& M, const DevState& S, const DevState& U0, const DevWork& W,
                                 const StageArgs& A, const DevAdj& J, int flux, int ws, int opts,
-                                const double* wnb_in, int u0mode, hipStream_t st);
+                                const double* wnb_in, int u0mode, unsigned int npw, hipStream_t st);
+// the pass-2 list lengths of n stage slots whose counters lie stride bytes
+// apart from first: zeroed (out null) or copied to out[n]
+hipError_t launch_slot_counts(unsigned int* first, int64_t stride, int n, unsigned int* out, hipStream_t st);
 // one stage of the tangent-linear step: values (S, U0 -> O, intermediates
 // in W) exactly as the forward stage, tangents (Sd, U0d, Q.zd -> Od,
 // intermediates in Q); fixed dt (A.dt_fixed)
