@@ -242,9 +242,11 @@ int swe_state_stats(swe_solver* s, double* out5);
  * and stage plus 24 per cell and stage state), then reversed; the last
  * segment's recomputation is the forward run.  Branch decisions (wet/dry,
  * limiter, wave-speed selections) are the forward ones; the Riemann
- * solver's and the partially wet reconstructions' local Jacobians are
- * exact (dual numbers; implicit derivatives of the cube-root and bisection
- * levels), the node maxima's adjoints go to the cells attaining them.  The
+ * solver is differentiated in reverse mode (the transpose of its
+ * dual-number derivative) and the partially wet reconstructions by local
+ * Jacobians on dual numbers (implicit derivatives of the cube-root and
+ * bisection levels); the node maxima's adjoints go to the cells attaining
+ * them.  The
  * HEAD-compatibility options give SWE_ERR_ARG.  ms (optional, 2 doubles):
  * forward sweep and reverse sweep (recomputation included) milliseconds.
  * Single-mesh handles only. */

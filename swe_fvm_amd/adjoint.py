@@ -71,7 +71,9 @@ if torch is not None:
     class ShallowWaterRun(torch.autograd.Function):
         """final_prim = ShallowWaterRun.apply(prim0, z, sd, nsteps, scheme, dt,
         checkpoint_every): prim0 (T x 3) and z (V) float64 CPU tensors; the
-        backward gives their gradients through the adjoint of every stage."""
+        backward gives their gradients through the adjoint of every stage.
+        sd is a swe_fvm_amd.SpaceDisc (ctypes) or a Solver of the pybind
+        module SWE_FVM; both call the same C ABI."""
 
         @staticmethod
         def forward(ctx, prim0, z, sd, nsteps, scheme="SSPRK2", dt=1e-3, checkpoint_every=50):

@@ -162,3 +162,14 @@ def wet_dry_mesh(n, seed, dry_frac=0.08, jitter=0.25):
     v = rng.normal(0, 0.1, T) * (h > 0)
     prim = np.stack([bc + h, u, v], axis=1)
     return xyz, topo, prim
+
+
+def swe_fvm_module():
+    """The pybind module SWE_FVM under the name the reference's scripts
+    import it by (one name per process: pybind registers its types once)."""
+    import importlib
+    import sys
+    d = os.path.join(os.path.dirname(HERE), "swe_fvm_amd")
+    if d not in sys.path:
+        sys.path.insert(0, d)
+    return importlib.import_module("SWE_FVM")

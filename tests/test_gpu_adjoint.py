@@ -474,3 +474,40 @@ def test_autograd_function_matches():
     assert abs(loss.item() - J) <= 1e-12 * abs(J)
     assert np.allclose(p.grad.numpy(), s0, rtol=1e-12, atol=1e-15)
     assert np.allclose(zt.grad.numpy(), zb, rtol=1e-12, atol=1e-15)
+
+
+def test_autograd_function_through_the_pybind_solver():
+    """The autograd Function on the pybind module's Solver: loss and
+    gradients equal those through swe_fvm_amd.SpaceDisc bit for bit (both
+    call the same C ABI), and the Solver's tangent satisfies the duality
+    with its adjoint."""
+    torch = pytest.importorskip("torch")
+    from tests import _common as C
+    SWE_FVM = C.swe_fvm_module()
+    dom, prim0 = wet_case(n=12, seed=9)
+    xyz_p, topo_p = SWE_FVM.jittered_mesh(12, 0.25, 9)
+    assert np.array_equal(xyz_p[:, :2], dom.geometry[:, :2])
+    ps = SWE_FVM.Solver(topo_p, dom.geometry, "HLLC", "Einfeldt")
+    sd = swe.SpaceDisc(swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt], dom, prim0)
+    z = dom.geometry[:, 2].copy()
+    res = []
+    for solver in (sd, ps):
+        p = torch.tensor(prim0, dtype=torch.float64, requires_grad=True)
+        zt = torch.tensor(z, dtype=torch.float64, requires_grad=True)
+        out = AD.ShallowWaterRun.apply(p, zt, solver, 10, "SSPRK2", 2e-3, 4)
+        loss = (out[:, 0] ** 2).sum()
+        loss.backward()
+        res.append((loss.item(), p.grad.numpy().copy(), zt.grad.numpy().copy()))
+    assert res[0][0] == res[1][0]
+    assert np.array_equal(res[0][1], res[1][1]) and np.array_equal(res[0][2], res[1][2])
+    rng = np.random.default_rng(4)
+    dp, dz = rng.standard_normal(prim0.shape), rng.standard_normal(z.shape)
+    ps.set_bathymetry(z)
+    ps.SetVolField(prim0)
+    fin_dot, _ = ps.tangent_run(dp, 10, z_dot=dz, scheme="SSPRK2", dt=2e-3)
+    fin = ps.GetVolField()
+    ybar = np.zeros_like(fin)
+    ybar[:, 0] = 2 * fin[:, 0]
+    lhs = float(np.sum(ybar * fin_dot))
+    rhs = float(np.sum(res[1][1] * dp) + np.sum(res[1][2] * dz))
+    assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + abs(rhs))

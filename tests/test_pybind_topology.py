@@ -64,3 +64,22 @@ def test_mesh_sources_sizes():
     xyz, topo = SWE_FVM.jittered_mesh(20, 0.25, 7)
     assert (topo.num_elements(), topo.num_nodes()) == (800, 441)
     assert topo.num_edges() == 441 + 800 - 1
+
+
+def test_solver_binding_checks_its_arguments_before_the_device():
+    """SWE_FVM.Solver (the device solver for the autograd Function) exports
+    the Function's methods and rejects bad arguments with ValueError before
+    any device call; without a device it fails with RuntimeError."""
+    for name in ("SetVolField", "GetVolField", "set_bathymetry", "run", "adjoint_run", "tangent_run"):
+        assert hasattr(SWE_FVM.Solver, name)
+    xyz, topo = SWE_FVM.struct_mesh(4, 4, 0.25)
+    with pytest.raises(ValueError, match="flux"):
+        SWE_FVM.Solver(topo, xyz, "HLLX", "Einfeldt")
+    with pytest.raises(ValueError, match="wavespeed"):
+        SWE_FVM.Solver(topo, xyz, "HLL", "Roe")
+    with pytest.raises(ValueError, match="xyz"):
+        SWE_FVM.Solver(topo, xyz[:3], "HLLC", "Einfeldt")
+    import torch
+    if not torch.cuda.is_available():
+        with pytest.raises(RuntimeError):
+            SWE_FVM.Solver(topo, xyz, "HLLC", "Einfeldt")
