@@ -120,7 +120,7 @@ class OracleSolver:
             raise ValueError("unsupported boundary tag (SOLID_WALL = -1 and FREE_FLOW = -2 only)")
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and lib is not None:  # lib is None at interpreter exit
             lib().orc_destroy(self.h)
             self.h = None
 

@@ -1,13 +1,12 @@
 """The adjoint of the step (swe_adjoint_run) against central finite
-differences of the GPU forward (itself bit-identical to the oracle), in
-random directions of the initial state and of the node bathymetry; the
-checkpoint interval does not change the result; unsupported states are
-refused; the torch.autograd.Function gives the same gradients.  (The full
-Jacobians of Euler, SSP-RK2 and SSP-RK3 steps on a 72-cell mesh, state and
-bathymetry columns, were also compared entry by entry with central
-differences of the oracle at 1e-9: agreement to 4e-7, the differences' own
-error.)  Finite differences must not cross the limiter's all-or-nothing
-decisions: the perturbations are ~1e-8 against level differences ~1e-3."""
+differences of the oracle (the CPU restatement of the reference solver) on
+a 72-cell mesh and of the GPU forward (itself bit-identical to the oracle)
+on larger ones, in random directions of the initial state and of the node
+bathymetry; against the tangent-linear build by duality; the checkpoint
+interval does not change the result; the HEAD options are refused; the
+torch.autograd.Function gives the same gradients through either binding.
+Finite differences must not cross the limiter's all-or-nothing decisions:
+the perturbations are ~1e-8 against level differences ~1e-3."""
 import numpy as np
 import pytest
 
@@ -82,6 +81,52 @@ def test_adjoint_matches_finite_differences(scheme, nsteps, flux):
         print(f"{scheme} {flux} x{nsteps} trial {trial}: adjoint {ad:.12e} fd {fd:.12e} "
               f"rel {abs(ad - fd) / abs(fd):.2e}", flush=True)
         assert abs(ad - fd) <= 1e-5 * abs(fd) + 1e-12
+        checked += 1
+    assert checked >= 2
+
+
+@pytest.mark.parametrize("scheme,nsteps,flux", [("Euler", 1, "HLLC"), ("SSPRK2", 3, "HLL"),
+                                                ("SSPRK3", 2, "HLLC")])
+def test_adjoint_matches_differences_of_the_oracle(scheme, nsteps, flux):
+    """The GPU adjoint against directional central differences of the oracle
+    (the CPU restatement of the reference solver), in state and bathymetry
+    directions, on a 72-cell mesh."""
+    from oracle import oracle as O
+    dom, prim0 = wet_case(n=6, seed=11)
+    assert prim0.shape[0] == 72
+    fl = getattr(swe.Fluxes, flux)[swe.Wavespeeds.Einfeldt]
+    sd = swe.SpaceDisc(fl, dom, prim0, cor=0.3)
+    xyz0 = dom.geometry.copy()
+    z = xyz0[:, 2].copy()
+    tables = dom.topology.tables()
+    rng = np.random.default_rng(17)
+    cvec = rng.standard_normal(prim0.shape)
+    dt = 2e-3
+    J, s0, zb, _ = AD.run_and_adjoint(sd, prim0, z, nsteps, lambda p: (float(np.sum(cvec * p)), cvec),
+                                      scheme=scheme, dt=dt, checkpoint_every=2)
+
+    def J_oracle(p, zz):
+        xyz = xyz0.copy()
+        xyz[:, 2] = zz
+        o = O.OracleSolver(xyz, tables, flux, "Einfeldt", cor=0.3)
+        o.set_state(p)
+        o.run(scheme, nsteps, dt=dt)
+        return float(np.sum(cvec * o.state()))
+
+    assert abs(J_oracle(prim0, z) - J) <= 1e-13 * abs(J)  # the forward is the oracle's
+    checked = 0
+    for trial in range(4):
+        dp = rng.standard_normal(prim0.shape) * 1e-2
+        dz = rng.standard_normal(z.shape) * 1e-2 if trial > 0 else np.zeros_like(z)
+        cd = [(J_oracle(prim0 + e * dp, z + e * dz) - J_oracle(prim0 - e * dp, z - e * dz)) / (2 * e)
+              for e in (1e-6, 2.5e-7)]
+        if abs(cd[0] - cd[1]) > 1e-5 * abs(cd[0]) + 1e-12:
+            print(f"{scheme} {flux} trial {trial}: the difference crossed a switch", flush=True)
+            continue
+        ad = float(np.sum(s0 * dp) + np.sum(zb * dz))
+        print(f"{scheme} {flux} x{nsteps} trial {trial}: adjoint {ad:.12e} oracle fd {cd[0]:.12e} "
+              f"rel {abs(ad - cd[0]) / abs(cd[0]):.2e}", flush=True)
+        assert abs(ad - cd[0]) <= 1e-5 * abs(cd[0]) + 1e-12
         checked += 1
     assert checked >= 2
 
