@@ -648,10 +648,11 @@ int swe_batch_run(swe_solver* s, int scheme, double cfl, double t_end, int64_t m
   const int64_t chunk = 64;
   while (done < max_steps && act > 0) {
     if (use_graph) {
-      // re-capture when the scheme changed or the window shrank by a fifth
+      // re-capture when the scheme changed or the window shrank (members
+      // finished: at most one capture per check, every 64 steps)
       const int64_t w = s->M.c1 - s->M.c0, wg = s->graph_c1 - s->graph_c0;
       if (s->graph == nullptr || s->graph_scheme != scheme || s->graph_cfl != cfl ||
-          s->graph_tend != t_end || s->M.c0 < s->graph_c0 || s->M.c1 > s->graph_c1 || 5 * w < 4 * wg)
+          s->graph_tend != t_end || s->M.c0 < s->graph_c0 || s->M.c1 > s->graph_c1 || w < wg)
         if (int rc = batch_capture(s, scheme, cfl, t_end, per_graph)) return rc;
     }
     int64_t k = std::min<int64_t>(chunk, max_steps - done);
