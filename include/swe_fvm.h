@@ -126,7 +126,10 @@ int swe_mesh_jittered(int64_t n, double jitter, uint64_t seed, double x0, double
 
 /* SpaceDisc(fluxer, Domain(geometry, topology), v0, cor, tau)
  * (reference include/SpaceDisc.h:24, include/Bathymetry.h:14).  xyz is the
- * node geometry (x, y, bathymetry z), V x 3.  device: HIP ordinal. */
+ * node geometry (x, y, bathymetry z), V x 3.  device: HIP ordinal.  The
+ * tables are validated before any device work (SWE_ERR_ARG for indices out
+ * of range or a mesh without triangles, SWE_ERR_DOMAIN for unsupported
+ * boundary tags). */
 int swe_create(const swe_topology* topo, const double* xyz, int flux, int wavespeed, double cor,
                double tau, int options, int device, swe_solver** out);
 void swe_destroy(swe_solver* s);

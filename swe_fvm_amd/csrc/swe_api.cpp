@@ -133,6 +133,7 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
     return fail(SWE_ERR_ARG, "unknown option bits");
   if (m.T > 0x7fffffff || m.E > 0x7fffffff || m.V > 0x7fffffff)
     return fail(SWE_ERR_ARG, "mesh too large for int32 device indices");
+  if (m.T <= 0) return fail(SWE_ERR_ARG, "mesh without triangles");
   const int64_t T = m.T, E = m.E, V = m.V;
   // host-side validation and SoA packing
   std::vector<int> tp(size_t(3 * T)), te(size_t(3 * T)), tt(size_t(3 * T));
@@ -305,6 +306,7 @@ int swe_batch_create(int64_t n, const swe_topology* const* topos, const double* 
   for (int64_t k = 0; k < n; ++k) {
     if (!topos[k] || !xyzs[k]) return fail(SWE_ERR_ARG, "null ensemble member");
     const Topo& m = topos[k]->m;
+    if (m.T <= 0) return fail(SWE_ERR_ARG, "ensemble member without triangles");
     const int64_t V0 = all.V, E0 = all.E, T0 = all.T;
     for (int64_t q = 0; q < 2 * m.E; ++q) all.ep.push_back(m.ep[size_t(q)] + V0);
     for (int64_t q = 0; q < 2 * m.E; ++q) {

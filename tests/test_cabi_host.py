@@ -173,3 +173,34 @@ def test_unknown_fluxer_is_rejected():
     xyz, topo = swe.jittered_mesh(4, 0.0, 1)
     with pytest.raises(swe.SolverError):
         swe.SpaceDisc(lambda *a: None, swe.Domain(xyz, topo), np.zeros((topo.NumTriangles(), 3)))
+
+
+def test_empty_and_out_of_range_tables_are_rejected_before_the_device():
+    """Degenerate inputs fail with a clear error on the host, with or
+    without a device: a mesh without triangles, node / edge / element
+    indices out of range, ragged tables, a degenerate triangle."""
+    xyz, topo = swe.jittered_mesh(4, 0.0, 1)
+    t = topo.tables()
+    V = topo.NumNodes()
+    fl = swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt]
+    z2, z3 = np.zeros((0, 2), np.int64), np.zeros((0, 3), np.int64)
+    empty = swe.Topology.create(V, z2, z2, z3, z3, z3)
+    assert empty.NumTriangles() == 0
+    with pytest.raises(ValueError, match="without triangles"):
+        swe.SpaceDisc(fl, swe.Domain(xyz, empty), np.zeros((0, 3)))
+    from swe_fvm_amd.ensemble import Ensemble
+    with pytest.raises(ValueError, match="without triangles"):
+        Ensemble(fl, [swe.Domain(xyz, topo), swe.Domain(xyz, empty)],
+                 [np.zeros((topo.NumTriangles(), 3)), np.zeros((0, 3))])
+    for key, row, col, value, msg in [("tp", 0, 0, V + 5, "element node"), ("te", 0, 0, -1, "element edge"),
+                                      ("ep", 0, 1, V, "edge node")]:
+        tab = {k: v.copy() for k, v in t.items()}
+        tab[key][row, col] = value
+        with pytest.raises(swe.DomainError, match=msg):
+            swe.Topology.create(V, tab["ep"], tab["et"], tab["tp"], tab["te"], tab["tt"])
+    with pytest.raises(ValueError):
+        swe.Topology.create(V, t["ep"][:-1], t["et"], t["tp"], t["te"], t["tt"])
+    tri = t["tp"].copy()
+    tri[0, 1] = tri[0, 0]
+    with pytest.raises(ValueError):
+        swe.Topology.build(V, tri, np.zeros((0, 2), np.int64))
