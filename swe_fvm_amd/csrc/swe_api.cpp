@@ -257,12 +257,10 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
     if ((e = hipMemset(s->su[b], 0, sizeof(double) * size_t(std::max<int64_t>(T, 1)))) != hipSuccess) return bail(e, "hipMemset");
     if ((e = hipMemset(s->sv[b], 0, sizeof(double) * size_t(std::max<int64_t>(T, 1)))) != hipSuccess) return bail(e, "hipMemset");
   }
+  // the whole mesh is the visited window; cinst / bdt / bactive are set for
+  // ensembles below
   s->M = DevMesh{dpx, dpy, dpz, dtp, dte, dtt, dfl, dep0, dep1, det0, det1, doff, dn2c, T, E, V,
-                 G.cx, G.cy, G.bc, G.area, G.edmin, G.crec};
-  s->M.c0 = 0;
-  s->M.c1 = T;
-  s->M.e0 = 0;
-  s->M.e1 = E;
+                 G.cx, G.cy, G.bc, G.area, G.edmin, G.crec, nullptr, nullptr, nullptr, 0, T, 0, E};
   s->G = G;
   if ((e = launch_setup_geometry(s->M, G, s->stream)) != hipSuccess) return bail(e, "setup_geometry");
   if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) return bail(e, "hipStreamSynchronize");
