@@ -24,7 +24,9 @@
 //   D9  Einfeldt Roe velocity (cl*ul + cl*ur)/(cl+cr), bug-compatible
 //       (src/Fluxes.cpp:22).
 //   D10 HLLC CFL denominator max(tol, max(al, ar)) (include/Fluxes.h:89).
-//   D11 only SOLID_WALL (-1) boundary tags; others are rejected.
+//   D11 SOLID_WALL (-1) as the reference, FREE_FLOW (-2) as a transmissive
+//       edge (declared, unimplemented in the reference); other tags are
+//       rejected.
 //   D12 CFL mode: dt = cfl * r with r from the stage-1 flux pass, r <= 1.
 //
 // Floating point: built with -ffp-contract=off; every expression is written
