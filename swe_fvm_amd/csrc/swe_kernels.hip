@@ -325,6 +325,8 @@ __device__ __forceinline__ void block_min_bits(double v, unsigned long long* tar
 // NI edges kBlock apart; all their loads are issued before any flux is
 // computed, so a wave keeps NI times as many gathers in flight.  NI is a
 // launch-time choice (swe_autotune); the results do not depend on it.
+// CFL false: no length-to-wavespeed minimum (a fixed-dt pass that never
+// reads it); the fluxes are the same.
 struct FluxIn {
   V3 Pa, Pb;  // P(ep0), P(ep1)
   double edmin, hf, cxf, cyf, cxt, cyt;
@@ -333,7 +335,7 @@ struct FluxIn {
   int t, inst;
   bool on;
 };
-template <int FLUX, int WS, int NI>
+template <int FLUX, int WS, int NI, bool CFL>
 __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, double cor,
                                               unsigned long long* rbits) {
   const int64_t base = M.e0 + (int64_t)logical_block() * (kBlock * NI) + threadIdx.x;
@@ -346,7 +348,7 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, 
     q.e = e;
     q.Pa = node(M, M.ep0[es]);
     q.Pb = node(M, M.ep1[es]);
-    q.edmin = M.edmin[es];
+    q.edmin = CFL ? M.edmin[es] : 1.;
     const int f = M.et0[es], t = M.et1[es];
     const int tr = t < 0 ? f : t;  // boundary edges read their own cell twice: R = L
     q.t = t;
@@ -390,6 +392,7 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, 
     W.f1[q.e] = F[1];
     W.f2[q.e] = F[2];
   }
+  if (!CFL) return;
   if (!M.cinst) {
     block_min_bits(lw, rbits);
     return;
@@ -896,12 +899,16 @@ hipError_t launch_recon2(const DevMesh& M, const DevState& S, const DevWork& W, 
 }
 hipError_t launch_flux(const DevMesh& M, const DevState& S, const DevWork& W, int flux, int ws,
                        double cor, unsigned long long* rbits, int items, hipStream_t st, bool reset) {
+  // without the reset (a fixed-dt pass of a single mesh) the minimum is not
+  // taken at all; ensembles always take their per-member minima
+  const bool cfl = reset || M.cinst;
   if (reset && !M.cinst) hipLaunchKernelGGL(k_reset_r, dim3(1), dim3(1), 0, st, rbits);
   if (M.e1 <= M.e0) return hipGetLastError();
   dim3 g = grid_for((M.e1 - M.e0 + items - 1) / items), b(256);
 #define SWE_FLUX_CASE(F, Wv, N)                                                        \
   if (flux == F && ws == Wv && items == N) {                                           \
-    hipLaunchKernelGGL((k_flux<F, Wv, N>), g, b, 0, st, M, S, W, cor, rbits);          \
+    if (cfl) hipLaunchKernelGGL((k_flux<F, Wv, N, true>), g, b, 0, st, M, S, W, cor, rbits);   \
+    else hipLaunchKernelGGL((k_flux<F, Wv, N, false>), g, b, 0, st, M, S, W, cor, rbits);      \
     return hipGetLastError();                                                          \
   }
 #define SWE_FLUX_CASES(N)                                                              \

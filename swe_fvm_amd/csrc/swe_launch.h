@@ -184,8 +184,8 @@ hipError_t launch_recon2(const DevMesh& M, const DevState& S, const DevWork& W, 
                          hipStream_t st);
 // items: edges per thread of the flux (1, 2 or 4), cells per thread of the
 // update (1 or 2); results are identical for every choice
-// reset false: the reduction slot is not reset first (a fixed-dt pass that
-// never reads it)
+// reset false (single mesh): no CFL minimum at all, neither reset nor taken
+// (a fixed-dt pass that never reads it); the fluxes are the same
 hipError_t launch_flux(const DevMesh& M, const DevState& S, const DevWork& W, int flux, int ws,
                        double cor, unsigned long long* rbits, int items, hipStream_t st, bool reset = true);
 // opts & 32 (SWE_OPT_DRAIN_LENGTH): outflow F0 * L in the draining dt
