@@ -92,7 +92,8 @@ def bench_ensemble(args):
     fl = swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt]
     ens = Ensemble(fl, [d for d, _ in mem], [p for _, p in mem])
     tune = ens.autotune(reps=3)
-    print(f"[bench] autotune: flux x{tune['flux_items']} update x{tune['update_items']}",
+    print(f"[bench] autotune: flux x{tune['flux_items']}/{tune['flux_block']} "
+          f"update x{tune['update_items']}/{tune['update_block']}",
           file=sys.stderr, flush=True)
     ens.run("SSPRK2", cfl=0.15, t_end=args.t_end, max_steps=2, graph=True)  # warm-up + capture
     ens.reset(0.0)
@@ -158,7 +159,9 @@ def bench_ensemble(args):
                    "cells_min": int(T.min()), "cells_max": int(T.max()), "scheme": "SSPRK2",
                    "flux": "HLLC<Einfeldt>", "dt": "per-member CFL 0.15",
                    "tuning": {"flux_edges_per_thread": tune["flux_items"],
-                              "update_cells_per_thread": tune["update_items"]}},
+                              "update_cells_per_thread": tune["update_items"],
+                              "flux_block_threads": tune["flux_block"],
+                              "update_block_threads": tune["update_block"]}},
         "dtype": "f64", "data": "synthetic", "n_gpus": 1}))
 
 
@@ -366,9 +369,11 @@ def main():
     if args.warmup:
         sd.run("SSPRK2", args.warmup, dt=0.0, cfl=0.15)
     # work items per thread: the fastest bit-identical kernel instantiation
-    tune = {"flux_items": sd.tuning()[0], "update_items": sd.tuning()[1]} if args.no_autotune \
-        else sd.autotune(reps=5)
-    print(f"[bench] autotune: flux x{tune['flux_items']} update x{tune['update_items']}",
+    tn = sd.tuning()
+    tune = {"flux_items": tn[0], "update_items": tn[1], "flux_block": tn[2], "update_block": tn[3]} \
+        if args.no_autotune else sd.autotune(reps=5)
+    print(f"[bench] autotune: flux x{tune['flux_items']}/{tune['flux_block']} "
+          f"update x{tune['update_items']}/{tune['update_block']}",
           file=sys.stderr, flush=True)
     sd.synchronize()
     t1 = time.perf_counter()
@@ -408,7 +413,9 @@ def main():
                    "scheme": "SSPRK2", "flux": "HLLC<Einfeldt>", "dt": "CFL 0.15",
                    "parallelism": "single GPU",
                    "tuning": {"flux_edges_per_thread": tune["flux_items"],
-                              "update_cells_per_thread": tune["update_items"]}},
+                              "update_cells_per_thread": tune["update_items"],
+                              "flux_block_threads": tune["flux_block"],
+                              "update_block_threads": tune["update_block"]}},
         "roofline": {"bound": "hbm", "kernel": dom_k, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
                      "traffic": traffic},

@@ -279,11 +279,18 @@ int swe_set_bathymetry(swe_solver* s, const double* z);
  * Defaults: 2 and 1. */
 int swe_set_tuning(swe_solver* s, int flux_items, int update_items);
 int swe_get_tuning(const swe_solver* s, int* flux_items, int* update_items);
-/* Times every instantiation reps times on the current state, keeps the
- * fastest and returns it; times (optional, 5 doubles) receives the average
- * per-launch milliseconds of flux x1, x2, x4 and update x1, x2.  The state
- * is unchanged; the reconstruction and flux fields afterwards are those of
- * the current state. */
+/* Threads per workgroup of the flux and update kernels (256 or 512); no
+ * reference counterpart.  Results do not depend on the choice.  Defaults:
+ * 256 and 256. */
+int swe_set_tuning_blocks(swe_solver* s, int flux_block, int update_block);
+int swe_get_tuning_blocks(const swe_solver* s, int* flux_block, int* update_block);
+/* Times every instantiation (work items x block size) reps times on the
+ * current state, keeps the fastest and returns its work items (the block
+ * sizes through swe_get_tuning_blocks); times (optional, 5 doubles)
+ * receives the average per-launch milliseconds of flux x1, x2, x4 and
+ * update x1, x2, each at its faster block size.  The state is unchanged;
+ * the reconstruction and flux fields afterwards are those of the current
+ * state. */
 int swe_autotune(swe_solver* s, int reps, int* flux_items, int* update_items, double* times);
 
 #ifdef __cplusplus

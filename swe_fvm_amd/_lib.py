@@ -66,6 +66,8 @@ SIGNATURES = [
     ("swe_tangent_run", _int, [_p, _int, _d, _p, _i64, _p, _p, _p, _p]),
     ("swe_set_bathymetry", _int, [_p, _p]),
     ("swe_set_tuning", _int, [_p, _int, _int]),
+    ("swe_set_tuning_blocks", _int, [_p, _int, _int]),
+    ("swe_get_tuning_blocks", _int, [_p, _p, _p]),
     ("swe_get_tuning", _int, [_p, _p, _p]),
     ("swe_autotune", _int, [_p, _int, _p, _p, _p]),
 ]

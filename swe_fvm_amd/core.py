@@ -517,10 +517,11 @@ class SpaceDisc:
                                         ctypes.byref(ms), None))
         return ms.value
 
-    def set_tuning(self, flux_items=2, update_items=1):
-        """Edges per thread of the flux kernel (1, 2, 4) and cells per thread
-        of the update kernel (1, 2); results are identical for every choice."""
-        _check(_lib.load().swe_set_tuning(self._h, int(flux_items), int(update_items)))
+    def set_tuning(self, flux_items=2, update_items=1, flux_block=256, update_block=256):
+        """Edges per thread of the flux kernel (1, 2, 4), cells per thread of
+        the update kernel (1, 2) and threads per workgroup of either (256,
+        512); results are identical for every choice."""
+        _set_tuning(self._h, flux_items, update_items, flux_block, update_block)
 
     def tuning(self):
         return _tuning(self._h)
@@ -531,17 +532,25 @@ class SpaceDisc:
         return _autotune(self._h, reps)
 
 
+def _set_tuning(h, flux_items, update_items, flux_block, update_block):
+    _check(_lib.load().swe_set_tuning(h, int(flux_items), int(update_items)))
+    _check(_lib.load().swe_set_tuning_blocks(h, int(flux_block), int(update_block)))
+
+
 def _tuning(h):
-    fi, ui = ctypes.c_int(), ctypes.c_int()
+    """(flux items, update items, flux block, update block)"""
+    fi, ui, fb, ub = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     _check(_lib.load().swe_get_tuning(h, ctypes.byref(fi), ctypes.byref(ui)))
-    return fi.value, ui.value
+    _check(_lib.load().swe_get_tuning_blocks(h, ctypes.byref(fb), ctypes.byref(ub)))
+    return fi.value, ui.value, fb.value, ub.value
 
 
 def _autotune(h, reps):
     fi, ui = ctypes.c_int(), ctypes.c_int()
     t = np.zeros(5, np.float64)
     _check(_lib.load().swe_autotune(h, int(reps), ctypes.byref(fi), ctypes.byref(ui), _ptr(t)))
-    return {"flux_items": fi.value, "update_items": ui.value,
+    _, _, fb, ub = _tuning(h)
+    return {"flux_items": fi.value, "update_items": ui.value, "flux_block": fb, "update_block": ub,
             "flux_ms": {1: t[0], 2: t[1], 4: t[2]}, "update_ms": {1: t[3], 2: t[4]}}
 
 

@@ -19,6 +19,7 @@ struct swe_solver {
   hipStream_t stream = nullptr;
   int flux = 1, ws = 2, opts = 0;
   int flux_items = 2, upd_items = 1;  // work items per thread (swe_autotune)
+  int flux_block = 256, upd_block = 256;  // threads per workgroup (swe_autotune)
   double cor = 0., tau = 0.;
   int64_t T = 0, E = 0, V = 0;
   std::vector<void*> allocs;
@@ -89,10 +90,10 @@ int run_stage(swe_solver* s, int in, int u0, int out, double a0, double a1, doub
   DevState S = state_of(s, in);
   SWE_CHECK(launch_recon1(s->M, S, s->W, s->opts, s->stream), "recon1");
   SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
-  SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + slot, s->flux_items, s->stream), "flux");
+  SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + slot, s->flux_items, s->flux_block, s->stream), "flux");
   SWE_CHECK(launch_drain(s->M, S, s->W, s->opts, s->stream), "drain");
   StageArgs A{a0, a1, coef, dt_fixed, cfl, s->cor, s->rbits, dts_log, friction(s)};
-  SWE_CHECK(launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->upd_items, s->stream), "update");
+  SWE_CHECK(launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->upd_items, s->upd_block, s->stream), "update");
   s->last_r_slot = slot;
   return SWE_OK;
 }
@@ -386,7 +387,7 @@ int swe_compute_fluxes(swe_solver* s, double* r) {
   if (int rc = single_mesh(s)) return rc;
   if (int rc = set_device(s)) return rc;
   DevState S = state_of(s, s->cur);
-  SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + 3, s->flux_items, s->stream), "flux");
+  SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + 3, s->flux_items, s->flux_block, s->stream), "flux");
   s->last_r_slot = 3;
   unsigned long long bits;
   SWE_CHECK(hipMemcpyAsync(&bits, s->rbits + 3, 8, hipMemcpyDeviceToHost, s->stream), "hipMemcpy");
@@ -563,11 +564,11 @@ static int batch_step(swe_solver* s, int scheme, double cfl, double t_end) {
     SWE_CHECK(launch_batch_reset_r(s->B, q, s->stream), "batch_reset_r");
     SWE_CHECK(launch_recon1(s->M, S, s->W, s->opts, s->stream), "recon1");
     SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
-    SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->B.rbits + q, s->flux_items, s->stream), "flux");
+    SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->B.rbits + q, s->flux_items, s->flux_block, s->stream), "flux");
     if (q == 0) SWE_CHECK(launch_batch_dt(s->B, cfl, t_end, s->stream), "batch_dt");
     SWE_CHECK(launch_drain(s->M, S, s->W, s->opts, s->stream), "drain");
     StageArgs A{st[q].a0, st[q].a1, st[q].coef, 0., cfl, s->cor, nullptr, nullptr, friction(s)};
-    SWE_CHECK(launch_update(s->M, S, state_of(s, st[q].u0), out_of(s, st[q].out), s->W, A, s->upd_items, s->stream),
+    SWE_CHECK(launch_update(s->M, S, state_of(s, st[q].u0), out_of(s, st[q].out), s->W, A, s->upd_items, s->upd_block, s->stream),
               "update");
   }
   SWE_CHECK(launch_batch_advance(s->B, t_end, s->stream), "batch_advance");
@@ -767,13 +768,13 @@ int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nstep
         if (e == hipSuccess) e = hipEventRecord(ev[2], s->stream);
         if (e == hipSuccess) e = hipEventRecord(ev[3], s->stream);
         if (e == hipSuccess)
-          e = launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + st, s->flux_items, s->stream);
+          e = launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + st, s->flux_items, s->flux_block, s->stream);
         if (e == hipSuccess) e = hipEventRecord(ev[4], s->stream);
         if (e == hipSuccess) e = hipEventRecord(ev[5], s->stream);
         if (e == hipSuccess) e = launch_drain(s->M, S, s->W, s->opts, s->stream);
         if (e == hipSuccess) e = hipEventRecord(ev[6], s->stream);
         if (e == hipSuccess) e = hipEventRecord(ev[7], s->stream);
-        if (e == hipSuccess) e = launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->upd_items, s->stream);
+        if (e == hipSuccess) e = launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->upd_items, s->upd_block, s->stream);
         if (e == hipSuccess) e = hipEventRecord(ev[8], s->stream);
         if (e == hipSuccess) e = hipEventSynchronize(ev[8]);
         if (e != hipSuccess) { rc = dev_fail(e, "kernel"); break; }
@@ -833,6 +834,27 @@ int swe_get_tuning(const swe_solver* s, int* flux_items, int* update_items) {
 // rewrites F and a CFL slot no step reads, the update writes a free state
 // buffer), keeps the fastest, and reports the per-launch times in ms as
 // {flux x1, x2, x4, update x1, x2}.  The state is left unchanged.
+int swe_set_tuning_blocks(swe_solver* s, int flux_block, int update_block) {
+  if (!s) return fail(SWE_ERR_ARG, "null solver");
+  if ((flux_block != 256 && flux_block != 512) || (update_block != 256 && update_block != 512))
+    return fail(SWE_ERR_ARG, "blocks must be 256 or 512 threads");
+  if (s->flux_block != flux_block || s->upd_block != update_block) {
+    s->flux_block = flux_block;
+    s->upd_block = update_block;
+    if (s->graph) {  // a captured ensemble step holds the old launches
+      (void)hipGraphExecDestroy(s->graph);
+      s->graph = nullptr;
+      s->graph_scheme = -1;
+    }
+  }
+  return SWE_OK;
+}
+int swe_get_tuning_blocks(const swe_solver* s, int* flux_block, int* update_block) {
+  if (!s) return fail(SWE_ERR_ARG, "null solver");
+  if (flux_block) *flux_block = s->flux_block;
+  if (update_block) *update_block = s->upd_block;
+  return SWE_OK;
+}
 int swe_autotune(swe_solver* s, int reps, int* flux_items, int* update_items, double* times) {
   if (!s) return fail(SWE_ERR_ARG, "null solver");
   if (reps < 1) return fail(SWE_ERR_ARG, "reps must be >= 1");
@@ -845,45 +867,57 @@ int swe_autotune(swe_solver* s, int reps, int* flux_items, int* update_items, do
   hipEvent_t e0, e1;
   SWE_CHECK(hipEventCreate(&e0), "hipEventCreate");
   SWE_CHECK(hipEventCreate(&e1), "hipEventCreate");
-  const int fi[3] = {1, 2, 4}, ui[2] = {1, 2};
-  double t[5];
+  const int fi[3] = {1, 2, 4}, ui[2] = {1, 2}, bs[2] = {256, 512};
+  double t[5][2];
   int rc = SWE_OK;
-  auto timed = [&](auto&& launch, double& out) {
-    hipError_t e = launch();  // warm-up
-    if (e == hipSuccess) e = hipEventRecord(e0, s->stream);
+  // round 0 only warms every variant up (clocks, caches), so that the
+  // first one timed is not the one that pays for it; each variant keeps the
+  // faster of rounds 1 and 2
+  for (int q = 0; q < 5; ++q) t[q][0] = t[q][1] = 1e300;
+  auto timed = [&](auto&& launch, double& out, bool keep) {
+    hipError_t e = hipEventRecord(e0, s->stream);
     for (int q = 0; q < reps && e == hipSuccess; ++q) e = launch();
     if (e == hipSuccess) e = hipEventRecord(e1, s->stream);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float f = 0.f;
     if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
     if (e != hipSuccess) return dev_fail(e, "autotune");
-    out = double(f) / reps;
+    if (keep && double(f) / reps < out) out = double(f) / reps;
     return SWE_OK;
   };
-  for (int q = 0; q < 3 && !rc; ++q)
-    rc = timed([&] { return launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, rslot, fi[q], s->stream); },
-               t[q]);
-  if (!rc) {
-    hipError_t e = launch_drain(s->M, S, s->W, s->opts, s->stream);
-    if (e != hipSuccess) rc = dev_fail(e, "drain");
-  }
   // Euler-form stage with a small fixed step into the free buffer
   StageArgs A{0., 0., 1., 1e-6, 0., s->cor, rslot, nullptr, friction(s)};
-  for (int q = 0; q < 2 && !rc; ++q)
-    rc = timed([&] {
-      return launch_update(s->M, S, S, out_of(s, scratch), s->W, A, ui[q], s->stream);
-    }, t[3 + q]);
+  for (int round = 0; round < 3 && !rc; ++round) {
+    for (int q = 0; q < 3 && !rc; ++q)
+      for (int b = 0; b < 2 && !rc; ++b)
+        rc = timed([&] { return launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, rslot, fi[q], bs[b], s->stream); },
+                   t[q][b], round > 0);
+    if (!rc) {
+      hipError_t e = launch_drain(s->M, S, s->W, s->opts, s->stream);
+      if (e != hipSuccess) rc = dev_fail(e, "drain");
+    }
+    for (int q = 0; q < 2 && !rc; ++q)
+      for (int b = 0; b < 2 && !rc; ++b)
+        rc = timed([&] {
+          return launch_update(s->M, S, S, out_of(s, scratch), s->W, A, ui[q], bs[b], s->stream);
+        }, t[3 + q][b], round > 0);
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  int bf = 0, bu = 0;
-  for (int q = 1; q < 3; ++q) if (t[q] < t[bf]) bf = q;
-  if (t[4] < t[3]) bu = 1;
+  int bf = 0, bfb = 0, bu = 0, bub = 0;
+  for (int q = 0; q < 3; ++q)
+    for (int b = 0; b < 2; ++b)
+      if (t[q][b] < t[bf][bfb]) { bf = q; bfb = b; }
+  for (int q = 0; q < 2; ++q)
+    for (int b = 0; b < 2; ++b)
+      if (t[3 + q][b] < t[3 + bu][bub]) { bu = q; bub = b; }
   if (int r2 = swe_set_tuning(s, fi[bf], ui[bu])) return r2;
+  if (int r2 = swe_set_tuning_blocks(s, bs[bfb], bs[bub])) return r2;
   if (s->n_inst == 0) s->last_r_slot = 3;
   if (flux_items) *flux_items = fi[bf];
   if (update_items) *update_items = ui[bu];
-  if (times) for (int q = 0; q < 5; ++q) times[q] = t[q];
+  if (times) for (int q = 0; q < 5; ++q) times[q] = t[q][0] < t[q][1] ? t[q][0] : t[q][1];
   return SWE_OK;
 }
 
@@ -1161,10 +1195,10 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
         e = launch_recon1(s->M, Sq, Wq, s->opts, s->stream, false);
         if (e == hipSuccess) e = launch_recon2(s->M, Sq, Wq, s->opts, s->stream);
         if (e == hipSuccess)  // fixed dt: the CFL reduction is not read
-          e = launch_flux(s->M, Sq, Wq, s->flux, s->ws, s->cor, s->rbits + 3, s->flux_items, s->stream, false);
+          e = launch_flux(s->M, Sq, Wq, s->flux, s->ws, s->cor, s->rbits + 3, s->flux_items, s->flux_block, s->stream, false);
         if (e == hipSuccess) e = launch_drain(s->M, Sq, Wq, s->opts, s->stream);
         StageArgs A{sc[q].a0, sc[q].a1, sc[q].coef, dtn(n), 0., s->cor, s->rbits, nullptr, friction(s)};
-        if (e == hipSuccess) e = launch_update(s->M, Sq, Un, O, Wq, A, s->upd_items, s->stream);
+        if (e == hipSuccess) e = launch_update(s->M, Sq, Un, O, Wq, A, s->upd_items, s->upd_block, s->stream);
         Sq = DevState{O.w, O.u, O.v};
       }
     }

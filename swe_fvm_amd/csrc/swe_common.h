@@ -74,11 +74,12 @@ __device__ __forceinline__ Slot slot_value(const Muscl& m, double cx, double cy,
 
 // grid of 256-thread workgroups covering n items, a multiple of 8 (one
 // contiguous range per XCD, see logical_block)
-inline dim3 grid_for(int64_t n) {
-  int64_t nb = (n + 255) / 256;
+inline dim3 grid_for_bs(int64_t n, int64_t bs) {
+  int64_t nb = (n + bs - 1) / bs;
   nb = ((nb + 7) / 8) * 8;
   if (nb < 8) nb = 8;
   return dim3((unsigned)nb);
 }
+inline dim3 grid_for(int64_t n) { return grid_for_bs(n, kBlock); }
 
 }  // namespace swe

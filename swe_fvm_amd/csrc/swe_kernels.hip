@@ -313,7 +313,7 @@ __device__ __forceinline__ void block_min_bits(double v, unsigned long long* tar
   if (lane == 0) red[wid] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int nw = kBlock / 64;
+    const int nw = (int)(blockDim.x >> 6);
     double m = red[0];
     for (int q = 1; q < nw; ++q) m = dmin(m, red[q]);
     // positive doubles order like their bit patterns
@@ -322,9 +322,10 @@ __device__ __forceinline__ void block_min_bits(double v, unsigned long long* tar
 }
 
 // SpaceDisc::ComputeFluxes (src/SpaceDisc.cpp:54-74).  Each thread handles
-// NI edges kBlock apart; all their loads are issued before any flux is
-// computed, so a wave keeps NI times as many gathers in flight.  NI is a
-// launch-time choice (swe_autotune); the results do not depend on it.
+// NI edges BS apart (BS threads per workgroup); all their loads are issued
+// before any flux is computed, so a wave keeps NI times as many gathers in
+// flight.  NI and BS are launch-time choices (swe_autotune); the results do
+// not depend on them.
 // CFL false: no length-to-wavespeed minimum (a fixed-dt pass that never
 // reads it); the fluxes are the same.
 struct FluxIn {
@@ -335,14 +336,14 @@ struct FluxIn {
   int t, inst;
   bool on;
 };
-template <int FLUX, int WS, int NI, bool CFL>
-__global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, DevWork W, double cor,
+template <int FLUX, int WS, int NI, bool CFL, int BS>
+__global__ void __launch_bounds__(BS) k_flux(DevMesh M, DevState S, DevWork W, double cor,
                                               unsigned long long* rbits) {
-  const int64_t base = M.e0 + (int64_t)logical_block() * (kBlock * NI) + threadIdx.x;
+  const int64_t base = M.e0 + (int64_t)logical_block() * (BS * NI) + threadIdx.x;
   FluxIn in[NI];
 #pragma unroll
   for (int it = 0; it < NI; ++it) {
-    const int64_t e = base + it * kBlock;
+    const int64_t e = base + it * BS;
     const int64_t es = e < M.e1 ? e : M.e0;  // out-of-range items read a valid edge and are dropped
     FluxIn& q = in[it];
     q.e = e;
@@ -547,12 +548,12 @@ __device__ __forceinline__ double stage_dt(const StageArgs& A) {
   return dt;
 }
 
-// NI cells per thread, kBlock apart, loads of all of them first (NI is a
-// launch-time choice like the flux's).
-template <int NI>
-__global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, DevState U0, DevOut O,
+// NI cells per thread, BS apart (BS threads per workgroup), loads of all of
+// them first (NI and BS are launch-time choices like the flux's).
+template <int NI, int BS>
+__global__ void __launch_bounds__(BS) k_update(DevMesh M, DevState S, DevState U0, DevOut O,
                                                 DevWork W, StageArgs A) {
-  const int64_t base = M.c0 + (int64_t)logical_block() * (kBlock * NI) + threadIdx.x;
+  const int64_t base = M.c0 + (int64_t)logical_block() * (BS * NI) + threadIdx.x;
   const double dt = M.cinst ? 0. : stage_dt(A);
   if (A.dts_log && blockIdx.x == 0 && threadIdx.x == 0) A.dts_log[0] = dt;
   RhsIn in[NI];
@@ -561,7 +562,7 @@ __global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, DevState 
   const bool comb = A.a0 != 0.;
 #pragma unroll
   for (int it = 0; it < NI; ++it) {
-    const int64_t i = base + it * kBlock;
+    const int64_t i = base + it * BS;
     const int is = (int)(i < M.c1 ? i : M.c0);  // out-of-range items read a valid cell and are dropped
     rhs_load(M, S, W, is, in[it]);
     w[it] = S.w[is];
@@ -576,7 +577,7 @@ __global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, DevState 
   }
 #pragma unroll
   for (int it = 0; it < NI; ++it) {
-    const int64_t i = base + it * kBlock;
+    const int64_t i = base + it * BS;
     if (i >= M.c1) continue;
     double dts = A.coef * dt;
     if (M.cinst) {
@@ -898,17 +899,22 @@ hipError_t launch_recon2(const DevMesh& M, const DevState& S, const DevWork& W, 
   return hipGetLastError();
 }
 hipError_t launch_flux(const DevMesh& M, const DevState& S, const DevWork& W, int flux, int ws,
-                       double cor, unsigned long long* rbits, int items, hipStream_t st, bool reset) {
+                       double cor, unsigned long long* rbits, int items, int block, hipStream_t st,
+                       bool reset) {
   // without the reset (a fixed-dt pass of a single mesh) the minimum is not
   // taken at all; ensembles always take their per-member minima
   const bool cfl = reset || M.cinst;
   if (reset && !M.cinst) hipLaunchKernelGGL(k_reset_r, dim3(1), dim3(1), 0, st, rbits);
   if (M.e1 <= M.e0) return hipGetLastError();
-  dim3 g = grid_for((M.e1 - M.e0 + items - 1) / items), b(256);
+  if (block != 256 && block != 512) return hipErrorInvalidValue;
+  dim3 g = grid_for_bs((M.e1 - M.e0 + items - 1) / items, block), b(block);
+#define SWE_FLUX_LAUNCH(F, Wv, N, BSV)                                                                \
+  if (cfl) hipLaunchKernelGGL((k_flux<F, Wv, N, true, BSV>), g, b, 0, st, M, S, W, cor, rbits);         \
+  else hipLaunchKernelGGL((k_flux<F, Wv, N, false, BSV>), g, b, 0, st, M, S, W, cor, rbits);
 #define SWE_FLUX_CASE(F, Wv, N)                                                        \
   if (flux == F && ws == Wv && items == N) {                                           \
-    if (cfl) hipLaunchKernelGGL((k_flux<F, Wv, N, true>), g, b, 0, st, M, S, W, cor, rbits);   \
-    else hipLaunchKernelGGL((k_flux<F, Wv, N, false>), g, b, 0, st, M, S, W, cor, rbits);      \
+    if (block == 256) { SWE_FLUX_LAUNCH(F, Wv, N, 256) }                               \
+    else { SWE_FLUX_LAUNCH(F, Wv, N, 512) }                                            \
     return hipGetLastError();                                                          \
   }
 #define SWE_FLUX_CASES(N)                                                              \
@@ -917,6 +923,7 @@ hipError_t launch_flux(const DevMesh& M, const DevState& S, const DevWork& W, in
   SWE_FLUX_CASES(1) SWE_FLUX_CASES(2) SWE_FLUX_CASES(4)
 #undef SWE_FLUX_CASES
 #undef SWE_FLUX_CASE
+#undef SWE_FLUX_LAUNCH
   return hipErrorInvalidValue;
 }
 hipError_t launch_drain(const DevMesh& M, const DevState& S, const DevWork& W, int opts,
@@ -928,10 +935,13 @@ hipError_t launch_drain(const DevMesh& M, const DevState& S, const DevWork& W, i
   return hipGetLastError();
 }
 hipError_t launch_update(const DevMesh& M, const DevState& S, const DevState& U0, const DevOut& O,
-                         const DevWork& W, const StageArgs& A, int items, hipStream_t st) {
-  dim3 g = grid_for((M.c1 - M.c0 + items - 1) / items), b(256);
-  if (items == 1) hipLaunchKernelGGL((k_update<1>), g, b, 0, st, M, S, U0, O, W, A);
-  else if (items == 2) hipLaunchKernelGGL((k_update<2>), g, b, 0, st, M, S, U0, O, W, A);
+                         const DevWork& W, const StageArgs& A, int items, int block, hipStream_t st) {
+  if (block != 256 && block != 512) return hipErrorInvalidValue;
+  dim3 g = grid_for_bs((M.c1 - M.c0 + items - 1) / items, block), b(block);
+  if (items == 1 && block == 256) hipLaunchKernelGGL((k_update<1, 256>), g, b, 0, st, M, S, U0, O, W, A);
+  else if (items == 2 && block == 256) hipLaunchKernelGGL((k_update<2, 256>), g, b, 0, st, M, S, U0, O, W, A);
+  else if (items == 1 && block == 512) hipLaunchKernelGGL((k_update<1, 512>), g, b, 0, st, M, S, U0, O, W, A);
+  else if (items == 2 && block == 512) hipLaunchKernelGGL((k_update<2, 512>), g, b, 0, st, M, S, U0, O, W, A);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }

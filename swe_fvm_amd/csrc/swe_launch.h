@@ -183,16 +183,18 @@ hipError_t launch_recon1(const DevMesh& M, const DevState& S, const DevWork& W, 
 hipError_t launch_recon2(const DevMesh& M, const DevState& S, const DevWork& W, int opts,
                          hipStream_t st);
 // items: edges per thread of the flux (1, 2 or 4), cells per thread of the
-// update (1 or 2); results are identical for every choice
+// update (1 or 2); block: threads per workgroup of either (256 or 512);
+// results are identical for every choice
 // reset false (single mesh): no CFL minimum at all, neither reset nor taken
 // (a fixed-dt pass that never reads it); the fluxes are the same
 hipError_t launch_flux(const DevMesh& M, const DevState& S, const DevWork& W, int flux, int ws,
-                       double cor, unsigned long long* rbits, int items, hipStream_t st, bool reset = true);
+                       double cor, unsigned long long* rbits, int items, int block, hipStream_t st,
+                       bool reset = true);
 // opts & 32 (SWE_OPT_DRAIN_LENGTH): outflow F0 * L in the draining dt
 hipError_t launch_drain(const DevMesh& M, const DevState& S, const DevWork& W, int opts,
                         hipStream_t st);
 hipError_t launch_update(const DevMesh& M, const DevState& S, const DevState& U0, const DevOut& O,
-                         const DevWork& W, const StageArgs& A, int items, hipStream_t st);
+                         const DevWork& W, const StageArgs& A, int items, int block, hipStream_t st);
 hipError_t launch_batch_reset_r(const DevBatch& B, int slot, hipStream_t st);
 hipError_t launch_batch_dt(const DevBatch& B, double cfl, double t_end, hipStream_t st);
 hipError_t launch_batch_advance(const DevBatch& B, double t_end, hipStream_t st);

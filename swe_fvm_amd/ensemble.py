@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .core import SCHEMES, Fluxer, SolverError, _autotune, _check, _ptr, _tuning
+from .core import SCHEMES, Fluxer, SolverError, _autotune, _check, _ptr, _set_tuning, _tuning
 
 
 class Ensemble:
@@ -73,8 +73,8 @@ class Ensemble:
         _check(_lib.load().swe_batch_time(self._h, _ptr(t), _ptr(st)))
         return t, st
 
-    def set_tuning(self, flux_items=2, update_items=1):
-        _check(_lib.load().swe_set_tuning(self._h, int(flux_items), int(update_items)))
+    def set_tuning(self, flux_items=2, update_items=1, flux_block=256, update_block=256):
+        _set_tuning(self._h, flux_items, update_items, flux_block, update_block)
 
     def tuning(self):
         return _tuning(self._h)

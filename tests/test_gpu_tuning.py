@@ -1,5 +1,5 @@
-"""Work items per thread (swe_set_tuning / swe_autotune): every choice gives
-bit-identical results, the autotuner leaves the state alone, and ensemble
+"""Work items per thread and block sizes (swe_set_tuning,
+swe_set_tuning_blocks, swe_autotune): every choice gives bit-identical results, the autotuner leaves the state alone, and ensemble
 handles are refused by the single-mesh entry points.  The cell layout:
 morton_order of a shuffled mesh gives the same run, faster."""
 import itertools
@@ -21,18 +21,18 @@ def test_every_tuning_gives_identical_steps():
     xyz, topo, prim = C.wet_dry_mesh(40, seed=9)
     dom = swe.Domain(xyz, topo)
     ref = None
-    for fi, ui in itertools.product((1, 2, 4), (1, 2)):
+    for fi, ui, fb, ub in itertools.product((1, 2, 4), (1, 2), (256, 512), (256, 512)):
         sd = swe.SpaceDisc(FL, dom, prim)
-        sd.set_tuning(fi, ui)
-        assert sd.tuning() == (fi, ui)
+        sd.set_tuning(fi, ui, fb, ub)
+        assert sd.tuning() == (fi, ui, fb, ub)
         sd.run("SSPRK2", 6, dt=0.0, cfl=0.15)
         got = (sd.GetVolField(), sd.dts(6), sd.GetFluxes())
-        print(f"tuning flux x{fi} update x{ui}", flush=True)
+        print(f"tuning flux x{fi}/{fb} update x{ui}/{ub}", flush=True)
         if ref is None:
             ref = got
         else:
             for a, b in zip(got, ref):
-                assert np.array_equal(a, b), (fi, ui)
+                assert np.array_equal(a, b), (fi, ui, fb, ub)
 
 
 def test_bad_tuning_rejected():
@@ -41,7 +41,10 @@ def test_bad_tuning_rejected():
     for fi, ui in ((3, 1), (0, 1), (2, 3), (8, 2)):
         with pytest.raises(ValueError):
             sd.set_tuning(fi, ui)
-    assert sd.tuning() == (2, 1)
+    for fb, ub in ((128, 256), (256, 1024), (0, 512)):
+        with pytest.raises(ValueError):
+            sd.set_tuning(2, 1, fb, ub)
+    assert sd.tuning() == (2, 1, 256, 256)
 
 
 def test_autotune_keeps_state_and_results():
@@ -53,7 +56,8 @@ def test_autotune_keeps_state_and_results():
     res = sd.autotune(reps=3)
     print("autotune:", res, flush=True)
     assert res["flux_items"] in (1, 2, 4) and res["update_items"] in (1, 2)
-    assert sd.tuning() == (res["flux_items"], res["update_items"])
+    assert res["flux_block"] in (256, 512) and res["update_block"] in (256, 512)
+    assert sd.tuning() == (res["flux_items"], res["update_items"], res["flux_block"], res["update_block"])
     assert all(t > 0 for t in res["flux_ms"].values()) and all(t > 0 for t in res["update_ms"].values())
     assert np.array_equal(sd.GetVolField(), prim)
     sd.run("SSPRK2", 3, dt=0.0, cfl=0.15)
@@ -67,7 +71,7 @@ def test_ensemble_retune_recaptures_graph():
     ref.run("SSPRK2", cfl=0.15, t_end=1e9, max_steps=8, graph=True)
     ens = Ensemble(FL, doms, prims)
     ens.run("SSPRK2", cfl=0.15, t_end=1e9, max_steps=4, graph=True)
-    ens.set_tuning(4, 2)
+    ens.set_tuning(4, 2, 512, 512)
     ens.run("SSPRK2", cfl=0.15, t_end=1e9, max_steps=4, graph=True)
     for a, b in zip(ens.states(), ref.states()):
         assert np.array_equal(a, b)
