@@ -556,3 +556,23 @@ def test_autograd_function_through_the_pybind_solver():
     lhs = float(np.sum(ybar * fin_dot))
     rhs = float(np.sum(res[1][1] * dp) + np.sum(res[1][2] * dz))
     assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + abs(rhs))
+
+
+def test_adjoint_is_independent_of_the_tuning():
+    """The recomputation inside the adjoint sweep launches with the solver's
+    tuning (work items, workgroup sizes): every choice gives the same
+    gradients bit for bit."""
+    dom, prim0 = wet_case(n=14, seed=21)
+    z = dom.geometry[:, 2].copy()
+    rng = np.random.default_rng(2)
+    cvec = rng.standard_normal(prim0.shape)
+    res = []
+    for tune in [(2, 1, 256, 256), (1, 2, 512, 512), (4, 1, 512, 256)]:
+        sd = swe.SpaceDisc(swe.Fluxes.HLLC[swe.Wavespeeds.Davis], dom, prim0)
+        sd.set_tuning(*tune)
+        J, s0, zb, _ = AD.run_and_adjoint(sd, prim0, z, 8, lambda p: (float(np.sum(cvec * p)), cvec),
+                                          scheme="SSPRK3", dt=2e-3, checkpoint_every=3)
+        res.append((J, s0, zb))
+    for J, s0, zb in res[1:]:
+        assert J == res[0][0]
+        assert np.array_equal(s0, res[0][1]) and np.array_equal(zb, res[0][2])
