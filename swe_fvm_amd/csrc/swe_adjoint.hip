@@ -171,7 +171,7 @@ __device__ inline void slot_bar(const Muscl& m, double cx, double cy, const V3& 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_adj_update(DevMesh M, DevState S, DevState U0, DevWork W,
                                                     StageArgs A, DevAdj J, const double* wnb_in,
                                                     int u0mode) {
-  const int64_t i64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i64 = global_index();  // XCD-aware, as the forward kernels
   if (i64 >= M.T) return;
   const int i = (int)i64;
   const int64_t T = M.T;
@@ -439,7 +439,7 @@ __device__ inline void side_flux_bar(const DevMesh& M, const DevState& S, const 
 // reconstruction.
 template <int FLUX, int WS, bool LEN>
 __global__ void __launch_bounds__(256) k_adj_flux(DevMesh M, DevState S, DevWork W, DevAdj J, double dts) {
-  const int64_t e64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t e64 = global_index();  // XCD-aware, as the forward kernels
   if (e64 >= M.E) return;
   const int e = (int)e64;
   const int64_t Ecnt = M.E;
@@ -569,7 +569,7 @@ __global__ void __launch_bounds__(256) k_adj_pwnode(DevMesh M, DevState S, DevWo
 // the cell (its own update and the slots of its three edges), then maps it
 // to the w adjoints of the cell and its neighbours and to corner z.
 __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWork W, DevAdj J, double dts) {
-  const int64_t i64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i64 = global_index();  // XCD-aware, as the forward kernels
   if (i64 >= M.T) return;
   const int i = (int)i64;
   const int64_t T = M.T, Ecnt = M.E;
@@ -826,10 +826,10 @@ hipError_t launch_adjoint_stage(const DevMesh& M, const DevState& S, const DevSt
                                 const StageArgs& A, const DevAdj& J, int flux, int ws, int opts,
                                 const double* wnb_in, int u0mode, unsigned int npw, hipStream_t st) {
   const double dts = A.coef * A.dt_fixed;
-  hipLaunchKernelGGL(k_adj_update, grid_plain(M.T), dim3(256), 0, st, M, S, U0, W, A, J, wnb_in, u0mode);
+  hipLaunchKernelGGL(k_adj_update, grid_for(M.T), dim3(256), 0, st, M, S, U0, W, A, J, wnb_in, u0mode);
 #define SWE_ADJ_FLUX(F, Wv, L)                                                                   \
   if (flux == F && ws == Wv && len == L)                                                         \
-    hipLaunchKernelGGL((k_adj_flux<F, Wv, L>), grid_plain(M.E), dim3(256), 0, st, M, S, W, J, dts);
+    hipLaunchKernelGGL((k_adj_flux<F, Wv, L>), grid_for(M.E), dim3(256), 0, st, M, S, W, J, dts);
   for (int li = 0; li < 2; ++li) {
     const bool len = li == 1;
     if (len != ((opts & kOptDrainLength) != 0)) continue;
@@ -846,7 +846,7 @@ hipError_t launch_adjoint_stage(const DevMesh& M, const DevState& S, const DevSt
     hipLaunchKernelGGL(k_adj_pw, g, dim3(256), 0, st, M, S, W, J);
     hipLaunchKernelGGL(k_adj_pwnode, g, dim3(256), 0, st, M, S, W, J);
   }
-  hipLaunchKernelGGL(k_adj_recon, grid_plain(M.T), dim3(256), 0, st, M, S, W, J, dts);
+  hipLaunchKernelGGL(k_adj_recon, grid_for(M.T), dim3(256), 0, st, M, S, W, J, dts);
   return hipGetLastError();
 }
 
