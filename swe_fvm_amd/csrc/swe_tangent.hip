@@ -122,7 +122,7 @@ __device__ __forceinline__ SlotT<D> slot_value_t(const MusclT& m, double cx, dou
 // pass 1 (k_recon1): dry, fully wet and the partially wet cells' PartWet1
 // level (kept in wrec1 for the node maxima of pass 2)
 __global__ void __launch_bounds__(256) k_tl_recon(DevMesh M, DevState S, DevState Sd, DevWork W, DevTan Q) {
-  const int64_t i64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i64 = global_index();  // XCD-aware, as the forward kernels
   if (i64 >= M.T) return;
   const int i = (int)i64;
   const int64_t T = M.T;
@@ -229,7 +229,7 @@ __device__ inline D node_max_t(const DevMesh& M, const DevWork& W, const DevTan&
 }
 
 __global__ void __launch_bounds__(256) k_tl_recon2(DevMesh M, DevState S, DevState Sd, DevWork W, DevTan Q) {
-  const int64_t i64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i64 = global_index();  // XCD-aware, as the forward kernels
   if (i64 >= M.T) return;
   const int i = (int)i64;
   if (W.cls[i] != 1) return;
@@ -263,7 +263,7 @@ __global__ void __launch_bounds__(256) k_tl_recon2(DevMesh M, DevState S, DevSta
 template <int FLUX, int WS>
 __global__ void __launch_bounds__(256) k_tl_flux(DevMesh M, DevState S, DevState Sd, DevWork W, DevTan Q,
                                                  double cor) {
-  const int64_t e64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t e64 = global_index();  // XCD-aware, as the forward kernels
   if (e64 >= M.E) return;
   const int e = (int)e64;
   const int64_t T = M.T;
@@ -299,7 +299,7 @@ __global__ void __launch_bounds__(256) k_tl_flux(DevMesh M, DevState S, DevState
 // ---------------------------------------------------------- draining dt --
 template <bool LEN>
 __global__ void __launch_bounds__(256) k_tl_drain(DevMesh M, DevState S, DevState Sd, DevWork W, DevTan Q) {
-  const int64_t i64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i64 = global_index();  // XCD-aware, as the forward kernels
   if (i64 >= M.T) return;
   const int i = (int)i64;
   const int64_t T = M.T;
@@ -330,7 +330,7 @@ __global__ void __launch_bounds__(256) k_tl_drain(DevMesh M, DevState S, DevStat
 __global__ void __launch_bounds__(256) k_tl_update(DevMesh M, DevState S, DevState Sd, DevState U0,
                                                    DevState U0d, DevOut O, DevOut Od, DevWork W, DevTan Q,
                                                    StageArgs A) {
-  const int64_t i64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i64 = global_index();  // XCD-aware, as the forward kernels
   if (i64 >= M.T) return;
   const int i = (int)i64;
   const int64_t T = M.T;
@@ -413,26 +413,22 @@ __global__ void __launch_bounds__(256) k_tl_update(DevMesh M, DevState S, DevSta
   Od.w[i] = ow.d[0]; Od.u[i] = ou.d[0]; Od.v[i] = ov.d[0];
 }
 
-static inline dim3 grid_plain(int64_t n) {
-  int64_t nb = (n + kBlock - 1) / kBlock;
-  return dim3((unsigned)(nb > 0 ? nb : 1));
-}
 
 hipError_t launch_tangent_stage(const DevMesh& M, const DevState& S, const DevState& Sd, const DevState& U0,
                                 const DevState& U0d, const DevOut& O, const DevOut& Od, const DevWork& W,
                                 const DevTan& Q, const StageArgs& A, int flux, int ws, int opts,
                                 hipStream_t st) {
-  hipLaunchKernelGGL(k_tl_recon, grid_plain(M.T), dim3(256), 0, st, M, S, Sd, W, Q);
-  hipLaunchKernelGGL(k_tl_recon2, grid_plain(M.T), dim3(256), 0, st, M, S, Sd, W, Q);
+  hipLaunchKernelGGL(k_tl_recon, grid_for(M.T), dim3(256), 0, st, M, S, Sd, W, Q);
+  hipLaunchKernelGGL(k_tl_recon2, grid_for(M.T), dim3(256), 0, st, M, S, Sd, W, Q);
 #define SWE_TL_FLUX(F, Wv)                                                                         \
   if (flux == F && ws == Wv)                                                                       \
-    hipLaunchKernelGGL((k_tl_flux<F, Wv>), grid_plain(M.E), dim3(256), 0, st, M, S, Sd, W, Q, A.cor);
+    hipLaunchKernelGGL((k_tl_flux<F, Wv>), grid_for(M.E), dim3(256), 0, st, M, S, Sd, W, Q, A.cor);
   SWE_TL_FLUX(0, 0) SWE_TL_FLUX(0, 1) SWE_TL_FLUX(0, 2)
   SWE_TL_FLUX(1, 0) SWE_TL_FLUX(1, 1) SWE_TL_FLUX(1, 2)
 #undef SWE_TL_FLUX
-  if (opts & kOptDrainLength) hipLaunchKernelGGL(k_tl_drain<true>, grid_plain(M.T), dim3(256), 0, st, M, S, Sd, W, Q);
-  else hipLaunchKernelGGL(k_tl_drain<false>, grid_plain(M.T), dim3(256), 0, st, M, S, Sd, W, Q);
-  hipLaunchKernelGGL(k_tl_update, grid_plain(M.T), dim3(256), 0, st, M, S, Sd, U0, U0d, O, Od, W, Q, A);
+  if (opts & kOptDrainLength) hipLaunchKernelGGL(k_tl_drain<true>, grid_for(M.T), dim3(256), 0, st, M, S, Sd, W, Q);
+  else hipLaunchKernelGGL(k_tl_drain<false>, grid_for(M.T), dim3(256), 0, st, M, S, Sd, W, Q);
+  hipLaunchKernelGGL(k_tl_update, grid_for(M.T), dim3(256), 0, st, M, S, Sd, U0, U0d, O, Od, W, Q, A);
   return hipGetLastError();
 }
 
