@@ -168,6 +168,7 @@ __device__ inline void slot_bar(const Muscl& m, double cx, double cy, const V3& 
 // k_adj_flux weighs each edge side), the draining-dt adjoints (own,
 // neighbours'), the MUSCL adjoint, the U0 adjoint, the first part of the S
 // adjoint and the corner z adjoint.
+template <bool NT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_adj_update(DevMesh M, DevState S, DevState U0, DevWork W,
                                                     StageArgs A, DevAdj J, const double* wnb_in,
                                                     int u0mode) {
@@ -302,9 +303,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     bcb -= hhb;
   }
   double zc[3] = {0., 0., 0.};
-  J.rb[i] = rb0;  // the flux adjoint of side k is -dtk sg cek rb (k_adj_flux)
-  J.rb[T + i] = rb1;
-  J.rb[2 * T + i] = rb2;
+  // outputs read only by later kernels: out_store (nontemporal on large meshes)
+  out_store<NT>(&J.rb[i], rb0);  // the flux adjoint of side k is -dtk sg cek rb (k_adj_flux)
+  out_store<NT>(&J.rb[T + i], rb1);
+  out_store<NT>(&J.rb[2 * T + i], rb2);
   for (int k = 0; k < 3; ++k) {
     const bool own = sgn_pos(fl, k);
     const double sg = own ? 1. : -1.;
@@ -331,10 +333,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     } else {
       if (dtik[k] < dts) nbb = dtkb;
     }
-    if (dtik[k] < dts) J.dtd_nb[k * T + i] = nbb;
+    if (dtik[k] < dts) out_store<NT>(&J.dtd_nb[k * T + i], nbb);
   }
   (void)ek;
-  if (dti < dts) J.dtd_own[i] = dtib;
+  if (dti < dts) out_store<NT>(&J.dtd_own[i], dtib);
   // the update's own MUSCL adjoint, applied here where the reconstruction's
   // reverse is a fixed map: the level to the cell's w (full-wet cells, and
   // partially wet ones above every vertex, which pass 2 leaves at w) or to
@@ -350,16 +352,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     su += mb[3];
     sv += mb[4];
   }
-  if (listed) J.m_cell[i] = mb[0];
+  if (listed) out_store<NT>(&J.m_cell[i], mb[0]);
   if (cls != 1 || listed) {
-    J.m_cell[T + i] = mb[1];
-    J.m_cell[2 * T + i] = mb[2];
+    out_store<NT>(&J.m_cell[T + i], mb[1]);
+    out_store<NT>(&J.m_cell[2 * T + i], mb[2]);
   }
   const double cthird = 1. / 3.;
-  for (int k = 0; k < 3; ++k) J.zc[k * T + i] += zc[k] + cthird * bcb;  // over every stage
-  J.sbar[i] = sw;
-  J.sbar[T + i] = su;
-  J.sbar[2 * T + i] = sv;
+  for (int k = 0; k < 3; ++k)  // over every stage
+    out_store<NT>(&J.zc[k * T + i], J.zc[k * T + i] + (zc[k] + cthird * bcb));
+  out_store<NT>(&J.sbar[i], sw);
+  out_store<NT>(&J.sbar[T + i], su);
+  out_store<NT>(&J.sbar[2 * T + i], sv);
 }
 
 // ---------------------------------------------------------------- drain --
@@ -410,7 +413,7 @@ __device__ inline bool drain_bar(const DevMesh& M, const DevState& S, const DevW
 // flux in the cell's update, plus the draining term on the mass flux when
 // the cell's draining dt was active; the side of the cell's local edge 0
 // also stores the draining depth term for k_adj_recon.
-template <bool LEN>
+template <bool LEN, bool NT>
 __device__ inline void side_flux_bar(const DevMesh& M, const DevState& S, const DevWork& W, const DevAdj& J,
                                      int c, int o, int k, double sg, double f0, double le, double dts,
                                      double fb[3]) {
@@ -426,7 +429,7 @@ __device__ inline void side_flux_bar(const DevMesh& M, const DevState& S, const 
   if (dtc < dts) {
     double sb, hb, out[3], len[3];
     const bool act = drain_bar<LEN>(M, S, W, J, c, sb, hb, out, len);
-    if (k == 0) J.drain_h[c] = act ? hb : 0.;  // once per cell, from its local edge 0
+    if (k == 0) out_store<NT>(&J.drain_h[c], act ? hb : 0.);  // once per cell, from its local edge 0
     if (act && out[k] > 0.) fb[0] += sb * len[k] * sg;
   }
 }
@@ -437,7 +440,7 @@ __device__ inline void side_flux_bar(const DevMesh& M, const DevState& S, const 
 // (inputs L (w, u, v), R (w, u, v); edge z by the chain rule), into the
 // adjoints of the two slot values; k_adj_recon takes them through the slot
 // reconstruction.
-template <int FLUX, int WS, bool LEN>
+template <int FLUX, int WS, bool LEN, bool NT>
 __global__ void __launch_bounds__(256) k_adj_flux(DevMesh M, DevState S, DevWork W, DevAdj J, double dts) {
   const int64_t e64 = global_index();  // XCD-aware, as the forward kernels
   if (e64 >= M.E) return;
@@ -448,9 +451,9 @@ __global__ void __launch_bounds__(256) k_adj_flux(DevMesh M, DevState S, DevWork
   const double le = edge_len(Pa, Pb);
   const double f0 = W.f0[e];
   double Fb[3], fb[3];
-  side_flux_bar<LEN>(M, S, W, J, f, t, J.eloc[e], 1., f0, le, dts, Fb);
+  side_flux_bar<LEN, NT>(M, S, W, J, f, t, J.eloc[e], 1., f0, le, dts, Fb);
   if (t >= 0) {
-    side_flux_bar<LEN>(M, S, W, J, t, f, J.eloc[Ecnt + e], -1., f0, le, dts, fb);
+    side_flux_bar<LEN, NT>(M, S, W, J, t, f, J.eloc[Ecnt + e], -1., f0, le, dts, fb);
     for (int q = 0; q < 3; ++q) Fb[q] += fb[q];
   }
   double g[7] = {0., 0., 0., 0., 0., 0., 0.};
@@ -487,8 +490,8 @@ __global__ void __launch_bounds__(256) k_adj_flux(DevMesh M, DevState S, DevWork
   // the edge z term g[6] = -(g[0] + g[3]) is formed by k_adj_recon from the
   // two w slots (free flow: g[3] is merged into g[0], the sum unchanged)
   for (int q = 0; q < 3; ++q) {
-    J.sl_edge[(q * 2 + 0) * Ecnt + e] = g[q];
-    J.sl_edge[(q * 2 + 1) * Ecnt + e] = g[3 + q];
+    out_store<NT>(&J.sl_edge[(q * 2 + 0) * Ecnt + e], g[q]);
+    out_store<NT>(&J.sl_edge[(q * 2 + 1) * Ecnt + e], g[3 + q]);
   }
 }
 
@@ -568,6 +571,7 @@ __global__ void __launch_bounds__(256) k_adj_pwnode(DevMesh M, DevState S, DevWo
 // Reverse of k_recon1 / k_recon2 for cell i: gathers the MUSCL adjoint of
 // the cell (its own update and the slots of its three edges), then maps it
 // to the w adjoints of the cell and its neighbours and to corner z.
+template <bool NT>
 __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWork W, DevAdj J, double dts) {
   const int64_t i64 = global_index();  // XCD-aware, as the forward kernels
   if (i64 >= M.T) return;
@@ -761,10 +765,10 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
       }
     }
   }
-  for (int k = 0; k < 3; ++k) J.w_nb[k * T + i] = wnb[k];
-  J.sbar[i] += wb;
+  for (int k = 0; k < 3; ++k) out_store<NT>(&J.w_nb[k * T + i], wnb[k]);
+  out_store<NT>(&J.sbar[i], J.sbar[i] + wb);
   const double c = 1. / 3.;
-  for (int k = 0; k < 3; ++k) J.zc[k * T + i] += zc[k] + c * bcb;
+  for (int k = 0; k < 3; ++k) out_store<NT>(&J.zc[k * T + i], J.zc[k * T + i] + (zc[k] + c * bcb));
 }
 
 // the neighbours' contributions to the w adjoint of cell c (after the last
@@ -826,10 +830,12 @@ hipError_t launch_adjoint_stage(const DevMesh& M, const DevState& S, const DevSt
                                 const StageArgs& A, const DevAdj& J, int flux, int ws, int opts,
                                 const double* wnb_in, int u0mode, unsigned int npw, hipStream_t st) {
   const double dts = A.coef * A.dt_fixed;
-  hipLaunchKernelGGL(k_adj_update, grid_for(M.T), dim3(256), 0, st, M, S, U0, W, A, J, wnb_in, u0mode);
+  if (M.nt) hipLaunchKernelGGL(k_adj_update<true>, grid_for(M.T), dim3(256), 0, st, M, S, U0, W, A, J, wnb_in, u0mode);
+  else hipLaunchKernelGGL(k_adj_update<false>, grid_for(M.T), dim3(256), 0, st, M, S, U0, W, A, J, wnb_in, u0mode);
 #define SWE_ADJ_FLUX(F, Wv, L)                                                                   \
   if (flux == F && ws == Wv && len == L)                                                         \
-    hipLaunchKernelGGL((k_adj_flux<F, Wv, L>), grid_for(M.E), dim3(256), 0, st, M, S, W, J, dts);
+    { if (M.nt) hipLaunchKernelGGL((k_adj_flux<F, Wv, L, true>), grid_for(M.E), dim3(256), 0, st, M, S, W, J, dts); \
+      else hipLaunchKernelGGL((k_adj_flux<F, Wv, L, false>), grid_for(M.E), dim3(256), 0, st, M, S, W, J, dts); }
   for (int li = 0; li < 2; ++li) {
     const bool len = li == 1;
     if (len != ((opts & kOptDrainLength) != 0)) continue;
@@ -846,7 +852,8 @@ hipError_t launch_adjoint_stage(const DevMesh& M, const DevState& S, const DevSt
     hipLaunchKernelGGL(k_adj_pw, g, dim3(256), 0, st, M, S, W, J);
     hipLaunchKernelGGL(k_adj_pwnode, g, dim3(256), 0, st, M, S, W, J);
   }
-  hipLaunchKernelGGL(k_adj_recon, grid_for(M.T), dim3(256), 0, st, M, S, W, J, dts);
+  if (M.nt) hipLaunchKernelGGL(k_adj_recon<true>, grid_for(M.T), dim3(256), 0, st, M, S, W, J, dts);
+  else hipLaunchKernelGGL(k_adj_recon<false>, grid_for(M.T), dim3(256), 0, st, M, S, W, J, dts);
   return hipGetLastError();
 }
 

@@ -261,7 +261,8 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
   // the whole mesh is the visited window; cinst / bdt / bactive are set for
   // ensembles below
   s->M = DevMesh{dpx, dpy, dpz, dtp, dte, dtt, dfl, dep0, dep1, det0, det1, doff, dn2c, T, E, V,
-                 G.cx, G.cy, G.bc, G.area, G.edmin, G.crec, nullptr, nullptr, nullptr, 0, T, 0, E};
+                 G.cx, G.cy, G.bc, G.area, G.edmin, G.crec, nullptr, nullptr, nullptr, 0, T, 0, E,
+                 T >= (int64_t(1) << 20) ? 1 : 0};
   s->G = G;
   if ((e = launch_setup_geometry(s->M, G, s->stream)) != hipSuccess) return bail(e, "setup_geometry");
   if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) return bail(e, "hipStreamSynchronize");

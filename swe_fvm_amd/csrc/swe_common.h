@@ -74,6 +74,15 @@ __device__ __forceinline__ Slot slot_value(const Muscl& m, double cx, double cy,
 
 // grid of 256-thread workgroups covering n items, a multiple of 8 (one
 // contiguous range per XCD, see logical_block)
+// a store of a kernel output; NT (a template parameter: a runtime branch
+// between the two forms is merged into a plain store) chosen at launch from
+// DevMesh::nt
+template <bool NT>
+__device__ __forceinline__ void out_store(double* p, double v) {
+  if (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 inline dim3 grid_for_bs(int64_t n, int64_t bs) {
   int64_t nb = (n + bs - 1) / bs;
   nb = ((nb + 7) / 8) * 8;

@@ -74,6 +74,10 @@ struct DevMesh {
   // cells [c0, c1) and edges [e0, e1) the step kernels visit (the whole mesh
   // except when finished ensemble members are trimmed off the front)
   int64_t c0, c1, e0, e1;
+  // outputs that only later kernels read are stored nontemporally (meshes of
+  // 2^20 cells or more: a stage's data exceeds the 256 MB Infinity Cache;
+  // smaller ones keep them cached for the next kernel)
+  int nt;
 };
 
 // per-instance control of a batched ensemble
