@@ -871,9 +871,9 @@ int swe_autotune(swe_solver* s, int reps, int* flux_items, int* update_items, do
   const int fi[3] = {1, 2, 4}, ui[2] = {1, 2}, bs[2] = {256, 512};
   double t[5][2];
   int rc = SWE_OK;
-  // round 0 only warms every variant up (clocks, caches), so that the
+  // pass 0 only warms every variant up (clocks, caches), so that the
   // first one timed is not the one that pays for it; each variant keeps the
-  // faster of rounds 1 and 2
+  // faster of passes 1 and 2
   for (int q = 0; q < 5; ++q) t[q][0] = t[q][1] = 1e300;
   auto timed = [&](auto&& launch, double& out, bool keep) {
     hipError_t e = hipEventRecord(e0, s->stream);
@@ -888,11 +888,11 @@ int swe_autotune(swe_solver* s, int reps, int* flux_items, int* update_items, do
   };
   // Euler-form stage with a small fixed step into the free buffer
   StageArgs A{0., 0., 1., 1e-6, 0., s->cor, rslot, nullptr, friction(s)};
-  for (int round = 0; round < 3 && !rc; ++round) {
+  for (int pass = 0; pass < 3 && !rc; ++pass) {
     for (int q = 0; q < 3 && !rc; ++q)
       for (int b = 0; b < 2 && !rc; ++b)
         rc = timed([&] { return launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, rslot, fi[q], bs[b], s->stream); },
-                   t[q][b], round > 0);
+                   t[q][b], pass > 0);
     if (!rc) {
       hipError_t e = launch_drain(s->M, S, s->W, s->opts, s->stream);
       if (e != hipSuccess) rc = dev_fail(e, "drain");
@@ -901,7 +901,7 @@ int swe_autotune(swe_solver* s, int reps, int* flux_items, int* update_items, do
       for (int b = 0; b < 2 && !rc; ++b)
         rc = timed([&] {
           return launch_update(s->M, S, S, out_of(s, scratch), s->W, A, ui[q], bs[b], s->stream);
-        }, t[3 + q][b], round > 0);
+        }, t[3 + q][b], pass > 0);
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
