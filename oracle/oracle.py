@@ -74,7 +74,8 @@ def lib():
     L.orc_bisection_cubic.restype = _d
     L.orc_bisection_cubic.argtypes = [_d, _d, _d, _d, _d]
     L.orc_gradient.argtypes = [_p, _p]
-    L.orc_flux_states.argtypes = [_int, _int, _p, _d, _d, _d, _d, _d, _d, _p]
+    L.orc_flux_edge.restype = _int
+    L.orc_flux_edge.argtypes = [_int, _int, _d, _p, _p, _p, _int, _p, _p, _p]
     L.orc_wavespeed.argtypes = [_int, _d, _d, _d, _d, _p]
     _lib = L
     return L
@@ -211,11 +212,22 @@ def gradient(p):
     return g
 
 
-def flux_states(flux, wave, n, hl, ul, vl, hr, ur, vr):
-    n = np.ascontiguousarray(n, dtype=np.float64)
-    F = np.empty(3, np.float64)
-    lib().orc_flux_states(FLUX[flux], WAVE[wave], _ptr(n), hl, ul, vl, hr, ur, vr, _ptr(F))
-    return F
+def flux_edge(flux, wave, xyz, left, right=None, to_tag=0, cor=0.0):
+    """Flux of the edge A B of a two-cell mesh through the oracle's
+    ComputeFluxes (Solver::HLL / HLLC, walls, free flow).  xyz: 4 x 3 nodes
+    A, B, C, D (cells A B C = `from` and B A D = `to`, counter-clockwise);
+    left / right: primitive (w, u, v) edge states; to_tag < 0: boundary
+    edge of that tag with no `to` cell (xyz then needs 3 rows).
+    Returns (F, n, r): flux, Norm(e, from) and the pass's CFL length."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    Lp = np.ascontiguousarray(left, dtype=np.float64)
+    Rp = np.ascontiguousarray(left if right is None else right, dtype=np.float64)
+    F, n, r = np.empty(3), np.empty(2), np.empty(1)
+    rc = lib().orc_flux_edge(FLUX[flux], WAVE[wave], cor, _ptr(xyz), _ptr(Lp), _ptr(Rp), to_tag,
+                             _ptr(F), _ptr(n), _ptr(r))
+    if rc != 0:
+        raise ValueError("orc_flux_edge: bad two-cell mesh")
+    return F, n, float(r[0])
 
 
 def wavespeed(wave, ul, hl, ur, hr):

@@ -946,47 +946,58 @@ double orc_bisection_cubic(double b, double c, double d, double xmin, double xma
 void orc_gradient(const double* p /*3x3 rows = points*/, double* g) {
   Gradient({p[0], p[1], p[2]}, {p[3], p[4], p[5]}, {p[6], p[7], p[8]}, g);
 }
-// Numerical flux of one edge from explicit left/right primitive states
-// (w, u, v) at the edge, with edge bathymetry be, tangent-derived normal of
-// `from`.  Used to pin HLL/HLLC against textbook formulas.
-void orc_flux_states(int flux, int ws, const double* n, double hl, double ul_x, double vl_y,
-                     double hr, double ur_x, double vr_y, double* F) {
+// Numerical flux of one edge through the solver's own ComputeFluxes /
+// Solver::HLL / Solver::HLLC (src/SpaceDisc.cpp:54-74, include/Fluxes.h:14-111)
+// on a two-cell mesh: xyz holds 4 nodes (x, y, z rows) A, B (the edge), C
+// (third vertex of the `from` cell, triangle A B C) and D (third vertex of
+// the `to` cell, triangle B A D); both must be counter-clockwise.  The edge
+// slots are set to the given primitive states Lp (seen from `from`) and Rp;
+// to_tag < 0 drops the `to` cell and tags the edge (SOLID_WALL, FREE_FLOW),
+// and a wall's flux then reads the `from` cell's average Lp.  Returns the
+// flux, the normal Norm(e, from) and the CFL length r of that pass.
+int orc_flux_edge(int flux, int ws, double cor, const double* xyz, const double* Lp,
+                  const double* Rp, int to_tag, double* F, double* n_out, double* r_out) {
   Solver s;
-  s.flux = flux; s.ws = ws;
-  double t[2] = {-n[1], n[0]};  // n = (t_y, -t_x)
-  double ul = ul_x * n[0] + vl_y * n[1], vl = ul_x * t[0] + vl_y * t[1];
-  double ur = ur_x * n[0] + vr_y * n[1], vr = ur_x * t[0] + vr_y * t[1];
-  F[0] = F[1] = F[2] = 0.;
-  if (hl + hr <= 1e-10) return;
-  double a[2];
-  s.Wavespeed(ul, hl, ur, hr, a);
-  double Ul[3] = {hl, hl * ul_x, hl * vl_y}, Ur[3] = {hr, hr * ur_x, hr * vr_y};
-  if (flux == FLUX_HLL) {
-    double al = dmin(0., a[0]), ar = dmax(0., a[1]);
-    if (ar - al <= 1e-10) return;
-    double Fl[3], Fr[3];
-    Solver::ElemFlux(n, Ul, Fl); Solver::ElemFlux(n, Ur, Fr);
-    for (int c = 0; c < 3; ++c)
-      F[c] = (ar * Fl[c] - al * Fr[c] + (al * ar) * (Ur[c] - Ul[c])) / (ar - al);
-    return;
+  s.flux = flux; s.ws = ws; s.cor = cor;
+  Mesh& m = s.m;
+  const bool inner = to_tag >= 0;
+  m.V = inner ? 4 : 3;
+  for (Idx i = 0; i < m.V; ++i) {
+    m.px.push_back(xyz[3 * i]); m.py.push_back(xyz[3 * i + 1]); m.pz.push_back(xyz[3 * i + 2]);
   }
-  double al = a[0], ar = a[1];
-  double ustar = (ar - ur) * hr * ur - (al - ul) * hl * ul + 0.5 * (hl * hl - hr * hr);
-  ustar /= (hr * (ar - ur) - hl * (al - ul));
-  double Fe[3];
-  if (ustar <= 0) {
-    double us = vr * t[0] + ustar * t[1], vs = vr * n[0] + ustar * n[1];
-    double hs = hr * (ar - ur) / (ar - ustar);
-    double Us[3] = {hs, hs * us, hs * vs};
-    Solver::ElemFlux(n, Ur, Fe);
-    for (int c = 0; c < 3; ++c) F[c] = Fe[c] + dmax(0., ar) * (Us[c] - Ur[c]);
-  } else {
-    double us = vl * t[0] + ustar * t[1], vs = vl * n[0] + ustar * n[1];
-    double hs = hl * (al - ul) / (al - ustar);
-    double Us[3] = {hs, hs * us, hs * vs};
-    Solver::ElemFlux(n, Ul, Fe);
-    for (int c = 0; c < 3; ++c) F[c] = Fe[c] + dmin(0., al) * (Us[c] - Ul[c]);
+  // the later cell of an interior edge is its et0 (the `from` cell): list
+  // B A D first
+  const Idx tri2[6] = {1, 0, 3, 0, 1, 2}, tri1[3] = {0, 1, 2};
+  const Idx bnd2[8] = {1, 3, 3, 0, 1, 2, 2, 0}, bnd1[6] = {0, 1, 1, 2, 2, 0};
+  if (inner) BuildTopology(m, tri2, 2, bnd2, 4);
+  else BuildTopology(m, tri1, 1, bnd1, 3);
+  Idx e = -1;
+  for (Idx q = 0; q < m.E; ++q)
+    if (std::min(m.ep[2 * q], m.ep[2 * q + 1]) == 0 && std::max(m.ep[2 * q], m.ep[2 * q + 1]) == 1) e = q;
+  const Idx from = inner ? 1 : 0, to = inner ? 0 : to_tag;
+  if (e < 0 || m.et[2 * e] != from || (inner && m.et[2 * e + 1] != to)) return -1;
+  if (!inner) {
+    if (to_tag < FREE_FLOW) return -1;
+    m.et[2 * e + 1] = to_tag;
+    m.tt[3 * from + 0] = to_tag;  // edge (A, B) is local edge 0 of A B C
   }
+  s.vol.assign(3 * m.T, 0.);
+  s.edg.assign(6 * m.E, 0.);
+  s.src.assign(6 * m.E, 0.);
+  s.f.assign(3 * m.E, 0.);
+  for (int c = 0; c < 3; ++c) s.vol[3 * from + c] = Lp[c];
+  if (inner)
+    for (int c = 0; c < 3; ++c) s.vol[3 * to + c] = Rp[c];
+  const Idx sl = Solver::Slot(e, from, to), sr = inner ? Solver::Slot(e, to, from) : sl;
+  for (int c = 0; c < 3; ++c) {
+    s.edg[3 * sl + c] = Lp[c];
+    if (inner) s.edg[3 * sr + c] = Rp[c];
+  }
+  s.ComputeFluxes();
+  for (int c = 0; c < 3; ++c) F[c] = s.f[3 * e + c];
+  s.Norm(e, from, n_out);
+  *r_out = s.r;
+  return 0;
 }
 void orc_wavespeed(int ws, double ul, double hl, double ur, double hr, double* a) {
   Solver s;

@@ -106,6 +106,65 @@ def test_friction_decays_a_uniform_stream():
     assert np.array_equal(a.state(), b.state())
 
 
+def _coriolis_map(scheme, k):
+    """Amplification matrix of one step on (hu, hv) for the source
+    cor (h v, -h u) alone: M = I + k J with k = cor dt, J = [[0, 1], [-1, 0]];
+    Shu-Osher stages (src/Solvers.cpp:5-59) combine it into M (Euler),
+    (I + M^2) / 2 (SSPRK2) and I / 3 + M / 2 + M^3 / 6 (SSPRK3)."""
+    I = np.eye(2)
+    M = I + k * np.array([[0.0, 1.0], [-1.0, 0.0]])
+    if scheme == "Euler":
+        return M
+    if scheme == "SSPRK2":
+        return 0.5 * (I + M @ M)
+    return I / 3 + 0.5 * M + (M @ M @ M) / 6
+
+
+def _coriolis_case():
+    xyz, topo = _flat(10, 7)
+    ff = _tagged(topo, swe.FREE_FLOW)
+    return xyz, ff, _uniform(topo)
+
+
+@pytest.mark.parametrize("opts", [0, O.OPT_DRAIN_LENGTH])
+@pytest.mark.parametrize("scheme", ["Euler", "SSPRK2", "SSPRK3"])
+def test_coriolis_rotates_a_uniform_stream(scheme, opts):
+    """The Coriolis source (src/SpaceDisc.cpp:29, src/TimeDisc.cpp:34) on a
+    uniform stream over a flat bottom through free-flow edges: the fluxes
+    cancel, the depth stays put and (u, v) follows the closed-form map of
+    each scheme to 1e-13; the opposite sign of cor rotates the other way."""
+    xyz, ff, prim = _coriolis_case()
+    cor, dt, n = 0.9, 2e-3, 25
+    s = O.OracleSolver(xyz, ff.tables(), "HLLC", "Einfeldt", cor=cor, opts=opts)
+    s.set_state(prim)
+    s.run(scheme, n, dt=dt)
+    st = s.state()
+    want = np.linalg.matrix_power(_coriolis_map(scheme, cor * dt), n) @ prim[0, 1:]
+    assert np.allclose(st[:, 0], 1.0, rtol=0, atol=1e-13)
+    assert np.max(np.abs(st[:, 1:] - want)) <= 1e-13 * np.abs(want).max()
+    wrong = np.linalg.matrix_power(_coriolis_map(scheme, -cor * dt), n) @ prim[0, 1:]
+    assert np.max(np.abs(st[:, 1:] - wrong)) > 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme", ["Euler", "SSPRK2", "SSPRK3"])
+def test_gpu_coriolis_rotates_a_uniform_stream(scheme):
+    xyz, ff, prim = _coriolis_case()
+    cor, dt, n = 0.9, 2e-3, 25
+    fl = swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt]
+    sd = swe.SpaceDisc(fl, swe.Domain(xyz, ff), prim, cor=cor)
+    sd.run(scheme, n, dt=dt)
+    g = sd.GetVolField()
+    want = np.linalg.matrix_power(_coriolis_map(scheme, cor * dt), n) @ prim[0, 1:]
+    print(f"{scheme}: max |(u, v) - closed form| {np.max(np.abs(g[:, 1:] - want)):.2e}", flush=True)
+    assert np.allclose(g[:, 0], 1.0, rtol=0, atol=1e-13)
+    assert np.max(np.abs(g[:, 1:] - want)) <= 1e-13 * np.abs(want).max()
+    orc = O.OracleSolver(xyz, ff.tables(), "HLLC", "Einfeldt", cor=cor)
+    orc.set_state(prim)
+    orc.run(scheme, n, dt=dt)
+    assert np.array_equal(g, orc.state())
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("drain_len", [False, True])
 @pytest.mark.parametrize("scheme", ["Euler", "SSPRK2", "SSPRK3"])

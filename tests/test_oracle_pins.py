@@ -135,64 +135,126 @@ def _rotate_back(Fr, n):
     return np.array([Fr[0], Fr[1] * n[0] + Fr[2] * t[0], Fr[1] * n[1] + Fr[2] * t[1]])
 
 
+def _edge_geometry(rng, ang=None):
+    """Nodes A, B (the edge), C (`from` side) and D (`to` side) of a two-cell
+    mesh around a random edge; the `from` cell lies on the -n side."""
+    ang = rng.uniform(0, 2 * np.pi) if ang is None else ang
+    t = np.array([math.cos(ang), math.sin(ang)])
+    n = np.array([t[1], -t[0]])
+    L = rng.uniform(0.05, 2.0)
+    M = rng.uniform(-3, 3, 2)
+    z = rng.uniform(-1, 1, 4)
+    A, B = M - 0.5 * L * t, M + 0.5 * L * t
+    C, D = M - 0.6 * L * n, M + 0.4 * L * n
+    return np.array([[*A, z[0]], [*B, z[1]], [*C, z[2]], [*D, z[3]]]), L
+
+
+def _area(p, q, r):
+    return 0.5 * abs((q[0] - p[0]) * (r[1] - p[1]) - (r[0] - p[0]) * (q[1] - p[1]))
+
+
 @pytest.mark.parametrize("wave", ["Rusanov", "Davis", "Einfeldt"])
 def test_flux_against_textbook(wave):
+    """Solver::HLL / HLLC (include/Fluxes.h:14-111), reached through the
+    oracle's ComputeFluxes on a two-cell mesh with prescribed edge slots,
+    against Toro's HLL and HLLC on distinct left / right states, and the CFL
+    length each pass leaves in r against min(2 A_l / L, 2 A_r / L) / speed.
+    Einfeldt is compared with the textbook Roe average where the reference's
+    weight bug vanishes (equal depths) and with the reference's formula
+    (src/Fluxes.cpp:22) elsewhere."""
     rng = np.random.default_rng(4)
-    for _ in range(400):
-        ang = rng.uniform(0, 2 * np.pi)
-        n = np.array([math.cos(ang), math.sin(ang)])
-        t = np.array([-n[1], n[0]])
+    for it in range(400):
+        xyz, L = _edge_geometry(rng)
+        be = 0.5 * (xyz[0, 2] + xyz[1, 2])
         hl, hr = rng.uniform(0.05, 2.0, 2)
+        bug = True
+        if wave == "Einfeldt" and it % 2 == 0:
+            hr, bug = hl, False
         vel_l, vel_r = rng.normal(0, 0.6, 2), rng.normal(0, 0.6, 2)
-        ul, vl = vel_l @ n, vel_l @ t
-        ur, vr = vel_r @ n, vel_r @ t
-        if wave == "Einfeldt":
-            # the Roe-weight bug vanishes for equal depths: compare there
-            hr = hl
-        SL, SR = _wave_textbook(wave, ul, hl, ur, hr, roe_bug=False)
-        a = O.wavespeed(wave, ul, hl, ur, hr)
-        assert abs(a[0] - SL) < 1e-13 and abs(a[1] - SR) < 1e-13
-        FL, FR = _physical(hl, ul, vl), _physical(hr, ur, vr)
-        UL, UR = np.array([hl, hl * ul, hl * vl]), np.array([hr, hr * ur, hr * vr])
-        ref_hll = _rotate_back(_hll_textbook(SL, SR, FL, FR, UL, UR), n)
-        got = O.flux_states("HLL", wave, n, hl, vel_l[0], vel_l[1], hr, vel_r[0], vel_r[1])
-        assert np.allclose(got, ref_hll, rtol=1e-12, atol=1e-12)
-        ref_hllc = _rotate_back(_hllc_textbook(SL, SR, hl, ul, vl, hr, ur, vr), n)
-        got = O.flux_states("HLLC", wave, n, hl, vel_l[0], vel_l[1], hr, vel_r[0], vel_r[1])
-        assert np.allclose(got, ref_hllc, rtol=1e-11, atol=1e-12)
+        cor = 0.0 if it % 3 else rng.uniform(-2, 2)
+        left, right = [hl + be, *vel_l], [hr + be, *vel_r]
+        for flux in ("HLL", "HLLC"):
+            got, n, r = O.flux_edge(flux, wave, xyz, left, right, cor=cor)
+            tt = np.array([-n[1], n[0]])
+            ul, vl = vel_l @ n, vel_l @ tt
+            ur, vr = vel_r @ n, vel_r @ tt
+            SL, SR = _wave_textbook(wave, ul, hl, ur, hr, roe_bug=bug)
+            a = O.wavespeed(wave, ul, hl, ur, hr)
+            assert abs(a[0] - SL) < 1e-13 and abs(a[1] - SR) < 1e-13
+            FL, FR = _physical(hl, ul, vl), _physical(hr, ur, vr)
+            UL, UR = np.array([hl, hl * ul, hl * vl]), np.array([hr, hr * ur, hr * vr])
+            if flux == "HLL":
+                ref = _rotate_back(_hll_textbook(SL, SR, FL, FR, UL, UR), n)
+                speed = max(-min(0.0, SL), max(0.0, SR))
+            else:
+                ref = _rotate_back(_hllc_textbook(SL, SR, hl, ul, vl, hr, ur, vr), n)
+                speed = max(1e-13, max(SL, SR))  # D10: include/Fluxes.h:89
+            assert np.allclose(got, ref, rtol=1e-11, atol=1e-12), (flux, it)
+            d = min(2 * _area(*xyz[[0, 1, 2], :2]) / L, 2 * _area(*xyz[[1, 0, 3], :2]) / L)
+            assert abs(r - min(1.0, d / (abs(cor) + speed))) <= 1e-12 * r, (flux, it)
 
 
-def test_einfeldt_roe_weight_is_reference_formula():
-    """src/Fluxes.cpp:22 weights ur by cl: with unequal depths and velocities
-    the speeds differ from the textbook Roe average in exactly that way."""
-    ul, hl, ur, hr = 0.3, 0.5, -0.2, 1.7
-    a = O.wavespeed("Einfeldt", ul, hl, ur, hr)
-    SL, SR = _wave_textbook("Einfeldt", ul, hl, ur, hr, roe_bug=True)
-    assert abs(a[0] - SL) < 1e-15 and abs(a[1] - SR) < 1e-15
-    SLt, SRt = _wave_textbook("Einfeldt", ul, hl, ur, hr, roe_bug=False)
-    assert abs(a[0] - SLt) > 1e-3 or abs(a[1] - SRt) > 1e-3
+def test_flux_cfl_length_and_orientation_from_the_solver():
+    """Norm(e, from) points from the `from` cell into the `to` cell
+    (src/Bathymetry.cpp:69-80), and swapping the two states reverses the
+    mass flux of a symmetric wave speed (Davis) exactly."""
+    rng = np.random.default_rng(9)
+    for _ in range(50):
+        xyz, L = _edge_geometry(rng)
+        be = 0.5 * (xyz[0, 2] + xyz[1, 2])
+        t = (xyz[1, :2] - xyz[0, :2]) / L
+        left = [be + 0.7, 0.2, -0.1]
+        right = [be + 0.4, -0.3, 0.25]
+        F, n, _ = O.flux_edge("HLLC", "Davis", xyz, left, right)
+        assert np.allclose(n, [t[1], -t[0]], atol=1e-14)
+        # the `to` side of the edge holds D
+        assert (xyz[3, :2] - xyz[0, :2]) @ n > 0
+        G, _, _ = O.flux_edge("HLLC", "Davis", xyz[[1, 0, 3, 2]], right, left)
+        assert abs(F[0] + G[0]) <= 1e-14 * max(1.0, abs(F[0]))
 
 
 @pytest.mark.parametrize("flux", ["HLL", "HLLC"])
 @pytest.mark.parametrize("wave", ["Rusanov", "Davis", "Einfeldt"])
 def test_flux_consistency(flux, wave):
-    """F(U, U) equals the physical flux ElemFlux(n, U) (include/SpaceDisc.h:4-18)."""
+    """F(U, U) equals the physical flux ElemFlux(n, U) (include/SpaceDisc.h:4-18),
+    on an interior edge and on a free-flow edge (whose outside state is the
+    inside one)."""
     rng = np.random.default_rng(5)
     for _ in range(100):
-        ang = rng.uniform(0, 2 * np.pi)
-        n = np.array([math.cos(ang), math.sin(ang)])
+        xyz, _ = _edge_geometry(rng)
+        be = 0.5 * (xyz[0, 2] + xyz[1, 2])
         h = rng.uniform(0.01, 3.0)
         u, v = rng.normal(0, 0.5, 2)
+        got, n, _ = O.flux_edge(flux, wave, xyz, [h + be, u, v], [h + be, u, v])
         un = u * n[0] + v * n[1]
         ref = np.array([h * un, h * un * u + 0.5 * h * h * n[0], h * un * v + 0.5 * h * h * n[1]])
-        got = O.flux_states(flux, wave, n, h, u, v, h, u, v)
         assert np.allclose(got, ref, rtol=1e-12, atol=1e-14)
+        ff, n2, _ = O.flux_edge(flux, wave, xyz[:3], [h + be, u, v], to_tag=-2)
+        assert np.allclose(n2, n, atol=1e-15)
+        assert np.allclose(ff, ref, rtol=1e-12, atol=1e-14)
+
+
+def test_wall_flux_is_the_cell_pressure():
+    """A SOLID_WALL edge carries (0, 1/2 h^2 n) of the cell average h
+    (src/SpaceDisc.cpp:66-69), whatever the velocity."""
+    rng = np.random.default_rng(6)
+    for _ in range(50):
+        xyz, _ = _edge_geometry(rng)
+        xyz[:, 2] = 0.0  # flat cell: h = w
+        h = rng.uniform(0.01, 3.0)
+        F, n, r = O.flux_edge("HLLC", "Einfeldt", xyz[:3], [h, *rng.normal(0, 1, 2)], to_tag=-1)
+        assert F[0] == 0.0 and np.allclose(F[1:], 0.5 * h * h * n, rtol=1e-14)
+        assert r == 1.0  # walls leave the CFL length alone
 
 
 def test_dry_states_give_zero_flux():
-    n = np.array([0.6, 0.8])
-    assert np.all(O.flux_states("HLLC", "Einfeldt", n, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0) == 0.0)
-    assert np.all(O.flux_states("HLL", "Davis", n, 3e-11, 0.0, 0.0, 4e-11, 0.0, 0.0) == 0.0)
+    rng = np.random.default_rng(7)
+    xyz, _ = _edge_geometry(rng)
+    be = 0.5 * (xyz[0, 2] + xyz[1, 2])
+    F, _, r = O.flux_edge("HLLC", "Einfeldt", xyz, [be, 0.0, 0.0], [be, 0.0, 0.0])
+    assert np.all(F == 0.0) and r == 1.0
+    F, _, r = O.flux_edge("HLL", "Davis", xyz, [be + 3e-11, 0.0, 0.0], [be + 4e-11, 0.0, 0.0])
+    assert np.all(F == 0.0) and r == 1.0
 
 
 # --------------------------------------------------------- whole steps --
