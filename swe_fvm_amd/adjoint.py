@@ -1,5 +1,5 @@
 """Gradients of functionals of the final state with respect to the initial
-state and the node bathymetry, through the hand-written adjoint of the step
+state and the node (or per-cell) bathymetry, through the hand-written adjoint of the step
 (swe_adjoint_run): a torch.autograd.Function and the gauge-misfit loss of
 the bathymetry-inversion workload.
 
@@ -16,11 +16,45 @@ except ImportError:  # the numpy interface below works without torch
     torch = None
 
 
+class CellBathymetry:
+    """Per-cell bathymetry b (T values) on a mesh whose solver reads node
+    heights z (V values; the reference's Domain holds node z,
+    include/Bathymetry.h:43, and a cell's bathymetry is its centroid mean,
+    src/ValueField.cpp:8-10): z = M b with z_n the mean of b over the cells
+    incident to node n.  The gradient of a function of z with respect to b
+    is M^T dJ/dz."""
+
+    def __init__(self, tp, num_nodes):
+        tp = np.asarray(tp, dtype=np.int64).reshape(-1, 3)
+        self.T, self.V = tp.shape[0], int(num_nodes)
+        self.rows = tp.reshape(-1)                      # node of each corner
+        self.cols = np.repeat(np.arange(self.T), 3)     # cell of each corner
+        deg = np.bincount(self.rows, minlength=self.V).astype(np.float64)
+        if np.any(deg == 0):
+            raise ValueError("a node belongs to no cell")
+        self.inv_deg = 1.0 / deg
+
+    def to_nodes(self, b):
+        """z = M b"""
+        b = np.asarray(b, dtype=np.float64).reshape(self.T)
+        return np.bincount(self.rows, weights=b[self.cols], minlength=self.V) * self.inv_deg
+
+    def gradient(self, zbar):
+        """M^T zbar: the per-cell gradient from the node one"""
+        zbar = np.asarray(zbar, dtype=np.float64).reshape(self.V)
+        return np.bincount(self.cols, weights=(zbar * self.inv_deg)[self.rows], minlength=self.T)
+
+
 def run_and_adjoint(sd, prim0, z, nsteps, final_bar_fn, scheme="SSPRK2", dt=None, dts=None,
-                    checkpoint_every=50):
+                    checkpoint_every=50, cell_bathymetry=None):
     """Sets (prim0, z), runs nsteps steps, evaluates final_bar_fn(final prim)
-    -> (J, dJ/dfinal), and returns (J, dJ/dprim0, dJ/dz, [forward ms,
-    reverse ms])."""
+    -> (J, dJ/dprim0, dJ/dz, [forward ms, reverse ms]).  With
+    cell_bathymetry (a CellBathymetry of the mesh) z holds per-cell values
+    (node heights M z) and the gradient returned is the per-cell one."""
+    if cell_bathymetry is not None:
+        J, s0, zb, ms = run_and_adjoint(sd, prim0, cell_bathymetry.to_nodes(z), nsteps, final_bar_fn,
+                                        scheme, dt, dts, checkpoint_every)
+        return J, s0, cell_bathymetry.gradient(zb), ms
     sd.set_bathymetry(z)
     sd.SetVolField(prim0)
     # forward pass for the loss value (the adjoint sweep repeats it)
@@ -70,20 +104,28 @@ def gauge_misfit(gauges, target):
 if torch is not None:
     class ShallowWaterRun(torch.autograd.Function):
         """final_prim = ShallowWaterRun.apply(prim0, z, sd, nsteps, scheme, dt,
-        checkpoint_every): prim0 (T x 3) and z (V) float64 CPU tensors; the
+        checkpoint_every, cells): prim0 (T x 3) and z float64 CPU tensors; the
         backward gives their gradients through the adjoint of every stage.
-        sd is a swe_fvm_amd.SpaceDisc (ctypes) or a Solver of the pybind
-        module SWE_FVM; both call the same C ABI."""
+        z holds the node heights (V), or with cells (a CellBathymetry of the
+        mesh) per-cell bathymetry values (T) mapped to the nodes.  dt <= 0
+        runs in CFL mode and the backward uses the step sizes it took.  sd
+        is a swe_fvm_amd.SpaceDisc (ctypes) or a Solver of the pybind module
+        SWE_FVM; both call the same C ABI."""
 
         @staticmethod
-        def forward(ctx, prim0, z, sd, nsteps, scheme="SSPRK2", dt=1e-3, checkpoint_every=50):
+        def forward(ctx, prim0, z, sd, nsteps, scheme="SSPRK2", dt=1e-3, checkpoint_every=50, cells=None):
             p0 = prim0.detach().cpu().numpy().astype(np.float64)
             zz = z.detach().cpu().numpy().astype(np.float64)
+            ctx.cells = cells
+            if cells is not None:
+                zz = cells.to_nodes(zz)
             sd.set_bathymetry(zz)
             sd.SetVolField(p0)
             sd.run(scheme, nsteps, dt=dt)
             out = torch.from_numpy(sd.GetVolField().copy())
             ctx.sd, ctx.p0, ctx.z = sd, p0, zz
+            # a CFL run (dt <= 0) is differentiated with the step sizes it took
+            ctx.dts = None if (dt is not None and dt > 0) else sd.dts(nsteps)
             ctx.args = (nsteps, scheme, dt, checkpoint_every)
             return out
 
@@ -94,6 +136,9 @@ if torch is not None:
             sd.set_bathymetry(ctx.z)
             sd.SetVolField(ctx.p0)
             s0, zb, ms = sd.adjoint_run(gout.detach().cpu().numpy().astype(np.float64), nsteps, scheme=scheme,
-                                        dt=dt, checkpoint_every=ck)
+                                        dt=None if ctx.dts is not None else dt, dts=ctx.dts,
+                                        checkpoint_every=ck)
             sd.last_adjoint_ms = ms  # forward-sweep and reverse-sweep milliseconds
-            return torch.from_numpy(s0), torch.from_numpy(zb), None, None, None, None, None
+            if ctx.cells is not None:
+                zb = ctx.cells.gradient(zb)
+            return torch.from_numpy(s0), torch.from_numpy(zb), None, None, None, None, None, None

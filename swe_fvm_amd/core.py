@@ -434,6 +434,11 @@ class SpaceDisc:
         the static geometry; the state (w, u, v) is kept."""
         z = np.ascontiguousarray(z, dtype=np.float64).reshape(self.V)
         _check(_lib.load().swe_set_bathymetry(self._h, _ptr(z)))
+        # GetDomain() reflects the device bathymetry (a new Domain: the
+        # caller's geometry array is left alone)
+        g = self.domain.geometry.copy()
+        g[:, 2] = z
+        self.domain = Domain(g, self.domain.topology)
 
     def adjoint_run(self, final_bar, nsteps, scheme="SSPRK2", dt=None, dts=None, checkpoint_every=50,
                     want_z=True):

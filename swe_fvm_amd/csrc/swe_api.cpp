@@ -43,6 +43,7 @@ struct swe_solver {
   int graph_scheme = -1;
   double graph_cfl = 0., graph_tend = 0.;
   int64_t graph_c0 = -1, graph_c1 = -1;
+  int graph_cur = -1;  // state buffer the captured launches start from
 };
 
 namespace {
@@ -593,20 +594,38 @@ int swe_batch_reset(swe_solver* s, double t0) {
 }
 
 // Cells and edges of the members still running: finished members at the
-// two ends of the member list are left out of the launches.
-static void batch_window(swe_solver* s, const std::vector<int>& act) {
+// two ends of the member list are left out of the launches.  Inside the
+// window a finished member's state is carried along by the update (copied
+// from the stage input); an Euler step swaps the two state buffers, so the
+// cells that leave the window are copied into the other buffer as well,
+// and both buffers then hold their final state whatever the later parity.
+static int batch_window(swe_solver* s, const std::vector<int>& act, bool euler) {
+  const int64_t oc0 = s->M.c0, oc1 = s->M.c1;
   int k0 = 0, k1 = s->n_inst - 1;
   while (k0 <= k1 && !act[size_t(k0)]) ++k0;
   while (k1 >= k0 && !act[size_t(k1)]) --k1;
   if (k0 > k1) {
     s->M.c0 = s->M.c1 = 0;
     s->M.e0 = s->M.e1 = 0;
-    return;
+  } else {
+    s->M.c0 = s->cell_off[size_t(k0)];
+    s->M.c1 = s->cell_off[size_t(k1) + 1];
+    s->M.e0 = s->edge_off[size_t(k0)];
+    s->M.e1 = s->edge_off[size_t(k1) + 1];
   }
-  s->M.c0 = s->cell_off[size_t(k0)];
-  s->M.c1 = s->cell_off[size_t(k1) + 1];
-  s->M.e0 = s->edge_off[size_t(k0)];
-  s->M.e1 = s->edge_off[size_t(k1) + 1];
+  if (!euler) return SWE_OK;
+  const int a = s->cur, b = a == 0 ? 1 : 0;
+  auto copy = [&](int64_t c0, int64_t c1) -> int {
+    if (c1 <= c0) return SWE_OK;
+    const size_t off = size_t(c0), bytes = sizeof(double) * size_t(c1 - c0);
+    SWE_CHECK(hipMemcpyAsync(s->sw[b] + off, s->sw[a] + off, bytes, hipMemcpyDeviceToDevice, s->stream), "hipMemcpy");
+    SWE_CHECK(hipMemcpyAsync(s->su[b] + off, s->su[a] + off, bytes, hipMemcpyDeviceToDevice, s->stream), "hipMemcpy");
+    SWE_CHECK(hipMemcpyAsync(s->sv[b] + off, s->sv[a] + off, bytes, hipMemcpyDeviceToDevice, s->stream), "hipMemcpy");
+    return SWE_OK;
+  };
+  if (s->M.c1 <= s->M.c0) return copy(oc0, oc1);
+  if (int rc = copy(oc0, std::min(oc1, s->M.c0))) return rc;
+  return copy(std::max(oc0, s->M.c1), oc1);
 }
 
 static int batch_capture(swe_solver* s, int scheme, double cfl, double t_end, int per_graph) {
@@ -628,6 +647,7 @@ static int batch_capture(swe_solver* s, int scheme, double cfl, double t_end, in
   s->graph_tend = t_end;
   s->graph_c0 = s->M.c0;
   s->graph_c1 = s->M.c1;
+  s->graph_cur = cur0;
   return SWE_OK;
 }
 
@@ -643,28 +663,30 @@ int swe_batch_run(swe_solver* s, int scheme, double cfl, double t_end, int64_t m
   SWE_CHECK(hipMemcpyAsync(act_flags.data(), s->B.active, sizeof(int) * act_flags.size(),
                            hipMemcpyDeviceToHost, s->stream), "hipMemcpy");
   SWE_CHECK(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
-  batch_window(s, act_flags);
+  const bool euler = scheme == SWE_SCHEME_EULER;
+  if (int rc = batch_window(s, act_flags, euler)) return rc;
   int64_t done = 0;
   int act = 0;
   for (int f : act_flags) act += f;
   const int64_t chunk = 64;
   while (done < max_steps && act > 0) {
     if (use_graph) {
-      // re-capture when the scheme changed or the window shrank (members
-      // finished: at most one capture per check, every 64 steps)
+      // re-capture when the scheme changed, the window shrank (members
+      // finished: at most one capture per check, every 64 steps) or the
+      // state moved to the other buffer since the capture (an odd number of
+      // Euler steps taken without the graph)
       const int64_t w = s->M.c1 - s->M.c0, wg = s->graph_c1 - s->graph_c0;
       if (s->graph == nullptr || s->graph_scheme != scheme || s->graph_cfl != cfl ||
-          s->graph_tend != t_end || s->M.c0 < s->graph_c0 || s->M.c1 > s->graph_c1 || w < wg)
+          s->graph_tend != t_end || s->M.c0 < s->graph_c0 || s->M.c1 > s->graph_c1 || w < wg ||
+          s->cur != s->graph_cur)
         if (int rc = batch_capture(s, scheme, cfl, t_end, per_graph)) return rc;
     }
-    int64_t k = std::min<int64_t>(chunk, max_steps - done);
-    if (use_graph) k = (k / per_graph) * per_graph;
-    if (k == 0) break;
+    const int64_t k = std::min<int64_t>(chunk, max_steps - done);
     for (int64_t q = 0; q < k;) {
-      if (use_graph) {
+      if (use_graph && k - q >= per_graph) {
         SWE_CHECK(hipGraphLaunch(s->graph, s->stream), "hipGraphLaunch");
         q += per_graph;
-      } else {
+      } else {  // without the graph, or the odd last step of an Euler graph
         if (int rc = batch_step(s, scheme, cfl, t_end)) return rc;
         q += 1;
       }
@@ -677,7 +699,7 @@ int swe_batch_run(swe_solver* s, int scheme, double cfl, double t_end, int64_t m
     for (int f : act_flags) act += f;
     // a captured graph keeps its own window until the next capture; members
     // inside it that finished are skipped by the kernels
-    batch_window(s, act_flags);
+    if (int rc = batch_window(s, act_flags, euler)) return rc;
   }
   // restore the whole-mesh window for the single-mesh entry points
   s->M.c0 = 0;

@@ -134,6 +134,12 @@ struct Solver {
     }
     check(rc);
   }
+  // step sizes of the last run (swe_get_dts)
+  F64 dts(int64_t nsteps) const {
+    F64 out({py::ssize_t(nsteps)});
+    check(swe_get_dts(h.get(), out.mutable_data(), nsteps));
+    return out;
+  }
   const double* steps(const py::object& dts, int64_t nsteps, F64& keep) const {
     if (dts.is_none()) return nullptr;
     keep = dts.cast<F64>();
@@ -271,6 +277,7 @@ PYBIND11_MODULE(SWE_FVM, m) {
       .def("set_bathymetry", &Solver::set_bathymetry, "z"_a, "node bathymetry (V); the state is kept")
       .def("run", &Solver::run, "scheme"_a = "SSPRK2", "nsteps"_a = 1, "dt"_a = 1e-3, "cfl"_a = 0.15,
            "nsteps steps; dt > 0 fixed, dt <= 0 CFL mode")
+      .def("dts", &Solver::dts, "nsteps"_a, "the step sizes of the last run (swe_get_dts)")
       .def("adjoint_run", &Solver::adjoint_run, "final_bar"_a, "nsteps"_a, "scheme"_a = "SSPRK2",
            "dt"_a = py::none(), "dts"_a = py::none(), "checkpoint_every"_a = 50, "want_z"_a = true,
            "swe_adjoint_run: (adjoint of the initial state T x 3, of the node bathymetry V or None, "

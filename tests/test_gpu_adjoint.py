@@ -619,3 +619,82 @@ def test_gradient_descent_lowers_the_gauge_misfit():
         Js.append(J)
         print(f"descent iteration {it}: step {step:.3e} misfit {J:.6e}", flush=True)
     assert all(b < a for a, b in zip(Js, Js[1:]))
+
+
+def test_cell_bathymetry_gradient_matches_differences_of_the_oracle():
+    """Per-cell bathymetry b with node heights z = M b (z_n the mean of b
+    over the cells at node n): the adjoint's per-cell gradient M^T dJ/dz
+    against directional central differences of the oracle in b, together
+    with the initial free surface (eta = w) direction."""
+    from oracle import oracle as O
+    dom, prim0 = wet_case(n=6, seed=13)
+    tables = dom.topology.tables()
+    cells = AD.CellBathymetry(tables["tp"], dom.geometry.shape[0])
+    xyz0 = dom.geometry.copy()
+    b0 = xyz0[tables["tp"], 2].mean(1)
+    fl = swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt]
+    sd = swe.SpaceDisc(fl, dom, prim0)
+    rng = np.random.default_rng(23)
+    gauges = rng.choice(prim0.shape[0], 12, replace=False)
+    fn = AD.gauge_misfit(gauges, prim0[gauges, 0] + 0.005)
+    dt, nsteps = 2e-3, 6
+    J, s0, bb, _ = AD.run_and_adjoint(sd, prim0, b0, nsteps, fn, scheme="SSPRK2", dt=dt,
+                                      checkpoint_every=2, cell_bathymetry=cells)
+    assert bb.shape == (prim0.shape[0],)
+
+    def J_oracle(p, b):
+        xyz = xyz0.copy()
+        xyz[:, 2] = cells.to_nodes(b)
+        o = O.OracleSolver(xyz, tables, "HLLC", "Einfeldt")
+        o.set_state(p)
+        o.run("SSPRK2", nsteps, dt=dt)
+        return fn(o.state())[0]
+
+    assert abs(J_oracle(prim0, b0) - J) <= 1e-13 * abs(J)
+    checked = 0
+    for trial in range(4):
+        db = rng.standard_normal(b0.shape) * 1e-2
+        dp = np.zeros_like(prim0)
+        if trial % 2:
+            dp[:, 0] = rng.standard_normal(prim0.shape[0]) * 1e-2  # initial eta
+        cd = [(J_oracle(prim0 + e * dp, b0 + e * db) - J_oracle(prim0 - e * dp, b0 - e * db)) / (2 * e)
+              for e in (1e-6, 2.5e-7)]
+        if abs(cd[0] - cd[1]) > 1e-5 * abs(cd[0]) + 1e-12:
+            print(f"cell bathymetry trial {trial}: the difference crossed a switch", flush=True)
+            continue
+        ad = float(np.sum(bb * db) + np.sum(s0 * dp))
+        print(f"cell bathymetry trial {trial}: adjoint {ad:.12e} oracle fd {cd[0]:.12e} "
+              f"rel {abs(ad - cd[0]) / abs(cd[0]):.2e}", flush=True)
+        assert abs(ad - cd[0]) <= 1e-5 * abs(cd[0]) + 1e-12
+        checked += 1
+    assert checked >= 2
+
+
+def test_autograd_function_cell_bathymetry_and_cfl_mode():
+    """ShallowWaterRun with per-cell bathymetry equals run_and_adjoint with
+    the same map, and a CFL-mode forward (dt <= 0) is differentiated with
+    the step sizes it took."""
+    torch = pytest.importorskip("torch")
+    dom, prim0 = wet_case(n=12, seed=9)
+    tables = dom.topology.tables()
+    cells = AD.CellBathymetry(tables["tp"], dom.geometry.shape[0])
+    b = dom.geometry[tables["tp"], 2].mean(1)
+    sd = swe.SpaceDisc(swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt], dom, prim0)
+    fn = lambda q: (float(np.sum(q[:, 0] ** 2)), np.stack([2 * q[:, 0], 0 * q[:, 0], 0 * q[:, 0]], 1))
+    for dt in (2e-3, 0.0):
+        p = torch.tensor(prim0, dtype=torch.float64, requires_grad=True)
+        bt = torch.tensor(b, dtype=torch.float64, requires_grad=True)
+        out = AD.ShallowWaterRun.apply(p, bt, sd, 10, "SSPRK2", dt, 4, cells)
+        loss = (out[:, 0] ** 2).sum()
+        loss.backward()
+        dts = None
+        if dt <= 0:
+            sd.set_bathymetry(cells.to_nodes(b))
+            sd.SetVolField(prim0)
+            sd.run("SSPRK2", 10, dt=0.0)
+            dts = sd.dts(10)
+        J, s0, bb, _ = AD.run_and_adjoint(sd, prim0, b, 10, fn, dt=dt if dt > 0 else None, dts=dts,
+                                          checkpoint_every=4, cell_bathymetry=cells)
+        assert abs(loss.item() - J) <= 1e-12 * abs(J)
+        assert np.allclose(p.grad.numpy(), s0, rtol=1e-12, atol=1e-15)
+        assert np.allclose(bt.grad.numpy(), bb, rtol=1e-12, atol=1e-15)

@@ -62,3 +62,68 @@ def test_members_stop_exactly_at_t_end():
                 tk = t_end
         assert tk == t_end
         assert np.array_equal(got[k], orc.state()), k
+
+
+def _oracle_replay(dom, prim, scheme, t_end, nsteps):
+    """A member advanced alone on the oracle: CFL steps, the last one
+    clipped to land on t_end (k_batch_dt)."""
+    orc = O.OracleSolver(dom.geometry, dom.topology.tables(), "HLLC", "Einfeldt")
+    orc.set_state(prim)
+    tk = 0.0
+    for _ in range(nsteps):
+        orc.compute_interface_values()
+        d = 0.15 * orc.compute_fluxes()
+        rest = t_end - tk
+        if d < rest:
+            orc.run(scheme, 1, dt=0.0, cfl=0.15)
+            tk = tk + d
+        else:
+            orc.run(scheme, 1, dt=rest)
+            tk = t_end
+    return orc.state(), tk
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_euler_members_finished_in_odd_chunks(graph):
+    """Euler swaps the two state buffers every step; members that finish
+    and leave the launch window must keep their final state whatever the
+    parity of the steps taken after (odd max_steps per call)."""
+    mem = members()
+    ens = Ensemble(FL, [d for d, _ in mem], [p for _, p in mem])
+    t_end = 0.015
+    calls = 0
+    while True:
+        done, act = ens.run("Euler", cfl=0.15, t_end=t_end, max_steps=3, graph=graph)
+        calls += 1
+        if act == 0:
+            break
+        assert calls < 500
+    t, steps = ens.times()
+    assert np.all(t == t_end)
+    assert len(set(steps.tolist())) > 1  # the members finish at different steps
+    got = ens.states()
+    for k, (dom, prim) in enumerate(mem):
+        ref, tk = _oracle_replay(dom, prim, "Euler", t_end, int(steps[k]))
+        assert tk == t_end
+        assert np.array_equal(got[k], ref), k
+    # further calls change nothing
+    before = [g.copy() for g in got]
+    ens.run("Euler", cfl=0.15, t_end=t_end, max_steps=1, graph=graph)
+    assert all(np.array_equal(a, b) for a, b in zip(before, ens.states()))
+
+
+def test_euler_graph_and_plain_runs_mixed():
+    """An Euler graph captured on one state buffer is re-captured after an
+    odd number of steps taken without it."""
+    mem = members()
+    ens = Ensemble(FL, [d for d, _ in mem], [p for _, p in mem])
+    plan = [(True, 4), (False, 3), (True, 4), (True, 1), (False, 2), (True, 5)]
+    for graph, n in plan:
+        done, act = ens.run("Euler", cfl=0.15, t_end=1e9, max_steps=n, graph=graph)
+        assert done == n and act == len(mem)
+    nsteps = sum(n for _, n in plan)
+    got = ens.states()
+    for k, (dom, prim) in enumerate(mem):
+        sd = swe.SpaceDisc(FL, dom, prim)
+        sd.run("Euler", nsteps, dt=0.0, cfl=0.15)
+        assert np.array_equal(got[k], sd.GetVolField()), k
