@@ -43,10 +43,14 @@ def kernel_alg_bytes(T, E, V):
         "drain": T * (8 + 12 + 12 + 4) + E * 8 + nodes + T * 8,
         # state (+ U0), tp, te, tt, cflag, MUSCL, class, F, dtd, nodes in; state out
         "update": T * (24 + 24 + 36 + 4 + 24 + 1 + 8) + E * 24 + nodes + T * 24,
+        # state (+ U0), tp, tile codes, cflag, MUSCL, class, centroids, bc,
+        # area, edmin, nodes in; state out (fluxes and draining steps stay
+        # in LDS)
+        "fused": T * (24 + 24 + 12 + 12 + 4 + 24 + 1 + 16 + 8 + 8) + E * 8 + nodes + T * 24,
     }
 
 
-KERNEL_NAMES = ["recon1", "recon2", "flux", "drain", "update"]
+KERNEL_NAMES = ["recon1", "recon2", "flux", "drain", "update", "fused"]
 
 
 def load_traffic(workload, kernel):
@@ -389,7 +393,8 @@ def main():
     alg = kernel_alg_bytes(T, E, V)
     per = {k: {"avg_ms": float(kms[i]), "alg_bytes": alg[k],
                "gbs": alg[k] / (kms[i] * 1e-3) / 1e9 if kms[i] > 0 else None}
-           for i, k in enumerate(KERNEL_NAMES)}
+           for i, k in enumerate(KERNEL_NAMES) if kms[i] > 0}
+    KERNEL_NAMES[:] = [k for k in KERNEL_NAMES if k in per]
     dom_k = max(KERNEL_NAMES, key=lambda k: per[k]["avg_ms"])
     achieved = per[dom_k]["gbs"]
     traffic = load_traffic(workload, dom_k)

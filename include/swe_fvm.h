@@ -158,6 +158,24 @@ int swe_get_cell_class(swe_solver* s, int8_t* cls);
 
 /* TimeDisc::RHS(i, dt) (src/TimeDisc.cpp:3-41) after ComputeFluxes. */
 int swe_rhs(swe_solver* s, int64_t cell, double dt, double* out3);
+/* MUSCLObject::ReconstructDryCell / PartWetCell1 / FullWetCell /
+ * PartWetCell2 (include/MUSCLObject.h:57-60, src/MUSCLObject.cpp:31-191) of
+ * one cell of the current state (kind 0, 1, 2, 3), evaluated on the device
+ * with the step's own code after recomputing the interface values (pass 2
+ * reads the pass-1 node maximum).  out5 = origin (w, u, v) at the centroid
+ * and the gradient of w (x, y); u and v have zero gradient in every
+ * reconstruction.  A wall neighbour of a full-wet plane is the cell itself
+ * (the reference reads index -1 there). */
+int swe_reconstruct(swe_solver* s, int64_t cell, int kind, double* out5);
+/* Fluxes::HLL<W> / HLLC<W> (include/Fluxes.h:14-111) of edge e seen from
+ * cell `from` towards cell `to` (to < 0: a boundary edge, the right state
+ * is the left one), flux SWE_FLUX_* and wave-speed estimate SWE_WAVE_*, with
+ * the edge slots of the last interface values: out3 = flux in the normal
+ * outward of `from`; *r (may be null) = min(*r, min(2 A_from / L, 2 A_to /
+ * L) / wave speed), left unchanged when the flux returns before the
+ * speeds (dry edge, vanishing speeds), as in the reference. */
+int swe_edge_flux(swe_solver* s, int flux, int wavespeed, int64_t e, int64_t from, int64_t to, double* r,
+                  double* out3);
 
 /* Solvers::Euler / SSPRK2 / SSPRK3 (src/Solvers.cpp:5-59) repeated nsteps
  * times on the device without host synchronisation.  dt > 0: fixed step;
@@ -204,9 +222,10 @@ int swe_batch_info(swe_solver* s, int64_t* n, int64_t* cell_off, int64_t* edge_o
 int swe_batch_time(swe_solver* s, double* t, int64_t* steps);
 
 /* Time, with HIP events on the solver's stream, nsteps steps of swe_run:
- * *ms = total milliseconds; kernel_ms (optional, 5 doubles) = average
- * per-launch duration of the reconstruction, wet/dry, flux, draining and
- * update kernels, measured by bracketing each launch with events. */
+ * *ms = total milliseconds; kernel_ms (optional, 6 doubles) = average
+ * per-launch duration of the reconstruction, wet/dry, flux, draining,
+ * update and fused flux-draining-update kernels (0 for a kernel the run did
+ * not use), measured by bracketing each launch with events. */
 int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nsteps, double* ms,
                  double* kernel_ms);
 
@@ -220,6 +239,14 @@ int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nstep
  * a fixed-order tree (identical run to run); per_cell (optional, 3 T
  * doubles) receives the cell terms.  Single-mesh handles only. */
 int swe_integrals(swe_solver* s, double* out3, double* per_cell);
+/* Switches the fused flux-draining-update kernel off (on = 0) or back on
+ * (no reference counterpart).  When on (the default, on meshes whose tiles
+ * fit the kernel), every stage whose step size is known before its flux
+ * pass (fixed dt; stages 2 and 3 of a CFL step) runs as reconstruction +
+ * one fused kernel instead of reconstruction + flux + draining + update;
+ * results are identical bit for bit.  *active (may be null) receives
+ * whether the kernel will be used. */
+int swe_set_fused(swe_solver* s, int on, int* active);
 /* Final per-cell reconstruction after swe_compute_interface_values: level
  * w0 at the centroid and gradient (gx, gy); velocity is the cell's (zero
  * for dry cells).  Any pointer may be null. */

@@ -45,7 +45,26 @@ struct Array {
   Array operator+(const Array& o) const { Array r; for (size_t i = 0; i < k; ++i) r.v[i] = v[i] + o.v[i]; return r; }
   Array operator-(const Array& o) const { Array r; for (size_t i = 0; i < k; ++i) r.v[i] = v[i] - o.v[i]; return r; }
   Array operator*(double s) const { Array r; for (size_t i = 0; i < k; ++i) r.v[i] = v[i] * s; return r; }
+  Array operator/(double s) const { Array r; for (size_t i = 0; i < k; ++i) r.v[i] = v[i] / s; return r; }
+  Array operator-() const { Array r; for (size_t i = 0; i < k; ++i) r.v[i] = -v[i]; return r; }
+  Array& operator+=(const Array& o) { for (size_t i = 0; i < k; ++i) v[i] += o.v[i]; return *this; }
+  Array& operator-=(const Array& o) { for (size_t i = 0; i < k; ++i) v[i] -= o.v[i]; return *this; }
+  Array& operator*=(double s) { for (size_t i = 0; i < k; ++i) v[i] *= s; return *this; }
   friend Array operator*(double s, const Array& a) { return a * s; }
+  // dot product summed in index order (Eigen's order for fixed sizes <= 3)
+  double dot(const Array& o) const { double d = v[0] * o.v[0]; for (size_t i = 1; i < k; ++i) d += v[i] * o.v[i]; return d; }
+  static constexpr size_t size() { return k; }
+};
+
+// 2-vectors of the geometry getters (Eigen::Vector2d in the reference)
+using Vector2d = Array<2>;
+
+// 3 x 2 gradient of a reconstruction (Eigen::Matrix32d): rows w, u, v
+struct Matrix32d {
+  double m[3][2] = {{0., 0.}, {0., 0.}, {0., 0.}};
+  static Matrix32d Zero() { return Matrix32d(); }
+  double& operator()(int r, int c) { return m[r][c]; }
+  double operator()(int r, int c) const { return m[r][c]; }
 };
 
 // k x N column-major storage (Eigen::Array<double, k, Dynamic>)
@@ -170,6 +189,40 @@ struct Topology {
   TriangTagArray tp_, te_, tt_;
 };
 
+// ------------------------------------------------- include/PointOperations.h
+// src/PointOperations.cpp:4-24, 42-48
+inline double Len(const Point& a, const Point& b) noexcept {
+  return std::sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]));
+}
+inline double Det(const Point& a, const Point& b) noexcept { return a[0] * b[1] - b[0] * a[1]; }
+inline double TriangArea(const Point& a, const Point& b, const Point& c) noexcept {
+  return 0.5 * std::fabs(Det(b - a, c - a));
+}
+// intersection of the lines a1 a2 and b1 b2; SolverError when the
+// determinant of their directions is below tol in magnitude (the reference
+// tests den < tol, which also refuses every pair crossing the other way
+// round, half of a mesh's interior edges in Domain::C)
+inline Point Intersection(const Point& a1, const Point& a2, const Point& b1, const Point& b2) {
+  const double den = Det(a2 - a1, b2 - b1);
+  if (std::fabs(den) < tol) throw SolverError("Intersection: the lines are (nearly) parallel");
+  return a1 + (a2 - a1) * Det(b1 - a1, b2 - b1) / den;
+}
+// plane through three points (x, y, value): the 2 x 2 system of the
+// differences to the first point, solved by partial-pivot LU
+inline Vector2d Gradient(const Point& p0, const Point& p1, const Point& p2) {
+  double a00 = p1[0] - p0[0], a01 = p1[1] - p0[1], b0 = p1[2] - p0[2];
+  double a10 = p2[0] - p0[0], a11 = p2[1] - p0[1], b1 = p2[2] - p0[2];
+  if (std::fabs(a10) > std::fabs(a00)) {
+    std::swap(a00, a10); std::swap(a01, a11); std::swap(b0, b1);
+  }
+  const double l = a10 / a00;
+  const double u11 = a11 - l * a01;
+  const double y1 = b1 - l * b0;
+  const double x1 = y1 / u11;
+  const double x0 = (b0 - a01 * x1) / a00;
+  return {x0, x1};
+}
+
 // ------------------------------------------------------ include/Bathymetry.h
 constexpr inline bool IsWet(double h) noexcept { return h > 1e-12; }
 inline Point DryState(double b) noexcept { return {b, 0., 0.}; }
@@ -206,28 +259,142 @@ struct Domain {
     Point a = P(tp[0]), b = P(tp[1]), c = P(tp[2]);
     return 0.5 * std::fabs((b[0] - a[0]) * (c[1] - a[1]) - (c[0] - a[0]) * (b[1] - a[1]));
   }
+  double MinX() const { return extreme(0, true); }
+  double MaxX() const { return extreme(0, false); }
+  double MinY() const { return extreme(1, true); }
+  double MaxY() const { return extreme(1, false); }
+  // Domain::Tang (src/Bathymetry.cpp:69-76): unit edge direction, turned
+  // so that cell t lies on its right (Norm then points out of t)
+  Vector2d Tang(NodeTag e, NodeTag t) const {
+    auto ep = m_topology.EdgePoints(e);
+    const Point a = P(ep[0]), b = P(ep[1]);
+    const double l = L(e);
+    double tx = (b[0] - a[0]) / l, ty = (b[1] - a[1]) / l;
+    const Point c = T(t);
+    if ((c[0] - a[0]) * ty - tx * (c[1] - a[1]) > 0.) { tx = -tx; ty = -ty; }
+    return {tx, ty};
+  }
+  // Domain::Norm (src/Bathymetry.cpp:78-80): (t_y, -t_x), outward of t
+  Vector2d Norm(NodeTag e, NodeTag t) const {
+    const Vector2d tn = Tang(e, t);
+    return {tn[1], -tn[0]};
+  }
+  // Domain::TriangSlope (src/Bathymetry.cpp:10-13): bathymetry gradient
+  Vector2d TriangSlope(NodeTag t) const {
+    auto tp = m_topology.TriangPoints(t);
+    return Gradient(P(tp[0]), P(tp[1]), P(tp[2]));
+  }
+  // Domain::Z (src/Bathymetry.cpp:15-18): slope . ((x, y) - first vertex)
+  double Z(NodeTag t, double x, double y) const {
+    auto tp = m_topology.TriangPoints(t);
+    const Point p0 = P(tp[0]);
+    return TriangSlope(t).dot(Vector2d{x - p0[0], y - p0[1]});
+  }
+  // Domain::C (src/Bathymetry.cpp:33-40): the edge midpoint on a boundary
+  // edge, else where the line between the two centroids crosses the edge
+  Point C(NodeTag e) const {
+    if (m_topology.IsEdgeBoundary(e)) return E(e);
+    auto ep = m_topology.EdgePoints(e);
+    auto et = m_topology.EdgeTriangs(e);
+    return Intersection(P(ep[0]), P(ep[1]), T(et[0]), T(et[1]));
+  }
 
  private:
+  double extreme(int row, bool lo) const {
+    double r = m_geometry(row, 0);
+    for (Idx i = 1; i < m_geometry.cols(); ++i) {
+      const double x = m_geometry(row, i);
+      if (lo ? (x < r) : (r < x)) r = x;
+    }
+    return r;
+  }
   PointArray m_geometry;
   Topology m_topology;
 };
 
+// ------------------------------------------------------- include/Assigners.h
+// Writes through a value field in primitive or conservative form with the
+// reference's dry clamp and small-depth velocity desingularisation
+// (include/Assigners.h:4-50, src/Assigners.cpp:8-44).  b is the
+// bathymetry under the written column.
+template <class Derived, size_t k>
+struct BaseAssigner {
+  BaseAssigner(Storage<k>* field, double b, Idx col) : m_field(field), m_b(b), m_col(col) {}
+  void operator+=(const Array<k>& rhs) { self() = self().Get() + rhs; }
+  void operator-=(const Array<k>& rhs) { self() = self().Get() - rhs; }
+  void operator*=(double scalar) { self() = self().Get() * scalar; }
+  void operator/=(double scalar) {
+    if (scalar == 0.) throw std::runtime_error("value field write divided by zero");
+    self() *= (1. / scalar);
+  }
+  operator Array<k>() const { return static_cast<const Derived&>(*this).Get(); }
+
+ protected:
+  Storage<k>* const m_field;
+  double m_b;
+  Idx m_col;
+
+ private:
+  Derived& self() { return static_cast<Derived&>(*this); }
+};
+
+struct PrimAssigner : BaseAssigner<PrimAssigner, 3> {
+  using BaseAssigner<PrimAssigner, 3>::BaseAssigner;
+  using BaseAssigner<PrimAssigner, 3>::operator+=;
+  Array<3> Get() const { return m_field->col(m_col); }
+  void operator=(const Array<3>& rhs) {
+    const double h = rhs[0] - m_b;
+    if (!IsWet(h)) { m_field->set_col(m_col, DryState(m_b)); return; }
+    Array<3> c = rhs;
+    if (h < 1e-3) {
+      const double f = std::sqrt(2.) * h / std::sqrt(h * h + 1e-6);
+      c[1] *= f;
+      c[2] *= f;
+    }
+    m_field->set_col(m_col, c);
+  }
+};
+
+struct ConsAssigner : BaseAssigner<ConsAssigner, 3> {
+  using BaseAssigner<ConsAssigner, 3>::BaseAssigner;
+  Array<3> Get() const {
+    Array<3> c = m_field->col(m_col);
+    const double h = c[0] - m_b;
+    return {h, c[1] * h, c[2] * h};
+  }
+  void operator=(const Array<3>& rhs) {
+    const double h = rhs[0];
+    if (!IsWet(h)) { m_field->set_col(m_col, DryState(m_b)); return; }
+    double ih;
+    if (h < 1e-3) ih = std::sqrt(2.) * h / std::sqrt(h * h * h * h + 1e-12);
+    else ih = 1. / h;
+    m_field->set_col(m_col, Array<3>{h + m_b, rhs[1] * ih, rhs[2] * ih});
+  }
+};
+
 // ------------------------------------------------------ include/ValueField.h
-// Primitive (w, u, v) per cell; b = centroid bathymetry.
+// Primitive (w, u, v) per cell; b = centroid bathymetry.  The const
+// accessors read; prim(i) / cons(i) on a mutable field return assigners, so
+// `field.cons(i) = U` and `field.prim(i) += dW` write as in the reference.
 struct VolumeField {
   VolumeField(const Domain& b, size_t size = 0) : m_str(Idx(size)), m_b(&b) {}
   size_t Size() const { return size_t(m_str.cols()); }
   void Resize(size_t size) { m_str.resize(Idx(size)); }
   double b(Idx i) const { return m_b->T(i)[2]; }
   Array<3> prim(Idx i) const { return m_str.col(i); }
+  Array<3> cons(Idx i) const { return {h(i), hu(i), hv(i)}; }
+  PrimAssigner prim(Idx i) { return PrimAssigner(&m_str, b(i), i); }
+  ConsAssigner cons(Idx i) { return ConsAssigner(&m_str, b(i), i); }
   void set_prim(Idx i, const Array<3>& p) { m_str.set_col(i, p); }
   double w(Idx i) const { return m_str(0, i); }
   double u(Idx i) const { return m_str(1, i); }
   double v(Idx i) const { return m_str(2, i); }
+  double& u(Idx i) { return m_str(1, i); }
+  double& v(Idx i) { return m_str(2, i); }
+  Array<2> vel(Idx i) const { return {u(i), v(i)}; }
   double h(Idx i) const { return w(i) - b(i); }
   double hu(Idx i) const { return h(i) * u(i); }
   double hv(Idx i) const { return h(i) * v(i); }
-  Array<3> cons(Idx i) const { return {h(i), hu(i), hv(i)}; }
   Storage<3>& storage() { return m_str; }
   const Storage<3>& storage() const { return m_str; }
 
@@ -243,16 +410,72 @@ struct EdgeField {
   static Idx Id(Idx e, Idx from, Idx to) { return 2 * e + Idx(from < to); }
   double b(Idx e, Idx, Idx) const { return m_b->E(e)[2]; }
   Array<3> prim(Idx e, Idx f, Idx t) const { return m_str.col(Id(e, f, t)); }
+  Array<3> cons(Idx e, Idx f, Idx t) const { return {h(e, f, t), hu(e, f, t), hv(e, f, t)}; }
+  PrimAssigner prim(Idx e, Idx f, Idx t) { return PrimAssigner(&m_str, b(e, f, t), Id(e, f, t)); }
+  ConsAssigner cons(Idx e, Idx f, Idx t) { return ConsAssigner(&m_str, b(e, f, t), Id(e, f, t)); }
   double w(Idx e, Idx f, Idx t) const { return m_str(0, Id(e, f, t)); }
   double u(Idx e, Idx f, Idx t) const { return m_str(1, Id(e, f, t)); }
   double v(Idx e, Idx f, Idx t) const { return m_str(2, Id(e, f, t)); }
   double h(Idx e, Idx f, Idx t) const { return w(e, f, t) - b(e, f, t); }
+  double hu(Idx e, Idx f, Idx t) const { return h(e, f, t) * u(e, f, t); }
+  double hv(Idx e, Idx f, Idx t) const { return h(e, f, t) * v(e, f, t); }
   Array<2> vel(Idx e, Idx f, Idx t) const { return {u(e, f, t), v(e, f, t)}; }
   Storage<3>& storage() { return m_str; }
 
  private:
   Storage<3> m_str;
   const Domain* m_b = nullptr;
+};
+
+// -------------------------------------------------------- ElemFlux (SpaceDisc.h)
+// physical flux of U = (h, hu, hv) through normal n (include/SpaceDisc.h:4-18)
+inline Array<3> ElemFlux(const Vector2d& n, const Array<3>& U) {
+  Array<3> res;
+  const double h = U[0];
+  if (IsWet(h)) {
+    const double hveln = U[1] * n[0] + U[2] * n[1];
+    res[0] = hveln;
+    res[1] = (hveln / h) * U[1] + (0.5 * h * h) * n[0];
+    res[2] = (hveln / h) * U[2] + (0.5 * h * h) * n[1];
+  }
+  return res;
+}
+
+// ------------------------------------------------------ include/MUSCLObject.h
+struct MUSCLObject {
+  // A cell's reconstruction (include/MUSCLObject.h:15-55): value at the
+  // centroid and gradient; AtPoint clamps to the dry state below ground.
+  struct MUSCL {
+    MUSCL(const Domain& b, const Array<3>& at_origin, const Matrix32d& grad, Idx i)
+        : m_b(&b), m_at_origin(at_origin), m_grad(grad), m_i(i) {}
+    Idx OriginId() const noexcept { return m_i; }
+    Array<3> AtOrigin() const noexcept { return m_at_origin; }
+    Array<3> AtPoint(const Point& p) const {
+      const Point c = m_b->T(m_i);
+      const double dx = p[0] - c[0], dy = p[1] - c[1];
+      Array<3> a;
+      for (int r = 0; r < 3; ++r) a[size_t(r)] = m_at_origin[size_t(r)] + (m_grad(r, 0) * dx + m_grad(r, 1) * dy);
+      if ((a[0] - p[2]) >= 0) return a;
+      return DryState(p[2]);
+    }
+    Matrix32d dryGradient() const {
+      Matrix32d g;
+      const Vector2d s = m_b->TriangSlope(m_i);
+      g(0, 0) = s[0];
+      g(0, 1) = s[1];
+      return g;
+    }
+    Matrix32d Gradient(const Point& p) const {
+      if (AtPoint(p)[0] - p[2] >= 0) return m_grad;
+      return dryGradient();
+    }
+
+   private:
+    const Domain* m_b;
+    Array<3> m_at_origin;
+    Matrix32d m_grad;
+    Idx m_i;
+  };
 };
 
 // ------------------------------------------------- include/Fluxes.h (decl)
@@ -295,7 +518,7 @@ Array<3> HLLC(SpaceDisc* const sd, Idx e, Idx from, Idx to, double* r);
 }  // namespace Fluxes
 
 // ------------------------------------------------------- include/SpaceDisc.h
-struct SpaceDisc {
+struct SpaceDisc : MUSCLObject {
   using Fluxer = std::function<Array<3>(SpaceDisc* const, Idx, Idx, Idx, double*)>;
   using FluxFn = Array<3> (*)(SpaceDisc* const, Idx, Idx, Idx, double*);
 
@@ -312,8 +535,14 @@ struct SpaceDisc {
     swe_detail::check(swe_set_state(s, v0.storage().data()));
   }
 
-  // VolumeField on the host: reading downloads, writing through the
-  // mutable accessor is uploaded before the next device operation.
+  // VolumeField on the host: reading downloads what the device changed;
+  // writing through the mutable accessor (field.cons(i) = ..., prim(i) =
+  // ..., u(i) = ...) is uploaded before the next ComputeInterfaceValues,
+  // ComputeFluxes, Run or field getter.  TimeDisc::RHS does not upload: it
+  // evaluates on the state of the last interface values, so a loop that
+  // writes cells one by one between RHS calls (src/Solvers.cpp:5-59) gets
+  // every draining time step from the stage-start state (the Jacobi
+  // reading of src/TimeDisc.cpp:43-66).
   const VolumeField& GetVolField() const { pull(); return m_vol; }
   VolumeField& GetVolField() { pull(); m_dirty = true; return m_vol; }
   const Domain& GetDomain() const noexcept { return m_b; }
@@ -368,7 +597,24 @@ struct SpaceDisc {
   }
   swe_solver* handle() const { return m_h.get(); }
 
+  // MUSCLObject::Reconstruct* (src/MUSCLObject.cpp:31-191) of cell i of the
+  // current state, evaluated on the device by the step's own code (the
+  // interface values are recomputed first)
+  MUSCL ReconstructDryCell(Idx i) const { return reconstruct(i, 0); }
+  MUSCL ReconstructPartWetCell1(Idx i) const { return reconstruct(i, 1); }
+  MUSCL ReconstructFullWetCell(Idx i) const { return reconstruct(i, 2); }
+  MUSCL ReconstructPartWetCell2(Idx i) const { return reconstruct(i, 3); }
+
  private:
+  MUSCL reconstruct(Idx i, int kind) const {
+    push();
+    double o[5];
+    swe_detail::check(swe_reconstruct(m_h.get(), i, kind, o));
+    Matrix32d g;
+    g(0, 0) = o[3];
+    g(0, 1) = o[4];
+    return MUSCL(m_b, Array<3>{o[0], o[1], o[2]}, g, i);
+  }
   template <WavespeedFn W>
   static bool is_flux(FluxFn f, int& flux, int& ws) {
     if (f == &Fluxes::HLL<W>) { flux = SWE_FLUX_HLL; ws = swe_detail::wave_code(W); return true; }
@@ -389,7 +635,6 @@ struct SpaceDisc {
     return m_cls[size_t(i)];
   }
   void pull() const {
-    push();
     if (m_stale) {
       swe_detail::check(swe_get_state(m_h.get(), m_vol.storage().data()));
       m_stale = false;
@@ -415,17 +660,22 @@ struct SpaceDisc {
 
 // ------------------------------------------------------- include/Fluxes.h
 namespace Fluxes {
-// Flux of edge e from the last ComputeFluxes (the device evaluates every
-// edge; from/to follow the edge's orientation et0 -> et1).
+// Flux of edge e seen from cell `from` towards `to` with the edge slots of
+// the last ComputeInterfaceValues, evaluated on the device by the step's
+// own flux code; *r (if given) takes the min with this edge's
+// length-to-wavespeed candidate (include/Fluxes.h:14-111).  ComputeFluxes
+// evaluates every edge on the device itself.
 template <WavespeedFn W>
-Array<3> HLL(SpaceDisc* const sd, Idx e, Idx, Idx, double* r) {
-  if (r) *r = sd->GetMinLenToWavespeed();
-  return sd->GetFluxes().col(e);
+Array<3> HLL(SpaceDisc* const sd, Idx e, Idx from, Idx to, double* r) {
+  Array<3> out;
+  swe_detail::check(swe_edge_flux(sd->handle(), SWE_FLUX_HLL, swe_detail::wave_code(W), e, from, to, r, out.v));
+  return out;
 }
 template <WavespeedFn W>
-Array<3> HLLC(SpaceDisc* const sd, Idx e, Idx, Idx, double* r) {
-  if (r) *r = sd->GetMinLenToWavespeed();
-  return sd->GetFluxes().col(e);
+Array<3> HLLC(SpaceDisc* const sd, Idx e, Idx from, Idx to, double* r) {
+  Array<3> out;
+  swe_detail::check(swe_edge_flux(sd->handle(), SWE_FLUX_HLLC, swe_detail::wave_code(W), e, from, to, r, out.v));
+  return out;
 }
 }  // namespace Fluxes
 

@@ -77,6 +77,7 @@ def lib():
     L.orc_flux_edge.restype = _int
     L.orc_flux_edge.argtypes = [_int, _int, _d, _p, _p, _p, _int, _p, _p, _p]
     L.orc_wavespeed.argtypes = [_int, _d, _d, _d, _d, _p]
+    L.orc_edge_flux.argtypes = [_p, _int, _int, _i64, _i64, _i64, _p, _p]
     _lib = L
     return L
 
@@ -188,6 +189,14 @@ class OracleSolver:
         dts = np.empty(nsteps, np.float64)
         lib().orc_run(self.h, SCHEME[scheme], dt, cfl, nsteps, _ptr(dts))
         return dts
+
+    def edge_flux(self, flux, wave, e, frm, to, r=1.0):
+        """Fluxes::HLL / HLLC of edge e from cell frm towards to on the
+        current edge slots: (F, min(r, candidate))."""
+        F = np.empty(3, np.float64)
+        rr = np.array([r], np.float64)
+        lib().orc_edge_flux(self.h, FLUX[flux], WAVE[wave], int(e), int(frm), int(to), _ptr(rr), _ptr(F))
+        return F, float(rr[0])
 
     def time_run(self, scheme="SSPRK2", nsteps=1, dt=1e-3, cfl=0.15):
         return lib().orc_time_run(self.h, SCHEME[scheme], dt, cfl, nsteps)

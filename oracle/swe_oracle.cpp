@@ -999,6 +999,16 @@ int orc_flux_edge(int flux, int ws, double cor, const double* xyz, const double*
   *r_out = s.r;
   return 0;
 }
+// Solver::HLL / HLLC of edge e seen from `from` towards `to` on the edge
+// slots of the last ComputeInterfaceValues (include/Fluxes.h:14-111);
+// *r = min(*r, candidate) as there.
+void orc_edge_flux(Solver* s, int flux, int ws, int64_t e, int64_t from, int64_t to, double* r, double* F) {
+  const int ws0 = s->ws;
+  s->ws = ws;
+  if (flux == FLUX_HLL) s->HLL(e, from, to, r, F);
+  else s->HLLC(e, from, to, r, F);
+  s->ws = ws0;
+}
 void orc_wavespeed(int ws, double ul, double hl, double ur, double hr, double* a) {
   Solver s;
   s.ws = ws;

@@ -6,8 +6,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libswe_fvm_amd.so")
-SOURCES = ["swe_kernels.hip", "swe_adjoint.hip", "swe_tangent.hip", "swe_api.cpp", "swe_mesh.cpp"]
-HEADERS = ["swe_device.h", "swe_dual.h", "swe_common.h", "swe_launch.h", "swe_internal.h"]
+SOURCES = ["swe_kernels.hip", "swe_fused.hip", "swe_adjoint.hip", "swe_tangent.hip", "swe_api.cpp",
+           "swe_mesh.cpp", "swe_tiles.cpp"]
+HEADERS = ["swe_device.h", "swe_dual.h", "swe_common.h", "swe_launch.h", "swe_internal.h", "swe_tiles.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

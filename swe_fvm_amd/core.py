@@ -511,16 +511,25 @@ class SpaceDisc:
         """Milliseconds of nsteps steps (HIP events on the solver stream).
 
         per_kernel: also return the average per-launch milliseconds of the
-        recon1, recon2, flux, drain and update kernels."""
+        recon1, recon2, flux, drain, update and fused flux-draining-update
+        kernels (0 for a kernel that did not run)."""
         ms = ctypes.c_double()
         if per_kernel:
-            k = np.zeros(5, np.float64)
+            k = np.zeros(6, np.float64)
             _check(_lib.load().swe_time_run(self._h, SCHEMES[scheme], float(dt), float(cfl),
                                             int(nsteps), ctypes.byref(ms), _ptr(k)))
             return ms.value, k
         _check(_lib.load().swe_time_run(self._h, SCHEMES[scheme], float(dt), float(cfl), int(nsteps),
                                         ctypes.byref(ms), None))
         return ms.value
+
+    def set_fused(self, on=True):
+        """Uses (default) or not the fused flux-draining-update kernel for
+        the stages whose step size is known; returns whether it is used.
+        Results are identical either way."""
+        act = ctypes.c_int()
+        _check(_lib.load().swe_set_fused(self._h, 1 if on else 0, ctypes.byref(act)))
+        return bool(act.value)
 
     def set_tuning(self, flux_items=2, update_items=1, flux_block=256, update_block=256):
         """Edges per thread of the flux kernel (1, 2, 4), cells per thread of

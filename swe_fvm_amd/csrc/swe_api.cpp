@@ -11,6 +11,7 @@
 #include "../../include/swe_fvm.h"
 #include "swe_internal.h"
 #include "swe_launch.h"
+#include "swe_tiles.h"
 
 using namespace swe;
 
@@ -44,6 +45,14 @@ struct swe_solver {
   double graph_cfl = 0., graph_tend = 0.;
   int64_t graph_c0 = -1, graph_c1 = -1;
   int graph_cur = -1;  // state buffer the captured launches start from
+  // fused flux / draining / update kernel over tiles (single meshes whose
+  // tiles fit its LDS): used for every stage whose step size is known
+  DevTiles tiles{};
+  bool fused_ok = false, fused_on = true;
+  // after a fused stage W holds no fluxes: they are recomputed from the
+  // stage's input buffer on demand (materialize_fluxes)
+  bool flux_stale = false;
+  int stale_in = -1, stale_slot = 0;
 };
 
 namespace {
@@ -85,16 +94,44 @@ int set_device(swe_solver* s) {
   return SWE_OK;
 }
 
+// The fluxes of the last fused stage, recomputed into W from that stage's
+// input (still intact: no stage writes its own input) and its
+// reconstruction, for the getters that read W's fluxes and before W's
+// reconstruction is overwritten.  Bit-identical to what the separate flux
+// kernel would have left.
+int materialize_fluxes(swe_solver* s) {
+  if (!s->flux_stale) return SWE_OK;
+  SWE_CHECK(launch_flux(s->M, state_of(s, s->stale_in), s->W, s->flux, s->ws, s->cor, s->rbits + s->stale_slot,
+                        s->flux_items, s->flux_block, s->stream), "flux");
+  s->flux_stale = false;
+  return SWE_OK;
+}
+
+// a stage takes the fused kernel when its step size is known before the
+// flux pass: a fixed dt, or a later stage of a CFL step (slot > 0)
+bool use_fused(const swe_solver* s, double dt_fixed, int slot) {
+  return s->fused_ok && s->fused_on && s->n_inst == 0 && (dt_fixed > 0. || slot > 0);
+}
+
 // One stage: ComputeInterfaceValues, ComputeFluxes, update loop.
 int run_stage(swe_solver* s, int in, int u0, int out, double a0, double a1, double coef,
               double dt_fixed, double cfl, int slot, double* dts_log) {
   DevState S = state_of(s, in);
   SWE_CHECK(launch_recon1(s->M, S, s->W, s->opts, s->stream), "recon1");
   SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
-  SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + slot, s->flux_items, s->flux_block, s->stream), "flux");
-  SWE_CHECK(launch_drain(s->M, S, s->W, s->opts, s->stream), "drain");
   StageArgs A{a0, a1, coef, dt_fixed, cfl, s->cor, s->rbits, dts_log, friction(s)};
-  SWE_CHECK(launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->upd_items, s->upd_block, s->stream), "update");
+  if (use_fused(s, dt_fixed, slot)) {
+    SWE_CHECK(launch_fused(s->M, s->tiles, S, state_of(s, u0), out_of(s, out), s->W, A, s->flux, s->ws, s->opts,
+                           s->rbits + slot, s->stream), "fused");
+    s->flux_stale = true;
+    s->stale_in = in;
+    s->stale_slot = slot;
+  } else {
+    SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + slot, s->flux_items, s->flux_block, s->stream), "flux");
+    s->flux_stale = false;
+    SWE_CHECK(launch_drain(s->M, S, s->W, s->opts, s->stream), "drain");
+    SWE_CHECK(launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->upd_items, s->upd_block, s->stream), "update");
+  }
   s->last_r_slot = slot;
   return SWE_OK;
 }
@@ -265,6 +302,27 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
                  G.cx, G.cy, G.bc, G.area, G.edmin, G.crec, nullptr, nullptr, nullptr, 0, T, 0, E,
                  T >= (int64_t(1) << 20) ? 1 : 0};
   s->G = G;
+  if (n_inst == 0) {
+    TileTables tt_host;
+    if (build_tiles(T, tt.data(), te.data(), flag.data(), et1.data(), tt_host)) {
+      uint32_t *dcode, *drcode;
+      int *droff, *drcell, *devoff;
+      uint16_t* dev;
+#define ALLOC(ptr, n) if ((e = dalloc(s, &ptr, n)) != hipSuccess) return bail(e, "hipMalloc")
+#define H2D(dst, v)                                                                                      \
+  if (!v.empty() && (e = hipMemcpy(dst, v.data(), sizeof(v[0]) * v.size(), hipMemcpyHostToDevice)) != hipSuccess) \
+    return bail(e, "hipMemcpy");
+      ALLOC(dcode, int64_t(tt_host.code.size())); ALLOC(droff, int64_t(tt_host.ring_off.size()));
+      ALLOC(drcell, int64_t(tt_host.ring_cell.size())); ALLOC(drcode, int64_t(tt_host.ring_code.size()));
+      ALLOC(devoff, int64_t(tt_host.ev_off.size())); ALLOC(dev, int64_t(tt_host.ev.size()));
+      H2D(dcode, tt_host.code); H2D(droff, tt_host.ring_off); H2D(drcell, tt_host.ring_cell);
+      H2D(drcode, tt_host.ring_code); H2D(devoff, tt_host.ev_off); H2D(dev, tt_host.ev);
+#undef H2D
+#undef ALLOC
+      s->tiles = DevTiles{dcode, droff, drcell, drcode, devoff, dev, tt_host.ntiles};
+      s->fused_ok = true;
+    }
+  }
   if ((e = launch_setup_geometry(s->M, G, s->stream)) != hipSuccess) return bail(e, "setup_geometry");
   if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) return bail(e, "hipStreamSynchronize");
   if (n_inst > 0) {
@@ -378,6 +436,7 @@ int swe_get_state(swe_solver* s, double* prim) {
 int swe_compute_interface_values(swe_solver* s) {
   if (!s) return fail(SWE_ERR_ARG, "null solver");
   if (int rc = set_device(s)) return rc;
+  if (int rc = materialize_fluxes(s)) return rc;
   DevState S = state_of(s, s->cur);
   SWE_CHECK(launch_recon1(s->M, S, s->W, s->opts, s->stream), "recon1");
   SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
@@ -390,6 +449,7 @@ int swe_compute_fluxes(swe_solver* s, double* r) {
   if (int rc = set_device(s)) return rc;
   DevState S = state_of(s, s->cur);
   SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + 3, s->flux_items, s->flux_block, s->stream), "flux");
+  s->flux_stale = false;
   s->last_r_slot = 3;
   unsigned long long bits;
   SWE_CHECK(hipMemcpyAsync(&bits, s->rbits + 3, 8, hipMemcpyDeviceToHost, s->stream), "hipMemcpy");
@@ -436,6 +496,7 @@ int swe_get_src_field(swe_solver* s, double* src) {
 int swe_get_fluxes(swe_solver* s, double* f) {
   if (!s || !f) return fail(SWE_ERR_ARG, "null argument");
   if (int rc = set_device(s)) return rc;
+  if (int rc = materialize_fluxes(s)) return rc;
   std::vector<double> a(size_t(s->E)), b(size_t(s->E)), c(size_t(s->E));
   if (int rc = download(s, a.data(), s->W.f0, sizeof(double) * a.size())) return rc;
   if (int rc = download(s, b.data(), s->W.f1, sizeof(double) * b.size())) return rc;
@@ -464,10 +525,54 @@ int swe_get_cell_class(swe_solver* s, int8_t* cls) {
   return download(s, cls, s->W.cls, size_t(s->T));
 }
 
+int swe_reconstruct(swe_solver* s, int64_t cell, int kind, double* out5) {
+  if (int rc = single_mesh(s)) return rc;
+  if (!out5) return fail(SWE_ERR_ARG, "null argument");
+  if (cell < 0 || cell >= s->T) return fail(SWE_ERR_ARG, "cell index out of range");
+  if (kind < 0 || kind > 3) return fail(SWE_ERR_ARG, "kind must be 0 (dry), 1, 2 (full wet) or 3");
+  if (int rc = set_device(s)) return rc;
+  if (int rc = materialize_fluxes(s)) return rc;
+  DevState S = state_of(s, s->cur);
+  SWE_CHECK(launch_recon1(s->M, S, s->W, s->opts, s->stream), "recon1");
+  SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
+  double* d = nullptr;
+  SWE_CHECK(hipMalloc((void**)&d, 5 * sizeof(double)), "hipMalloc");
+  hipError_t e = launch_reconstruct_one(s->M, S, s->W, int(cell), kind, s->opts, d, s->stream);
+  int rc = (e != hipSuccess) ? dev_fail(e, "reconstruct") : download(s, out5, d, 5 * sizeof(double));
+  (void)hipStreamSynchronize(s->stream);
+  (void)hipFree(d);
+  return rc;
+}
+
+int swe_edge_flux(swe_solver* s, int flux, int wavespeed, int64_t e, int64_t from, int64_t to, double* r,
+                  double* out3) {
+  if (int rc = single_mesh(s)) return rc;
+  if (!out3) return fail(SWE_ERR_ARG, "null argument");
+  if (flux < 0 || flux > 1 || wavespeed < 0 || wavespeed > 2)
+    return fail(SWE_ERR_ARG, "unknown flux or wave-speed estimate");
+  if (e < 0 || e >= s->E) return fail(SWE_ERR_ARG, "edge index out of range");
+  if (from < 0 || from >= s->T || to >= s->T || to < -2 || to == from)
+    return fail(SWE_ERR_ARG, "cell index out of range");
+  if (int rc = set_device(s)) return rc;
+  double* d = nullptr;
+  SWE_CHECK(hipMalloc((void**)&d, 4 * sizeof(double)), "hipMalloc");
+  hipError_t he = launch_edge_flux_one(s->M, state_of(s, s->cur), s->W, flux, wavespeed, int(e), int(from),
+                                       int(to), s->cor, d, s->stream);
+  double o[4] = {0., 0., 0., 0.};
+  int rc = (he != hipSuccess) ? dev_fail(he, "edge_flux") : download(s, o, d, 4 * sizeof(double));
+  (void)hipStreamSynchronize(s->stream);
+  (void)hipFree(d);
+  if (rc) return rc;
+  out3[0] = o[0]; out3[1] = o[1]; out3[2] = o[2];
+  if (r && o[3] < *r) *r = o[3];  // include/Fluxes.h:42-48: r = min(r, candidate)
+  return SWE_OK;
+}
+
 int swe_rhs(swe_solver* s, int64_t cell, double dt, double* out3) {
   if (!s || !out3) return fail(SWE_ERR_ARG, "null argument");
   if (cell < 0 || cell >= s->T) return fail(SWE_ERR_ARG, "cell index out of range");
   if (int rc = set_device(s)) return rc;
+  if (int rc = materialize_fluxes(s)) return rc;
   DevState S = state_of(s, s->cur);
   double* d = nullptr;
   SWE_CHECK(hipMalloc((void**)&d, 3 * sizeof(double)), "hipMalloc");
@@ -759,10 +864,11 @@ int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nstep
     if (e != hipSuccess && !rc) rc = dev_fail(e, "timing events");
     *ms = f;
   } else {
-    // bracket each kernel of each stage with its own event pair
-    const int nk = 5;
-    double acc[nk] = {0, 0, 0, 0, 0};
-    int64_t cnt = 0;
+    // bracket each kernel of each stage with its own event pair: recon1,
+    // recon2, then flux, drain, update or the fused kernel
+    const int nk = 6;
+    double acc[nk] = {0, 0, 0, 0, 0, 0};
+    int64_t cnt[nk] = {0, 0, 0, 0, 0, 0};
     std::vector<hipEvent_t> ev(2 * nk, nullptr);
     for (auto& x : ev)
       if (!rc && hipEventCreate(&x) != hipSuccess) rc = fail(SWE_ERR_DEVICE, "hipEventCreate");
@@ -784,37 +890,52 @@ int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nstep
         else { in = c; u0 = a; out = a; a0 = 1. / 3.; a1 = 2. / 3.; coef = 2. / 3.; }
         DevState S = state_of(s, in);
         StageArgs A{a0, a1, coef, dt, cfl, s->cor, s->rbits, st == 0 ? s->dts + k : nullptr, friction(s)};
-        hipError_t e = hipEventRecord(ev[0], s->stream);
-        if (e == hipSuccess) e = launch_recon1(s->M, S, s->W, s->opts, s->stream);
-        if (e == hipSuccess) e = hipEventRecord(ev[1], s->stream);
-        if (e == hipSuccess) e = launch_recon2(s->M, S, s->W, s->opts, s->stream);
-        if (e == hipSuccess) e = hipEventRecord(ev[2], s->stream);
-        if (e == hipSuccess) e = hipEventRecord(ev[3], s->stream);
-        if (e == hipSuccess)
-          e = launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + st, s->flux_items, s->flux_block, s->stream);
-        if (e == hipSuccess) e = hipEventRecord(ev[4], s->stream);
-        if (e == hipSuccess) e = hipEventRecord(ev[5], s->stream);
-        if (e == hipSuccess) e = launch_drain(s->M, S, s->W, s->opts, s->stream);
-        if (e == hipSuccess) e = hipEventRecord(ev[6], s->stream);
-        if (e == hipSuccess) e = hipEventRecord(ev[7], s->stream);
-        if (e == hipSuccess) e = launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->upd_items, s->upd_block, s->stream);
-        if (e == hipSuccess) e = hipEventRecord(ev[8], s->stream);
-        if (e == hipSuccess) e = hipEventSynchronize(ev[8]);
+        bool used[nk] = {false, false, false, false, false, false};
+        hipError_t e = hipSuccess;
+        auto timed = [&](int q, auto&& launch) {
+          if (e == hipSuccess) e = hipEventRecord(ev[size_t(2 * q)], s->stream);
+          if (e == hipSuccess) e = launch();
+          if (e == hipSuccess) e = hipEventRecord(ev[size_t(2 * q + 1)], s->stream);
+          used[q] = true;
+        };
+        timed(0, [&] { return launch_recon1(s->M, S, s->W, s->opts, s->stream); });
+        timed(1, [&] { return launch_recon2(s->M, S, s->W, s->opts, s->stream); });
+        if (use_fused(s, dt, st)) {
+          timed(5, [&] {
+            return launch_fused(s->M, s->tiles, S, state_of(s, u0), out_of(s, out), s->W, A, s->flux, s->ws,
+                                s->opts, s->rbits + st, s->stream);
+          });
+          s->flux_stale = true;
+          s->stale_in = in;
+          s->stale_slot = st;
+        } else {
+          timed(2, [&] {
+            return launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + st, s->flux_items, s->flux_block,
+                               s->stream);
+          });
+          s->flux_stale = false;
+          timed(3, [&] { return launch_drain(s->M, S, s->W, s->opts, s->stream); });
+          timed(4, [&] {
+            return launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->upd_items, s->upd_block,
+                                 s->stream);
+          });
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
         if (e != hipSuccess) { rc = dev_fail(e, "kernel"); break; }
-        const int pairs[5][2] = {{0, 1}, {1, 2}, {3, 4}, {5, 6}, {7, 8}};
-        for (int q = 0; q < 5; ++q) {
+        for (int q = 0; q < nk; ++q) {
+          if (!used[q]) continue;
           float f = 0.f;
-          if (hipEventElapsedTime(&f, ev[size_t(pairs[q][0])], ev[size_t(pairs[q][1])]) != hipSuccess) f = 0.f;
+          if (hipEventElapsedTime(&f, ev[size_t(2 * q)], ev[size_t(2 * q + 1)]) != hipSuccess) f = 0.f;
           acc[q] += f;
+          cnt[q] += 1;
           total += f;
         }
-        ++cnt;
         s->last_r_slot = st;
       }
       if (scheme == SWE_SCHEME_EULER) s->cur = b;
     }
     s->last_nsteps = nsteps;
-    for (int q = 0; q < nk; ++q) kernel_ms[q] = cnt ? acc[q] / double(cnt) : 0.;
+    for (int q = 0; q < nk; ++q) kernel_ms[q] = cnt[q] ? acc[q] / double(cnt[q]) : 0.;
     *ms = total;
     for (auto& x : ev)
       if (x) (void)hipEventDestroy(x);
@@ -826,6 +947,15 @@ int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nstep
     if (e != hipSuccess) rc = dev_fail(e, "run");
   }
   return rc;
+}
+
+int swe_set_fused(swe_solver* s, int on, int* active) {
+  if (!s) return fail(SWE_ERR_ARG, "null solver");
+  if (int rc = set_device(s)) return rc;
+  if (int rc = materialize_fluxes(s)) return rc;
+  s->fused_on = on != 0;
+  if (active) *active = (s->fused_ok && s->fused_on && s->n_inst == 0) ? 1 : 0;
+  return SWE_OK;
 }
 
 int swe_set_tuning(swe_solver* s, int flux_items, int update_items) {
@@ -882,6 +1012,7 @@ int swe_autotune(swe_solver* s, int reps, int* flux_items, int* update_items, do
   if (!s) return fail(SWE_ERR_ARG, "null solver");
   if (reps < 1) return fail(SWE_ERR_ARG, "reps must be >= 1");
   if (int rc = set_device(s)) return rc;
+  if (int rc = materialize_fluxes(s)) return rc;  // the free buffer may hold a fused stage's input
   DevState S = state_of(s, s->cur);
   const int scratch = (s->cur + 1) % 3;
   unsigned long long* rslot = (s->n_inst > 0 ? s->B.rbits : s->rbits) + 3;
@@ -950,6 +1081,7 @@ int swe_integrals(swe_solver* s, double* out3, double* per_cell) {
   if (int rc = single_mesh(s)) return rc;
   if (!out3) return fail(SWE_ERR_ARG, "null argument");
   if (int rc = set_device(s)) return rc;
+  if (int rc = materialize_fluxes(s)) return rc;
   DevState S = state_of(s, s->cur);
   SWE_CHECK(launch_recon1(s->M, S, s->W, s->opts, s->stream), "recon1");
   SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
@@ -1089,6 +1221,7 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
     return fail(SWE_ERR_ARG, "the adjoint does not support the HEAD-compatibility options");
   const int64_t K = checkpoint_every > 0 ? std::min(checkpoint_every, nsteps) : nsteps;
   if (int rc = set_device(s)) return rc;
+  if (int rc = materialize_fluxes(s)) return rc;
   const int64_t T = s->T, E = s->E, V = s->V;
   StageCoef sc[3];
   const int ns = stage_list(scheme, sc);
@@ -1295,6 +1428,7 @@ int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int
   if (s->opts & (SWE_OPT_HEAD_GRADIENT | SWE_OPT_HEAD_PARTWET2))
     return fail(SWE_ERR_ARG, "the tangent run does not support the HEAD-compatibility options");
   if (int rc = set_device(s)) return rc;
+  if (int rc = materialize_fluxes(s)) return rc;
   const int64_t T = s->T, E = s->E, V = s->V;
   // tangent states (three, rotating like the values), node z tangent,
   // intermediates
@@ -1363,6 +1497,7 @@ int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int
 int swe_set_bathymetry(swe_solver* s, const double* z) {
   if (!s || !z) return fail(SWE_ERR_ARG, "null argument");
   if (int rc = set_device(s)) return rc;
+  if (int rc = materialize_fluxes(s)) return rc;
   SWE_CHECK(hipMemcpyAsync(const_cast<double*>(s->M.pz), z, sizeof(double) * size_t(s->V),
                            hipMemcpyHostToDevice, s->stream), "hipMemcpy");
   if (s->graph) {

@@ -72,6 +72,63 @@ __device__ __forceinline__ Slot slot_value(const Muscl& m, double cx, double cy,
   return s;
 }
 
+// TimeDisc::RHS (src/TimeDisc.cpp:3-41) of one cell from its gathered
+// inputs (RhsIn): fluxes of its three edges, its own and its neighbours'
+// draining time steps, its MUSCL and vertices.
+struct RhsIn {
+  uint32_t fl;
+  Muscl mu;
+  double wi;  // cell average level
+  V3 p[3];  // the cell's vertices
+  double dti;
+  double F0[3], F1[3], F2[3], dtik[3];
+};
+__device__ __forceinline__ void rhs_compute(const RhsIn& q, double dts, double cor, double fric,
+                                            double& r0, double& r1, double& r2) {
+  r0 = 0.; r1 = 0.; r2 = 0.;
+  // Domain::T / Area (src/Bathymetry.cpp:24-27,84-90)
+  const V3 c = centroid(q.p[0], q.p[1], q.p[2]);
+  const double i_area = 1. / tri_area(q.p[0], q.p[1], q.p[2]);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const bool own = sgn_pos(q.fl, k);
+    const double sg = own ? 1. : -1.;
+    const double F0 = q.F0[k], F1 = q.F1[k], F2 = q.F2[k];
+    const double dtk = (sg * F0) > 0. ? dmin(dts, q.dti) : dmin(dts, q.dtik[k]);
+    // Domain::L / E / Tang of edge k (src/Bathymetry.cpp:29-76); the tangent
+    // is taken outward of this cell, as Norm(e, i) is
+    const V3 Pa = pick(q.p[0], q.p[1], q.p[2], ep0_local(q.fl, k));
+    const V3 Pb = pick(q.p[0], q.p[1], q.p[2], ep1_local(q.fl, k));
+    const double le = edge_len(Pa, Pb);
+    const double c_ek = i_area * le;
+    const V3 E = midpoint(Pa, Pb);
+    const Slot s = slot_value(q.mu, c.x, c.y, E);
+    const double h_ek = s.w - E.z;
+    double src1 = q.mu.gx, src2 = q.mu.gy;
+    src1 += cor * (-s.v);
+    src2 += cor * s.u;
+    double tx, ty;
+    tangent(Pa, Pb, le, c, tx, ty);
+    const double nx = ty, ny = -tx;
+    const double cf = dtk * sg * c_ek;
+    r0 -= cf * F0;
+    r1 -= cf * F1;
+    r2 -= cf * F2;
+    const double cs = dts * (1. / 3.);
+    r1 -= cs * src1 * h_ek;
+    r2 -= cs * src2 * h_ek;
+    const double q2 = 0.5 * h_ek * h_ek;
+    r1 += dtk * (nx * c_ek * q2);
+    r2 += dtk * (ny * c_ek * q2);
+  }
+  // linear friction -tau (hu, hv) of the cell average (opt-in; the
+  // reference stores tau unused, include/SpaceDisc.h:47)
+  if (fric != 0.) {
+    const double hh = q.wi - c.z;
+    r1 -= dts * (fric * (hh * q.mu.u0));
+    r2 -= dts * (fric * (hh * q.mu.v0));
+  }
+}
 // grid of 256-thread workgroups covering n items, a multiple of 8 (one
 // contiguous range per XCD, see logical_block)
 // a store of a kernel output; NT (a template parameter: a runtime branch

@@ -69,6 +69,68 @@ __global__ void __launch_bounds__(256) k_setup_edges(DevMesh M, DevGeomOut G) {
   G.edmin[e] = dm;
 }
 
+// ReconstructFullWetCell (src/MUSCLObject.cpp:38-84): the limited plane of
+// cell (vertices p0..p2, flags fl, level w, centroid cx, cy) from its
+// neighbours' levels wn, records rn and ids jn (a partially wet neighbour's
+// full record is read from M.crec).
+template <bool HEAD_GRAD>
+__device__ __forceinline__ void fullwet_plane(const DevMesh& M, const V3& p0, const V3& p1, const V3& p2,
+                                              uint32_t fl, double w, double cx, double cy, const double wn[3],
+                                              const NRec rn[3], const int jn[3], double& gx, double& gy) {
+  gx = 0.;
+  gy = 0.;
+  V3 pts[3];
+  V3 E[3];
+  bool any_dry = false;
+#pragma unroll
+  for (int kk = 0; kk < 3; ++kk) {
+    E[kk] = edge_mid(p0, p1, p2, fl, kk);
+    const double wj = wn[kk], bcj = rn[kk].bc;
+    if (rn[kk].zmx < wj) {  // interior full-wet neighbour (zmx = +inf on boundary cells)
+      pts[kk] = {rn[kk].cx, rn[kk].cy, HEAD_GRAD ? bcj : wj};
+    } else if (!is_wet(wj - bcj)) {
+      any_dry = true;
+      pts[kk] = {0., 0., 0.};
+    } else {
+      const CellRec rj = M.crec[jn[kk]];  // partially wet neighbour: the full record
+      double w0j = partwet1_w(rj.zmin, rj.zmin, rj.zmax, bcj, wj);
+      double dx = E[kk].x - rn[kk].cx, dy = E[kk].y - rn[kk].cy;
+      double a = w0j + (0. * dx + 0. * dy);
+      if (!((a - E[kk].z) >= 0)) a = E[kk].z;
+      double val = 0.5 * (w + a);
+      pts[kk] = {E[kk].x, E[kk].y, HEAD_GRAD ? E[kk].z : val};
+    }
+  }
+  if (any_dry) return;
+  gradient(pts[0], pts[1], pts[2], gx, gy);
+  // vertex positivity (src/MUSCLObject.cpp:66-72)
+  const double c23 = 1. - 1. / 3., c13 = -1. / 3.;
+  bool all_wet = true;
+  {
+    double dxm = (p0.x * c23 + p1.x * c13) + p2.x * c13, dym = (p0.y * c23 + p1.y * c13) + p2.y * c13;
+    if (!is_wet(((gx * dxm + gy * dym) + w) - p0.z)) all_wet = false;
+  }
+  {
+    double dxm = (p0.x * c13 + p1.x * c23) + p2.x * c13, dym = (p0.y * c13 + p1.y * c23) + p2.y * c13;
+    if (!is_wet(((gx * dxm + gy * dym) + w) - p1.z)) all_wet = false;
+  }
+  {
+    double dxm = (p0.x * c13 + p1.x * c13) + p2.x * c23, dym = (p0.y * c13 + p1.y * c13) + p2.y * c23;
+    if (!is_wet(((gx * dxm + gy * dym) + w) - p2.z)) all_wet = false;
+  }
+  if (!all_wet) { gx = 0.; gy = 0.; }
+  // TVD limiter on w (src/MUSCLObject.cpp:74-81)
+  double tvd = 1.;
+#pragma unroll
+  for (int kk = 0; kk < 3; ++kk) {
+    double vtmin = dmin(w, wn[kk]), vtmax = dmax(w, wn[kk]);
+    double vek = w + (gx * (E[kk].x - cx) + gy * (E[kk].y - cy));
+    if (!((vtmin <= vek) && (vek <= vtmax))) tvd = 0.;
+  }
+  gx = tvd * gx;
+  gy = tvd * gy;
+}
+
 // ---------------------------------------------------------------- pass 1 --
 // SpaceDisc::ComputeInterfaceValues pass 1 (src/SpaceDisc.cpp:37-45) with
 // ReconstructDryCell / ReconstructPartWetCell1 / ReconstructFullWetCell
@@ -131,57 +193,7 @@ __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W
     W.wrec1[i] = w0;
   } else {
     k = 2;
-    V3 pts[3];
-    V3 E[3];
-    bool any_dry = false;
-#pragma unroll
-    for (int kk = 0; kk < 3; ++kk) {
-      E[kk] = edge_mid(p0, p1, p2, fl, kk);
-      const double wj = wn[kk], bcj = rn[kk].bc;
-      if (rn[kk].zmx < wj) {  // interior full-wet neighbour (zmx = +inf on boundary cells)
-        pts[kk] = {rn[kk].cx, rn[kk].cy, HEAD_GRAD ? bcj : wj};
-      } else if (!is_wet(wj - bcj)) {
-        any_dry = true;
-        pts[kk] = {0., 0., 0.};
-      } else {
-        const CellRec rj = M.crec[jn[kk]];  // partially wet neighbour: the full record
-        double w0j = partwet1_w(rj.zmin, rj.zmin, rj.zmax, bcj, wj);
-        double dx = E[kk].x - rn[kk].cx, dy = E[kk].y - rn[kk].cy;
-        double a = w0j + (0. * dx + 0. * dy);
-        if (!((a - E[kk].z) >= 0)) a = E[kk].z;
-        double val = 0.5 * (w + a);
-        pts[kk] = {E[kk].x, E[kk].y, HEAD_GRAD ? E[kk].z : val};
-      }
-    }
-    if (!any_dry) {
-      gradient(pts[0], pts[1], pts[2], gx, gy);
-      // vertex positivity (src/MUSCLObject.cpp:66-72)
-      const double c23 = 1. - 1. / 3., c13 = -1. / 3.;
-      bool all_wet = true;
-      {
-        double dxm = (p0.x * c23 + p1.x * c13) + p2.x * c13, dym = (p0.y * c23 + p1.y * c13) + p2.y * c13;
-        if (!is_wet(((gx * dxm + gy * dym) + w) - p0.z)) all_wet = false;
-      }
-      {
-        double dxm = (p0.x * c13 + p1.x * c23) + p2.x * c13, dym = (p0.y * c13 + p1.y * c23) + p2.y * c13;
-        if (!is_wet(((gx * dxm + gy * dym) + w) - p1.z)) all_wet = false;
-      }
-      {
-        double dxm = (p0.x * c13 + p1.x * c13) + p2.x * c23, dym = (p0.y * c13 + p1.y * c13) + p2.y * c23;
-        if (!is_wet(((gx * dxm + gy * dym) + w) - p2.z)) all_wet = false;
-      }
-      if (!all_wet) { gx = 0.; gy = 0.; }
-      // TVD limiter on w (src/MUSCLObject.cpp:74-81)
-      double tvd = 1.;
-#pragma unroll
-      for (int kk = 0; kk < 3; ++kk) {
-        double vtmin = dmin(w, wn[kk]), vtmax = dmax(w, wn[kk]);
-        double vek = w + (gx * (E[kk].x - cx) + gy * (E[kk].y - cy));
-        if (!((vtmin <= vek) && (vek <= vtmax))) tvd = 0.;
-      }
-      gx = tvd * gx;
-      gy = tvd * gy;
-    }
+    fullwet_plane<HEAD_GRAD>(M, p0, p1, p2, fl, w, cx, cy, wn, rn, jn, gx, gy);
   }
   // outputs read only by later kernels: out_store, nontemporal on large
   // meshes, keeps L2 for the neighbour gathers (-1.5 % per step at 16M cells)
@@ -231,8 +243,11 @@ __device__ inline double node_max_pass1(const DevMesh& M, const DevWork& W, int 
   return mx;
 }
 
+// the final MUSCL (w0, gx, gy) of a partially wet cell i; w1: its pass-1
+// (PartWet1) level, the result where pass 2 falls back to it
 template <bool HEAD_PW2>
-__device__ inline void recon2_cell(const DevMesh& M, const DevState& S, const DevWork& W, int i) {
+__device__ inline void partwet2_muscl(const DevMesh& M, const DevState& S, const DevWork& W, int i, double w1,
+                                      double& w0_out, double& gx_out, double& gy_out) {
   int i0 = M.tp[i], i1 = M.tp[M.T + i], i2 = M.tp[2 * M.T + i];
   V3 Q0 = node(M, i0), Q1 = node(M, i1), Q2 = node(M, i2);
   const double cx = M.cx[i], cy = M.cy[i], bc = M.bc[i];
@@ -247,7 +262,7 @@ __device__ inline void recon2_cell(const DevMesh& M, const DevState& S, const De
   (void)i2;
   double w = S.w[i];
   double b23 = Q0.z, b12 = Q1.z, b13 = Q2.z;
-  double w0 = W.wrec1[i], gx = 0., gy = 0.;  // fallback: pass-1 (PartWet1) result
+  double w0 = w1, gx = 0., gy = 0.;  // fallback: pass-1 (PartWet1) result
   bool fallback = (w > b13) || (b13 - b23 < kTol);
   if (!fallback) {
     double w23 = node_max_pass1(M, W, i0, Q0);
@@ -291,6 +306,15 @@ __device__ inline void recon2_cell(const DevMesh& M, const DevState& S, const De
       w0 = w23 + (gx * (cx - Q0.x) + gy * (cy - Q0.y));
     }
   }
+  w0_out = w0;
+  gx_out = gx;
+  gy_out = gy;
+}
+
+template <bool HEAD_PW2>
+__device__ inline void recon2_cell(const DevMesh& M, const DevState& S, const DevWork& W, int i) {
+  double w0, gx, gy;
+  partwet2_muscl<HEAD_PW2>(M, S, W, i, W.wrec1[i], w0, gx, gy);
   // in place: readers of pass-1 values take partially wet cells from wrec1
   W.mw[i] = w0;
   W.mgx[i] = gx;
@@ -462,14 +486,6 @@ __global__ void __launch_bounds__(256) k_drain(DevMesh M, DevState S, DevWork W)
 // TimeDisc::RHS (src/TimeDisc.cpp:3-41) + stage combination
 // (src/Solvers.cpp:11-13,23-25,30-32,42-58) + ConsAssigner.  The inputs of a
 // cell are gathered first (RhsIn), the arithmetic follows.
-struct RhsIn {
-  uint32_t fl;
-  Muscl mu;
-  double wi;  // cell average level
-  V3 p[3];  // the cell's vertices
-  double dti;
-  double F0[3], F1[3], F2[3], dtik[3];
-};
 __device__ __forceinline__ void rhs_load(const DevMesh& M, const DevState& S, const DevWork& W,
                                          int i, RhsIn& q) {
   q.fl = M.cflag[i];
@@ -488,52 +504,6 @@ __device__ __forceinline__ void rhs_load(const DevMesh& M, const DevState& S, co
     q.F2[k] = W.f2[e];
     const double d = W.dtd[nb < 0 ? i : nb];
     q.dtik[k] = nb < 0 ? INFINITY : d;
-  }
-}
-__device__ __forceinline__ void rhs_compute(const RhsIn& q, double dts, double cor, double fric,
-                                            double& r0, double& r1, double& r2) {
-  r0 = 0.; r1 = 0.; r2 = 0.;
-  // Domain::T / Area (src/Bathymetry.cpp:24-27,84-90)
-  const V3 c = centroid(q.p[0], q.p[1], q.p[2]);
-  const double i_area = 1. / tri_area(q.p[0], q.p[1], q.p[2]);
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const bool own = sgn_pos(q.fl, k);
-    const double sg = own ? 1. : -1.;
-    const double F0 = q.F0[k], F1 = q.F1[k], F2 = q.F2[k];
-    const double dtk = (sg * F0) > 0. ? dmin(dts, q.dti) : dmin(dts, q.dtik[k]);
-    // Domain::L / E / Tang of edge k (src/Bathymetry.cpp:29-76); the tangent
-    // is taken outward of this cell, as Norm(e, i) is
-    const V3 Pa = pick(q.p[0], q.p[1], q.p[2], ep0_local(q.fl, k));
-    const V3 Pb = pick(q.p[0], q.p[1], q.p[2], ep1_local(q.fl, k));
-    const double le = edge_len(Pa, Pb);
-    const double c_ek = i_area * le;
-    const V3 E = midpoint(Pa, Pb);
-    const Slot s = slot_value(q.mu, c.x, c.y, E);
-    const double h_ek = s.w - E.z;
-    double src1 = q.mu.gx, src2 = q.mu.gy;
-    src1 += cor * (-s.v);
-    src2 += cor * s.u;
-    double tx, ty;
-    tangent(Pa, Pb, le, c, tx, ty);
-    const double nx = ty, ny = -tx;
-    const double cf = dtk * sg * c_ek;
-    r0 -= cf * F0;
-    r1 -= cf * F1;
-    r2 -= cf * F2;
-    const double cs = dts * (1. / 3.);
-    r1 -= cs * src1 * h_ek;
-    r2 -= cs * src2 * h_ek;
-    const double q2 = 0.5 * h_ek * h_ek;
-    r1 += dtk * (nx * c_ek * q2);
-    r2 += dtk * (ny * c_ek * q2);
-  }
-  // linear friction -tau (hu, hv) of the cell average (opt-in; the
-  // reference stores tau unused, include/SpaceDisc.h:47)
-  if (fric != 0.) {
-    const double hh = q.wi - c.z;
-    r1 -= dts * (fric * (hh * q.mu.u0));
-    r2 -= dts * (fric * (hh * q.mu.v0));
   }
 }
 __device__ __forceinline__ void rhs_cell(const DevMesh& M, const DevState& S, const DevWork& W,
@@ -872,6 +842,68 @@ __global__ void __launch_bounds__(256) k_stats_final(const double* partial, int6
   }
 }
 
+// One reconstruction of one cell for the MUSCLObject::Reconstruct* getters
+// (src/MUSCLObject.cpp:31-191): kind 0 dry, 1 PartWet1, 2 FullWet, 3
+// PartWet2 (with the pass-1 node maximum of the last interface values).
+// out = (origin w, u, v, gradient of w x, y).
+template <bool HEAD_GRAD, bool HEAD_PW2>
+__global__ void k_reconstruct_one(DevMesh M, DevState S, DevWork W, int i, int kind, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const CellRec ri = M.crec[i];
+  const uint32_t fl = M.cflag[i];
+  const V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[M.T + i]), p2 = node(M, M.tp[2 * M.T + i]);
+  const double w = S.w[i];
+  double o0 = w, o1 = S.u[i], o2 = S.v[i], gx = 0., gy = 0.;
+  const double lvl1 = partwet1_w(ri.zmin, ri.zmin, ri.zmax, ri.bc, w);
+  if (kind == 0) {
+    o0 = ri.bc; o1 = 0.; o2 = 0.;
+    gradient(p0, p1, p2, gx, gy);
+  } else if (kind == 1) {
+    o0 = lvl1;
+  } else if (kind == 2) {
+    double wn[3];
+    NRec rn[3];
+    int jn[3];
+    for (int kk = 0; kk < 3; ++kk) {
+      int j = M.tt[kk * M.T + i];
+      j = j < 0 ? i : j;  // a wall has no neighbour: the cell stands in, as in pass 1
+      jn[kk] = j;
+      wn[kk] = S.w[j];
+      const CellRec r = M.crec[j];
+      rn[kk] = NRec{r.cx, r.cy, (r.fl & CELL_BND) ? INFINITY : r.zmax, r.bc};
+    }
+    fullwet_plane<HEAD_GRAD>(M, p0, p1, p2, fl, w, ri.cx, ri.cy, wn, rn, jn, gx, gy);
+  } else {
+    partwet2_muscl<HEAD_PW2>(M, S, W, i, lvl1, o0, gx, gy);
+  }
+  out[0] = o0; out[1] = o1; out[2] = o2; out[3] = gx; out[4] = gy;
+}
+
+// Fluxes::HLL<W> / HLLC<W> (include/Fluxes.h:14-111) of edge e seen from
+// cell `from` (normal outward of it, left state its slot), with the slots
+// of the last interface values; `to` < 0 takes the left state for the
+// right one.  out = flux (3), then the length-to-wavespeed candidate (or
+// +inf when the flux returned before it).
+template <int FLUX, int WS>
+__global__ void k_edge_flux_one(DevMesh M, DevState S, DevWork W, int e, int from, int to, double cor,
+                                double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const V3 Pa = node(M, M.ep0[e]), Pb = node(M, M.ep1[e]);
+  const double le = edge_len(Pa, Pb);
+  const V3 E = midpoint(Pa, Pb);
+  double tx, ty;
+  tangent(Pa, Pb, le, V3{M.cx[from], M.cy[from], 0.}, tx, ty);
+  const double nx = ty, ny = -tx;
+  const Slot L = slot_value(load_muscl(S, W, from), M.cx[from], M.cy[from], E);
+  const Slot R = to < 0 ? L : slot_value(load_muscl(S, W, to), M.cx[to], M.cy[to], E);
+  const double dl = 2. * M.area[from] / le, dr = to < 0 ? dl : 2. * M.area[to] / le;
+  double lw = INFINITY;
+  double F[3];
+  if (FLUX == 0) flux_hll<WS>(nx, ny, E.z, L, R, dmin(dl, dr), cor, lw, F);
+  else flux_hllc<WS>(tx, ty, E.z, L, R, dmin(dl, dr), cor, lw, F);
+  out[0] = F[0]; out[1] = F[1]; out[2] = F[2]; out[3] = lw;
+}
+
 __global__ void k_rhs_one(DevMesh M, DevState S, DevWork W, int i, double dt, double cor,
                           double fric, double* out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -996,6 +1028,23 @@ hipError_t launch_integrals(const DevMesh& M, const DevState& S, const DevWork& 
     hipLaunchKernelGGL(k_integrals, dim3((unsigned)nb), dim3(256), 0, st, M, S, W, partial, per_cell);
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, partial, nb, out);
   return hipGetLastError();
+}
+hipError_t launch_reconstruct_one(const DevMesh& M, const DevState& S, const DevWork& W, int i, int kind,
+                                  int opts, double* out, hipStream_t st) {
+  const bool hg = (opts & 1) != 0, hp = (opts & 2) != 0;
+  if (hg && hp) hipLaunchKernelGGL((k_reconstruct_one<true, true>), dim3(1), dim3(64), 0, st, M, S, W, i, kind, out);
+  else if (hg) hipLaunchKernelGGL((k_reconstruct_one<true, false>), dim3(1), dim3(64), 0, st, M, S, W, i, kind, out);
+  else if (hp) hipLaunchKernelGGL((k_reconstruct_one<false, true>), dim3(1), dim3(64), 0, st, M, S, W, i, kind, out);
+  else hipLaunchKernelGGL((k_reconstruct_one<false, false>), dim3(1), dim3(64), 0, st, M, S, W, i, kind, out);
+  return hipGetLastError();
+}
+hipError_t launch_edge_flux_one(const DevMesh& M, const DevState& S, const DevWork& W, int flux, int ws, int e,
+                                int from, int to, double cor, double* out, hipStream_t st) {
+#define SWE_EF(F, Wv) \
+  if (flux == F && ws == Wv) { hipLaunchKernelGGL((k_edge_flux_one<F, Wv>), dim3(1), dim3(64), 0, st, M, S, W, e, from, to, cor, out); return hipGetLastError(); }
+  SWE_EF(0, 0) SWE_EF(0, 1) SWE_EF(0, 2) SWE_EF(1, 0) SWE_EF(1, 1) SWE_EF(1, 2)
+#undef SWE_EF
+  return hipErrorInvalidValue;
 }
 hipError_t launch_rhs_one(const DevMesh& M, const DevState& S, const DevWork& W, int i, double dt,
                           double cor, double fric, double* out, hipStream_t st) {

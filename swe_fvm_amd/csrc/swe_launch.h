@@ -245,6 +245,24 @@ hipError_t launch_adjoint_tables(const DevMesh& M, const DevAdj& J, hipStream_t 
 // after the last reverse stage: J.sbar's w part gathers J.w_nb; zbar (if not
 // null) += the corners of J.zc at every node
 hipError_t launch_adjoint_finish(const DevMesh& M, const DevAdj& J, double* zbar, hipStream_t st);
+// Fluxes, draining time steps and update of one stage in one kernel over
+// tiles (swe_fused.hip); the reconstruction of S must be in W.  The stage's
+// step size must be known (A.dt_fixed > 0, or the CFL r of an earlier pass
+// in A.rbits[0]); the pass's own CFL length-to-wavespeed minimum goes to
+// *rbits.  W's fluxes and draining time steps are not written.  Single
+// meshes only.
+struct DevTiles;
+hipError_t launch_fused(const DevMesh& M, const DevTiles& P, const DevState& S, const DevState& U0,
+                        const DevOut& O, const DevWork& W, const StageArgs& A, int flux, int ws, int opts,
+                        unsigned long long* rbits, hipStream_t st);
+// one reconstruction (kind 0 dry, 1 PartWet1, 2 FullWet, 3 PartWet2) of
+// cell i: out[5] = origin (w, u, v), gradient of w (x, y)
+hipError_t launch_reconstruct_one(const DevMesh& M, const DevState& S, const DevWork& W, int i, int kind,
+                                  int opts, double* out, hipStream_t st);
+// flux of edge e from cell `from` to cell `to` (< 0: right state = left):
+// out[4] = flux, length-to-wavespeed candidate (+inf if none)
+hipError_t launch_edge_flux_one(const DevMesh& M, const DevState& S, const DevWork& W, int flux, int ws, int e,
+                                int from, int to, double cor, double* out, hipStream_t st);
 hipError_t launch_rhs_one(const DevMesh& M, const DevState& S, const DevWork& W, int i, double dt,
                           double cor, double fric, double* out, hipStream_t st);
 

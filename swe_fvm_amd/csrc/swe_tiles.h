@@ -1,0 +1,43 @@
+// Tiles of the fused flux / draining / update kernel: host-built static
+// tables (swe_tiles.cpp) and their device view (swe_fused.hip).
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+namespace swe {
+
+constexpr int kTileCells = 256;     // cells per tile (= threads per workgroup)
+constexpr int kTileMaxLocal = 512;  // tile cells + ring
+constexpr int kTileMaxEval = 768;   // edge evaluations per tile (LDS flux slots)
+constexpr uint32_t kNbWall = 0xffffu, kNbFree = 0xfffeu, kNbOut = 0xfffdu;
+
+struct TileTables {
+  int64_t ntiles = 0;
+  std::vector<uint32_t> code;       // [3][T] tile cells: slot << 16 | neighbour code
+  std::vector<int> ring_off;        // [ntiles + 1]
+  std::vector<int> ring_cell;       // ring cells (global ids), tile by tile
+  std::vector<uint32_t> ring_code;  // [ring entry][3] codes of the ring cells
+  std::vector<int> ev_off;          // [ntiles + 1]
+  std::vector<uint16_t> ev;         // evaluations: local cell << 2 | k
+  int max_eval = 0, max_ring = 0;
+};
+
+// false when a tile exceeds the kernel's LDS capacity (a mesh far from a
+// cell-local order) or the tables are inconsistent: the fused kernel is
+// then not used
+bool build_tiles(int64_t T, const int* tt, const int* te, const uint32_t* cflag, const int* et1,
+                 TileTables& out);
+
+struct DevTiles {
+  const uint32_t* code;
+  const int* ring_off;
+  const int* ring_cell;
+  const uint32_t* ring_code;
+  const int* ev_off;
+  const uint16_t* ev;
+  int64_t ntiles;
+};
+
+}  // namespace swe
