@@ -328,6 +328,8 @@ def main():
     ap.add_argument("--adj-fd", type=int, default=16, help="finite-difference directions")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default work items per thread (profiling runs)")
+    ap.add_argument("--no-fused", action="store_true",
+                    help="separate flux, draining and update kernels in every stage (A/B runs)")
     args = ap.parse_args()
     if args.ensemble:
         bench_ensemble(args)
@@ -370,6 +372,7 @@ def main():
 
     fl = swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt]
     sd = swe.SpaceDisc(fl, dom, prim, device=0)
+    fused = sd.set_fused(not args.no_fused)
     if args.warmup:
         sd.run("SSPRK2", args.warmup, dt=0.0, cfl=0.15)
     # work items per thread: the fastest bit-identical kernel instantiation
@@ -416,7 +419,7 @@ def main():
         "config": {"workload": workload, "cells": T, "edges": E, "nodes": V,
                    "mesh": f"seeded Delaunay of {args.n}x{args.n} jittered grid, Morton order",
                    "scheme": "SSPRK2", "flux": "HLLC<Einfeldt>", "dt": "CFL 0.15",
-                   "parallelism": "single GPU",
+                   "parallelism": "single GPU", "fused_stages": fused,
                    "tuning": {"flux_edges_per_thread": tune["flux_items"],
                               "update_cells_per_thread": tune["update_items"],
                               "flux_block_threads": tune["flux_block"],

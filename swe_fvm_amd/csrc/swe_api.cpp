@@ -94,11 +94,11 @@ int set_device(swe_solver* s) {
   return SWE_OK;
 }
 
-// The fluxes of the last fused stage, recomputed into W from that stage's
-// input (still intact: no stage writes its own input) and its
-// reconstruction, for the getters that read W's fluxes and before W's
-// reconstruction is overwritten.  Bit-identical to what the separate flux
-// kernel would have left.
+// The fluxes and CFL minimum of the last fused stage, recomputed into W and
+// its r slot from that stage's input (still intact: no stage writes its own
+// input) and its reconstruction, for the getters that read them and before
+// W's reconstruction is overwritten.  Bit-identical to what the separate
+// flux kernel would have left.
 int materialize_fluxes(swe_solver* s) {
   if (!s->flux_stale) return SWE_OK;
   SWE_CHECK(launch_flux(s->M, state_of(s, s->stale_in), s->W, s->flux, s->ws, s->cor, s->rbits + s->stale_slot,
@@ -122,7 +122,7 @@ int run_stage(swe_solver* s, int in, int u0, int out, double a0, double a1, doub
   StageArgs A{a0, a1, coef, dt_fixed, cfl, s->cor, s->rbits, dts_log, friction(s)};
   if (use_fused(s, dt_fixed, slot)) {
     SWE_CHECK(launch_fused(s->M, s->tiles, S, state_of(s, u0), out_of(s, out), s->W, A, s->flux, s->ws, s->opts,
-                           s->rbits + slot, s->stream), "fused");
+                           s->stream), "fused");
     s->flux_stale = true;
     s->stale_in = in;
     s->stale_slot = slot;
@@ -304,19 +304,19 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
   s->G = G;
   if (n_inst == 0) {
     TileTables tt_host;
-    if (build_tiles(T, tt.data(), te.data(), flag.data(), et1.data(), tt_host)) {
+    if (build_tiles(T, tt.data(), te.data(), flag.data(), et0.data(), et1.data(), ep0.data(), ep1.data(), tt_host)) {
       uint32_t *dcode, *drcode;
       int *droff, *drcell, *devoff;
-      uint16_t* dev;
+      EdgeRec* dev;
 #define ALLOC(ptr, n) if ((e = dalloc(s, &ptr, n)) != hipSuccess) return bail(e, "hipMalloc")
 #define H2D(dst, v)                                                                                      \
   if (!v.empty() && (e = hipMemcpy(dst, v.data(), sizeof(v[0]) * v.size(), hipMemcpyHostToDevice)) != hipSuccess) \
     return bail(e, "hipMemcpy");
       ALLOC(dcode, int64_t(tt_host.code.size())); ALLOC(droff, int64_t(tt_host.ring_off.size()));
       ALLOC(drcell, int64_t(tt_host.ring_cell.size())); ALLOC(drcode, int64_t(tt_host.ring_code.size()));
-      ALLOC(devoff, int64_t(tt_host.ev_off.size())); ALLOC(dev, int64_t(tt_host.ev.size()));
+      ALLOC(devoff, int64_t(tt_host.ev_off.size())); ALLOC(dev, int64_t(tt_host.evrec.size()));
       H2D(dcode, tt_host.code); H2D(droff, tt_host.ring_off); H2D(drcell, tt_host.ring_cell);
-      H2D(drcode, tt_host.ring_code); H2D(devoff, tt_host.ev_off); H2D(dev, tt_host.ev);
+      H2D(drcode, tt_host.ring_code); H2D(devoff, tt_host.ev_off); H2D(dev, tt_host.evrec);
 #undef H2D
 #undef ALLOC
       s->tiles = DevTiles{dcode, droff, drcell, drcode, devoff, dev, tt_host.ntiles};
@@ -619,6 +619,7 @@ int swe_cfl_dt(swe_solver* s, double* dt) {
   if (int rc = single_mesh(s)) return rc;
   if (!dt) return fail(SWE_ERR_ARG, "null argument");
   if (int rc = set_device(s)) return rc;
+  if (int rc = materialize_fluxes(s)) return rc;
   unsigned long long bits;
   if (int rc = download(s, &bits, s->rbits + s->last_r_slot, 8)) return rc;
   double r;
@@ -631,6 +632,7 @@ int swe_min_len_to_wavespeed(swe_solver* s, double* r) {
   if (int rc = single_mesh(s)) return rc;
   if (!r) return fail(SWE_ERR_ARG, "null argument");
   if (int rc = set_device(s)) return rc;
+  if (int rc = materialize_fluxes(s)) return rc;
   unsigned long long bits;
   if (int rc = download(s, &bits, s->rbits + s->last_r_slot, 8)) return rc;
   std::memcpy(r, &bits, 8);
@@ -903,7 +905,7 @@ int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nstep
         if (use_fused(s, dt, st)) {
           timed(5, [&] {
             return launch_fused(s->M, s->tiles, S, state_of(s, u0), out_of(s, out), s->W, A, s->flux, s->ws,
-                                s->opts, s->rbits + st, s->stream);
+                                s->opts, s->stream);
           });
           s->flux_stale = true;
           s->stale_in = in;

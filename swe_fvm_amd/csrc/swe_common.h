@@ -72,19 +72,24 @@ __device__ __forceinline__ Slot slot_value(const Muscl& m, double cx, double cy,
   return s;
 }
 
-// TimeDisc::RHS (src/TimeDisc.cpp:3-41) of one cell from its gathered
-// inputs (RhsIn): fluxes of its three edges, its own and its neighbours'
-// draining time steps, its MUSCL and vertices.
-struct RhsIn {
+// TimeDisc::RHS (src/TimeDisc.cpp:3-41) of one cell from its inputs: its
+// MUSCL, vertices and draining time step (RhsCell), and per local edge k
+// the edge's flux and the neighbour's draining time step, which the
+// accessor edge(k, F0, F1, F2, dtik) supplies (from registers in the
+// update kernel, from LDS in the fused kernel: the same arithmetic).
+struct RhsCell {
   uint32_t fl;
   Muscl mu;
   double wi;  // cell average level
   V3 p[3];  // the cell's vertices
   double dti;
+};
+struct RhsIn : RhsCell {
   double F0[3], F1[3], F2[3], dtik[3];
 };
-__device__ __forceinline__ void rhs_compute(const RhsIn& q, double dts, double cor, double fric,
-                                            double& r0, double& r1, double& r2) {
+template <class EdgeIn>
+__device__ __forceinline__ void rhs_compute_by(const RhsCell& q, EdgeIn edge, double dts, double cor,
+                                               double fric, double& r0, double& r1, double& r2) {
   r0 = 0.; r1 = 0.; r2 = 0.;
   // Domain::T / Area (src/Bathymetry.cpp:24-27,84-90)
   const V3 c = centroid(q.p[0], q.p[1], q.p[2]);
@@ -93,8 +98,9 @@ __device__ __forceinline__ void rhs_compute(const RhsIn& q, double dts, double c
   for (int k = 0; k < 3; ++k) {
     const bool own = sgn_pos(q.fl, k);
     const double sg = own ? 1. : -1.;
-    const double F0 = q.F0[k], F1 = q.F1[k], F2 = q.F2[k];
-    const double dtk = (sg * F0) > 0. ? dmin(dts, q.dti) : dmin(dts, q.dtik[k]);
+    double F0, F1, F2, dtik;
+    edge(k, F0, F1, F2, dtik);
+    const double dtk = (sg * F0) > 0. ? dmin(dts, q.dti) : dmin(dts, dtik);
     // Domain::L / E / Tang of edge k (src/Bathymetry.cpp:29-76); the tangent
     // is taken outward of this cell, as Norm(e, i) is
     const V3 Pa = pick(q.p[0], q.p[1], q.p[2], ep0_local(q.fl, k));
@@ -128,6 +134,15 @@ __device__ __forceinline__ void rhs_compute(const RhsIn& q, double dts, double c
     r1 -= dts * (fric * (hh * q.mu.u0));
     r2 -= dts * (fric * (hh * q.mu.v0));
   }
+}
+__device__ __forceinline__ void rhs_compute(const RhsIn& q, double dts, double cor, double fric,
+                                            double& r0, double& r1, double& r2) {
+  rhs_compute_by(
+      q,
+      [&](int k, double& F0, double& F1, double& F2, double& dtik) {
+        F0 = q.F0[k]; F1 = q.F1[k]; F2 = q.F2[k]; dtik = q.dtik[k];
+      },
+      dts, cor, fric, r0, r1, r2);
 }
 // grid of 256-thread workgroups covering n items, a multiple of 8 (one
 // contiguous range per XCD, see logical_block)

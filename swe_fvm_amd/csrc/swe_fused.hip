@@ -15,9 +15,8 @@
 //
 // The stage step size must be known before the kernel: a fixed dt, or the
 // CFL step of an earlier flux pass of the same step (stages 2 and 3).  The
-// CFL length-to-wavespeed minimum of this pass is reduced as the flux kernel
-// does (duplicated evaluations repeat a candidate, which leaves the minimum
-// unchanged).
+// pass's own CFL length-to-wavespeed minimum is not taken: the solver
+// recomputes it with the fluxes when a getter asks for them.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -33,27 +32,9 @@ __device__ __forceinline__ double stage_dt_fused(const StageArgs& A) {
   return dt;
 }
 
-__device__ __forceinline__ void block_min_fused(double v, unsigned long long* target) {
-  __shared__ double red[kTileCells / 64];
-  for (int off = 32; off > 0; off >>= 1) v = dmin(v, __shfl_xor(v, off, 64));
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) red[wid] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = red[0];
-    for (int q = 1; q < kTileCells / 64; ++q) m = dmin(m, red[q]);
-    if (m < 1.0) atomicMin(target, (unsigned long long)__double_as_longlong(m));
-  }
-}
-
-__global__ void k_reset_r_fused(unsigned long long* rbits) {
-  *rbits = (unsigned long long)__double_as_longlong(1.0);
-}
-
 template <int FLUX, int WS, bool LEN, bool NT>
-__global__ void __launch_bounds__(kTileCells) k_fused(DevMesh M, DevTiles P, DevState S, DevState U0,
-                                                      DevOut O, DevWork W, StageArgs A,
-                                                      unsigned long long* rbits) {
+__global__ void __launch_bounds__(kTileCells, 4) k_fused(DevMesh M, DevTiles P, DevState S, DevState U0,
+                                                      DevOut O, DevWork W, StageArgs A) {
   __shared__ double sF0[kTileMaxEval], sF1[kTileMaxEval], sF2[kTileMaxEval];
   __shared__ double sD[kTileMaxLocal];
   const int64_t tile = logical_block();
@@ -64,30 +45,15 @@ __global__ void __launch_bounds__(kTileCells) k_fused(DevMesh M, DevTiles P, Dev
   const int e0 = P.ev_off[tile], ne = P.ev_off[tile + 1] - e0;
   const int tid = threadIdx.x;
   const int64_t T = M.T;
-  auto global = [&](int l) -> int { return l < kTileCells ? (int)(c0 + l) : P.ring_cell[rg0 + l - kTileCells]; };
-  auto code_of = [&](int l, int g, int k) -> uint32_t {
-    return l < kTileCells ? P.code[k * T + g] : P.ring_code[3 * (rg0 + l - kTileCells) + k];
-  };
 
-  // ---- edge fluxes (k_flux's operations), one evaluation per slot
-  double lw = 1.0;
+  // ---- edge fluxes (k_flux's operations), one evaluation per LDS slot;
+  // the edge record gives both cells and nodes (one load level, as the
+  // separate flux kernel's edge tables)
   for (int j = tid; j < ne; j += kTileCells) {
-    const uint32_t ev = P.ev[e0 + j];
-    const int l = (int)(ev >> 2), k = (int)(ev & 3u);
-    const int x = global(l);
-    const uint32_t nb = code_of(l, x, k) & 0xffffu;
-    const uint32_t fl = M.cflag[x];
-    int y, t;
-    if (nb == kNbWall) { y = -1; t = kSolidWall; }
-    else if (nb == kNbFree) { y = -1; t = -2; }
-    else { y = nb == kNbOut ? M.tt[k * T + x] : global((int)nb); t = y; }
-    const bool own0 = sgn_pos(fl, k);  // x is the edge's et0 cell
-    const int f = own0 ? x : y;
-    if (!own0) t = x;
-    const int tr = t < 0 ? f : t;
-    const V3 Pa = node(M, M.tp[ep0_local(fl, k) * T + x]);
-    const V3 Pb = node(M, M.tp[ep1_local(fl, k) * T + x]);
-    const double edmin = M.edmin[M.te[k * T + x]];
+    const EdgeRec r = P.evrec[e0 + j];
+    const int f = r.et0, t = r.et1;
+    const int tr = t < 0 ? f : t;  // boundary edges read their own cell twice: R = L
+    const V3 Pa = node(M, r.ep0), Pb = node(M, r.ep1);
     const Muscl mf = load_muscl(S, W, f), mt = load_muscl(S, W, tr);
     const double cxf = M.cx[f], cyf = M.cy[f], cxt = M.cx[tr], cyt = M.cy[tr];
     const double hf = t == kSolidWall ? S.w[f] - M.bc[f] : 0.;
@@ -97,27 +63,30 @@ __global__ void __launch_bounds__(kTileCells) k_fused(DevMesh M, DevTiles P, Dev
     tangent(Pa, Pb, le, V3{cxf, cyf, 0.}, tx, ty);
     const double nx = ty, ny = -tx;
     double F[3];
-    double lwi = 1.0;
+    double lw = 1.0;  // the CFL candidate is not taken here (unused)
     if (t == kSolidWall) {
       elem_flux(nx, ny, hf, 0., 0., F[0], F[1], F[2]);
     } else {
       const Slot L = slot_value(mf, cxf, cyf, E);
       const Slot R = slot_value(mt, cxt, cyt, E);
-      if (FLUX == 0) flux_hll<WS>(nx, ny, E.z, L, R, edmin, A.cor, lwi, F);
-      else flux_hllc<WS>(tx, ty, E.z, L, R, edmin, A.cor, lwi, F);
+      if (FLUX == 0) flux_hll<WS>(nx, ny, E.z, L, R, 1.0, A.cor, lw, F);
+      else flux_hllc<WS>(tx, ty, E.z, L, R, 1.0, A.cor, lw, F);
     }
-    lw = dmin(lw, lwi);
     sF0[j] = F[0];
     sF1[j] = F[1];
     sF2[j] = F[2];
   }
-  block_min_fused(lw, rbits);  // synchronises the workgroup as well
+  __syncthreads();
 
   // ---- draining time steps of every local cell (k_drain's operations)
   const int nl = kTileCells + nr;
   for (int l = tid; l < nl; l += kTileCells) {
     if (l >= nc && l < kTileCells) continue;  // beyond the last tile's cells
-    const int x = global(l);
+    const bool own = l < kTileCells;
+    const int x = own ? (int)(c0 + l) : P.ring_cell[rg0 + l - kTileCells];
+    uint32_t code[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) code[k] = own ? P.code[k * T + x] : P.ring_code[3 * (rg0 + l - kTileCells) + k];
     const double h = S.w[x] - M.bc[x];
     double d;
     if (!is_wet(h)) {
@@ -133,7 +102,7 @@ __global__ void __launch_bounds__(kTileCells) k_fused(DevMesh M, DevTiles P, Dev
       double sum = 0.;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const double fe = sF0[code_of(l, x, k) >> 16];
+        const double fe = sF0[code[k] >> 16];
         double out = sgn_pos(fl, k) ? fe : -fe;
         if (LEN)
           out = out * edge_len(pick(p0, p1, p2, ep0_local(fl, k)), pick(p0, p1, p2, ep1_local(fl, k)));
@@ -145,12 +114,16 @@ __global__ void __launch_bounds__(kTileCells) k_fused(DevMesh M, DevTiles P, Dev
   }
   __syncthreads();
 
-  // ---- update of the tile's cells (k_update's operations)
+  // ---- update of the tile's cells (k_update's operations; the fluxes and
+  // neighbour draining steps are read from LDS edge by edge)
   if (tid >= nc) return;
   const double dt = stage_dt_fused(A);
   if (A.dts_log && tile == 0 && tid == 0) A.dts_log[0] = dt;
   const int i = (int)(c0 + tid);
-  RhsIn q;
+  uint32_t code[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) code[k] = P.code[k * T + i];
+  RhsCell q;
   q.fl = M.cflag[i];
   q.mu = load_muscl(S, W, i);
   q.wi = S.w[i];
@@ -158,30 +131,30 @@ __global__ void __launch_bounds__(kTileCells) k_fused(DevMesh M, DevTiles P, Dev
   q.p[1] = node(M, M.tp[T + i]);
   q.p[2] = node(M, M.tp[2 * T + i]);
   q.dti = sD[tid];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const uint32_t c = P.code[k * T + i];
-    const int s = (int)(c >> 16);
-    const uint32_t nb = c & 0xffffu;
-    q.F0[k] = sF0[s];
-    q.F1[k] = sF1[s];
-    q.F2[k] = sF2[s];
-    q.dtik[k] = nb >= kNbOut ? INFINITY : sD[nb];
-  }
   const bool comb = A.a0 != 0.;
   const double w = S.w[i], u = S.u[i], v = S.v[i];
-  const double bc = centroid(q.p[0], q.p[1], q.p[2]).z;
   double w0 = 0., u0 = 0., v0 = 0.;
   if (comb) {
     w0 = U0.w[i];
     u0 = U0.u[i];
     v0 = U0.v[i];
   }
+  const double bc = centroid(q.p[0], q.p[1], q.p[2]).z;
   const double dts = A.coef * dt;
   double R0, R1, R2;
-  rhs_compute(q, dts, A.cor, A.fric, R0, R1, R2);
-  const double h = w - bc;
-  const double Uc0 = h, Uc1 = h * u, Uc2 = h * v;
+  rhs_compute_by(
+      q,
+      [&](int k, double& F0, double& F1, double& F2, double& dtik) {
+        const int s = (int)(code[k] >> 16);
+        const uint32_t nb = code[k] & 0xffffu;
+        F0 = sF0[s];
+        F1 = sF1[s];
+        F2 = sF2[s];
+        dtik = nb >= kNbOut ? INFINITY : sD[nb];
+      },
+      dts, A.cor, A.fric, R0, R1, R2);
+  const double hh = w - bc;
+  const double Uc0 = hh, Uc1 = hh * u, Uc2 = hh * v;
   double r0, r1, r2;
   if (comb) {
     const double h0 = w0 - bc;
@@ -203,17 +176,16 @@ __global__ void __launch_bounds__(kTileCells) k_fused(DevMesh M, DevTiles P, Dev
 
 hipError_t launch_fused(const DevMesh& M, const DevTiles& P, const DevState& S, const DevState& U0,
                         const DevOut& O, const DevWork& W, const StageArgs& A, int flux, int ws, int opts,
-                        unsigned long long* rbits, hipStream_t st) {
+                        hipStream_t st) {
   if (M.cinst || M.c0 != 0 || M.c1 != M.T) return hipErrorInvalidValue;  // single meshes only
-  hipLaunchKernelGGL(k_reset_r_fused, dim3(1), dim3(1), 0, st, rbits);
   const dim3 g = grid_for_bs(P.ntiles, 1), b(kTileCells);
   const bool len = (opts & kOptDrainLength) != 0;
 #define SWE_FUSED(F, Wv)                                                                             \
   if (flux == F && ws == Wv) {                                                                       \
-    if (len && M.nt) hipLaunchKernelGGL((k_fused<F, Wv, true, true>), g, b, 0, st, M, P, S, U0, O, W, A, rbits);   \
-    else if (len) hipLaunchKernelGGL((k_fused<F, Wv, true, false>), g, b, 0, st, M, P, S, U0, O, W, A, rbits);     \
-    else if (M.nt) hipLaunchKernelGGL((k_fused<F, Wv, false, true>), g, b, 0, st, M, P, S, U0, O, W, A, rbits);    \
-    else hipLaunchKernelGGL((k_fused<F, Wv, false, false>), g, b, 0, st, M, P, S, U0, O, W, A, rbits);             \
+    if (len && M.nt) hipLaunchKernelGGL((k_fused<F, Wv, true, true>), g, b, 0, st, M, P, S, U0, O, W, A);   \
+    else if (len) hipLaunchKernelGGL((k_fused<F, Wv, true, false>), g, b, 0, st, M, P, S, U0, O, W, A);     \
+    else if (M.nt) hipLaunchKernelGGL((k_fused<F, Wv, false, true>), g, b, 0, st, M, P, S, U0, O, W, A);    \
+    else hipLaunchKernelGGL((k_fused<F, Wv, false, false>), g, b, 0, st, M, P, S, U0, O, W, A);             \
     return hipGetLastError();                                                                        \
   }
   SWE_FUSED(0, 0) SWE_FUSED(0, 1) SWE_FUSED(0, 2) SWE_FUSED(1, 0) SWE_FUSED(1, 1) SWE_FUSED(1, 2)

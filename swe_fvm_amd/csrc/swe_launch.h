@@ -248,13 +248,12 @@ hipError_t launch_adjoint_finish(const DevMesh& M, const DevAdj& J, double* zbar
 // Fluxes, draining time steps and update of one stage in one kernel over
 // tiles (swe_fused.hip); the reconstruction of S must be in W.  The stage's
 // step size must be known (A.dt_fixed > 0, or the CFL r of an earlier pass
-// in A.rbits[0]); the pass's own CFL length-to-wavespeed minimum goes to
-// *rbits.  W's fluxes and draining time steps are not written.  Single
-// meshes only.
+// in A.rbits[0]).  Neither W's fluxes and draining time steps nor this
+// pass's CFL minimum are written.  Single meshes only.
 struct DevTiles;
 hipError_t launch_fused(const DevMesh& M, const DevTiles& P, const DevState& S, const DevState& U0,
                         const DevOut& O, const DevWork& W, const StageArgs& A, int flux, int ws, int opts,
-                        unsigned long long* rbits, hipStream_t st);
+                        hipStream_t st);
 // one reconstruction (kind 0 dry, 1 PartWet1, 2 FullWet, 3 PartWet2) of
 // cell i: out[5] = origin (w, u, v), gradient of w (x, y)
 hipError_t launch_reconstruct_one(const DevMesh& M, const DevState& S, const DevWork& W, int i, int kind,

@@ -22,8 +22,8 @@
 
 namespace swe {
 
-bool build_tiles(int64_t T, const int* tt, const int* te, const uint32_t* cflag, const int* et1,
-                 TileTables& out) {
+bool build_tiles(int64_t T, const int* tt, const int* te, const uint32_t* cflag, const int* et0, const int* et1,
+                 const int* ep0, const int* ep1, TileTables& out) {
   const int64_t nt = (T + kTileCells - 1) / kTileCells;
   out = TileTables{};
   out.ntiles = nt;
@@ -54,7 +54,6 @@ bool build_tiles(int64_t T, const int* tt, const int* te, const uint32_t* cflag,
     codes.assign(size_t(3 * nl), 0u);
     slot.assign(size_t(3 * nl), -1);
     int nev = 0;
-    const size_t ev0 = out.ev.size();
     for (int l = 0; l < nl; ++l) {
       if (!local_ok(l)) continue;
       const int64_t x = global(l);
@@ -76,6 +75,8 @@ bool build_tiles(int64_t T, const int* tt, const int* te, const uint32_t* cflag,
         if (evaluator) {
           slot[size_t(3 * l + k)] = nev++;
           out.ev.push_back(uint16_t((l << 2) | k));
+          const int e = te[size_t(k * T + x)];
+          out.evrec.push_back(EdgeRec{et0[e], et1[e], ep0[e], ep1[e]});
         }
       }
     }
@@ -107,7 +108,6 @@ bool build_tiles(int64_t T, const int* tt, const int* te, const uint32_t* cflag,
     for (int y : ring) out.ring_cell.push_back(y);
     out.ring_off[size_t(t) + 1] = int(out.ring_cell.size());
     out.ev_off[size_t(t) + 1] = int(out.ev.size());
-    (void)ev0;
     for (int64_t i = c0; i < c1; ++i) loc[size_t(i)] = -1;
     for (int y : ring) loc[size_t(y)] = -1;
   }

@@ -13,6 +13,11 @@ constexpr int kTileMaxLocal = 512;  // tile cells + ring
 constexpr int kTileMaxEval = 768;   // edge evaluations per tile (LDS flux slots)
 constexpr uint32_t kNbWall = 0xffffu, kNbFree = 0xfffeu, kNbOut = 0xfffdu;
 
+// one evaluated edge: its cells and nodes (one 16-byte load)
+struct alignas(16) EdgeRec {
+  int et0, et1, ep0, ep1;
+};
+
 struct TileTables {
   int64_t ntiles = 0;
   std::vector<uint32_t> code;       // [3][T] tile cells: slot << 16 | neighbour code
@@ -21,14 +26,15 @@ struct TileTables {
   std::vector<uint32_t> ring_code;  // [ring entry][3] codes of the ring cells
   std::vector<int> ev_off;          // [ntiles + 1]
   std::vector<uint16_t> ev;         // evaluations: local cell << 2 | k
+  std::vector<EdgeRec> evrec;       // per evaluation: the edge's et0, et1, ep0, ep1
   int max_eval = 0, max_ring = 0;
 };
 
 // false when a tile exceeds the kernel's LDS capacity (a mesh far from a
 // cell-local order) or the tables are inconsistent: the fused kernel is
 // then not used
-bool build_tiles(int64_t T, const int* tt, const int* te, const uint32_t* cflag, const int* et1,
-                 TileTables& out);
+bool build_tiles(int64_t T, const int* tt, const int* te, const uint32_t* cflag, const int* et0, const int* et1,
+                 const int* ep0, const int* ep1, TileTables& out);
 
 struct DevTiles {
   const uint32_t* code;
@@ -36,7 +42,7 @@ struct DevTiles {
   const int* ring_cell;
   const uint32_t* ring_code;
   const int* ev_off;
-  const uint16_t* ev;
+  const EdgeRec* evrec;  // et0, et1, ep0, ep1 of each evaluated edge
   int64_t ntiles;
 };
 

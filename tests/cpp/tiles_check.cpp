@@ -19,13 +19,14 @@
 using namespace swe;
 
 extern "C" int tiles_check(int64_t T, const int* tt, const int* te, const uint32_t* cflag, const int* et0,
-                           const int* et1, int64_t E, int64_t* stats, char* msg, int msglen) {
+                           const int* et1, const int* ep0, const int* ep1, int64_t E, int64_t* stats, char* msg,
+                           int msglen) {
   auto bad = [&](const std::string& m) {
     std::snprintf(msg, size_t(msglen), "%s", m.c_str());
     return 1;
   };
   TileTables P;
-  if (!build_tiles(T, tt, te, cflag, et1, P)) return bad("build_tiles refused the mesh");
+  if (!build_tiles(T, tt, te, cflag, et0, et1, ep0, ep1, P)) return bad("build_tiles refused the mesh");
   if (P.code.size() != size_t(3 * T)) return bad("code size");
   int64_t evals = 0, ring = 0;
   for (int64_t tile = 0; tile < P.ntiles; ++tile) {
@@ -35,7 +36,7 @@ extern "C" int tiles_check(int64_t T, const int* tt, const int* te, const uint32
     const int e0 = P.ev_off[size_t(tile)], ne = P.ev_off[size_t(tile) + 1] - e0;
     if (nr < 0 || ne < 0 || ne > kTileMaxEval || kTileCells + nr > kTileMaxLocal) return bad("capacity");
     if (size_t(rg0 + nr) > P.ring_cell.size() || size_t(3 * (rg0 + nr)) > P.ring_code.size() ||
-        size_t(e0 + ne) > P.ev.size())
+        size_t(e0 + ne) > P.ev.size() || P.evrec.size() != P.ev.size())
       return bad("table range");
     const int nl = kTileCells + nr;
     auto valid_local = [&](int l) { return (l >= 0 && l < nc) || (l >= kTileCells && l < nl); };
@@ -71,6 +72,8 @@ extern "C" int tiles_check(int64_t T, const int* tt, const int* te, const uint32
       if (f < 0 || f >= T || f != et0[e]) return bad("f is not the edge's et0");
       if (t != et1[e]) return bad("t is not the edge's et1");
       if (!seen.insert(e).second) return bad("edge evaluated twice in a tile");
+      const EdgeRec& rec = P.evrec[size_t(e0 + j)];
+      if (rec.et0 != f || rec.et1 != et1[e] || rec.ep0 < 0 || rec.ep1 < 0) return bad("edge record");
       slot_edge[size_t(j)] = e;
     }
     for (int l = 0; l < nl; ++l) {

@@ -26,7 +26,7 @@ def checker(tmp_path_factory):
     assert res.returncode == 0, res.stderr
     L = ctypes.CDLL(out)
     L.tiles_check.restype = ctypes.c_int
-    L.tiles_check.argtypes = [ctypes.c_int64] + [ctypes.c_void_p] * 5 + [ctypes.c_int64, ctypes.c_void_p,
+    L.tiles_check.argtypes = [ctypes.c_int64] + [ctypes.c_void_p] * 7 + [ctypes.c_int64, ctypes.c_void_p,
                                                                           ctypes.c_char_p, ctypes.c_int]
     return L
 
@@ -38,18 +38,21 @@ def soa(topo):
     te = np.ascontiguousarray(t["te"].T.reshape(-1), dtype=np.int32)
     et0 = np.ascontiguousarray(t["et"][:, 0], dtype=np.int32)
     et1 = np.ascontiguousarray(t["et"][:, 1], dtype=np.int32)
+    ep0 = np.ascontiguousarray(t["ep"][:, 0], dtype=np.int32)
+    ep1 = np.ascontiguousarray(t["ep"][:, 1], dtype=np.int32)
     fl = np.zeros(T, np.uint32)
     for k in range(3):
         fl |= (et0[t["te"][:, k]] == np.arange(T)).astype(np.uint32) << k
-    return T, tt, te, fl, et0, et1
+    return T, tt, te, fl, et0, et1, ep0, ep1
 
 
 def run(checker, topo):
-    T, tt, te, fl, et0, et1 = soa(topo)
+    T, tt, te, fl, et0, et1, ep0, ep1 = soa(topo)
     stats = np.zeros(5, np.int64)
     msg = ctypes.create_string_buffer(256)
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
-    rc = checker.tiles_check(T, p(tt), p(te), p(fl), p(et0), p(et1), et0.shape[0], p(stats), msg, 256)
+    rc = checker.tiles_check(T, p(tt), p(te), p(fl), p(et0), p(et1), p(ep0), p(ep1), et0.shape[0], p(stats), msg,
+                             256)
     return rc, msg.value.decode(), stats
 
 
