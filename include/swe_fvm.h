@@ -247,6 +247,20 @@ int swe_integrals(swe_solver* s, double* out3, double* per_cell);
  * results are identical bit for bit.  *active (may be null) receives
  * whether the kernel will be used. */
 int swe_set_fused(swe_solver* s, int on, int* active);
+/* Allocation-free swe_integrals.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same results as
+ * swe_integrals, bit for bit, but out3_dev (3 doubles) and per_cell_dev
+ * (3 T doubles, may be null) are DEVICE pointers written asynchronously on
+ * the solver's stream (swe_synchronize before reading them on the host),
+ * and the block partials live in a static per-GPU scratch array of
+ * SWE_NOALLOC_MAX_CELLS / 256 x 3 doubles (1.5 MiB, held until the process
+ * ends): meshes above SWE_NOALLOC_MAX_CELLS cells are refused with
+ * SWE_ERR_ARG.  Calls no hipMalloc / hipFree.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (other streams or
+ * threads) race.  Single-mesh handles only. */
+#define SWE_NOALLOC_MAX_CELLS 16777216
+int swe_integrals_noalloc(swe_solver* s, double* out3_dev, double* per_cell_dev);
 /* Final per-cell reconstruction after swe_compute_interface_values: level
  * w0 at the centroid and gradient (gx, gy); velocity is the cell's (zero
  * for dry cells).  Any pointer may be null. */

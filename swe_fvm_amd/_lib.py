@@ -60,6 +60,7 @@ SIGNATURES = [
     ("swe_batch_info", _int, [_p, _pi64, _p, _p, _p]),
     ("swe_batch_time", _int, [_p, _p, _p]),
     ("swe_integrals", _int, [_p, _p, _p]),
+    ("swe_integrals_noalloc", _int, [_p, _p, _p]),
     ("swe_set_fused", _int, [_p, _int, _p]),
     ("swe_reconstruct", _int, [_p, _i64, _int, _p]),
     ("swe_edge_flux", _int, [_p, _int, _int, _i64, _i64, _i64, _p, _p]),

@@ -45,6 +45,7 @@ struct swe_solver {
   double graph_cfl = 0., graph_tend = 0.;
   int64_t graph_c0 = -1, graph_c1 = -1;
   int graph_cur = -1;  // state buffer the captured launches start from
+  double* scratch = nullptr;  // this device's static scratch array (swe_integrals_noalloc)
   // fused flux / draining / update kernel over tiles (single meshes whose
   // tiles fit its LDS): used for every stage whose step size is known
   DevTiles tiles{};
@@ -253,6 +254,7 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
   if ((e = hipSetDevice(device)) != hipSuccess) return bail(e, "hipSetDevice");
   if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess)
     return bail(e, "hipStreamCreate");
+  if ((e = scratch_address(&s->scratch)) != hipSuccess) return bail(e, "hipGetSymbolAddress(scratch)");
   double *dpx, *dpy, *dpz;
   int *dtp, *dte, *dtt, *dep0, *dep1, *det0, *det1, *doff, *dn2c;
   uint32_t* dfl;
@@ -1101,6 +1103,26 @@ int swe_integrals(swe_solver* s, double* out3, double* per_cell) {
   if (pc) (void)hipFree(pc);
   if (e == hipSuccess) e = e2;
   if (e != hipSuccess) return dev_fail(e, "integrals");
+  return SWE_OK;
+}
+
+// swe_integrals without allocation or host synchronisation: the block
+// partials live in the device's static scratch array, the results go to
+// device memory on the solver's stream.
+int swe_integrals_noalloc(swe_solver* s, double* out3_dev, double* per_cell_dev) {
+  if (int rc = single_mesh(s)) return rc;
+  if (!out3_dev) return fail(SWE_ERR_ARG, "null argument");
+  if (!s->scratch) return fail(SWE_ERR_DEVICE, "no scratch array on this device");
+  const int64_t nb = integrals_blocks(s->M);
+  if (nb > kScratchBlocks)
+    return fail(SWE_ERR_ARG, "mesh exceeds the allocation-free limit of " +
+                                 std::to_string(kScratchBlocks * 256) + " cells");
+  if (int rc = set_device(s)) return rc;
+  if (int rc = materialize_fluxes(s)) return rc;
+  DevState S = state_of(s, s->cur);
+  SWE_CHECK(launch_recon1(s->M, S, s->W, s->opts, s->stream), "recon1");
+  SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
+  SWE_CHECK(launch_integrals(s->M, S, s->W, s->scratch, per_cell_dev, out3_dev, s->stream), "integrals");
   return SWE_OK;
 }
 

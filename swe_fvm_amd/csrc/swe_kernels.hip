@@ -31,6 +31,9 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <mutex>
+#include <vector>
+
 #include "swe_common.h"
 
 namespace swe {
@@ -908,6 +911,31 @@ __global__ void k_rhs_one(DevMesh M, DevState S, DevWork W, int i, double dt, do
                           double fric, double* out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   rhs_cell(M, S, W, i, dt, cor, fric, out[0], out[1], out[2]);
+}
+
+// ------------------------------------------------------ static scratch --
+// Block partials of the allocation-free integrals (swe_integrals_noalloc):
+// a file-scope array, one copy per GPU, for meshes of up to kScratchBlocks
+// workgroups of 256 cells.  Host code never names it in a launch: it takes
+// the array's device address from scratch_address().
+__device__ __align__(256) double scratch[3 * kScratchBlocks];
+
+hipError_t scratch_address(double** out) {
+  static std::mutex mu;
+  static std::vector<double*> addr;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(mu);
+  if (addr.size() <= size_t(dev)) addr.resize(size_t(dev) + 1, nullptr);
+  if (!addr[size_t(dev)]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[size_t(dev)] = static_cast<double*>(p);
+  }
+  *out = addr[size_t(dev)];
+  return hipSuccess;
 }
 
 // -------------------------------------------------------------- launch --

@@ -254,6 +254,12 @@ struct DevTiles;
 hipError_t launch_fused(const DevMesh& M, const DevTiles& P, const DevState& S, const DevState& U0,
                         const DevOut& O, const DevWork& W, const StageArgs& A, int flux, int ws, int opts,
                         hipStream_t st);
+// The allocation-free integrals keep their block partials in a static
+// per-GPU array of 3 * kScratchBlocks doubles (meshes up to kScratchBlocks *
+// 256 cells); scratch_address gives its device address on the current
+// device (looked up once per device).
+constexpr int64_t kScratchBlocks = 65536;
+hipError_t scratch_address(double** out);
 // one reconstruction (kind 0 dry, 1 PartWet1, 2 FullWet, 3 PartWet2) of
 // cell i: out[5] = origin (w, u, v), gradient of w (x, y)
 hipError_t launch_reconstruct_one(const DevMesh& M, const DevState& S, const DevWork& W, int i, int kind,
