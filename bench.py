@@ -6,14 +6,22 @@ Fourier bathymetry (64 modes, amplitude 0.1), h = 1 inside r < 0.2, h = 0.1
 outside, 5 % dry pockets; FP64, HLLC<Einfeldt>, SSP-RK2, CFL mode 0.15.
 
 One step = one SSP-RK2 time step of every cell; value = cell-updates/s with
-the state resident in HBM.  Prints one JSON line.
+the state resident in HBM.  Prints one JSON line.  The same line carries
+the batched-ensemble and adjoint/forward ratios of shortened runs of the
+256-member ensemble and of the 32-member bathymetry inversion
+(`ensemble`, `adjoint`), the roofline of the dominant kernel and of the
+whole step against the compulsory bytes of a cell-update, and CPU rows
+(the oracle, one thread, on the host of the GPU) for the Thacker and
+lake-at-rest configurations.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 2828]
+    python bench.py --ensemble | --adjoint      (the full runs, one line each)
 """
 import argparse
+import gc
 import json
 import os
-import platform
+import statistics
 import sys
 import time
 
@@ -23,14 +31,36 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
-# Compulsory bytes per cell-update of SSP-RK2 (state in/out per stage, U^n in
-# stage 2, int32 cell indices, node records at V/T ~ 0.5).
-STEP_BYTES_PER_CELL = 192.0
+# Compulsory bytes per cell of one SSP-RK2 stage: read the stage state
+# (24), write the new state (24), 3 neighbour + 3 node ids (24), node
+# records at V/T ~ 0.5 (12); stage 2 also reads U^n (24).
+STAGE_BYTES = (84.0, 108.0)
+STEP_BYTES_PER_CELL = sum(STAGE_BYTES)  # 192
 
 
-def kernel_alg_bytes(T, E, V):
-    """Algorithmic bytes per launch of each kernel: every array the kernel
-    must touch, read or written once (int32 indices, FP64 values)."""
+def host_cpu():
+    """CPU model and the cores this process may use (lscpu's model name)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
+
+
+def kernel_own_bytes(T, E, V):
+    """Bytes of every array a kernel reads or writes, once each (int32
+    indices, FP64 values), per launch: the kernel's own implementation
+    arrays, intermediates included (not the compulsory bytes of a
+    cell-update, which the roofline uses)."""
     nodes = 24.0 * V
     return {
         # w, tp, tt, cflag, nodes in; MUSCL (w0, gx, gy) + class out
@@ -50,154 +80,212 @@ def kernel_alg_bytes(T, E, V):
     }
 
 
-KERNEL_NAMES = ["recon1", "recon2", "flux", "drain", "update", "fused"]
 
 
-def load_traffic(workload, kernel):
-    """HBM bytes per launch from committed rocprofv3 --pmc summaries, if any."""
+def load_traffic(workload):
+    """HBM bytes per launch of each kernel and launches per step, from the
+    committed rocprofv3 --pmc summary (profiles/pmc_traffic.json), if any."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if not os.path.exists(path):
-        return None
+        return {}
     with open(path) as f:
         data = json.load(f)
-    ent = data.get(workload, {}).get(kernel)
-    if not ent:
-        return None
-    return ent.get("bytes_per_launch")
+    return data.get(workload, {})
 
 
-def cpu_baseline(dom, prim, flux, wave, seconds_hint=""):
+def cpu_baseline(dom, prim, flux, wave, reps=3):
     """The oracle (scalar C++ restatement, 1 thread) on the same state: one
-    SSP-RK2 step in CFL mode."""
+    SSP-RK2 step in CFL mode after one warm-up step, median of reps."""
     from oracle import oracle as O
     orc = O.OracleSolver(dom.geometry, dom.topology.tables(), flux=flux, wave=wave)
     orc.set_state(prim)
-    el = orc.time_run("SSPRK2", 1, dt=0.0, cfl=0.15)
+    orc.time_run("SSPRK2", 1, dt=0.0, cfl=0.15)  # warm-up
+    times = []
+    for _ in range(reps):
+        orc.set_state(prim)
+        times.append(orc.time_run("SSPRK2", 1, dt=0.0, cfl=0.15))
+    el = statistics.median(times)
     T = prim.shape[0]
+    cpu = host_cpu()
     return {"value": T / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"1 SSP-RK2 step (CFL mode) of the same {T}-cell state, "
-                      f"oracle C++ restatement -O3 -ffp-contract=off, 1 thread, {el:.1f} s"}
+            "host_cpu": cpu["model"], "host_logical_cpus": cpu["logical_cpus"],
+            "sample": f"1 SSP-RK2 step (CFL mode) of the same {T}-cell state after a warm-up step, "
+                      f"median of {reps} ({', '.join(f'{t:.2f}' for t in times)} s), oracle C++ restatement "
+                      f"-O3 -ffp-contract=off, 1 thread of {cpu['model']}"}
 
 
-def bench_ensemble(args):
-    """256 seeded members of 10k-340k cells (~20M in total), HLLC<Einfeldt>,
-    SSP-RK2, per-member CFL 0.15 steps to t_end: the batched ensemble (one
-    graph-captured step for all members) against the same members advanced
-    one after the other with the single-mesh solver."""
-    import gc
+def cpu_rows(fl):
+    """BASELINE.md's CPU rows: the oracle (1 thread) against the GPU on the
+    Thacker configuration (40,000 cells, one period, CFL 0.5, in full) and
+    on the 1M-cell lake at rest (3 steps sampled, HLLC<Einfeldt>)."""
+    import swe_fvm_amd as swe
+    from oracle import oracle as O
+    from swe_fvm_amd import workloads as W
+    from swe_fvm_amd.ensemble import Ensemble
+    rows = []
+    # C1: the GPU runs to the period with per-step CFL dt (the last step
+    # clipped); the oracle replays the same steps
+    dom, prim, th = W.thacker(n=100, quad=100)
+    T = prim.shape[0]
+    ens = Ensemble(fl, [dom], [prim])
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.run("SSPRK2", cfl=0.5, t_end=th.period, graph=True)
+    ens.synchronize()
+    g_s = time.perf_counter() - t0
+    _, steps = ens.times()
+    n = int(steps[0])
+    orc = O.OracleSolver(dom.geometry, dom.topology.tables(), "HLLC", "Einfeldt")
+    orc.set_state(prim)
+    t0 = time.perf_counter()
+    tk = 0.0
+    for _ in range(n):
+        orc.compute_interface_values()
+        d = 0.5 * orc.compute_fluxes()
+        if d < th.period - tk:
+            orc.run("SSPRK2", 1, dt=0.0, cfl=0.5)
+            tk += d
+        else:
+            orc.run("SSPRK2", 1, dt=th.period - tk)
+            tk = th.period
+    c_s = time.perf_counter() - t0  # includes the oracle's extra r pass per step
+    same = bool(np.array_equal(ens.states()[0], orc.state()))
+    rows.append({"config": "thacker_100x100_T40000", "cells": T, "steps": n, "full": True,
+                 "cpu_cell_updates_per_s": T * n / c_s, "gpu_cell_updates_per_s": T * n / g_s,
+                 "gpu_over_cpu": c_s / g_s, "bitwise_equal": same,
+                 "note": "GPU: one graph-captured ensemble step per step (launch-bound at 40k cells); CPU time "
+                         "includes one extra interface-value and flux pass per step for the clipped-step replay"})
+    del ens
+    # C2: 3 steps sampled on the CPU, 100 on the GPU
+    dom, prim = W.lake_at_rest(n=707, seed=1)
+    T = prim.shape[0]
+    orc = O.OracleSolver(dom.geometry, dom.topology.tables(), "HLLC", "Einfeldt")
+    orc.set_state(prim)
+    orc.time_run("SSPRK2", 1, dt=0.0, cfl=0.15)
+    c_s = orc.time_run("SSPRK2", 3, dt=0.0, cfl=0.15)
+    sd = swe.SpaceDisc(fl, dom, prim)
+    sd.run("SSPRK2", 5, dt=0.0, cfl=0.15)
+    g_ms = sd.time_run("SSPRK2", 100, dt=0.0, cfl=0.15)
+    rows.append({"config": "lake_at_rest_T999698", "cells": T, "steps_cpu": 3, "steps_gpu": 100, "full": False,
+                 "cpu_cell_updates_per_s": T * 3 / c_s, "gpu_cell_updates_per_s": T * 100 / (g_ms * 1e-3),
+                 "gpu_over_cpu": (T * 100 / (g_ms * 1e-3)) / (T * 3 / c_s)})
+    del sd
+    return rows
+
+
+def ensemble_measure(n_members, cells, t_end, seq_steps, host_stepped=True, tune_reps=3):
+    """n_members seeded members (log-uniform sizes, ~cells in total),
+    HLLC<Einfeldt>, SSP-RK2, per-member CFL 0.15 steps to t_end: the batched
+    ensemble (one graph-captured step for all active members) against the
+    same members advanced one after the other with the single-mesh solver
+    (device-stepped: seq_steps steps per call; host-stepped: one
+    Solvers.SSPRK2 call per step, as the reference's drivers step)."""
     import swe_fvm_amd as swe
     from swe_fvm_amd import workloads as W
     from swe_fvm_amd.ensemble import Ensemble
     t0 = time.time()
-    mem = W.ensemble(n_inst=args.members, total=args.ensemble_cells, seed=11)
+    mem = W.ensemble(n_inst=n_members, total=cells, seed=11)
     T = np.array([p.shape[0] for _, p in mem])
     print(f"[bench] ensemble {len(mem)} members, {T.sum()} cells ({T.min()}..{T.max()}) in "
           f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     fl = swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt]
     ens = Ensemble(fl, [d for d, _ in mem], [p for _, p in mem])
-    tune = ens.autotune(reps=3)
-    print(f"[bench] autotune: flux x{tune['flux_items']}/{tune['flux_block']} "
-          f"update x{tune['update_items']}/{tune['update_block']}",
-          file=sys.stderr, flush=True)
-    ens.run("SSPRK2", cfl=0.15, t_end=args.t_end, max_steps=2, graph=True)  # warm-up + capture
+    tune = ens.autotune(reps=tune_reps)
+    ens.run("SSPRK2", cfl=0.15, t_end=t_end, max_steps=2, graph=True)  # warm-up + capture
     ens.reset(0.0)
     ens.set_states([p for _, p in mem])
     ens.synchronize()
     t1 = time.perf_counter()
-    done, act = ens.run("SSPRK2", cfl=0.15, t_end=args.t_end, graph=True)
+    done, act = ens.run("SSPRK2", cfl=0.15, t_end=t_end, graph=True)
     ens.synchronize()
     wall_b = time.perf_counter() - t1
     tt, steps = ens.times()
     updates = float(np.sum(T * steps))
     batched = updates / wall_b
     print(f"[bench] batched: {done} steps, {updates:.3e} cell-updates in {wall_b:.2f}s "
-          f"-> {batched:.3e}/s, all at t_end: {bool(np.all(tt == args.t_end))}", file=sys.stderr, flush=True)
+          f"-> {batched:.3e}/s, all at t_end: {bool(np.all(tt == t_end))}", file=sys.stderr, flush=True)
     del ens
     gc.collect()
-    # sequential: each member alone, args.seq_steps CFL steps (a bounded sample)
-    seq_updates = 0.0
-    seq_ms = 0.0
+    seq_updates, seq_s = 0.0, 0.0
     for dom, prim in mem:
         sd = swe.SpaceDisc(fl, dom, prim)
         sd.run("SSPRK2", 1, dt=0.0, cfl=0.15)
         sd.synchronize()
         t2 = time.perf_counter()
-        sd.run("SSPRK2", args.seq_steps, dt=0.0, cfl=0.15)
-        seq_ms += (time.perf_counter() - t2) * 1e3
-        seq_updates += prim.shape[0] * args.seq_steps
+        sd.run("SSPRK2", seq_steps, dt=0.0, cfl=0.15)
+        seq_s += time.perf_counter() - t2
+        seq_updates += prim.shape[0] * seq_steps
         del sd
-    sequential = seq_updates / (seq_ms * 1e-3)
-    print(f"[bench] sequential: {seq_updates:.3e} cell-updates in {seq_ms / 1e3:.2f}s -> "
-          f"{sequential:.3e}/s", file=sys.stderr, flush=True)
-    # sequential through the drop-in API, one Solvers.SSPRK2 call per step
-    # (each call returns after its step, as the reference's drivers step)
-    hs_updates = 0.0
-    hs_s = 0.0
-    for dom, prim in mem:
-        sd = swe.SpaceDisc(fl, dom, prim)
-        td = swe.TimeDisc(sd)
-        sd.ComputeInterfaceValues()
-        sd.ComputeFluxes()
-        dt = td.CFLdt()
-        swe.Solvers.SSPRK2(td, dt)
-        t2 = time.perf_counter()
-        for _ in range(args.seq_steps):
+    sequential = seq_updates / seq_s
+    print(f"[bench] sequential: {seq_updates:.3e} cell-updates in {seq_s:.2f}s -> {sequential:.3e}/s",
+          file=sys.stderr, flush=True)
+    out = {"value": batched / sequential, "unit": "ratio (batched / sequential device-stepped cell-updates/s)",
+           "batched_cell_updates_per_s": batched, "sequential_cell_updates_per_s": sequential,
+           "batched": {"steps": done, "members_active_at_end": act, "cell_updates": updates, "seconds": wall_b,
+                       "t_end": t_end, "all_at_t_end": bool(np.all(tt == t_end)), "graph": True},
+           "sequential": {"steps_per_member": seq_steps, "cell_updates": seq_updates, "seconds": seq_s},
+           "config": {"workload": f"ensemble_{len(mem)}_members_T{int(T.sum())}", "members": len(mem),
+                      "cells_min": int(T.min()), "cells_max": int(T.max()), "scheme": "SSPRK2",
+                      "flux": "HLLC<Einfeldt>", "dt": "per-member CFL 0.15",
+                      "tuning": {"flux_edges_per_thread": tune["flux_items"],
+                                 "update_cells_per_thread": tune["update_items"],
+                                 "flux_block_threads": tune["flux_block"],
+                                 "update_block_threads": tune["update_block"]}}}
+    if host_stepped:
+        hs_updates, hs_s = 0.0, 0.0
+        for dom, prim in mem:
+            sd = swe.SpaceDisc(fl, dom, prim)
+            td = swe.TimeDisc(sd)
+            sd.ComputeInterfaceValues()
+            sd.ComputeFluxes()
+            dt = td.CFLdt()
             swe.Solvers.SSPRK2(td, dt)
-        hs_s += time.perf_counter() - t2
-        hs_updates += prim.shape[0] * args.seq_steps
-        del sd
-    host_stepped = hs_updates / hs_s
-    print(f"[bench] sequential host-stepped: {hs_updates:.3e} cell-updates in {hs_s:.2f}s -> "
-          f"{host_stepped:.3e}/s", file=sys.stderr, flush=True)
-    print(json.dumps({
-        "metric": "batched/sequential aggregate FP64 cell-updates/s",
-        "value": batched / sequential, "unit": "ratio", "higher_is_better": True,
-        "batched_cell_updates_per_s": batched, "sequential_cell_updates_per_s": sequential,
-        "sequential_host_stepped_cell_updates_per_s": host_stepped,
-        "ratio_vs_host_stepped": batched / host_stepped,
-        "batched": {"steps": done, "members_active_at_end": act, "cell_updates": updates,
-                    "seconds": wall_b, "t_end": args.t_end, "graph": True},
-        "sequential": {"steps_per_member": args.seq_steps, "cell_updates": seq_updates,
-                       "seconds": seq_ms / 1e3},
-        "config": {"workload": f"ensemble_{len(mem)}_members_T{int(T.sum())}", "members": len(mem),
-                   "cells_min": int(T.min()), "cells_max": int(T.max()), "scheme": "SSPRK2",
-                   "flux": "HLLC<Einfeldt>", "dt": "per-member CFL 0.15",
-                   "tuning": {"flux_edges_per_thread": tune["flux_items"],
-                              "update_cells_per_thread": tune["update_items"],
-                              "flux_block_threads": tune["flux_block"],
-                              "update_block_threads": tune["update_block"]}},
-        "dtype": "f64", "data": "synthetic", "n_gpus": 1}))
+            t2 = time.perf_counter()
+            for _ in range(seq_steps):
+                swe.Solvers.SSPRK2(td, dt)
+            hs_s += time.perf_counter() - t2
+            hs_updates += prim.shape[0] * seq_steps
+            del sd
+        out["sequential_host_stepped_cell_updates_per_s"] = hs_updates / hs_s
+        out["ratio_vs_host_stepped"] = batched / (hs_updates / hs_s)
+    return out
 
 
-def bench_adjoint(args):
-    """Bathymetry inversion, a batch of args.adj_batch members on the
-    250,632-cell mesh stacked into one domain (every kernel advances all
-    members in one launch), HLLC<Einfeldt>, SSP-RK2, fixed dt,
-    args.adj_steps steps, checkpoints every args.adj_checkpoint steps.  The
-    gradient of the summed gauge misfits with respect to the initial state
-    and the node bathymetry comes from the torch.autograd.Function
-    (ShallowWaterRun).  Reports the plain batched forward, the adjoint's
-    forward sweep (checkpoints) and reverse sweep (segment recomputation +
-    reverse stages); value = reverse sweep / forward, the cost of the
-    adjoint step per forward step, recomputation included;
-    gradient_cost_ratio = (forward sweep + reverse sweep) / forward.
-    Checks: member 0 run alone reproduces its slice of the batch bit for
-    bit; in args.adj_fd random directions (bathymetry and initial surface
+def bench_ensemble(args):
+    """The full 256-member run (t_end = 1.0) as one JSON line."""
+    r = ensemble_measure(args.members, args.ensemble_cells, args.t_end, args.seq_steps)
+    r.update({"metric": "batched/sequential aggregate FP64 cell-updates/s", "higher_is_better": True,
+              "dtype": "f64", "data": "synthetic", "n_gpus": 1})
+    print(json.dumps(r))
+
+
+def adjoint_measure(B=32, n=354, N=2000, dt=2e-4, checkpoint=50, nfd=16, check_steps=100):
+    """Bathymetry inversion, a batch of B members on the n x n-grid mesh
+    (n = 354: 250,632 cells) stacked into one domain (every kernel advances
+    all members in one launch), HLLC<Einfeldt>, SSP-RK2, fixed dt, N steps,
+    checkpoints every `checkpoint` steps.  The gradient of the summed gauge
+    misfits with respect to the initial state and the node bathymetry comes
+    from the torch.autograd.Function (ShallowWaterRun).  Reports the plain
+    batched forward, the adjoint's forward sweep (checkpoints) and reverse
+    sweep (segment recomputation + reverse stages); value = reverse sweep /
+    forward, the cost of the adjoint step per forward step, recomputation
+    included; gradient_cost_ratio = (forward sweep + reverse sweep) /
+    forward.  Checks: member 0 run alone reproduces its slice of the batch
+    bit for bit; in nfd random directions (bathymetry and initial surface
     together) its gradient satisfies the duality with the tangent of the
     dual-number build of the forward over the full horizon, and is compared
-    with central differences over the full horizon and over
-    args.adj_check_steps steps, at eps and eps/4 (a pair that disagrees
-    crossed one of the step's switches, the limiter's all-or-nothing
-    decisions: inconclusive)."""
+    with central differences over the full horizon and over check_steps
+    steps, at eps and eps/4 (a pair that disagrees crossed one of the
+    step's switches, the limiter's all-or-nothing decisions:
+    inconclusive)."""
     import torch
 
     import swe_fvm_amd as swe
     from swe_fvm_amd import adjoint as AD
     from swe_fvm_amd import workloads as W
     t0 = time.time()
-    B, N, dt = args.adj_batch, args.adj_steps, args.adj_dt
-    ck = args.adj_checkpoint if args.adj_checkpoint > 0 else N
-    d = W.adjoint_inversion_batch(members=B, n=args.adj_n)
+    ck = checkpoint if checkpoint > 0 else N
+    d = W.adjoint_inversion_batch(members=B, n=n)
     dom1, prim1 = d["member"]
     T1, V1 = prim1.shape[0], dom1.geometry.shape[0]
     big, off = swe.stack_domains([dom1] * B)
@@ -243,7 +331,7 @@ def bench_adjoint(args):
     rng = np.random.default_rng(3)
     c1 = dom1.centroids()
     dirs = []
-    for _ in range(args.adj_fd):  # smooth directions: bathymetry and initial surface
+    for _ in range(nfd):  # smooth directions: bathymetry and initial surface
         dp = np.zeros_like(prim1)
         dp[:, 0] = W.smooth_field(c1[:, 0], c1[:, 1], rng, 1e-2)
         dirs.append((W.smooth_field(dom1.geometry[:, 0], dom1.geometry[:, 1], rng, 1e-2) * taper, dp))
@@ -278,7 +366,7 @@ def bench_adjoint(args):
     tangent = {"steps": N, "directions": len(dual), "max_rel": max(r["rel"] for r in dual),
                "tangent_run_ms": tl_ms / max(len(dual), 1), "results": dual}
     fd_full = fd_checks(N, f0, gz[:V1], gp[:T1])
-    nc = args.adj_check_steps
+    nc = check_steps
     fc = AD.gauge_misfit(g0, run1(d["z_true"][0], prim1, nc)[g0, 0].copy())
     _, s0c, zbc, _ = AD.run_and_adjoint(sd1, prim1, z1, nc, fc, dt=dt, checkpoint_every=min(ck, nc))
     fd_short = fd_checks(nc, fc, zbc, s0c)
@@ -295,6 +383,13 @@ def bench_adjoint(args):
                    "scheme": "SSPRK2", "flux": "HLLC<Einfeldt>", "gradient": "torch.autograd.Function"},
         "dtype": "f64", "data": "synthetic", "n_gpus": 1,
     }
+    return out
+
+
+def bench_adjoint(args):
+    """The full inversion workload (2,000 steps) as one JSON line."""
+    out = adjoint_measure(args.adj_batch, args.adj_n, args.adj_steps, args.adj_dt, args.adj_checkpoint,
+                          args.adj_fd, args.adj_check_steps)
     print(json.dumps(out))
 
 
@@ -330,6 +425,12 @@ def main():
                     help="keep the default work items per thread (profiling runs)")
     ap.add_argument("--no-fused", action="store_true",
                     help="separate flux, draining and update kernels in every stage (A/B runs)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the dam break (no ensemble, adjoint and CPU-row sub-objects)")
+    ap.add_argument("--ens-t-end", type=float, default=0.2,
+                    help="t_end of the default line's shortened ensemble run")
+    ap.add_argument("--adj-short-steps", type=int, default=200,
+                    help="steps of the default line's shortened adjoint run")
     args = ap.parse_args()
     if args.ensemble:
         bench_ensemble(args)
@@ -365,6 +466,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(dom, prim, "HLLC", "Einfeldt")
+        gc.collect()
         print(f"[bench] cpu baseline {cpu['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
     if args.cpu_only:
         print(json.dumps({"cpu_baseline": cpu}))
@@ -393,16 +495,61 @@ def main():
 
     # per-kernel average launch durations (events bracket each launch)
     kms_total, kms = sd.time_run("SSPRK2", args.kernel_steps, dt=0.0, cfl=0.15, per_kernel=True)
-    alg = kernel_alg_bytes(T, E, V)
-    per = {k: {"avg_ms": float(kms[i]), "alg_bytes": alg[k],
-               "gbs": alg[k] / (kms[i] * 1e-3) / 1e9 if kms[i] > 0 else None}
-           for i, k in enumerate(KERNEL_NAMES) if kms[i] > 0}
-    KERNEL_NAMES[:] = [k for k in KERNEL_NAMES if k in per]
-    dom_k = max(KERNEL_NAMES, key=lambda k: per[k]["avg_ms"])
-    achieved = per[dom_k]["gbs"]
-    traffic = load_traffic(workload, dom_k)
+    own = kernel_own_bytes(T, E, V)
+    # launches per SSP-RK2 step: two reconstructions; the separate flux,
+    # draining and update kernels run in stage 1 (CFL mode), the fused one
+    # in stage 2 (both stages separate without it)
+    per_step = {"recon1": 2, "recon2": 2, "flux": 1 if fused else 2, "drain": 1 if fused else 2,
+                "update": 1 if fused else 2, "fused": 1}
+    names = ["recon1", "recon2", "flux", "drain", "update", "fused"]
+    per = {k: {"avg_ms": float(kms[q]), "launches_per_step": per_step[k], "own_array_bytes": own[k],
+               "own_array_gbs": own[k] / (kms[q] * 1e-3) / 1e9}
+           for q, k in enumerate(names) if kms[q] > 0}
+    # the dominant kernel: most time per step; its achieved bandwidth counts
+    # the compulsory bytes of the stage it completes (the fused kernel and
+    # the stage-2 update: 108 B per cell; a stage-1 kernel: 84 B) against
+    # its own launch time
+    dom_k = max(per, key=lambda k: per[k]["avg_ms"] * per[k]["launches_per_step"])
+    stage_b = STAGE_BYTES[1] if dom_k == "fused" else STAGE_BYTES[0]
+    achieved = stage_b * T / (per[dom_k]["avg_ms"] * 1e-3) / 1e9
+    pmc = load_traffic(workload)
+    traffic = pmc.get(dom_k, {}).get("bytes_per_launch") if pmc.get("fused_stages", False) == fused else None
+    step_traffic = None
+    if traffic is not None:
+        step_traffic = sum(pmc[k]["bytes_per_launch"] * per_step[k] for k in per if k in pmc)
     step_gbs = STEP_BYTES_PER_CELL * value / 1e9
     h = sd.GetVolField()
+    finite = bool(np.isfinite(h).all())
+    del sd
+    gc.collect()
+    print(f"[bench] dominant kernel {dom_k}: {per[dom_k]['avg_ms']:.3f} ms, {achieved:.0f} GB/s compulsory",
+          file=sys.stderr, flush=True)
+
+    fl_rows = swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt]
+    extras = {}
+    if not args.no_extras:
+        t0 = time.time()
+        extras["ensemble"] = ensemble_measure(256, 20_000_000, args.ens_t_end, 10, host_stepped=False, tune_reps=2)
+        extras["ensemble"]["note"] = (f"config 4 shortened to t_end = {args.ens_t_end} (the full run to t = 1.0: "
+                                      f"python bench.py --ensemble)")
+        gc.collect()
+        print(f"[bench] ensemble {extras['ensemble']['value']:.2f}x in {time.time() - t0:.0f}s",
+              file=sys.stderr, flush=True)
+        t0 = time.time()
+        adj = adjoint_measure(B=32, n=354, N=args.adj_short_steps, dt=2e-4, checkpoint=50, nfd=4,
+                              check_steps=50)
+        for key in ("fd_full_horizon", "fd_short_horizon", "tangent_duality"):
+            adj[key].pop("results", None)
+        adj["note"] = (f"config 5 shortened to {args.adj_short_steps} steps, checkpoints every 50 "
+                       f"(the full 2,000 steps: python bench.py --adjoint)")
+        extras["adjoint"] = adj
+        gc.collect()
+        print(f"[bench] adjoint/forward {adj['value']:.2f} in {time.time() - t0:.0f}s", file=sys.stderr,
+              flush=True)
+        if cpu is not None:
+            t0 = time.time()
+            extras["cpu_rows"] = cpu_rows(fl_rows)
+            print(f"[bench] cpu rows in {time.time() - t0:.0f}s", file=sys.stderr, flush=True)
     out = {
         "metric": "FP64 cell-updates/s/GPU",
         "value": value,
@@ -425,15 +572,19 @@ def main():
                               "flux_block_threads": tune["flux_block"],
                               "update_block_threads": tune["update_block"]}},
         "roofline": {"bound": "hbm", "kernel": dom_k, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                     "traffic": traffic},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_cell": stage_b,
+                     "basis": "compulsory bytes of the stage the kernel completes per cell (SSP-RK2 stage "
+                              "1: 84 B, stage 2: 108 B) / the kernel's average launch time"},
         "step_roofline": {"bytes_per_cell_update": STEP_BYTES_PER_CELL, "achieved": step_gbs,
-                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS,
+                          "traffic_per_step": step_traffic,
+                          "traffic_ratio": step_traffic / (STEP_BYTES_PER_CELL * T) if step_traffic else None},
         "kernels": per,
         "cpu_baseline": cpu,
-        "host": platform.processor() or platform.machine(),
-        "finite": bool(np.isfinite(h).all()),
+        "finite": finite,
     }
+    out.update(extras)
     print(json.dumps(out))
 
 
