@@ -55,12 +55,14 @@ extern "C" {
 #define SWE_OPT_DRAIN_LENGTH 32
 
 /* Boundary tags of edge_elements[e][1] (include/Includes.h:21):
- * SOLID_WALL (src/SpaceDisc.cpp:66-69) and FREE_FLOW, declared but
+ * SOLID_WALL (src/SpaceDisc.cpp:66-69); FREE_FLOW, declared but
  * unimplemented in the reference: a transmissive edge whose outside state is
- * the inside edge state (the fluxer sees R = L).  PERIODIC and CUSTOM are
- * rejected. */
+ * the inside edge state (the fluxer sees R = L); PERIODIC, declared but
+ * never paired there: edges paired by swe_topology_set_periodic.  CUSTOM
+ * and an unpaired PERIODIC edge are rejected. */
 #define SWE_SOLID_WALL (-1)
 #define SWE_FREE_FLOW (-2)
+#define SWE_PERIODIC (-3)
 
 typedef struct swe_topology swe_topology;
 typedef struct swe_solver swe_solver;
@@ -73,7 +75,8 @@ const char* swe_last_error(void);
 /* Topology::create / Topology ctor (reference include/TriangMesh.h:16-40,
  * pybind/Topology.cpp:21-27).  Copies (owns) the tables: edge_nodes and
  * edge_elements are E x 2 (edge_elements[e][1] < 0 on the boundary: a tag,
- * of which swe_create accepts SWE_SOLID_WALL and SWE_FREE_FLOW), element_*
+ * of which swe_create accepts SWE_SOLID_WALL, SWE_FREE_FLOW and paired
+ * SWE_PERIODIC edges), element_*
  * are T x 3.
  * Indices out of range give SWE_ERR_DOMAIN. */
 int swe_topology_create(int64_t num_nodes, int64_t num_edges, const int64_t* edge_nodes,
@@ -102,6 +105,16 @@ int64_t swe_topology_num_edges(const swe_topology* t);
 int64_t swe_topology_num_elements(const swe_topology* t);
 /* EdgePoints / EdgeTriangs / TriangPoints / TriangEdges / TriangTriangs
  * (include/TriangMesh.h:46-52), whole tables; any pointer may be NULL. */
+/* PERIODIC boundaries (include/Includes.h:21, declared there and never
+ * paired): boundary edges pairs[2k] and pairs[2k+1] are tagged PERIODIC
+ * (-3) and paired; the outside state of each is its partner's inside slot,
+ * the flux of a pair is evaluated once (from the pair's lower edge) and
+ * applied with opposite signs, and the partner cell is the neighbour of
+ * the draining time step.  Pairs should join edges of equal length and
+ * opposite outward normals (a translation of the mesh boundary). */
+int swe_topology_set_periodic(swe_topology* t, int64_t n, const int64_t* pairs);
+/* partner[e] = the edge paired with edge e, -1 for unpaired edges. */
+int swe_topology_periodic(const swe_topology* t, int64_t* partner);
 int swe_topology_copy(const swe_topology* t, int64_t* edge_nodes, int64_t* edge_elements,
                       int64_t* element_nodes, int64_t* element_edges,
                       int64_t* element_neighbours);

@@ -51,7 +51,7 @@ def lib():
     L.orc_topology_copy.argtypes = [_p, _p, _p, _p, _p, _p]
     L.orc_topology_free.argtypes = [_p]
     L.orc_create.restype = _p
-    L.orc_create.argtypes = [_i64, _p, _i64, _p, _p, _i64, _p, _p, _p, _int, _int, _d, _d, _int]
+    L.orc_create.argtypes = [_i64, _p, _i64, _p, _p, _i64, _p, _p, _p, _int, _int, _d, _d, _int, _p]
     L.orc_destroy.argtypes = [_p]
     for n in ("orc_set_state", "orc_get_state", "orc_get_edge_field", "orc_get_src_field",
               "orc_get_fluxes", "orc_get_max_wp", "orc_classify"):
@@ -110,16 +110,18 @@ class OracleSolver:
     def __init__(self, xyz, topo, flux="HLLC", wave="Einfeldt", cor=0.0, tau=0.0, opts=0):
         L = lib()
         self.xyz = np.ascontiguousarray(xyz, dtype=np.float64)
-        self.topo = {k: np.ascontiguousarray(v, dtype=np.int64) for k, v in topo.items()}
+        self.topo = {k: np.ascontiguousarray(v, dtype=np.int64) for k, v in topo.items() if v is not None}
         t = self.topo
         self.V = self.xyz.shape[0]
         self.E = t["ep"].shape[0]
         self.T = t["tp"].shape[0]
+        pe = t.get("pe")
         self.h = L.orc_create(self.V, _ptr(self.xyz), self.E, _ptr(t["ep"]), _ptr(t["et"]),
                               self.T, _ptr(t["tp"]), _ptr(t["te"]), _ptr(t["tt"]),
-                              FLUX[flux], WAVE[wave], cor, tau, opts)
+                              FLUX[flux], WAVE[wave], cor, tau, opts, _ptr(pe) if pe is not None else None)
         if not self.h:
-            raise ValueError("unsupported boundary tag (SOLID_WALL = -1 and FREE_FLOW = -2 only)")
+            raise ValueError("unsupported boundary tag (SOLID_WALL = -1, FREE_FLOW = -2 and paired "
+                             "PERIODIC = -3 only)")
 
     def __del__(self):
         if getattr(self, "h", None) and lib is not None:  # lib is None at interpreter exit

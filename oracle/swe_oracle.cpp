@@ -25,8 +25,15 @@
 //       (src/Fluxes.cpp:22).
 //   D10 HLLC CFL denominator max(tol, max(al, ar)) (include/Fluxes.h:89).
 //   D11 SOLID_WALL (-1) as the reference, FREE_FLOW (-2) as a transmissive
-//       edge (declared, unimplemented in the reference); other tags are
+//       edge (declared, unimplemented in the reference); CUSTOM is
 //       rejected.
+//   D18 PERIODIC (-3, declared but never paired in the reference): edges
+//       paired by a partner table pe; the pair's flux is evaluated once,
+//       from its lower edge e (normal outward of et0(e), left state the
+//       slot of et0(e) at E(e), right state the slot of et0(pe[e]) at
+//       E(pe[e]), edge bathymetry of e, CFL lengths of both cells over
+//       L(e)), and the partner edge carries its negation; across the pair
+//       the draining time step of the neighbour is the partner cell's.
 //   D12 CFL mode: dt = cfl * r with r from the stage-1 flux pass, r <= 1.
 //
 // Floating point: built with -ffp-contract=off; every expression is written
@@ -68,7 +75,7 @@ enum Opt : int {
 // and FREE_FLOW = -2, declared but unimplemented in the reference: here the
 // outside state equals the inside edge state (a transmissive edge), so the
 // fluxer sees R = L and the CFL length uses the inside cell only.
-constexpr Idx SOLID_WALL = -1, FREE_FLOW = -2;
+constexpr Idx SOLID_WALL = -1, FREE_FLOW = -2, PERIODIC = -3;
 
 enum Flux : int { FLUX_HLL = 0, FLUX_HLLC = 1 };
 enum Wave : int { WS_RUSANOV = 0, WS_DAVIS = 1, WS_EINFELDT = 2 };
@@ -162,6 +169,7 @@ struct Mesh {
   Idx V = 0, E = 0, T = 0;
   std::vector<double> px, py, pz;
   std::vector<Idx> ep, et, tp, te, tt;  // row-major E x 2, E x 2, T x 3 x3
+  std::vector<Idx> pe;                  // partner of each PERIODIC edge (D18), -1 elsewhere
 };
 
 // Topology table builder — conventions of the reference's mesh reader,
@@ -618,10 +626,13 @@ struct Solver {
   }
 
   // Fluxes::HLL<W>: include/Fluxes.h:14-54
-  void HLL(Idx e, Idx from, Idx to, double* rr, double F[3]) const {
+  // sr_p >= 0: the right state's slot and to_p the cell whose area enters
+  // the CFL length (a periodic pair, D18)
+  void HLL(Idx e, Idx from, Idx to, double* rr, double F[3], Idx sr_p = -1, Idx to_p = -1) const {
     double n[2];
     Norm(e, from, n);
-    Idx sl = Slot(e, from, to), sr = to < 0 ? sl : Slot(e, to, from);
+    Idx sl = Slot(e, from, to), sr = sr_p >= 0 ? sr_p : (to < 0 ? sl : Slot(e, to, from));
+    if (to_p >= 0) to = to_p;
     double be = bedge(e);
     double ul = edg[3 * sl + 1] * n[0] + edg[3 * sl + 2] * n[1];
     double hl = edg[3 * sl] - be;
@@ -648,11 +659,12 @@ struct Solver {
   }
 
   // Fluxes::HLLC<W>: include/Fluxes.h:56-111
-  void HLLC(Idx e, Idx from, Idx to, double* rr, double F[3]) const {
+  void HLLC(Idx e, Idx from, Idx to, double* rr, double F[3], Idx sr_p = -1, Idx to_p = -1) const {
     double t[2], n[2];
     Tang(e, from, t);
     n[0] = t[1]; n[1] = -t[0];
-    Idx sl = Slot(e, from, to), sr = to < 0 ? sl : Slot(e, to, from);
+    Idx sl = Slot(e, from, to), sr = sr_p >= 0 ? sr_p : (to < 0 ? sl : Slot(e, to, from));
+    if (to_p >= 0) to = to_p;
     double be = bedge(e);
     double ul = edg[3 * sl + 1] * n[0] + edg[3 * sl + 2] * n[1];
     double vl = edg[3 * sl + 1] * t[0] + edg[3 * sl + 2] * t[1];
@@ -706,6 +718,13 @@ struct Solver {
         Norm(e, lf, n);
         double U[3] = {h(lf), 0., 0.};
         ElemFlux(n, U, F);
+      } else if (lt == PERIODIC) {  // D18: the pair's lower edge, its negation on the other
+        const Idx q = m.pe[e], p = std::min(e, q), s2 = std::max(e, q);
+        const Idx fp = m.et[2 * p], fs = m.et[2 * s2];
+        // slots of boundary edges: Slot(e, cell, -1) = 2 e
+        if (flux == FLUX_HLL) HLL(p, fp, PERIODIC, &r, F, 2 * s2, fs);
+        else HLLC(p, fp, PERIODIC, &r, F, 2 * s2, fs);
+        if (e != p) { F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2]; }
       } else if (flux == FLUX_HLL) {
         HLL(e, lf, lt, &r, F);
       } else {
@@ -740,7 +759,9 @@ struct Solver {
     for (int k = 0; k < 3; ++k) {
       Idx e = m.te[3 * i + k], nb = m.tt[3 * i + k];
       int sgn = (m.et[2 * e] == i) ? 1 : -1;
-      double dtik = ComputeDrainingDt(nb);
+      // D18: across a periodic pair the neighbour is the partner edge's cell
+      const Idx nbd = (nb < 0 && m.et[2 * e + 1] == PERIODIC) ? m.et[2 * m.pe[e]] : nb;
+      double dtik = ComputeDrainingDt(nbd);
       const double* F = &f[3 * e];
       double dtk = (sgn * F[0]) > 0. ? dmin(dt, dti) : dmin(dt, dtik);
       double c_ek = i_area * L(e);
@@ -849,9 +870,15 @@ void orc_topology_free(orc_topo* h) { delete h; }
 // Solver on given tables.  xyz: V x 3 row-major (x, y, z).
 Solver* orc_create(int64_t V, const double* xyz, int64_t E, const int64_t* ep,
                    const int64_t* et, int64_t T, const int64_t* tp, const int64_t* te,
-                   const int64_t* tt, int flux, int ws, double cor, double tau, int opts) {
-  for (int64_t e = 0; e < E; ++e)
-    if (et[2 * e + 1] < FREE_FLOW) return nullptr;  // D11: walls and free flow only
+                   const int64_t* tt, int flux, int ws, double cor, double tau, int opts,
+                   const int64_t* pe) {
+  for (int64_t e = 0; e < E; ++e) {
+    if (et[2 * e + 1] < PERIODIC) return nullptr;  // D11: CUSTOM is rejected
+    if (et[2 * e + 1] == PERIODIC) {               // D18: paired both ways
+      if (!pe || pe[e] < 0 || pe[e] >= E || pe[e] == e || et[2 * pe[e] + 1] != PERIODIC || pe[pe[e]] != e)
+        return nullptr;
+    }
+  }
   auto* s = new Solver;
   Mesh& m = s->m;
   m.V = V; m.E = E; m.T = T;
@@ -861,6 +888,7 @@ Solver* orc_create(int64_t V, const double* xyz, int64_t E, const int64_t* ep,
   }
   m.ep.assign(ep, ep + 2 * E); m.et.assign(et, et + 2 * E);
   m.tp.assign(tp, tp + 3 * T); m.te.assign(te, te + 3 * T); m.tt.assign(tt, tt + 3 * T);
+  if (pe) m.pe.assign(pe, pe + E);
   s->flux = flux; s->ws = ws; s->cor = cor; s->tau = tau; s->opts = opts;
   s->vol.assign(3 * T, 0.);
   s->edg.assign(6 * E, 0.);

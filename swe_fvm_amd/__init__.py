@@ -7,11 +7,11 @@ This package mirrors the reference's operator interface on top of it.
 from .core import (DomainError, DeviceError, Domain, Fluxer, Fluxes, SolverError, SpaceDisc,
                    Solvers, StructTriangMesh, TimeDisc, Topology, Wavespeeds, jittered_mesh,
                    read_msh, OPT_HEAD_GRADIENT, OPT_HEAD_PARTWET2, OPT_FRICTION, OPT_DRAIN_LENGTH, SOLID_WALL,
-                   FREE_FLOW, stack_domains, morton_order, reorder_cells)
+                   FREE_FLOW, PERIODIC, periodic_pairs, stack_domains, morton_order, reorder_cells)
 
 __all__ = [
     "DomainError", "DeviceError", "Domain", "Fluxer", "Fluxes", "SolverError", "SpaceDisc",
     "Solvers", "StructTriangMesh", "TimeDisc", "Topology", "Wavespeeds", "jittered_mesh",
     "read_msh", "OPT_HEAD_GRADIENT", "OPT_HEAD_PARTWET2", "OPT_FRICTION", "OPT_DRAIN_LENGTH", "SOLID_WALL",
-    "FREE_FLOW", "stack_domains", "morton_order", "reorder_cells",
+    "FREE_FLOW", "PERIODIC", "periodic_pairs", "stack_domains", "morton_order", "reorder_cells",
 ]

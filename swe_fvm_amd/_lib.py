@@ -23,6 +23,8 @@ SIGNATURES = [
     ("swe_last_error", ctypes.c_char_p, []),
     ("swe_topology_create", _int, [_i64, _i64, _p, _p, _i64, _p, _p, _p, _pp]),
     ("swe_topology_build", _int, [_i64, _i64, _p, _i64, _p, _pp]),
+    ("swe_topology_set_periodic", _int, [_p, _i64, _p]),
+    ("swe_topology_periodic", _int, [_p, _p]),
     ("swe_topology_read_msh", _int, [ctypes.c_char_p, _pp, _pi64, _pp]),
     ("swe_topology_num_nodes", _i64, [_p]),
     ("swe_topology_num_edges", _i64, [_p]),

@@ -27,6 +27,7 @@ OPT_FRICTION = 16  # linear friction -tau (hu, hv); the reference leaves tau unu
 OPT_DRAIN_LENGTH = 32  # draining dt from the outflow F0 * L (the reference omits L)
 SOLID_WALL = -1  # boundary tags, include/Includes.h:21
 FREE_FLOW = -2  # transmissive edge (declared but unimplemented in the reference)
+PERIODIC = -3  # paired edges (declared but never paired in the reference): Topology.with_periodic
 
 
 class DomainError(RuntimeError):
@@ -173,10 +174,43 @@ class Topology:
         te = np.empty((T, 3), np.int64)
         tt = np.empty((T, 3), np.int64)
         _check(_lib.load().swe_topology_copy(self._h, _ptr(ep), _ptr(et), _ptr(tp), _ptr(te), _ptr(tt)))
-        return dict(ep=ep, et=et, tp=tp, te=te, tt=tt)
+        out = dict(ep=ep, et=et, tp=tp, te=te, tt=tt)
+        if np.any(et[:, 1] == PERIODIC):  # the partner table of periodic pairs
+            pe = np.empty(E, np.int64)
+            _check(_lib.load().swe_topology_periodic(self._h, _ptr(pe)))
+            out["pe"] = pe
+        return out
 
     def IsEdgeBoundary(self, e):
         return bool(self.tables()["et"][e, 1] < 0)
+
+    @staticmethod
+    def _pairs(_unused, t):
+        """(n, 2) periodic pairs (lower edge first) of tables t."""
+        if "pe" not in t:
+            return np.zeros((0, 2), np.int64)
+        e = np.nonzero(t["pe"] >= 0)[0]
+        e = e[e < t["pe"][e]]
+        return np.stack([e, t["pe"][e]], 1)
+
+    @staticmethod
+    def _with_tables(num_nodes, t, pairs):
+        """A new Topology of tables t (periodic tags set by pairs)."""
+        et = t["et"].copy()
+        et[et[:, 1] == PERIODIC, 1] = SOLID_WALL
+        cp = Topology.create(num_nodes, t["ep"], et, t["tp"], t["te"], t["tt"])
+        if pairs.shape[0]:
+            pairs = np.ascontiguousarray(pairs, dtype=np.int64)
+            _check(_lib.load().swe_topology_set_periodic(cp._h, pairs.shape[0], _ptr(pairs)))
+        return cp
+
+    def with_periodic(self, pairs):
+        """A copy whose boundary edges pairs[k][0] and pairs[k][1] are PERIODIC
+        partners (swe_topology_set_periodic): the outside state of each is
+        the inside state of the other.  Existing pairs are kept."""
+        t = self.tables()
+        pairs = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+        return Topology._with_tables(self.NumNodes(), t, np.concatenate([Topology._pairs(None, t), pairs]))
 
     def with_boundary_tag(self, edges, tag):
         """A copy whose boundary edges `edges` carry `tag` (SOLID_WALL or
@@ -185,8 +219,38 @@ class Topology:
         edges = np.asarray(edges, dtype=np.int64)
         if np.any(t["et"][edges, 1] >= 0):
             raise DomainError("only boundary edges carry a tag")
+        if "pe" in t and np.any(t["pe"][edges] >= 0):
+            raise DomainError("periodic edges keep their tag")
+        if int(tag) == PERIODIC:
+            raise DomainError("PERIODIC edges come in pairs: Topology.with_periodic")
         t["et"][edges, 1] = int(tag)
-        return Topology.create(self.NumNodes(), t["ep"], t["et"], t["tp"], t["te"], t["tt"])
+        return Topology._with_tables(self.NumNodes(), t, Topology._pairs(None, t))
+
+
+def periodic_pairs(xyz, topo, shift, tol=1e-9):
+    """Pairs of boundary edges that map onto each other by the translation
+    shift = (dx, dy): edge a (midpoint m) pairs with the boundary edge b
+    whose midpoint lies within tol (relative to the domain size) of m +
+    shift.  Returns an (n, 2) array for Topology.with_periodic."""
+    t = topo.tables()
+    b = np.nonzero(t["et"][:, 1] < 0)[0]
+    mid = 0.5 * (xyz[t["ep"][b, 0], :2] + xyz[t["ep"][b, 1], :2])
+    shift = np.asarray(shift, dtype=np.float64)
+    eps = tol * max(1.0, float(np.abs(mid).max()))
+    q = 1e3 * eps  # hash cells much larger than the tolerance; neighbours checked
+    grid = {}
+    for k, m in enumerate(mid):
+        grid.setdefault(tuple(np.floor(m / q).astype(np.int64)), []).append(k)
+    pairs = []
+    for k, m in enumerate(mid):
+        target = m + shift
+        c = np.floor(target / q).astype(np.int64)
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for j in grid.get((c[0] + dx, c[1] + dy), ()):
+                    if np.all(np.abs(mid[j] - target) <= eps):
+                        pairs.append((b[k], b[j]))
+    return np.array(pairs, dtype=np.int64).reshape(-1, 2)
 
 
 def stack_domains(domains):
@@ -208,7 +272,8 @@ def stack_domains(domains):
     tp = np.concatenate([t["tp"] + nv[k] for k, t in enumerate(tabs)])
     te = np.concatenate([t["te"] + ne[k] for k, t in enumerate(tabs)])
     tt = np.concatenate([cells(t["tt"], k) for k, t in enumerate(tabs)])
-    topo = Topology.create(int(nv[-1]), ep, et, tp, te, tt)
+    pairs = np.concatenate([Topology._pairs(None, t) + ne[k] for k, t in enumerate(tabs)])
+    topo = Topology._with_tables(int(nv[-1]), dict(ep=ep, et=et, tp=tp, te=te, tt=tt), pairs)
     geo = np.concatenate([d.geometry for d in domains])
     return Domain(geo, topo), dict(nodes=nv, edges=ne, cells=nt)
 
@@ -242,8 +307,12 @@ def reorder_cells(domain, perm):
     b = np.nonzero(t["et"][:, 1] < 0)[0]
     topo = Topology.build(V, new_id[tri], new_id[t["ep"][b]])
     tags = t["et"][b, 1]
-    for tag in np.unique(tags[tags != SOLID_WALL]):
+    for tag in np.unique(tags[(tags != SOLID_WALL) & (tags != PERIODIC)]):
         topo = topo.with_boundary_tag(np.nonzero(tags == tag)[0], int(tag))
+    if "pe" in t:  # boundary edge b[k] is edge k of the rebuilt tables
+        new_e = np.full(t["ep"].shape[0], -1, np.int64)
+        new_e[b] = np.arange(b.shape[0])
+        topo = topo.with_periodic(new_e[Topology._pairs(None, t)])
     return Domain(domain.geometry[node_perm], topo), perm, node_perm
 
 

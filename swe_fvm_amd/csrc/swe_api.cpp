@@ -183,8 +183,13 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
     int64_t a = m.ep[2 * e], b = m.ep[2 * e + 1], f = m.et[2 * e], t = m.et[2 * e + 1];
     if (a < 0 || a >= V || b < 0 || b >= V) return fail(SWE_ERR_ARG, "edge node out of range");
     if (f < 0 || f >= T || t >= T) return fail(SWE_ERR_ARG, "edge element out of range");
-    if (t < -2)
-      return fail(SWE_ERR_DOMAIN, "boundary tag other than SOLID_WALL (-1) or FREE_FLOW (-2) is not supported");
+    if (t < -3)
+      return fail(SWE_ERR_DOMAIN, "boundary tag other than SOLID_WALL (-1), FREE_FLOW (-2) or PERIODIC (-3)");
+    if (t == -3) {
+      const int64_t q = m.pe.empty() ? -1 : m.pe[size_t(e)];
+      if (q < 0 || q >= E || m.et[2 * q + 1] != -3 || m.pe[size_t(q)] != e)
+        return fail(SWE_ERR_DOMAIN, "PERIODIC edge without a partner (swe_topology_set_periodic)");
+    }
     ep0[size_t(e)] = int(a); ep1[size_t(e)] = int(b); et0[size_t(e)] = int(f); et1[size_t(e)] = int(t);
   }
   std::vector<int> deg(size_t(V) + 1, 0);
@@ -230,6 +235,12 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
     for (int64_t i = 0; i < T; ++i)
       for (int k = 0; k < 3; ++k) n2c[size_t(fillp[size_t(m.tp[3 * i + k])]++)] = int(i);
   }
+  std::vector<int> pe;  // partner edges (periodic meshes only)
+  for (int64_t e = 0; e < E; ++e)
+    if (et1[size_t(e)] == -3) {
+      if (pe.empty()) pe.assign(size_t(E), -1);
+      pe[size_t(e)] = int(m.pe[size_t(e)]);
+    }
   std::vector<double> px(static_cast<size_t>(V)), py(static_cast<size_t>(V)), pz(static_cast<size_t>(V));
   for (int64_t n = 0; n < V; ++n) {
     px[size_t(n)] = xyz[3 * n]; py[size_t(n)] = xyz[3 * n + 1]; pz[size_t(n)] = xyz[3 * n + 2];
@@ -302,9 +313,18 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
   // ensembles below
   s->M = DevMesh{dpx, dpy, dpz, dtp, dte, dtt, dfl, dep0, dep1, det0, det1, doff, dn2c, T, E, V,
                  G.cx, G.cy, G.bc, G.area, G.edmin, G.crec, nullptr, nullptr, nullptr, 0, T, 0, E,
-                 T >= (int64_t(1) << 20) ? 1 : 0};
+                 T >= (int64_t(1) << 20) ? 1 : 0, nullptr};
   s->G = G;
-  if (n_inst == 0) {
+  if (!pe.empty()) {
+    int* dpe;
+    if ((e = dalloc(s, &dpe, E)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = hipMemcpy(dpe, pe.data(), sizeof(int) * size_t(E), hipMemcpyHostToDevice)) != hipSuccess)
+      return bail(e, "hipMemcpy");
+    s->M.pe = dpe;
+  }
+  // the fused kernel's tiles hold no periodic partners: periodic meshes keep
+  // the separate flux, draining and update kernels
+  if (n_inst == 0 && pe.empty()) {
     TileTables tt_host;
     if (build_tiles(T, tt.data(), te.data(), flag.data(), et0.data(), et1.data(), ep0.data(), ep1.data(), tt_host)) {
       uint32_t *dcode, *drcode;
@@ -379,6 +399,13 @@ int swe_batch_create(int64_t n, const swe_topology* const* topos, const double* 
     for (int64_t q = 0; q < 3 * m.T; ++q) {
       int64_t c = m.tt[size_t(q)];
       all.tt.push_back(c >= 0 ? c + T0 : c);
+    }
+    if (!m.pe.empty() || !all.pe.empty()) {
+      all.pe.resize(size_t(E0), -1);
+      for (int64_t q = 0; q < m.E; ++q) {
+        const int64_t c = m.pe.empty() ? -1 : m.pe[size_t(q)];
+        all.pe.push_back(c >= 0 ? c + E0 : -1);
+      }
     }
     xyz.insert(xyz.end(), xyzs[k], xyzs[k] + 3 * m.V);
     inst.insert(inst.end(), size_t(m.T), int(k));
@@ -1241,6 +1268,7 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
   if (dts)
     for (int64_t n = 0; n < nsteps; ++n)
       if (!(dts[n] > 0.)) return fail(SWE_ERR_ARG, "step sizes must be positive");
+  if (s->M.pe) return fail(SWE_ERR_ARG, "periodic boundaries are not differentiated");
   if (s->opts & (SWE_OPT_HEAD_GRADIENT | SWE_OPT_HEAD_PARTWET2))
     return fail(SWE_ERR_ARG, "the adjoint does not support the HEAD-compatibility options");
   const int64_t K = checkpoint_every > 0 ? std::min(checkpoint_every, nsteps) : nsteps;
@@ -1449,6 +1477,7 @@ int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int
   if (dts)
     for (int64_t n = 0; n < nsteps; ++n)
       if (!(dts[n] > 0.)) return fail(SWE_ERR_ARG, "step sizes must be positive");
+  if (s->M.pe) return fail(SWE_ERR_ARG, "periodic boundaries are not differentiated");
   if (s->opts & (SWE_OPT_HEAD_GRADIENT | SWE_OPT_HEAD_PARTWET2))
     return fail(SWE_ERR_ARG, "the tangent run does not support the HEAD-compatibility options");
   if (int rc = set_device(s)) return rc;

@@ -15,6 +15,8 @@ namespace swe {
 // boundary tags (include/Includes.h:21): walls (src/SpaceDisc.cpp:66-69) and
 // free flow, whose outside state is the inside edge state (transmissive)
 constexpr int kSolidWall = -1;
+// PERIODIC: the outside state is the partner edge's inside slot (M.pe)
+constexpr int kPeriodic = -3;
 constexpr int kOptDrainLength = 32;  // SWE_OPT_DRAIN_LENGTH
 
 __device__ __forceinline__ int logical_block() {

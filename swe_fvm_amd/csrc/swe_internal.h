@@ -13,6 +13,9 @@ namespace swe {
 struct Topo {
   int64_t V = 0, E = 0, T = 0;
   std::vector<int64_t> ep, et, tp, te, tt;
+  // partner edge of every PERIODIC (-3) boundary edge, -1 elsewhere; empty
+  // when the mesh has no periodic edges
+  std::vector<int64_t> pe;
 };
 
 int fail(int code, const std::string& msg);

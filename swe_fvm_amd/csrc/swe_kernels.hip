@@ -66,7 +66,9 @@ __global__ void __launch_bounds__(256) k_setup_edges(DevMesh M, DevGeomOut G) {
   double dm = 0.;
   if (t != kSolidWall) {  // interior, or free flow (the inside cell twice)
     const double le = edge_len(node(M, M.ep0[e]), node(M, M.ep1[e]));
-    const double dl = 2. * G.area[f] / le, dr = t < 0 ? dl : 2. * G.area[t] / le;
+    const double dl = 2. * G.area[f] / le;
+    double dr = t < 0 ? dl : 2. * G.area[t] / le;
+    if (t == kPeriodic) dr = 2. * G.area[M.et0[M.pe[e]]] / le;
     dm = dmin(dl, dr);
   }
   G.edmin[e] = dm;
@@ -359,11 +361,12 @@ __device__ __forceinline__ void block_min_bits(double v, unsigned long long* tar
 // reads it); the fluxes are the same.
 struct FluxIn {
   V3 Pa, Pb;  // P(ep0), P(ep1)
+  V3 Eq;      // periodic: the partner edge's midpoint (the right state's point)
   double edmin, hf, cxf, cyf, cxt, cyt;
   Muscl mf, mt;
   int64_t e;
   int t, inst;
-  bool on;
+  bool on, neg;  // neg: the second edge of a periodic pair takes -F of the first
 };
 template <int FLUX, int WS, int NI, bool CFL, int BS, bool NT>
 __global__ void __launch_bounds__(BS) k_flux(DevMesh M, DevState S, DevWork W, double cor,
@@ -376,11 +379,23 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, DevState S, DevWork W, d
     const int64_t es = e < M.e1 ? e : M.e0;  // out-of-range items read a valid edge and are dropped
     FluxIn& q = in[it];
     q.e = e;
-    q.Pa = node(M, M.ep0[es]);
-    q.Pb = node(M, M.ep1[es]);
-    q.edmin = CFL ? M.edmin[es] : 1.;
-    const int f = M.et0[es], t = M.et1[es];
-    const int tr = t < 0 ? f : t;  // boundary edges read their own cell twice: R = L
+    int f = M.et0[es];
+    const int t = M.et1[es];
+    int tr = t < 0 ? f : t;  // boundary edges read their own cell twice: R = L
+    int64_t eg = es;         // the edge whose geometry and orientation the flux takes
+    q.neg = false;
+    if (t == kPeriodic) {  // the pair's first edge, its cell left, the partner's right
+      const int qe = M.pe[es];
+      const int p = es < qe ? (int)es : qe, s2 = es < qe ? qe : (int)es;
+      eg = p;
+      f = M.et0[p];
+      tr = M.et0[s2];
+      q.neg = es != p;
+      q.Eq = midpoint(node(M, M.ep0[s2]), node(M, M.ep1[s2]));
+    }
+    q.Pa = node(M, M.ep0[eg]);
+    q.Pb = node(M, M.ep1[eg]);
+    q.edmin = CFL ? M.edmin[eg] : 1.;
     q.t = t;
     q.mf = load_muscl(S, W, f);
     q.mt = load_muscl(S, W, tr);
@@ -412,9 +427,10 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, DevState S, DevWork W, d
       elem_flux(nx, ny, q.hf, 0., 0., F[0], F[1], F[2]);
     } else {
       Slot L = slot_value(q.mf, q.cxf, q.cyf, E);
-      Slot R = slot_value(q.mt, q.cxt, q.cyt, E);
+      Slot R = slot_value(q.mt, q.cxt, q.cyt, q.t == kPeriodic ? q.Eq : E);
       if (FLUX == 0) flux_hll<WS>(nx, ny, E.z, L, R, q.edmin, cor, lwi, F);
       else flux_hllc<WS>(tx, ty, E.z, L, R, q.edmin, cor, lwi, F);
+      if (q.neg) { F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2]; }
     }
     lw = dmin(lw, lwi);
     lwi_item[it] = lwi;
@@ -507,6 +523,8 @@ __device__ __forceinline__ void rhs_load(const DevMesh& M, const DevState& S, co
     q.F2[k] = W.f2[e];
     const double d = W.dtd[nb < 0 ? i : nb];
     q.dtik[k] = nb < 0 ? INFINITY : d;
+    // across a periodic edge the neighbour is the partner edge's cell
+    if (nb < 0 && M.pe && M.et1[e] == kPeriodic) q.dtik[k] = W.dtd[M.et0[M.pe[e]]];
   }
 }
 __device__ __forceinline__ void rhs_cell(const DevMesh& M, const DevState& S, const DevWork& W,

@@ -7,7 +7,7 @@
 //            bits 0-2  et0(te[k]) == cell (flux orientation of edge k)
 //            bit  3    boundary triangle (some edge has et1 < 0)
 //            bits 4+4k, 6+4k  local vertex of ep0 / ep1 of edge k
-//   edges  ep0, ep1, et0, et1             int32[E]
+//   edges  ep0, ep1, et0, et1             int32[E], pe (periodic partner) int32[E]
 //   node -> cell map  n2c_off[V+1], n2c[3T]
 //   static geometry (computed once by the setup kernels with the step's own
 //   formulas, so every use is bit-identical to recomputing it):
@@ -78,6 +78,9 @@ struct DevMesh {
   // 2^20 cells or more: a stage's data exceeds the 256 MB Infinity Cache;
   // smaller ones keep them cached for the next kernel)
   int nt;
+  // partner edge of each PERIODIC edge (et1 == -3), -1 elsewhere; null when
+  // the mesh has none
+  const int* pe;
 };
 
 // per-instance control of a batched ensemble

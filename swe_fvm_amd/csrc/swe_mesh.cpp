@@ -532,6 +532,37 @@ int swe_topology_copy(const swe_topology* t, int64_t* edge_nodes, int64_t* edge_
   return SWE_OK;
 }
 
+// Pairs boundary edges a_k <-> b_k as PERIODIC (tag -3, include/Includes.h:21):
+// the outside state of each is the inside state of its partner.
+int swe_topology_set_periodic(swe_topology* t, int64_t n, const int64_t* pairs) {
+  if (!t) return fail(SWE_ERR_ARG, "null topology");
+  if (n < 0 || (n > 0 && !pairs)) return fail(SWE_ERR_ARG, "invalid periodic pairs");
+  Topo& m = t->m;
+  std::vector<int64_t> pe = m.pe.empty() ? std::vector<int64_t>(size_t(m.E), -1) : m.pe;
+  std::vector<int64_t> et = m.et;
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t a = pairs[2 * k], b = pairs[2 * k + 1];
+    if (a < 0 || a >= m.E || b < 0 || b >= m.E || a == b) return fail(SWE_ERR_ARG, "periodic pair out of range");
+    if (et[size_t(2 * a + 1)] >= 0 || et[size_t(2 * b + 1)] >= 0)
+      return fail(SWE_ERR_DOMAIN, "only boundary edges can be periodic");
+    if (pe[size_t(a)] >= 0 || pe[size_t(b)] >= 0) return fail(SWE_ERR_DOMAIN, "edge paired twice");
+    pe[size_t(a)] = b;
+    pe[size_t(b)] = a;
+    et[size_t(2 * a + 1)] = -3;
+    et[size_t(2 * b + 1)] = -3;
+  }
+  m.pe.swap(pe);
+  m.et.swap(et);
+  return SWE_OK;
+}
+
+int swe_topology_periodic(const swe_topology* t, int64_t* partner) {
+  if (!t || !partner) return fail(SWE_ERR_ARG, "null argument");
+  const Topo& m = t->m;
+  for (int64_t e = 0; e < m.E; ++e) partner[e] = m.pe.empty() ? -1 : m.pe[size_t(e)];
+  return SWE_OK;
+}
+
 void swe_topology_destroy(swe_topology* t) { delete t; }
 
 }  // extern "C"

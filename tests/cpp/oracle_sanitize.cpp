@@ -16,7 +16,7 @@ void orc_topology_copy(const orc_topo* h, int64_t* ep, int64_t* et, int64_t* tp,
 void orc_topology_free(orc_topo* h);
 Solver* orc_create(int64_t V, const double* xyz, int64_t E, const int64_t* ep, const int64_t* et, int64_t T,
                    const int64_t* tp, const int64_t* te, const int64_t* tt, int flux, int ws, double cor,
-                   double tau, int opts);
+                   double tau, int opts, const int64_t* pe);
 void orc_destroy(Solver* s);
 void orc_set_state(Solver* s, const double* prim);
 void orc_get_state(const Solver* s, double* prim);
@@ -74,7 +74,7 @@ int main() {
       for (int opts : opts_list)
         for (int scheme = 0; scheme < 3; ++scheme) {
           Solver* s = orc_create(V, xyz.data(), E, ep.data(), et.data(), T, tp.data(), te.data(), tt.data(),
-                                 flux, ws, 0.3, 0.5, opts);
+                                 flux, ws, 0.3, 0.5, opts, nullptr);
           if (!s) return 2;
           orc_set_state(s, prim.data());
           double dts[4];
