@@ -698,3 +698,116 @@ def test_autograd_function_cell_bathymetry_and_cfl_mode():
         assert abs(loss.item() - J) <= 1e-12 * abs(J)
         assert np.allclose(p.grad.numpy(), s0, rtol=1e-12, atol=1e-15)
         assert np.allclose(bt.grad.numpy(), bb, rtol=1e-12, atol=1e-15)
+
+
+def _inversion_batch(members, n, N, dt):
+    """The stacked bathymetry-inversion batch: the domain, initial state,
+    node bathymetry, gauge targets from one batched run over the members'
+    true bathymetries, and the member offsets."""
+    from swe_fvm_amd import workloads as W
+    d = W.adjoint_inversion_batch(members=members, n=n)
+    dom1, prim1 = d["member"]
+    big, off = swe.stack_domains([dom1] * members)
+    prim = np.tile(prim1, (members, 1))
+    z0 = np.tile(d["z"], members)
+    sd = swe.SpaceDisc(swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt], big, prim)
+    sd.set_bathymetry(np.concatenate(d["z_true"]))
+    sd.SetVolField(prim)
+    sd.run("SSPRK2", N, dt=dt)
+    fin = sd.GetVolField()
+    targets = [(g + off["cells"][k], fin[g + off["cells"][k], 0].copy()) for k, g in enumerate(d["gauges"])]
+    return d, big, off, sd, prim, z0, targets
+
+
+def test_inversion_batch_full_size():
+    """The bathymetry-inversion batch at full size: 250,632 cells per member,
+    2 members stacked, 2,000 SSP-RK2 steps at dt = 2e-4, checkpoints every
+    50.  Each member's slice of the batched final state and gradients equals
+    that member run alone, bit for bit; the batched gradient satisfies the
+    duality with the tangent-linear build over the full horizon to 1e-12 in
+    directions that move both members."""
+    from swe_fvm_amd import workloads as W
+    B, N, dt = 2, 2000, 2e-4
+    d, big, off, sd, prim, z0, targets = _inversion_batch(B, 354, N, dt)
+    dom1, prim1 = d["member"]
+    T1, V1 = prim1.shape[0], dom1.geometry.shape[0]
+    assert T1 == 250_632
+    fn = AD.batch_gauge_misfit(targets)
+    J, s0, zb, ms = AD.run_and_adjoint(sd, prim, z0, N, fn, dt=dt, checkpoint_every=50)
+    fin = sd.GetVolField()
+    print(f"inversion batch {B} x {T1} cells, {N} steps: J {J:.6e}, sweeps {ms[0]:.0f} + {ms[1]:.0f} ms",
+          flush=True)
+    assert J > 0 and np.all(np.isfinite(s0)) and np.all(np.isfinite(zb))
+    sd1 = swe.SpaceDisc(swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt], dom1, prim1)
+    for k in range(B):
+        cs, vs = slice(k * T1, (k + 1) * T1), slice(k * V1, (k + 1) * V1)
+        f1 = AD.gauge_misfit(d["gauges"][k], targets[k][1])
+        J1, a0, az, _ = AD.run_and_adjoint(sd1, prim1, d["z"], N, f1, dt=dt, checkpoint_every=50)
+        same = (np.array_equal(sd1.GetVolField(), fin[cs]), np.array_equal(a0, s0[cs]),
+                np.array_equal(az, zb[vs]))
+        print(f"member {k}: J {J1:.6e} final/state-gradient/bathymetry-gradient bitwise {same}", flush=True)
+        assert all(same)
+        assert J1 == fn.parts[k]
+    rng = np.random.default_rng(8)
+    c = big.centroids()
+    ybar = fn(fin)[1]
+    taper = W.wall_taper(big.geometry[:, 0]) * W.wall_taper(big.geometry[:, 1])
+    for trial in range(2):
+        pdot = np.zeros_like(prim)
+        pdot[:, 0] = W.smooth_field(c[:, 0], c[:, 1], rng, 1e-2)
+        zdot = W.smooth_field(big.geometry[:, 0], big.geometry[:, 1], rng, 1e-2) * taper
+        sd.set_bathymetry(z0)
+        sd.SetVolField(prim)
+        tdot, _ = sd.tangent_run(pdot, N, z_dot=zdot, dt=dt)
+        lhs = float(np.sum(ybar * tdot))
+        rhs = float(np.sum(s0 * pdot) + np.sum(zb * zdot))
+        scale = float(np.sum(np.abs(ybar * tdot)))
+        print(f"full-size duality {trial}: {lhs:.15e} {rhs:.15e} diff/scale {abs(lhs - rhs) / scale:.1e}",
+              flush=True)
+        assert abs(lhs - rhs) <= 1e-12 * scale
+
+
+def test_inversion_batch_against_differences_of_the_oracle():
+    """The inversion batch on a small mesh (2 members of 128 cells, 300
+    steps, checkpoints every 50): the batched loss is the oracle's, and the
+    GPU gradient matches central differences of the oracle's forward in
+    smooth directions of the initial surface and the node bathymetry."""
+    from oracle import oracle as O
+    from swe_fvm_amd import workloads as W
+    B, N, dt = 2, 300, 2e-3
+    d, big, off, sd, prim, z0, targets = _inversion_batch(B, 8, N, dt)
+    fn = AD.batch_gauge_misfit(targets)
+    J, s0, zb, _ = AD.run_and_adjoint(sd, prim, z0, N, fn, dt=dt, checkpoint_every=50)
+    xyz0 = big.geometry.copy()
+    tables = big.topology.tables()
+
+    def J_oracle(p, zz):
+        xyz = xyz0.copy()
+        xyz[:, 2] = zz
+        o = O.OracleSolver(xyz, tables, "HLLC", "Einfeldt")
+        o.set_state(p)
+        o.run("SSPRK2", N, dt=dt)
+        return fn(o.state())[0]
+
+    Jo = J_oracle(prim, z0)
+    print(f"inversion loss: GPU {J:.17e} oracle {Jo:.17e}", flush=True)
+    assert abs(Jo - J) <= 1e-13 * abs(J)  # the forward is the oracle's
+    rng = np.random.default_rng(21)
+    c = big.centroids()
+    taper = W.wall_taper(xyz0[:, 0]) * W.wall_taper(xyz0[:, 1])
+    checked = 0
+    for trial in range(6):
+        dp = np.zeros_like(prim)
+        dp[:, 0] = W.smooth_field(c[:, 0], c[:, 1], rng, 1e-2)
+        dz = W.smooth_field(xyz0[:, 0], xyz0[:, 1], rng, 1e-2) * taper
+        cd = [(J_oracle(prim + e * dp, z0 + e * dz) - J_oracle(prim - e * dp, z0 - e * dz)) / (2 * e)
+              for e in (1e-6, 2.5e-7)]
+        if abs(cd[0] - cd[1]) > 1e-5 * abs(cd[0]) + 1e-14:
+            print(f"inversion trial {trial}: the difference crossed a switch", flush=True)
+            continue
+        ad = float(np.sum(s0 * dp) + np.sum(zb * dz))
+        print(f"inversion trial {trial}: adjoint {ad:.12e} oracle fd {cd[0]:.12e} "
+              f"rel {abs(ad - cd[0]) / abs(cd[0]):.2e}", flush=True)
+        assert abs(ad - cd[0]) <= 1e-5 * abs(cd[0]) + 1e-14
+        checked += 1
+    assert checked >= 3
