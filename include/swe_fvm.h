@@ -53,6 +53,19 @@ extern "C" {
  * per unit length: its A h / sum is not a time, falls below the CFL step
  * in fully wet cells as the mesh is refined and stalls convergence. */
 #define SWE_OPT_DRAIN_LENGTH 32
+/* Groupings that keep a lake at rest to the bit wherever the two edge states
+ * are equal: HLLC's star depth h (a - u) / (a - u*) taken as
+ * h ((a - u) / (a - u*)) (include/Fluxes.h:96-108), HLL's flux written
+ * Fl + al (ar (Ur - Ul) - (Fr - Fl)) / (ar - al) (include/Fluxes.h:44-52),
+ * the RHS adding dt L / A (sgn F - n h_e^2 / 2) per edge instead of the
+ * flux and the pressure correction apart (src/TimeDisc.cpp:31,35), and the
+ * wall flux taking the inside edge depth h_e, the depth of that correction,
+ * instead of the cell mean (src/SpaceDisc.cpp:66-69), and the Shu-Osher
+ * stage combinations a0 U0 + a1 U written U + a0 (U0 - U) (a0 + a1 = 1,
+ * src/Solvers.cpp:23-25,42-58).  Results
+ * differ from the reference's in the last bits.  Not differentiated
+ * (swe_adjoint_run and swe_tangent_run refuse it). */
+#define SWE_OPT_EXACT_BALANCE 64
 
 /* Boundary tags of edge_elements[e][1] (include/Includes.h:21):
  * SOLID_WALL (src/SpaceDisc.cpp:66-69); FREE_FLOW, declared but

@@ -24,6 +24,7 @@ OPT_SEQ_DRAINING = 4
 OPT_SEQ_NODEMAX = 8
 OPT_FRICTION = 16
 OPT_DRAIN_LENGTH = 32
+OPT_EXACT_BALANCE = 64
 
 _lib = None
 

@@ -70,6 +70,7 @@ enum Opt : int {
   OPT_SEQ_NODEMAX = 8,        // D6 literal
   OPT_FRICTION = 16,          // linear friction -tau (hu, hv) (tau is unused in the reference)
   OPT_DRAIN_LENGTH = 32,      // D16 corrected: outflow F0 * L in the draining dt
+  OPT_EXACT_BALANCE = 64,     // D19: groupings that keep a lake at rest to the bit
 };
 // Boundaries (include/Includes.h:21): SOLID_WALL = -1 (src/SpaceDisc.cpp:66-69)
 // and FREE_FLOW = -2, declared but unimplemented in the reference: here the
@@ -654,6 +655,13 @@ struct Solver {
     double Ul[3], Ur[3], Fl[3], Fr[3];
     SlotCons(e, sl, Ul); SlotCons(e, sr, Ur);
     ElemFlux(n, Ul, Fl); ElemFlux(n, Ur, Fr);
+    if (opts & OPT_EXACT_BALANCE) {
+      // D19: the same flux as Fl + al (ar (Ur - Ul) - (Fr - Fl)) / (ar - al),
+      // which is Fl exactly when the two states are equal
+      for (int c = 0; c < 3; ++c)
+        F[c] = Fl[c] + al * (ar * (Ur[c] - Ul[c]) - (Fr[c] - Fl[c])) / (ar - al);
+      return;
+    }
     for (int c = 0; c < 3; ++c)
       F[c] = (ar * Fl[c] - al * Fr[c] + (al * ar) * (Ur[c] - Ul[c])) / (ar - al);
   }
@@ -691,7 +699,8 @@ struct Solver {
     if (ustar <= 0) {
       double us = vr * t[0] + ustar * t[1];
       double vs = vr * n[0] + ustar * n[1];
-      double hs = hr * (ar - ur) / (ar - ustar);
+      // D19: the ratio first, so that at rest (u* = ur = 0) hs is hr exactly
+      double hs = (opts & OPT_EXACT_BALANCE) ? hr * ((ar - ur) / (ar - ustar)) : hr * (ar - ur) / (ar - ustar);
       double Us[3] = {hs, hs * us, hs * vs};
       ElemFlux(n, Ur, Fe);
       double s = dmax(0., ar);
@@ -699,7 +708,7 @@ struct Solver {
     } else {
       double us = vl * t[0] + ustar * t[1];
       double vs = vl * n[0] + ustar * n[1];
-      double hs = hl * (al - ul) / (al - ustar);
+      double hs = (opts & OPT_EXACT_BALANCE) ? hl * ((al - ul) / (al - ustar)) : hl * (al - ul) / (al - ustar);
       double Us[3] = {hs, hs * us, hs * vs};
       ElemFlux(n, Ul, Fe);
       double s = dmin(0., al);
@@ -716,7 +725,10 @@ struct Solver {
       if (lt == SOLID_WALL) {
         double n[2];
         Norm(e, lf, n);
-        double U[3] = {h(lf), 0., 0.};
+        // D19: with OPT_EXACT_BALANCE the inside edge depth, the depth of
+        // the RHS's pressure correction, instead of the cell mean
+        const double hw = (opts & OPT_EXACT_BALANCE) ? edg[3 * Slot(e, lf, lt)] - bedge(e) : h(lf);
+        double U[3] = {hw, 0., 0.};
         ElemFlux(n, U, F);
       } else if (lt == PERIODIC) {  // D18: the pair's lower edge, its negation on the other
         const Idx q = m.pe[e], p = std::min(e, q), s2 = std::max(e, q);
@@ -767,6 +779,21 @@ struct Solver {
       double c_ek = i_area * L(e);
       Idx s = Slot(e, i, nb);
       double h_ek = edg[3 * s] - bedge(e);
+      if (opts & OPT_EXACT_BALANCE) {
+        // D19: the flux minus the cell's own hydrostatic edge pressure,
+        // zero to the bit where the edge states are equal and at rest
+        double n[2];
+        Norm(e, i, n);
+        double q = 0.5 * h_ek * h_ek;
+        double g0 = sgn * F[0], g1 = sgn * F[1] - n[0] * q, g2 = sgn * F[2] - n[1] * q;
+        res[0] -= dtk * (c_ek * g0);
+        res[1] -= dtk * (c_ek * g1);
+        res[2] -= dtk * (c_ek * g2);
+        double cs = dt * (1. / 3.);
+        res[1] -= cs * src[3 * s + 1] * h_ek;
+        res[2] -= cs * src[3 * s + 2] * h_ek;
+        continue;
+      }
       double cf = dtk * sgn * c_ek;
       res[0] -= cf * F[0];
       res[1] -= cf * F[1];
@@ -806,7 +833,12 @@ struct Solver {
       if (U0) {
         double Ua[3];
         cons_of(*U0, i, Ua);
-        for (int c = 0; c < 3; ++c) rhs[c] = (a0 * Ua[c] + a1 * Uc[c]) + R[c];
+        // D19: with OPT_EXACT_BALANCE a convex combination is written
+        // Uc + a0 (Ua - Uc), which is Uc to the bit where Ua = Uc
+        if ((opts & OPT_EXACT_BALANCE) && a0 + a1 == 1.)
+          for (int c = 0; c < 3; ++c) rhs[c] = (Uc[c] + a0 * (Ua[c] - Uc[c])) + R[c];
+        else
+          for (int c = 0; c < 3; ++c) rhs[c] = (a0 * Ua[c] + a1 * Uc[c]) + R[c];
       } else {
         for (int c = 0; c < 3; ++c) rhs[c] = Uc[c] + R[c];
       }

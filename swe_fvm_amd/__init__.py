@@ -6,12 +6,14 @@ This package mirrors the reference's operator interface on top of it.
 """
 from .core import (DomainError, DeviceError, Domain, Fluxer, Fluxes, SolverError, SpaceDisc,
                    Solvers, StructTriangMesh, TimeDisc, Topology, Wavespeeds, jittered_mesh,
-                   read_msh, OPT_HEAD_GRADIENT, OPT_HEAD_PARTWET2, OPT_FRICTION, OPT_DRAIN_LENGTH, SOLID_WALL,
-                   FREE_FLOW, PERIODIC, periodic_pairs, stack_domains, morton_order, reorder_cells)
+                   read_msh, OPT_HEAD_GRADIENT, OPT_HEAD_PARTWET2, OPT_FRICTION, OPT_DRAIN_LENGTH,
+                   OPT_EXACT_BALANCE, SOLID_WALL, FREE_FLOW, PERIODIC, periodic_pairs, stack_domains,
+                   morton_order, reorder_cells)
 
 __all__ = [
     "DomainError", "DeviceError", "Domain", "Fluxer", "Fluxes", "SolverError", "SpaceDisc",
     "Solvers", "StructTriangMesh", "TimeDisc", "Topology", "Wavespeeds", "jittered_mesh",
-    "read_msh", "OPT_HEAD_GRADIENT", "OPT_HEAD_PARTWET2", "OPT_FRICTION", "OPT_DRAIN_LENGTH", "SOLID_WALL",
-    "FREE_FLOW", "PERIODIC", "periodic_pairs", "stack_domains", "morton_order", "reorder_cells",
+    "read_msh", "OPT_HEAD_GRADIENT", "OPT_HEAD_PARTWET2", "OPT_FRICTION", "OPT_DRAIN_LENGTH",
+    "OPT_EXACT_BALANCE", "SOLID_WALL", "FREE_FLOW", "PERIODIC", "periodic_pairs", "stack_domains",
+    "morton_order", "reorder_cells",
 ]

@@ -25,6 +25,7 @@ OPT_HEAD_GRADIENT = 1
 OPT_HEAD_PARTWET2 = 2
 OPT_FRICTION = 16  # linear friction -tau (hu, hv); the reference leaves tau unused
 OPT_DRAIN_LENGTH = 32  # draining dt from the outflow F0 * L (the reference omits L)
+OPT_EXACT_BALANCE = 64  # flux / RHS groupings that keep equal edge states at rest to the bit
 SOLID_WALL = -1  # boundary tags, include/Includes.h:21
 FREE_FLOW = -2  # transmissive edge (declared but unimplemented in the reference)
 PERIODIC = -3  # paired edges (declared but never paired in the reference): Topology.with_periodic

@@ -169,7 +169,8 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
   if (!xyz || !out) return fail(SWE_ERR_ARG, "null argument");
   if (flux < 0 || flux > 1 || wavespeed < 0 || wavespeed > 2)
     return fail(SWE_ERR_ARG, "unknown flux or wave-speed estimate");
-  if (options & ~(SWE_OPT_HEAD_GRADIENT | SWE_OPT_HEAD_PARTWET2 | SWE_OPT_FRICTION | SWE_OPT_DRAIN_LENGTH))
+  if (options & ~(SWE_OPT_HEAD_GRADIENT | SWE_OPT_HEAD_PARTWET2 | SWE_OPT_FRICTION | SWE_OPT_DRAIN_LENGTH |
+                  SWE_OPT_EXACT_BALANCE))
     return fail(SWE_ERR_ARG, "unknown option bits");
   if (m.T > 0x7fffffff || m.E > 0x7fffffff || m.V > 0x7fffffff)
     return fail(SWE_ERR_ARG, "mesh too large for int32 device indices");
@@ -313,7 +314,7 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
   // ensembles below
   s->M = DevMesh{dpx, dpy, dpz, dtp, dte, dtt, dfl, dep0, dep1, det0, det1, doff, dn2c, T, E, V,
                  G.cx, G.cy, G.bc, G.area, G.edmin, G.crec, nullptr, nullptr, nullptr, 0, T, 0, E,
-                 T >= (int64_t(1) << 20) ? 1 : 0, nullptr};
+                 T >= (int64_t(1) << 20) ? 1 : 0, nullptr, (options & SWE_OPT_EXACT_BALANCE) ? 1 : 0};
   s->G = G;
   if (!pe.empty()) {
     int* dpe;
@@ -1269,8 +1270,8 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
     for (int64_t n = 0; n < nsteps; ++n)
       if (!(dts[n] > 0.)) return fail(SWE_ERR_ARG, "step sizes must be positive");
   if (s->M.pe) return fail(SWE_ERR_ARG, "periodic boundaries are not differentiated");
-  if (s->opts & (SWE_OPT_HEAD_GRADIENT | SWE_OPT_HEAD_PARTWET2))
-    return fail(SWE_ERR_ARG, "the adjoint does not support the HEAD-compatibility options");
+  if (s->opts & (SWE_OPT_HEAD_GRADIENT | SWE_OPT_HEAD_PARTWET2 | SWE_OPT_EXACT_BALANCE))
+    return fail(SWE_ERR_ARG, "the adjoint does not support the HEAD-compatibility or exact-balance options");
   const int64_t K = checkpoint_every > 0 ? std::min(checkpoint_every, nsteps) : nsteps;
   if (int rc = set_device(s)) return rc;
   if (int rc = materialize_fluxes(s)) return rc;
@@ -1478,8 +1479,8 @@ int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int
     for (int64_t n = 0; n < nsteps; ++n)
       if (!(dts[n] > 0.)) return fail(SWE_ERR_ARG, "step sizes must be positive");
   if (s->M.pe) return fail(SWE_ERR_ARG, "periodic boundaries are not differentiated");
-  if (s->opts & (SWE_OPT_HEAD_GRADIENT | SWE_OPT_HEAD_PARTWET2))
-    return fail(SWE_ERR_ARG, "the tangent run does not support the HEAD-compatibility options");
+  if (s->opts & (SWE_OPT_HEAD_GRADIENT | SWE_OPT_HEAD_PARTWET2 | SWE_OPT_EXACT_BALANCE))
+    return fail(SWE_ERR_ARG, "the tangent run does not support the HEAD-compatibility or exact-balance options");
   if (int rc = set_device(s)) return rc;
   if (int rc = materialize_fluxes(s)) return rc;
   const int64_t T = s->T, E = s->E, V = s->V;

@@ -89,9 +89,12 @@ struct RhsCell {
 struct RhsIn : RhsCell {
   double F0[3], F1[3], F2[3], dtik[3];
 };
+// eb (SWE_OPT_EXACT_BALANCE): each edge adds the flux minus the cell's own
+// hydrostatic edge pressure, sg F - n h_ek^2 / 2, which is zero to the bit
+// where the edge states on both sides are equal and at rest
 template <class EdgeIn>
 __device__ __forceinline__ void rhs_compute_by(const RhsCell& q, EdgeIn edge, double dts, double cor,
-                                               double fric, double& r0, double& r1, double& r2) {
+                                               double fric, bool eb, double& r0, double& r1, double& r2) {
   r0 = 0.; r1 = 0.; r2 = 0.;
   // Domain::T / Area (src/Bathymetry.cpp:24-27,84-90)
   const V3 c = centroid(q.p[0], q.p[1], q.p[2]);
@@ -118,6 +121,17 @@ __device__ __forceinline__ void rhs_compute_by(const RhsCell& q, EdgeIn edge, do
     double tx, ty;
     tangent(Pa, Pb, le, c, tx, ty);
     const double nx = ty, ny = -tx;
+    if (eb) {
+      const double q2 = 0.5 * h_ek * h_ek;
+      const double g0 = sg * F0, g1 = sg * F1 - nx * q2, g2 = sg * F2 - ny * q2;
+      r0 -= dtk * (c_ek * g0);
+      r1 -= dtk * (c_ek * g1);
+      r2 -= dtk * (c_ek * g2);
+      const double cs = dts * (1. / 3.);
+      r1 -= cs * src1 * h_ek;
+      r2 -= cs * src2 * h_ek;
+      continue;
+    }
     const double cf = dtk * sg * c_ek;
     r0 -= cf * F0;
     r1 -= cf * F1;
@@ -137,14 +151,21 @@ __device__ __forceinline__ void rhs_compute_by(const RhsCell& q, EdgeIn edge, do
     r2 -= dts * (fric * (hh * q.mu.v0));
   }
 }
-__device__ __forceinline__ void rhs_compute(const RhsIn& q, double dts, double cor, double fric,
+// stage combination a0 U0 + a1 Uc + R (src/Solvers.cpp:23-25,42-58); cvx
+// (SWE_OPT_EXACT_BALANCE with a0 + a1 = 1) writes it Uc + a0 (U0 - Uc) + R,
+// which is Uc + R to the bit where U0 = Uc
+__device__ __forceinline__ double stage_combine(bool cvx, double a0, double a1, double Ua, double Uc, double R) {
+  if (cvx) return (Uc + a0 * (Ua - Uc)) + R;
+  return (a0 * Ua + a1 * Uc) + R;
+}
+__device__ __forceinline__ void rhs_compute(const RhsIn& q, double dts, double cor, double fric, bool eb,
                                             double& r0, double& r1, double& r2) {
   rhs_compute_by(
       q,
       [&](int k, double& F0, double& F1, double& F2, double& dtik) {
         F0 = q.F0[k]; F1 = q.F1[k]; F2 = q.F2[k]; dtik = q.dtik[k];
       },
-      dts, cor, fric, r0, r1, r2);
+      dts, cor, fric, eb, r0, r1, r2);
 }
 // grid of 256-thread workgroups covering n items, a multiple of 8 (one
 // contiguous range per XCD, see logical_block)

@@ -214,11 +214,13 @@ using Slot = SlotT<double>;
 
 // Fluxes::HLL<W> (include/Fluxes.h:14-54).  lw: length-to-wavespeed
 // candidate for the CFL minimum (unchanged when the flux returns early);
-// dmn = min(2 Area(from) / L, 2 Area(to) / L).
+// dmn = min(2 Area(from) / L, 2 Area(to) / L).  eb (SWE_OPT_EXACT_BALANCE):
+// the same flux written Fl + al (ar (Ur - Ul) - (Fr - Fl)) / (ar - al),
+// which is Fl to the bit when the two states are equal.
 template <int WS, class T>
 __host__ __device__ __forceinline__ void flux_hll(double nx, double ny, const T& be, const SlotT<T>& L,
                                          const SlotT<T>& R, double dmn, double cor, double& lw,
-                                         T F[3]) {
+                                         T F[3], bool eb = false) {
   T ul = L.u * nx + L.v * ny;
   T hl = L.w - be;
   T ur = R.u * nx + R.v * ny;
@@ -237,16 +239,24 @@ __host__ __device__ __forceinline__ void flux_hll(double nx, double ny, const T&
   elem_flux(nx, ny, Ul0, Ul1, Ul2, Fl0, Fl1, Fl2);
   elem_flux(nx, ny, Ur0, Ur1, Ur2, Fr0, Fr1, Fr2);
   T aa = al * ar, den = ar - al;
+  if (eb) {
+    F[0] = Fl0 + al * (ar * (Ur0 - Ul0) - (Fr0 - Fl0)) / den;
+    F[1] = Fl1 + al * (ar * (Ur1 - Ul1) - (Fr1 - Fl1)) / den;
+    F[2] = Fl2 + al * (ar * (Ur2 - Ul2) - (Fr2 - Fl2)) / den;
+    return;
+  }
   F[0] = (ar * Fl0 - al * Fr0 + aa * (Ur0 - Ul0)) / den;
   F[1] = (ar * Fl1 - al * Fr1 + aa * (Ur1 - Ul1)) / den;
   F[2] = (ar * Fl2 - al * Fr2 + aa * (Ur2 - Ul2)) / den;
 }
 
-// Fluxes::HLLC<W> (include/Fluxes.h:56-111)
+// Fluxes::HLLC<W> (include/Fluxes.h:56-111).  eb (SWE_OPT_EXACT_BALANCE):
+// the star depth takes the speed ratio first, h (a - u) / (a - u*) as
+// h ((a - u) / (a - u*)), so that at rest (u = u* = 0) it is h exactly.
 template <int WS, class T>
 __host__ __device__ __forceinline__ void flux_hllc(double tx, double ty, const T& be, const SlotT<T>& L,
                                           const SlotT<T>& R, double dmn, double cor, double& lw,
-                                          T F[3]) {
+                                          T F[3], bool eb = false) {
   double nx = ty, ny = -tx;
   T ul = L.u * nx + L.v * ny;
   T vl = L.u * tx + L.v * ty;
@@ -266,7 +276,9 @@ __host__ __device__ __forceinline__ void flux_hllc(double tx, double ty, const T
   if (ustar <= 0) {
     T us = vr * tx + ustar * ty;
     T vs = vr * nx + ustar * ny;
-    T hs = hr * (ar - ur) / (ar - ustar);
+    T hs;
+    if (eb) hs = hr * ((ar - ur) / (ar - ustar));
+    else hs = hr * (ar - ur) / (ar - ustar);
     U0 = hr; U1 = hr * R.u; U2 = hr * R.v;
     elem_flux(nx, ny, U0, U1, U2, Fe0, Fe1, Fe2);
     T s = dmax(0., ar);
@@ -276,7 +288,9 @@ __host__ __device__ __forceinline__ void flux_hllc(double tx, double ty, const T
   } else {
     T us = vl * tx + ustar * ty;
     T vs = vl * nx + ustar * ny;
-    T hs = hl * (al - ul) / (al - ustar);
+    T hs;
+    if (eb) hs = hl * ((al - ul) / (al - ustar));
+    else hs = hl * (al - ul) / (al - ustar);
     U0 = hl; U1 = hl * L.u; U2 = hl * L.v;
     elem_flux(nx, ny, U0, U1, U2, Fe0, Fe1, Fe2);
     T s = dmin(0., al);

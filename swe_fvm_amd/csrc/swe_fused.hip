@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(kTileCells, 4) k_fused(DevMesh M, DevTiles P, 
     const V3 Pa = node(M, r.ep0), Pb = node(M, r.ep1);
     const Muscl mf = load_muscl(S, W, f), mt = load_muscl(S, W, tr);
     const double cxf = M.cx[f], cyf = M.cy[f], cxt = M.cx[tr], cyt = M.cy[tr];
-    const double hf = t == kSolidWall ? S.w[f] - M.bc[f] : 0.;
+    const double hf = (t == kSolidWall && !M.eb) ? S.w[f] - M.bc[f] : 0.;
     const double le = edge_len(Pa, Pb);
     const V3 E = midpoint(Pa, Pb);
     double tx, ty;
@@ -65,12 +65,14 @@ __global__ void __launch_bounds__(kTileCells, 4) k_fused(DevMesh M, DevTiles P, 
     double F[3];
     double lw = 1.0;  // the CFL candidate is not taken here (unused)
     if (t == kSolidWall) {
-      elem_flux(nx, ny, hf, 0., 0., F[0], F[1], F[2]);
+      // SWE_OPT_EXACT_BALANCE: the inside edge depth instead of the cell mean
+      const double hw = M.eb ? slot_value(mf, cxf, cyf, E).w - E.z : hf;
+      elem_flux(nx, ny, hw, 0., 0., F[0], F[1], F[2]);
     } else {
       const Slot L = slot_value(mf, cxf, cyf, E);
       const Slot R = slot_value(mt, cxt, cyt, E);
-      if (FLUX == 0) flux_hll<WS>(nx, ny, E.z, L, R, 1.0, A.cor, lw, F);
-      else flux_hllc<WS>(tx, ty, E.z, L, R, 1.0, A.cor, lw, F);
+      if (FLUX == 0) flux_hll<WS>(nx, ny, E.z, L, R, 1.0, A.cor, lw, F, M.eb != 0);
+      else flux_hllc<WS>(tx, ty, E.z, L, R, 1.0, A.cor, lw, F, M.eb != 0);
     }
     sF0[j] = F[0];
     sF1[j] = F[1];
@@ -152,16 +154,17 @@ __global__ void __launch_bounds__(kTileCells, 4) k_fused(DevMesh M, DevTiles P, 
         F2 = sF2[s];
         dtik = nb >= kNbOut ? INFINITY : sD[nb];
       },
-      dts, A.cor, A.fric, R0, R1, R2);
+      dts, A.cor, A.fric, M.eb != 0, R0, R1, R2);
   const double hh = w - bc;
   const double Uc0 = hh, Uc1 = hh * u, Uc2 = hh * v;
   double r0, r1, r2;
   if (comb) {
     const double h0 = w0 - bc;
     const double Ua0 = h0, Ua1 = h0 * u0, Ua2 = h0 * v0;
-    r0 = (A.a0 * Ua0 + A.a1 * Uc0) + R0;
-    r1 = (A.a0 * Ua1 + A.a1 * Uc1) + R1;
-    r2 = (A.a0 * Ua2 + A.a1 * Uc2) + R2;
+    const bool cvx = M.eb && A.a0 + A.a1 == 1.;
+    r0 = stage_combine(cvx, A.a0, A.a1, Ua0, Uc0, R0);
+    r1 = stage_combine(cvx, A.a0, A.a1, Ua1, Uc1, R1);
+    r2 = stage_combine(cvx, A.a0, A.a1, Ua2, Uc2, R2);
   } else {
     r0 = Uc0 + R0;
     r1 = Uc1 + R1;

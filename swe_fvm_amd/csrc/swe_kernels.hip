@@ -404,7 +404,8 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, DevState S, DevWork W, d
     q.cxt = M.cx[tr];
     q.cyt = M.cy[tr];
     // the cell average enters only the wall flux: walls alone load it
-    q.hf = t == kSolidWall ? S.w[f] - M.bc[f] : 0.;
+    // (with SWE_OPT_EXACT_BALANCE the wall takes the inside edge depth)
+    q.hf = (t == kSolidWall && !M.eb) ? S.w[f] - M.bc[f] : 0.;
     q.inst = M.cinst ? M.cinst[f] : 0;
     q.on = e < M.e1 && (!M.cinst || M.bactive[q.inst]);
   }
@@ -424,12 +425,13 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, DevState S, DevWork W, d
     double F[3];
     double lwi = 1.0;
     if (q.t == kSolidWall) {
-      elem_flux(nx, ny, q.hf, 0., 0., F[0], F[1], F[2]);
+      const double hw = M.eb ? slot_value(q.mf, q.cxf, q.cyf, E).w - E.z : q.hf;
+      elem_flux(nx, ny, hw, 0., 0., F[0], F[1], F[2]);
     } else {
       Slot L = slot_value(q.mf, q.cxf, q.cyf, E);
       Slot R = slot_value(q.mt, q.cxt, q.cyt, q.t == kPeriodic ? q.Eq : E);
-      if (FLUX == 0) flux_hll<WS>(nx, ny, E.z, L, R, q.edmin, cor, lwi, F);
-      else flux_hllc<WS>(tx, ty, E.z, L, R, q.edmin, cor, lwi, F);
+      if (FLUX == 0) flux_hll<WS>(nx, ny, E.z, L, R, q.edmin, cor, lwi, F, M.eb != 0);
+      else flux_hllc<WS>(tx, ty, E.z, L, R, q.edmin, cor, lwi, F, M.eb != 0);
       if (q.neg) { F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2]; }
     }
     lw = dmin(lw, lwi);
@@ -532,7 +534,7 @@ __device__ __forceinline__ void rhs_cell(const DevMesh& M, const DevState& S, co
                                          double& r1, double& r2) {
   RhsIn q;
   rhs_load(M, S, W, i, q);
-  rhs_compute(q, dts, cor, fric, r0, r1, r2);
+  rhs_compute(q, dts, cor, fric, M.eb != 0, r0, r1, r2);
 }
 
 __device__ __forceinline__ double stage_dt(const StageArgs& A) {
@@ -585,16 +587,17 @@ __global__ void __launch_bounds__(BS) k_update(DevMesh M, DevState S, DevState U
       dts = A.coef * M.bdt[inst];
     }
     double R0, R1, R2;
-    rhs_compute(in[it], dts, A.cor, A.fric, R0, R1, R2);
+    rhs_compute(in[it], dts, A.cor, A.fric, M.eb != 0, R0, R1, R2);
     const double h = w[it] - bc[it];
     const double Uc0 = h, Uc1 = h * u[it], Uc2 = h * v[it];
     double r0, r1, r2;
     if (comb) {
       const double h0 = w0[it] - bc[it];
       const double Ua0 = h0, Ua1 = h0 * u0[it], Ua2 = h0 * v0[it];
-      r0 = (A.a0 * Ua0 + A.a1 * Uc0) + R0;
-      r1 = (A.a0 * Ua1 + A.a1 * Uc1) + R1;
-      r2 = (A.a0 * Ua2 + A.a1 * Uc2) + R2;
+      const bool cvx = M.eb && A.a0 + A.a1 == 1.;
+      r0 = stage_combine(cvx, A.a0, A.a1, Ua0, Uc0, R0);
+      r1 = stage_combine(cvx, A.a0, A.a1, Ua1, Uc1, R1);
+      r2 = stage_combine(cvx, A.a0, A.a1, Ua2, Uc2, R2);
     } else {
       r0 = Uc0 + R0;
       r1 = Uc1 + R1;
@@ -920,8 +923,8 @@ __global__ void k_edge_flux_one(DevMesh M, DevState S, DevWork W, int e, int fro
   const double dl = 2. * M.area[from] / le, dr = to < 0 ? dl : 2. * M.area[to] / le;
   double lw = INFINITY;
   double F[3];
-  if (FLUX == 0) flux_hll<WS>(nx, ny, E.z, L, R, dmin(dl, dr), cor, lw, F);
-  else flux_hllc<WS>(tx, ty, E.z, L, R, dmin(dl, dr), cor, lw, F);
+  if (FLUX == 0) flux_hll<WS>(nx, ny, E.z, L, R, dmin(dl, dr), cor, lw, F, M.eb != 0);
+  else flux_hllc<WS>(tx, ty, E.z, L, R, dmin(dl, dr), cor, lw, F, M.eb != 0);
   out[0] = F[0]; out[1] = F[1]; out[2] = F[2]; out[3] = lw;
 }
 

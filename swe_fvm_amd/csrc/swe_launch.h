@@ -81,6 +81,9 @@ struct DevMesh {
   // partner edge of each PERIODIC edge (et1 == -3), -1 elsewhere; null when
   // the mesh has none
   const int* pe;
+  // SWE_OPT_EXACT_BALANCE: the flux and RHS groupings that keep equal edge
+  // states at rest to the bit
+  int eb;
 };
 
 // per-instance control of a batched ensemble

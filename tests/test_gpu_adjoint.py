@@ -496,12 +496,17 @@ def test_adjoint_through_shorelines(scheme, flux):
         assert abs(lhs - rhs) <= 1e-12 * scale
 
 
-def test_adjoint_refuses_head_options():
+@pytest.mark.parametrize("opts", [swe.OPT_HEAD_GRADIENT, swe.OPT_EXACT_BALANCE])
+def test_adjoint_refuses_head_options(opts):
+    """The HEAD-compatibility and exact-balance groupings are not
+    differentiated: the adjoint and tangent runs refuse them."""
     xyz, topo, prim = C.wet_dry_mesh(12, seed=2)
     sd = swe.SpaceDisc(swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt], swe.Domain(xyz, topo), prim,
-                       options=swe.OPT_HEAD_GRADIENT)
+                       options=opts)
     with pytest.raises(ValueError):
         sd.adjoint_run(np.ones_like(prim), 2, dt=1e-3)
+    with pytest.raises(ValueError):
+        sd.tangent_run(np.ones_like(prim), 2, dt=1e-3)
 
 
 def test_autograd_function_matches():

@@ -73,3 +73,37 @@ def test_lake_at_rest_one_million_cells():
     assert dev <= 1e-12
     assert mom <= 1e-12
     assert np.min(h) >= 0.0
+
+
+def test_lake_at_rest_one_million_cells_exact_balance():
+    """The same lake with SWE_OPT_EXACT_BALANCE, 1,000 steps: fully wet
+    edges carry equal states at rest, so the drift left comes from the
+    shorelines of the two islands alone, where the partially wet cells'
+    reconstruction (cube roots, bisections; src/MUSCLObject.cpp:86-190)
+    places the level within ~1e-13 of 1.  Measured: max|w - 1| 5.5e-13 and
+    max|hu|, |hv| 7.9e-13, the figures of the reference groupings, so the
+    shoreline is the whole of that drift.  Over bumps that are all
+    submerged (surface 1.5) the lake stays at rest bit for bit over the
+    1,000 steps, where the reference groupings leave round-off momentum in
+    most cells after one step."""
+    dom, prim = W.lake_at_rest(n=707, seed=1)
+    T = prim.shape[0]
+    xyz, tp = dom.geometry, dom.topology.tables()["tp"]
+    sd = swe.SpaceDisc(FL, dom, prim, options=swe.OPT_EXACT_BALANCE)
+    ms = sd.time_run("SSPRK2", 1000, dt=0.0, cfl=0.15)
+    st = sd.GetVolField()
+    h = st[:, 0] - C.cell_bath(xyz, tp)
+    full = np.max(xyz[tp, 2], axis=1) < 1.0
+    dev = np.max(np.abs(st[full, 0] - 1.0))
+    mom = max(np.max(np.abs(h * st[:, 1])), np.max(np.abs(h * st[:, 2])))
+    print(f"lake at rest 1M cells exact balance, 1000 steps: {ms:.1f} ms, max|w-1| {dev:.2e}, "
+          f"max|hu|,|hv| {mom:.2e}", flush=True)
+    assert dev <= 1e-12
+    assert mom <= 1e-12
+    dom, prim = W.lake_at_rest(n=707, seed=1, level=1.5)
+    assert np.all(np.max(dom.geometry[tp, 2], axis=1) < 1.5)
+    sd = swe.SpaceDisc(FL, dom, prim, options=swe.OPT_EXACT_BALANCE)
+    sd.run("SSPRK2", 1000, dt=0.0, cfl=0.15)
+    same = np.array_equal(sd.GetVolField(), prim)
+    print(f"submerged lake 1M cells exact balance, 1000 steps: unchanged bit for bit {same}", flush=True)
+    assert same

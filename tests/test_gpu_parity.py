@@ -190,3 +190,18 @@ def test_large_properties_determinism_mass_positivity():
     assert abs(m1 - m0) <= 1e-12 * abs(m0)
     h = outs[0][:, 0] - C.cell_bath(xyz, tp)
     assert np.min(h) >= 0.0
+
+
+@pytest.mark.parametrize("scheme", ["Euler", "SSPRK2", "SSPRK3"])
+@pytest.mark.parametrize("flux,wave", [("HLL", "Einfeldt"), ("HLLC", "Einfeldt"), ("HLLC", "Davis")])
+def test_exact_balance_matches_oracle(scheme, flux, wave):
+    """SWE_OPT_EXACT_BALANCE on a wet/dry mesh with Coriolis: CFL steps (the
+    separate flux / draining / update kernels for the first stage) and
+    fixed-dt steps (the fused tile kernel) against the oracle, bit for bit."""
+    xyz, topo, prim = C.wet_dry_mesh(24, seed=11)
+    tp = topo.tables()["tp"]
+    for dt in (0.0, 2e-3):
+        sd, orc = make_pair(xyz, topo, prim, flux, wave, cor=0.1, opts=swe.OPT_EXACT_BALANCE)
+        sd.run(scheme, 6, dt=dt, cfl=0.15)
+        orc.run(scheme, 6, dt=dt, cfl=0.15)
+        assert assert_state_close(sd, orc, xyz, tp, what=f"exact balance {scheme} {flux}<{wave}> dt={dt}")

@@ -463,3 +463,84 @@ def test_analytic_solutions_solve_the_swe(name, kw):
         assert max(abs(r) for r in res) < 1e-6, (name, kw, x, y, t, res)
         checked += 1
     assert checked > 20
+
+@pytest.mark.parametrize("flux", ["HLL", "HLLC"])
+@pytest.mark.parametrize("scheme", ["Euler", "SSPRK2", "SSPRK3"])
+def test_exact_balance_keeps_a_wet_lake_at_rest_to_the_bit(flux, scheme):
+    """A lake over bumps that are all submerged (surface 1.5): with
+    OPT_EXACT_BALANCE every edge carries equal states at rest, so every
+    increment is zero and the state stays the initial one bit for bit; the
+    reference's groupings leave round-off momentum in most cells after one
+    step."""
+    from swe_fvm_amd import workloads as W
+    dom, prim = W.lake_at_rest(n=24, seed=2, level=1.5)
+    xyz, t = dom.geometry, dom.topology.tables()
+    assert np.all(np.max(xyz[t["tp"], 2], axis=1) < 1.5)
+    s = O.OracleSolver(xyz, t, flux, "Einfeldt", opts=O.OPT_EXACT_BALANCE)
+    s.set_state(prim)
+    s.run(scheme, 100, dt=0.0, cfl=0.15)
+    assert np.array_equal(s.state(), prim)
+    d = O.OracleSolver(xyz, t, flux, "Einfeldt")
+    d.set_state(prim)
+    d.run(scheme, 1, dt=0.0, cfl=0.15)
+    moved = np.count_nonzero(d.state()[:, 1:])
+    print(f"{flux} {scheme}: reference groupings, cells with momentum after one step {moved} of {prim.shape[0]}")
+    assert moved > prim.shape[0] // 4
+
+
+@pytest.mark.parametrize("flux", ["HLL", "HLLC"])
+@pytest.mark.parametrize("scheme", ["Euler", "SSPRK2", "SSPRK3"])
+def test_exact_balance_regroups_round_off_only(flux, scheme):
+    """Away from walls the option only regroups: a smooth flow over a
+    varying bottom with Coriolis, FREE_FLOW boundaries, 20 CFL steps,
+    differs from the reference groupings by round-off (<= 1e-13 of each
+    component's scale; measured <= 6e-15), but does differ."""
+    import swe_fvm_amd as swe
+    xyz, topo = swe.jittered_mesh(16, 0.25, 5)
+    xyz[:, 2] = 0.1 * np.sin(3 * xyz[:, 0]) * np.cos(2 * xyz[:, 1])
+    b = np.nonzero(topo.tables()["et"][:, 1] < 0)[0]
+    t = topo.with_boundary_tag(b, swe.FREE_FLOW).tables()
+    tp = t["tp"]
+    c = xyz[tp].mean(1)
+    prim = np.stack([1 + 0.1 * np.exp(-((c[:, 0] - .4) ** 2 + (c[:, 1] - .5) ** 2) / .02),
+                     0.2 * np.sin(2 * c[:, 1]), -0.1 * np.cos(3 * c[:, 0])], 1)
+    out = []
+    for opts in (0, O.OPT_EXACT_BALANCE):
+        s = O.OracleSolver(xyz, t, flux, "Einfeldt", cor=0.2, opts=opts)
+        s.set_state(prim)
+        s.run(scheme, 20, dt=0.0, cfl=0.15)
+        out.append(prim_cons(s.state(), xyz, tp))
+    d = np.max(np.abs(out[0] - out[1]) / np.max(np.abs(out[0]), axis=0))
+    print(f"{flux} {scheme}: regrouped vs reference {d:.2e}")
+    assert 0 < d <= 1e-13
+    assert np.max(np.abs(out[0] - prim_cons(prim, xyz, tp))) > 1e-3  # the flow moved
+
+
+def test_exact_balance_wall_takes_the_edge_depth():
+    """With the option the wall flux is the hydrostatic pressure of the
+    inside edge depth, 0.5 h_e^2 n (h_e the reconstructed edge level minus
+    the edge bathymetry); the reference's takes the cell-mean depth."""
+    xyz, topo, prim = C.wet_dry_mesh(12, seed=3)
+    t = topo.tables()
+    for opts in (0, O.OPT_EXACT_BALANCE):
+        s = O.OracleSolver(xyz, t, "HLLC", "Einfeldt", opts=opts)
+        s.set_state(prim)
+        s.compute_interface_values()
+        s.compute_fluxes()
+        F, edg = s.fluxes(), s.edge_field()
+        walls = np.nonzero(t["et"][:, 1] == -1)[0]
+        ep = t["ep"]
+        be = 0.5 * (xyz[ep[walls, 0], 2] + xyz[ep[walls, 1], 2])
+        f = t["et"][walls, 0]
+        h = edg[2 * walls, 0] - be if opts else prim[f, 0] - C.cell_bath(xyz, t["tp"])[f]
+        h = np.where(h > 1e-12, h, 0.0)  # dry walls carry no pressure
+        q = 0.5 * h * h
+        pn = np.hypot(F[walls, 1], F[walls, 2])
+        assert np.all(F[walls, 0] == 0)
+        assert np.allclose(pn, q, rtol=1e-14, atol=0)
+        assert np.any(h > 0)
+
+
+def prim_cons(prim, xyz, tp):
+    h = prim[:, 0] - C.cell_bath(xyz, tp)
+    return np.stack([h, h * prim[:, 1], h * prim[:, 2]], 1)
