@@ -32,8 +32,21 @@ __device__ __forceinline__ double stage_dt_fused(const StageArgs& A) {
   return dt;
 }
 
+// waves per SIMD the register allocation must allow, and threads per tile
+// workgroup (kTileCells or 2 kTileCells: the flux and draining phases then
+// give each thread fewer items, the update leaves the second half idle);
+// build-time knobs
+#ifndef SWE_FUSED_MIN_WAVES
+#define SWE_FUSED_MIN_WAVES 4
+#endif
+#ifndef SWE_FUSED_THREADS
+#define SWE_FUSED_THREADS 256
+#endif
+constexpr int kFusedThreads = SWE_FUSED_THREADS;
+static_assert(kFusedThreads == kTileCells || kFusedThreads == 2 * kTileCells, "threads per tile");
+
 template <int FLUX, int WS, bool LEN, bool NT>
-__global__ void __launch_bounds__(kTileCells, 4) k_fused(DevMesh M, DevTiles P, DevState S, DevState U0,
+__global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(DevMesh M, DevTiles P, DevState S, DevState U0,
                                                       DevOut O, DevWork W, StageArgs A) {
   __shared__ double sF0[kTileMaxEval], sF1[kTileMaxEval], sF2[kTileMaxEval];
   __shared__ double sD[kTileMaxLocal];
@@ -49,8 +62,11 @@ __global__ void __launch_bounds__(kTileCells, 4) k_fused(DevMesh M, DevTiles P, 
   // ---- edge fluxes (k_flux's operations), one evaluation per LDS slot;
   // the edge record gives both cells and nodes (one load level, as the
   // separate flux kernel's edge tables)
-  for (int j = tid; j < ne; j += kTileCells) {
-    const EdgeRec r = P.evrec[e0 + j];
+  // the next evaluation's edge record is loaded before this one's gathers
+  EdgeRec rn = tid < ne ? P.evrec[e0 + tid] : EdgeRec{0, 0, 0, 0};
+  for (int j = tid; j < ne; j += kFusedThreads) {
+    const EdgeRec r = rn;
+    if (j + kFusedThreads < ne) rn = P.evrec[e0 + j + kFusedThreads];
     const int f = r.et0, t = r.et1;
     const int tr = t < 0 ? f : t;  // boundary edges read their own cell twice: R = L
     const V3 Pa = node(M, r.ep0), Pb = node(M, r.ep1);
@@ -80,67 +96,74 @@ __global__ void __launch_bounds__(kTileCells, 4) k_fused(DevMesh M, DevTiles P, 
   }
   __syncthreads();
 
-  // ---- draining time steps of every local cell (k_drain's operations)
-  const int nl = kTileCells + nr;
-  for (int l = tid; l < nl; l += kTileCells) {
-    if (l >= nc && l < kTileCells) continue;  // beyond the last tile's cells
-    const bool own = l < kTileCells;
-    const int x = own ? (int)(c0 + l) : P.ring_cell[rg0 + l - kTileCells];
-    uint32_t code[3];
+  // ---- draining time steps of every local cell (k_drain's operations).
+  // The thread's own cell (tid < nc) is also the cell it updates: its
+  // update inputs are loaded here with its draining inputs, so one latency
+  // covers both phases' loads.
+  const bool mine = tid < nc;
+  const int i = (int)(c0 + (mine ? tid : 0));
+  uint32_t code[3];
+  RhsCell q;
+  double w, u, v, w0 = 0., u0 = 0., v0 = 0., bci, areai;
+  const bool comb = A.a0 != 0.;
+  {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) code[k] = own ? P.code[k * T + x] : P.ring_code[3 * (rg0 + l - kTileCells) + k];
-    const double h = S.w[x] - M.bc[x];
-    double d;
-    if (!is_wet(h)) {
-      d = 0.;
-    } else {
-      const uint32_t fl = M.cflag[x];
-      V3 p0, p1, p2;
-      if (LEN) {
-        p0 = node(M, M.tp[x]);
-        p1 = node(M, M.tp[T + x]);
-        p2 = node(M, M.tp[2 * T + x]);
-      }
-      double sum = 0.;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const double fe = sF0[code[k] >> 16];
-        double out = sgn_pos(fl, k) ? fe : -fe;
-        if (LEN)
-          out = out * edge_len(pick(p0, p1, p2, ep0_local(fl, k)), pick(p0, p1, p2, ep1_local(fl, k)));
-        sum += dmax(0., out);
-      }
-      d = sum > kTol ? M.area[x] * h / sum : INFINITY;
+    for (int k = 0; k < 3; ++k) code[k] = P.code[k * T + i];
+    q.fl = M.cflag[i];
+    q.mu = load_muscl(S, W, i);
+    q.wi = S.w[i];
+    q.p[0] = node(M, M.tp[i]);
+    q.p[1] = node(M, M.tp[T + i]);
+    q.p[2] = node(M, M.tp[2 * T + i]);
+    w = S.w[i];
+    u = S.u[i];
+    v = S.v[i];
+    if (comb) {
+      w0 = U0.w[i];
+      u0 = U0.u[i];
+      v0 = U0.v[i];
     }
-    sD[l] = d;
+    bci = M.bc[i];
+    areai = M.area[i];
+  }
+  // the draining step of a cell from its depth, edges and flux slots
+  auto drain_dt = [&](double h, uint32_t fl, const uint32_t* cd, const V3& p0, const V3& p1, const V3& p2,
+                      double area) -> double {
+    if (!is_wet(h)) return 0.;
+    double sum = 0.;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double fe = sF0[cd[k] >> 16];
+      double out = sgn_pos(fl, k) ? fe : -fe;
+      if (LEN) out = out * edge_len(pick(p0, p1, p2, ep0_local(fl, k)), pick(p0, p1, p2, ep1_local(fl, k)));
+      sum += dmax(0., out);
+    }
+    return sum > kTol ? area * h / sum : INFINITY;
+  };
+  if (mine) sD[tid] = drain_dt(w - bci, q.fl, code, q.p[0], q.p[1], q.p[2], areai);
+  const int nl = kTileCells + nr;
+  for (int l = kTileCells + tid; l < nl; l += kFusedThreads) {
+    const int r = rg0 + l - kTileCells;
+    const int x = P.ring_cell[r];
+    uint32_t cd[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) cd[k] = P.ring_code[3 * r + k];
+    V3 p0, p1, p2;
+    if (LEN) {
+      p0 = node(M, M.tp[x]);
+      p1 = node(M, M.tp[T + x]);
+      p2 = node(M, M.tp[2 * T + x]);
+    }
+    sD[l] = drain_dt(S.w[x] - M.bc[x], M.cflag[x], cd, p0, p1, p2, M.area[x]);
   }
   __syncthreads();
 
   // ---- update of the tile's cells (k_update's operations; the fluxes and
   // neighbour draining steps are read from LDS edge by edge)
-  if (tid >= nc) return;
+  if (!mine) return;
   const double dt = stage_dt_fused(A);
   if (A.dts_log && tile == 0 && tid == 0) A.dts_log[0] = dt;
-  const int i = (int)(c0 + tid);
-  uint32_t code[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) code[k] = P.code[k * T + i];
-  RhsCell q;
-  q.fl = M.cflag[i];
-  q.mu = load_muscl(S, W, i);
-  q.wi = S.w[i];
-  q.p[0] = node(M, M.tp[i]);
-  q.p[1] = node(M, M.tp[T + i]);
-  q.p[2] = node(M, M.tp[2 * T + i]);
   q.dti = sD[tid];
-  const bool comb = A.a0 != 0.;
-  const double w = S.w[i], u = S.u[i], v = S.v[i];
-  double w0 = 0., u0 = 0., v0 = 0.;
-  if (comb) {
-    w0 = U0.w[i];
-    u0 = U0.u[i];
-    v0 = U0.v[i];
-  }
   const double bc = centroid(q.p[0], q.p[1], q.p[2]).z;
   const double dts = A.coef * dt;
   double R0, R1, R2;
@@ -181,7 +204,7 @@ hipError_t launch_fused(const DevMesh& M, const DevTiles& P, const DevState& S, 
                         const DevOut& O, const DevWork& W, const StageArgs& A, int flux, int ws, int opts,
                         hipStream_t st) {
   if (M.cinst || M.c0 != 0 || M.c1 != M.T) return hipErrorInvalidValue;  // single meshes only
-  const dim3 g = grid_for_bs(P.ntiles, 1), b(kTileCells);
+  const dim3 g = grid_for_bs(P.ntiles, 1), b(kFusedThreads);
   const bool len = (opts & kOptDrainLength) != 0;
 #define SWE_FUSED(F, Wv)                                                                             \
   if (flux == F && ws == Wv) {                                                                       \

@@ -8,9 +8,13 @@
 
 namespace swe {
 
-constexpr int kTileCells = 256;     // cells per tile (= threads per workgroup)
-constexpr int kTileMaxLocal = 512;  // tile cells + ring
-constexpr int kTileMaxEval = 768;   // edge evaluations per tile (LDS flux slots)
+#ifndef SWE_TILE_CELLS
+#define SWE_TILE_CELLS 256
+#endif
+constexpr int kTileCells = SWE_TILE_CELLS;      // cells per tile (a build-time knob: 256 or 512)
+constexpr int kTileMaxLocal = 2 * kTileCells;  // tile cells + ring
+constexpr int kTileMaxEval = 3 * kTileCells;   // edge evaluations per tile (LDS flux slots)
+static_assert(kTileCells == 256 || kTileCells == 512, "tile size");
 constexpr uint32_t kNbWall = 0xffffu, kNbFree = 0xfffeu, kNbOut = 0xfffdu;
 
 // one evaluated edge: its cells and nodes (one 16-byte load)

@@ -16,10 +16,12 @@ import swe_fvm_amd as swe
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-@pytest.fixture(scope="module")
-def checker(tmp_path_factory):
+@pytest.fixture(scope="module", params=[256, 512], ids=["tile256", "tile512"])
+def checker(tmp_path_factory, request):
+    """The checker built with the tables' tile size (the build-time knob
+    SWE_TILE_CELLS of the library)."""
     out = str(tmp_path_factory.mktemp("tiles") / "libtiles_check.so")
-    cmd = ["g++", "-std=c++17", "-O2", "-shared", "-fPIC", "-Wall", "-Wextra",
+    cmd = ["g++", "-std=c++17", "-O2", "-shared", "-fPIC", "-Wall", "-Wextra", f"-DSWE_TILE_CELLS={request.param}",
            os.path.join(ROOT, "tests", "cpp", "tiles_check.cpp"),
            os.path.join(ROOT, "swe_fvm_amd", "csrc", "swe_tiles.cpp"), "-o", out]
     res = subprocess.run(cmd, capture_output=True, text=True)
