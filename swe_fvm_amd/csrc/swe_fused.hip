@@ -62,6 +62,30 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
   // ---- edge fluxes (k_flux's operations), one evaluation per LDS slot;
   // the edge record gives both cells and nodes (one load level, as the
   // separate flux kernel's edge tables)
+  // index loads of the later phases, issued before the flux phase: the
+  // own cell's slot codes, flags and vertex ids, and the thread's ring
+  // entry (nl <= kTileMaxLocal = 2 kTileCells <= kTileCells + kFusedThreads:
+  // at most one per thread)
+  const bool mine = tid < nc;
+  const int i = (int)(c0 + (mine ? tid : 0));
+  uint32_t code[3];
+  int tpi[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    code[k] = P.code[k * T + i];
+    tpi[k] = M.tp[k * T + i];
+  }
+  const uint32_t fli = M.cflag[i];
+  const int nl = kTileCells + nr;
+  const int lr = kTileCells + tid;
+  int xr = 0;
+  uint32_t cdr[3] = {0u, 0u, 0u};
+  if (lr < nl) {
+    xr = P.ring_cell[rg0 + tid];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) cdr[k] = P.ring_code[3 * (rg0 + tid) + k];
+  }
+
   // the next evaluation's edge record is loaded before this one's gathers
   EdgeRec rn = tid < ne ? P.evrec[e0 + tid] : EdgeRec{0, 0, 0, 0};
   for (int j = tid; j < ne; j += kFusedThreads) {
@@ -100,21 +124,16 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
   // The thread's own cell (tid < nc) is also the cell it updates: its
   // update inputs are loaded here with its draining inputs, so one latency
   // covers both phases' loads.
-  const bool mine = tid < nc;
-  const int i = (int)(c0 + (mine ? tid : 0));
-  uint32_t code[3];
   RhsCell q;
   double w, u, v, w0 = 0., u0 = 0., v0 = 0., bci, areai;
   const bool comb = A.a0 != 0.;
   {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) code[k] = P.code[k * T + i];
-    q.fl = M.cflag[i];
+    q.fl = fli;
     q.mu = load_muscl(S, W, i);
     q.wi = S.w[i];
-    q.p[0] = node(M, M.tp[i]);
-    q.p[1] = node(M, M.tp[T + i]);
-    q.p[2] = node(M, M.tp[2 * T + i]);
+    q.p[0] = node(M, tpi[0]);
+    q.p[1] = node(M, tpi[1]);
+    q.p[2] = node(M, tpi[2]);
     w = S.w[i];
     u = S.u[i];
     v = S.v[i];
@@ -141,20 +160,14 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
     return sum > kTol ? area * h / sum : INFINITY;
   };
   if (mine) sD[tid] = drain_dt(w - bci, q.fl, code, q.p[0], q.p[1], q.p[2], areai);
-  const int nl = kTileCells + nr;
-  for (int l = kTileCells + tid; l < nl; l += kFusedThreads) {
-    const int r = rg0 + l - kTileCells;
-    const int x = P.ring_cell[r];
-    uint32_t cd[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) cd[k] = P.ring_code[3 * r + k];
+  if (lr < nl) {
     V3 p0, p1, p2;
     if (LEN) {
-      p0 = node(M, M.tp[x]);
-      p1 = node(M, M.tp[T + x]);
-      p2 = node(M, M.tp[2 * T + x]);
+      p0 = node(M, M.tp[xr]);
+      p1 = node(M, M.tp[T + xr]);
+      p2 = node(M, M.tp[2 * T + xr]);
     }
-    sD[l] = drain_dt(S.w[x] - M.bc[x], M.cflag[x], cd, p0, p1, p2, M.area[x]);
+    sD[lr] = drain_dt(S.w[xr] - M.bc[xr], M.cflag[xr], cdr, p0, p1, p2, M.area[xr]);
   }
   __syncthreads();
 
