@@ -151,37 +151,45 @@ __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W
   const int64_t blk0 = M.c0 + (int64_t)logical_block() * kBlock;
   const int64_t i = blk0 + threadIdx.x;
   const bool live = i < M.c1;
-  CellRec ri{};
-  double w = 0.;
-  if (live) {  // the cell's own record fills its LDS entry (no second read)
-    ri = M.crec[i];
-    w = S.w[i];
+  const int is = live ? (int)i : (int)M.c0;  // threads past the end read a valid cell
+  // every load that does not need the LDS window is issued before the
+  // barrier: the cell's own record (which fills its LDS entry), its flags,
+  // vertices and neighbour ids, and the neighbours outside the workgroup
+  const CellRec ri = M.crec[is];
+  const double w = S.w[is];
+  const uint32_t fl = M.cflag[is];
+  int jn[3];
+  bool inb[3];
+  double wn[3];
+  NRec rn[3];
+#pragma unroll
+  for (int kk = 0; kk < 3; ++kk) {
+    int j = M.tt[kk * M.T + is];
+    j = j < 0 ? is : j;  // boundary cells never take the full-wet branch
+    jn[kk] = j;
+    const int64_t lj = j - blk0;
+    inb[kk] = lj >= 0 && lj < kBlock && j < M.c1;
+    if (!inb[kk]) {  // the owner's workgroup reads the same record: an L2 hit mostly
+      wn[kk] = S.w[j];
+      const CellRec r = M.crec[j];
+      rn[kk] = NRec{r.cx, r.cy, (r.fl & CELL_BND) ? INFINITY : r.zmax, r.bc};
+    }
+  }
+  const V3 p0 = node(M, M.tp[is]), p1 = node(M, M.tp[M.T + is]), p2 = node(M, M.tp[2 * M.T + is]);
+  if (live) {
     s_w[threadIdx.x] = w;
     s_r[threadIdx.x] = NRec{ri.cx, ri.cy, (ri.fl & CELL_BND) ? INFINITY : ri.zmax, ri.bc};
   }
   __syncthreads();
   if (!live) return;
   if (M.cinst && !M.bactive[M.cinst[i]]) return;  // finished ensemble member
-  const uint32_t fl = M.cflag[i];
-  const V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[M.T + i]), p2 = node(M, M.tp[2 * M.T + i]);
-  double wn[3];
-  NRec rn[3];
-  int jn[3];
 #pragma unroll
-  for (int kk = 0; kk < 3; ++kk) {
-    int j = M.tt[kk * M.T + i];
-    j = j < 0 ? (int)i : j;  // boundary cells never take the full-wet branch
-    jn[kk] = j;
-    const int64_t lj = j - blk0;
-    if (lj >= 0 && lj < kBlock && j < M.c1) {
+  for (int kk = 0; kk < 3; ++kk)
+    if (inb[kk]) {
+      const int64_t lj = jn[kk] - blk0;
       wn[kk] = s_w[lj];
       rn[kk] = s_r[lj];
-    } else {  // the owner's workgroup reads the same record: an L2 hit mostly
-      wn[kk] = S.w[j];
-      const CellRec r = M.crec[j];
-      rn[kk] = NRec{r.cx, r.cy, (r.fl & CELL_BND) ? INFINITY : r.zmax, r.bc};
     }
-  }
   const double bc = ri.bc;
   const double zmx = ri.zmax, zmn = ri.zmin;
   const double cx = ri.cx, cy = ri.cy;
