@@ -177,6 +177,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   const int i = (int)i64;
   const int64_t T = M.T;
   const uint32_t fl = M.cflag[i];
+  const int cls = W.cls[i];  // loads come before the first store (see below)
   const Muscl mu = load_muscl(S, W, i);
   const V3 p[3] = {node(M, M.tp[i]), node(M, M.tp[T + i]), node(M, M.tp[2 * T + i])};
   const V3 c = centroid(p[0], p[1], p[2]);
@@ -262,6 +263,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     rb1 = ih * ou;
     rb2 = ih * ov;
   }
+  // the corner z adjoint accumulated over the stages, and the first stage's
+  // U0 adjoint, read before this kernel's first store: the compiler may not
+  // move a load of one buffer past a store to another
+  double zc_in[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) zc_in[k] = J.zc[k * T + i];
+  const bool into_early = (u0mode & kU0Into) && !comb;
+  double u0in[3] = {0., 0., 0.};
+  if (into_early)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) u0in[k] = J.u0bar[k * T + i];
   // stage combination
   double sw = 0., su = 0., sv = 0.;
   const double ac = comb ? A.a1 : 1.;
@@ -287,7 +299,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     }
     bcb -= hb;
   }
-  if (u0mode & kU0Into) {  // first stage: its input is U0 itself
+  if (into_early) {  // first stage: its input is U0 itself
+    sw += u0in[0];
+    su += u0in[1];
+    sv += u0in[2];
+  } else if (u0mode & kU0Into) {
     sw += J.u0bar[i];
     su += J.u0bar[T + i];
     sv += J.u0bar[2 * T + i];
@@ -343,7 +359,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   // the corners through b_cell (dry cells); u0, v0 to the cell's u, v (zero
   // for dry cells).  What stays for k_adj_recon / k_adj_pw: the gradient
   // part of dry and full-wet cells, all of it for cells in pass 2's list.
-  const int cls = W.cls[i];
   const bool listed = cls == 1 && !(w > dmax(dmax(p[0].z, p[1].z), p[2].z));
   if (cls == 0) {
     bcb += mb[0];
@@ -359,7 +374,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   }
   const double cthird = 1. / 3.;
   for (int k = 0; k < 3; ++k)  // over every stage
-    out_store<NT>(&J.zc[k * T + i], J.zc[k * T + i] + (zc[k] + cthird * bcb));
+    out_store<NT>(&J.zc[k * T + i], zc_in[k] + (zc[k] + cthird * bcb));
   out_store<NT>(&J.sbar[i], sw);
   out_store<NT>(&J.sbar[T + i], su);
   out_store<NT>(&J.sbar[2 * T + i], sv);
@@ -588,30 +603,44 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
   }
   double zc[3] = {0., 0., 0.};
   double wb = 0., bcb = 0.;
-  if (W.dtd[i] < dts) {  // the depth term of the draining dt (k_adj_flux)
-    const double hb = J.drain_h[i];
-    wb += hb;
-    bcb -= hb;
+  // every load of the edge part is issued before any branch on loaded data
+  // and before any store: the draining depth term, the edges' tags and both
+  // sides' w adjoints and this side's velocity adjoints
+  const double dtdi = W.dtd[i], dhi = J.drain_h[i];
+  int ek[3], tag[3];
+  double s0[3], s1[3], su[3], sv[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int e = M.te[k * T + i];
+    const int side = sgn_pos(fl, k) ? 0 : 1;
+    ek[k] = e;
+    tag[k] = M.et1[e];
+    s0[k] = J.sl_edge[e];
+    s1[k] = J.sl_edge[Ecnt + e];
+    su[k] = J.sl_edge[(1 * 2 + side) * Ecnt + e];
+    sv[k] = J.sl_edge[(2 * 2 + side) * Ecnt + e];
+  }
+  if (dtdi < dts) {  // the depth term of the draining dt (k_adj_flux)
+    wb += dhi;
+    bcb -= dhi;
   }
   const V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[T + i]), p2 = node(M, M.tp[2 * T + i]);
   const double cxi = M.cx[i], cyi = M.cy[i];
   {
     // the slot values of the cell's three edges (AtPoint at the midpoints)
     const Muscl mu = load_muscl(S, W, i);
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const int e = M.te[k * T + i];
       const bool own = sgn_pos(fl, k);
-      const int side = own ? 0 : 1;
-      if (M.et1[e] == kSolidWall) {  // the edge's tag (tt is -1 on every boundary edge)
-        const double hf = J.sl_edge[e];  // side 0 w: the depth adjoint
+      if (tag[k] == kSolidWall) {  // the edge's tag (tt is -1 on every boundary edge)
+        const double hf = s0[k];  // side 0 w: the depth adjoint
         wb += hf;
         bcb -= hf;
         continue;
       }
       // the flux's own z term, once: -(w adjoints of both sides)
-      double ez = own ? -(J.sl_edge[e] + J.sl_edge[Ecnt + e]) : 0.;
-      slot_bar(mu, cxi, cyi, edge_mid(p0, p1, p2, fl, k), J.sl_edge[(0 * 2 + side) * Ecnt + e],
-               J.sl_edge[(1 * 2 + side) * Ecnt + e], J.sl_edge[(2 * 2 + side) * Ecnt + e], mb, ez);
+      double ez = own ? -(s0[k] + s1[k]) : 0.;
+      slot_bar(mu, cxi, cyi, edge_mid(p0, p1, p2, fl, k), own ? s0[k] : s1[k], su[k], sv[k], mb, ez);
       zc[ep0_local(fl, k)] += 0.5 * ez;
       zc[ep1_local(fl, k)] += 0.5 * ez;
     }
@@ -621,12 +650,13 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
   // the node maxima this cell attains (k_adj_pwnode): adjoint of its pass-1
   // reconstruction's value at those vertices
   double mp[3] = {0., 0., 0.};
+  bool p1clear[3] = {false, false, false};  // consumed node-maximum adjoints, cleared at the end
   if (*W.pw_count) {
     const V3 pv[3] = {p0, p1, p2};
     for (int k = 0; k < 3; ++k) {
       const double pk = J.p1bar[k * T + i];
       if (pk != 0.) {
-        J.p1bar[k * T + i] = 0.;
+        p1clear[k] = true;
         mp[0] += pk;
         mp[1] += pk * (pv[k].x - cxi);
         mp[2] += pk * (pv[k].y - cyi);
@@ -648,8 +678,6 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
       zc[k] += mb[1] * gx + mb[2] * gy;
     }
   } else {
-    J.sbar[T + i] += mb[3];
-    J.sbar[2 * T + i] += mb[4];
     const double zmax = dmax(dmax(p0.z, p1.z), p2.z);
     if (cls == 1 && w > zmax) {
       // not in pass 2's list: the level is w with a zero gradient, also as
@@ -765,6 +793,14 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
       }
     }
   }
+  // the stores, after every load (the compiler may not move a load of one
+  // buffer past a store to another)
+  if (cls != 0) {
+    J.sbar[T + i] += mb[3];
+    J.sbar[2 * T + i] += mb[4];
+  }
+  for (int k = 0; k < 3; ++k)
+    if (p1clear[k]) J.p1bar[k * T + i] = 0.;
   for (int k = 0; k < 3; ++k) out_store<NT>(&J.w_nb[k * T + i], wnb[k]);
   out_store<NT>(&J.sbar[i], J.sbar[i] + wb);
   const double c = 1. / 3.;
