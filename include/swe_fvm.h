@@ -217,7 +217,9 @@ int swe_cfl_dt(swe_solver* s, double* dt);
  * flux pass (of swe_compute_fluxes or of the last stage of swe_run). */
 int swe_min_len_to_wavespeed(swe_solver* s, double* r);
 
-/* Sizes and raw device pointers (state w, u, v SoA) for zero-copy interop. */
+/* Sizes and raw device pointers (state w, u, v SoA) for zero-copy interop;
+ * swe_device_state refuses a solver whose state layout is not SoA
+ * (swe_set_layout). */
 int64_t swe_num_cells(const swe_solver* s);
 int64_t swe_num_edges(const swe_solver* s);
 int64_t swe_num_nodes(const swe_solver* s);
@@ -359,6 +361,24 @@ int swe_get_tuning_blocks(const swe_solver* s, int* flux_block, int* update_bloc
  * the reconstruction and flux fields afterwards are those of the current
  * state. */
 int swe_autotune(swe_solver* s, int reps, int* flux_items, int* update_items, double* times);
+
+/* Layout of the cell state (w, u, v) in device memory, a dimension of the
+ * autotuner: block 0 is SoA (three arrays); a power of two B up to 64 stores
+ * blocks of B cells as B values of w, then of u, then of v (AoSoA; B = 1 is
+ * AoS).  Results do not depend on it.  Changing it rewrites the current
+ * state on the device and drops a captured ensemble graph.  Pointers from
+ * swe_device_state, and swe_adjoint_run / swe_tangent_run, need SoA.  No
+ * counterpart in the reference (its VolumeField is an Eigen 3 x T array,
+ * include/ValueField.h:17-44). */
+int swe_set_layout(swe_solver* s, int block);
+int swe_get_layout(const swe_solver* s, int* block);
+/* Tunes the layout with the work-item variants: for each candidate layout
+ * (SoA, 64, 8, 1) swe_autotune, then one separate stage and, for a single
+ * mesh that fuses its later stages, one fused stage timed on the live state;
+ * keeps the layout and variants of least cost.  cost_ms (may be null)
+ * receives SWE_LAYOUT_CANDIDATES costs in that order. */
+#define SWE_LAYOUT_CANDIDATES 4
+int swe_autotune_layout(swe_solver* s, int reps, int* block, double* cost_ms);
 
 #ifdef __cplusplus
 }

@@ -76,6 +76,9 @@ SIGNATURES = [
     ("swe_get_tuning_blocks", _int, [_p, _p, _p]),
     ("swe_get_tuning", _int, [_p, _p, _p]),
     ("swe_autotune", _int, [_p, _int, _p, _p, _p]),
+    ("swe_set_layout", _int, [_p, _int]),
+    ("swe_get_layout", _int, [_p, _p]),
+    ("swe_autotune_layout", _int, [_p, _int, _p, _p]),
 ]
 
 _lib = None

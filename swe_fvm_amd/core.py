@@ -615,6 +615,38 @@ class SpaceDisc:
         keeps the fastest; returns the choice and the per-launch times."""
         return _autotune(self._h, reps)
 
+    def set_layout(self, block=0):
+        """Device layout of the cell state: 0 = SoA (three arrays), a power
+        of two B <= 64 = blocks of B cells (w, then u, then v; B = 1 is AoS).
+        Results are identical for every layout."""
+        _check(_lib.load().swe_set_layout(self._h, int(block)))
+
+    def layout(self):
+        return _layout(self._h)
+
+    def autotune_layout(self, reps=10):
+        """The autotuner with the layout dimension (swe_autotune_layout):
+        returns the chosen layout, the tuning kept and the cost per layout."""
+        return _autotune_layout(self._h, reps)
+
+
+LAYOUTS = (0, 64, 8, 1)  # the candidates of swe_autotune_layout, in its order
+
+
+def _layout(h):
+    b = ctypes.c_int()
+    _check(_lib.load().swe_get_layout(h, ctypes.byref(b)))
+    return b.value
+
+
+def _autotune_layout(h, reps):
+    b = ctypes.c_int()
+    c = np.zeros(len(LAYOUTS), np.float64)
+    _check(_lib.load().swe_autotune_layout(h, int(reps), ctypes.byref(b), _ptr(c)))
+    fi, ui, fb, ub = _tuning(h)
+    return {"layout": b.value, "flux_items": fi, "update_items": ui, "flux_block": fb, "update_block": ub,
+            "stage_ms": {("soa" if L == 0 else f"aosoa{L}"): float(x) for L, x in zip(LAYOUTS, c)}}
+
 
 def _set_tuning(h, flux_items, update_items, flux_block, update_block):
     _check(_lib.load().swe_set_tuning(h, int(flux_items), int(update_items)))

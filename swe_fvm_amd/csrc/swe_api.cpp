@@ -26,6 +26,12 @@ struct swe_solver {
   std::vector<void*> allocs;
   DevMesh M{};
   DevGeomOut G{};  // writable static geometry (swe_set_bathymetry)
+  // three state copies, each one allocation of 3 * Tpad doubles (Tpad = T
+  // rounded up to kLayoutMaxBlock cells) holding w, u, v in the layout
+  // lay_b (0: SoA, else blocks of lay_b cells); sw, su, sv point into it
+  double* sbuf[3] = {nullptr, nullptr, nullptr};
+  int64_t Tpad = 0;
+  int lay_b = 0;
   double* sw[3] = {nullptr, nullptr, nullptr};
   double* su[3] = {nullptr, nullptr, nullptr};
   double* sv[3] = {nullptr, nullptr, nullptr};
@@ -81,8 +87,21 @@ hipError_t dalloc(swe_solver* s, T** p, int64_t n) {
   return e;
 }
 
-DevState state_of(const swe_solver* s, int b) { return DevState{s->sw[b], s->su[b], s->sv[b]}; }
-DevOut out_of(const swe_solver* s, int b) { return DevOut{s->sw[b], s->su[b], s->sv[b]}; }
+uint32_t layout_mask(int block) { return block == 0 ? kLayoutSoA : uint32_t(block - 1); }
+// offsets of the u and v planes from w in a state allocation
+int64_t layout_plane(const swe_solver* s) { return s->lay_b == 0 ? s->Tpad : int64_t(s->lay_b); }
+void layout_ptrs(swe_solver* s) {
+  const int64_t d = layout_plane(s);
+  for (int b = 0; b < 3; ++b) {
+    s->sw[b] = s->sbuf[b];
+    s->su[b] = s->sbuf[b] ? s->sbuf[b] + d : nullptr;
+    s->sv[b] = s->sbuf[b] ? s->sbuf[b] + 2 * d : nullptr;
+  }
+}
+DevState state_of(const swe_solver* s, int b) {
+  return DevState{s->sw[b], s->su[b], s->sv[b], layout_mask(s->lay_b)};
+}
+DevOut out_of(const swe_solver* s, int b) { return DevOut{s->sw[b], s->su[b], s->sv[b], layout_mask(s->lay_b)}; }
 
 #define SWE_CHECK(call, what)                 \
   do {                                        \
@@ -275,7 +294,9 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
   ALLOC(dtp, 3 * T); ALLOC(dte, 3 * T); ALLOC(dtt, 3 * T); ALLOC(dfl, T);
   ALLOC(dep0, E); ALLOC(dep1, E); ALLOC(det0, E); ALLOC(det1, E);
   ALLOC(doff, V + 1); ALLOC(dn2c, 3 * T);
-  for (int b = 0; b < 3; ++b) { ALLOC(s->sw[b], T); ALLOC(s->su[b], T); ALLOC(s->sv[b], T); }
+  s->Tpad = (T + kLayoutMaxBlock - 1) / kLayoutMaxBlock * kLayoutMaxBlock;
+  for (int b = 0; b < 3; ++b) ALLOC(s->sbuf[b], 3 * s->Tpad);
+  layout_ptrs(s);
   ALLOC(s->W.mw, T); ALLOC(s->W.mgx, T); ALLOC(s->W.mgy, T);
   ALLOC(s->W.wrec1, T);
   ALLOC(s->W.cls, T);
@@ -306,9 +327,7 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
     return bail(e, "hipMemcpy");
   if ((e = hipMemset(s->W.pw_count, 0, sizeof(unsigned int))) != hipSuccess) return bail(e, "hipMemset");
   for (int b = 0; b < 3; ++b) {
-    if ((e = hipMemset(s->sw[b], 0, sizeof(double) * size_t(std::max<int64_t>(T, 1)))) != hipSuccess) return bail(e, "hipMemset");
-    if ((e = hipMemset(s->su[b], 0, sizeof(double) * size_t(std::max<int64_t>(T, 1)))) != hipSuccess) return bail(e, "hipMemset");
-    if ((e = hipMemset(s->sv[b], 0, sizeof(double) * size_t(std::max<int64_t>(T, 1)))) != hipSuccess) return bail(e, "hipMemset");
+    if ((e = hipMemset(s->sbuf[b], 0, sizeof(double) * size_t(3 * s->Tpad))) != hipSuccess) return bail(e, "hipMemset");
   }
   // the whole mesh is the visited window; cinst / bdt / bactive are set for
   // ensembles below
@@ -433,17 +452,20 @@ void swe_destroy(swe_solver* s) {  // teardown: failures cannot be reported
   delete s;
 }
 
+// the host image of a state allocation in the solver's layout (padding cells
+// zero) from / to the caller's (T, 3) primitive array
 int swe_set_state(swe_solver* s, const double* prim) {
   if (!s || !prim) return fail(SWE_ERR_ARG, "null argument");
   if (int rc = set_device(s)) return rc;
-  std::vector<double> w(size_t(s->T)), u(size_t(s->T)), v(size_t(s->T));
+  std::vector<double> h(size_t(3 * s->Tpad), 0.);
+  const uint32_t mask = layout_mask(s->lay_b);
+  const int64_t d = layout_plane(s);
   for (int64_t i = 0; i < s->T; ++i) {
-    w[size_t(i)] = prim[3 * i]; u[size_t(i)] = prim[3 * i + 1]; v[size_t(i)] = prim[3 * i + 2];
+    const int64_t a = state_at(i, mask);
+    h[size_t(a)] = prim[3 * i]; h[size_t(d + a)] = prim[3 * i + 1]; h[size_t(2 * d + a)] = prim[3 * i + 2];
   }
-  int b = s->cur;
-  SWE_CHECK(hipMemcpyAsync(s->sw[b], w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice, s->stream), "hipMemcpy");
-  SWE_CHECK(hipMemcpyAsync(s->su[b], u.data(), sizeof(double) * u.size(), hipMemcpyHostToDevice, s->stream), "hipMemcpy");
-  SWE_CHECK(hipMemcpyAsync(s->sv[b], v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, s->stream), "hipMemcpy");
+  SWE_CHECK(hipMemcpyAsync(s->sbuf[s->cur], h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, s->stream),
+            "hipMemcpy");
   SWE_CHECK(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
   return SWE_OK;
 }
@@ -451,14 +473,15 @@ int swe_set_state(swe_solver* s, const double* prim) {
 int swe_get_state(swe_solver* s, double* prim) {
   if (!s || !prim) return fail(SWE_ERR_ARG, "null argument");
   if (int rc = set_device(s)) return rc;
-  std::vector<double> w(size_t(s->T)), u(size_t(s->T)), v(size_t(s->T));
-  int b = s->cur;
-  SWE_CHECK(hipMemcpyAsync(w.data(), s->sw[b], sizeof(double) * w.size(), hipMemcpyDeviceToHost, s->stream), "hipMemcpy");
-  SWE_CHECK(hipMemcpyAsync(u.data(), s->su[b], sizeof(double) * u.size(), hipMemcpyDeviceToHost, s->stream), "hipMemcpy");
-  SWE_CHECK(hipMemcpyAsync(v.data(), s->sv[b], sizeof(double) * v.size(), hipMemcpyDeviceToHost, s->stream), "hipMemcpy");
+  std::vector<double> h(size_t(3 * s->Tpad));
+  SWE_CHECK(hipMemcpyAsync(h.data(), s->sbuf[s->cur], sizeof(double) * h.size(), hipMemcpyDeviceToHost, s->stream),
+            "hipMemcpy");
   SWE_CHECK(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  const uint32_t mask = layout_mask(s->lay_b);
+  const int64_t d = layout_plane(s);
   for (int64_t i = 0; i < s->T; ++i) {
-    prim[3 * i] = w[size_t(i)]; prim[3 * i + 1] = u[size_t(i)]; prim[3 * i + 2] = v[size_t(i)];
+    const int64_t a = state_at(i, mask);
+    prim[3 * i] = h[size_t(a)]; prim[3 * i + 1] = h[size_t(d + a)]; prim[3 * i + 2] = h[size_t(2 * d + a)];
   }
   return SWE_OK;
 }
@@ -675,6 +698,7 @@ int64_t swe_num_nodes(const swe_solver* s) { return s ? s->V : -1; }
 
 int swe_device_state(swe_solver* s, double** w, double** u, double** v) {
   if (!s) return fail(SWE_ERR_ARG, "null solver");
+  if (s->lay_b != 0) return fail(SWE_ERR_ARG, "the state is not in the SoA layout (swe_set_layout(s, 0))");
   if (w) *w = s->sw[s->cur];
   if (u) *u = s->su[s->cur];
   if (v) *v = s->sv[s->cur];
@@ -754,6 +778,16 @@ static int batch_window(swe_solver* s, const std::vector<int>& act, bool euler) 
   const int a = s->cur, b = a == 0 ? 1 : 0;
   auto copy = [&](int64_t c0, int64_t c1) -> int {
     if (c1 <= c0) return SWE_OK;
+    if (s->lay_b != 0) {
+      // blocked layouts: the whole blocks the cells lie in, one contiguous
+      // range (cells of running members that share a block are copied too:
+      // the other buffer's values of those are overwritten by their next
+      // stage before anything reads them)
+      const int64_t B = s->lay_b, k0 = c0 / B, k1 = (c1 + B - 1) / B;
+      SWE_CHECK(hipMemcpyAsync(s->sbuf[b] + 3 * B * k0, s->sbuf[a] + 3 * B * k0, sizeof(double) * size_t(3 * B * (k1 - k0)),
+                               hipMemcpyDeviceToDevice, s->stream), "hipMemcpy");
+      return SWE_OK;
+    }
     const size_t off = size_t(c0), bytes = sizeof(double) * size_t(c1 - c0);
     SWE_CHECK(hipMemcpyAsync(s->sw[b] + off, s->sw[a] + off, bytes, hipMemcpyDeviceToDevice, s->stream), "hipMemcpy");
     SWE_CHECK(hipMemcpyAsync(s->su[b] + off, s->su[a] + off, bytes, hipMemcpyDeviceToDevice, s->stream), "hipMemcpy");
@@ -1107,6 +1141,108 @@ int swe_autotune(swe_solver* s, int reps, int* flux_items, int* update_items, do
   return SWE_OK;
 }
 
+// The state layout: the current state is rewritten into a free buffer in
+// the new layout by a device kernel and becomes current; the other buffers
+// take the new layout's pointers (their contents are stage scratch).
+int swe_set_layout(swe_solver* s, int block) {
+  if (!s) return fail(SWE_ERR_ARG, "null solver");
+  if (block < 0 || block > kLayoutMaxBlock || (block & (block - 1)) != 0)
+    return fail(SWE_ERR_ARG, "layout block: 0 (SoA) or a power of two up to 64 cells");
+  if (block == s->lay_b) return SWE_OK;
+  if (int rc = set_device(s)) return rc;
+  if (int rc = materialize_fluxes(s)) return rc;  // it reads the stale stage input in the old layout
+  const DevState src = state_of(s, s->cur);
+  const int to = (s->cur + 1) % 3;
+  s->lay_b = block;
+  layout_ptrs(s);
+  SWE_CHECK(launch_relayout(src, out_of(s, to), s->T, s->stream), "relayout");
+  SWE_CHECK(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  s->cur = to;
+  if (s->graph) {  // captured launches hold the old pointers
+    (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+    s->graph_scheme = -1;
+  }
+  return SWE_OK;
+}
+int swe_get_layout(const swe_solver* s, int* block) {
+  if (!s || !block) return fail(SWE_ERR_ARG, "null argument");
+  *block = s->lay_b;
+  return SWE_OK;
+}
+
+// The layout dimension of the autotuner: for each candidate layout the
+// work-item and workgroup variants are tuned (swe_autotune), then one stage
+// of each kind the solver runs is timed on the live state with them: the
+// separate reconstruction / flux / draining / update stage, and for a
+// single mesh that fuses its later stages also the reconstruction + fused
+// stage.  The layout (with its variants) of the least cost, separate stage
+// + (fused or separate) stage (an SSP-RK2 step in CFL mode), is kept.
+int swe_autotune_layout(swe_solver* s, int reps, int* block, double* cost_ms) {
+  if (!s) return fail(SWE_ERR_ARG, "null solver");
+  if (reps < 1) return fail(SWE_ERR_ARG, "reps must be >= 1");
+  if (int rc = set_device(s)) return rc;
+  constexpr int kLayoutCandidates = SWE_LAYOUT_CANDIDATES;
+  const int cand[kLayoutCandidates] = {0, 64, 8, 1};
+  double cost[kLayoutCandidates];
+  int items[kLayoutCandidates][2], blocks[kLayoutCandidates][2];
+  hipEvent_t e0, e1;
+  SWE_CHECK(hipEventCreate(&e0), "hipEventCreate");
+  SWE_CHECK(hipEventCreate(&e1), "hipEventCreate");
+  int rc = SWE_OK;
+  const bool fused = s->n_inst == 0 && use_fused(s, 1e-6, 1);
+  unsigned long long* rslot = (s->n_inst > 0 ? s->B.rbits : s->rbits) + 3;
+  for (int c = 0; c < kLayoutCandidates && !rc; ++c) {
+    rc = swe_set_layout(s, cand[c]);
+    if (!rc) rc = swe_autotune(s, reps, &items[c][0], &items[c][1], nullptr);
+    if (!rc) rc = swe_get_tuning_blocks(s, &blocks[c][0], &blocks[c][1]);
+    if (rc) break;
+    const DevState S = state_of(s, s->cur);
+    const DevOut O = out_of(s, (s->cur + 1) % 3);
+    const StageArgs A{0., 0., 1., 1e-6, 0., s->cor, rslot, nullptr, friction(s)};
+    auto sep = [&]() -> hipError_t {
+      hipError_t e = launch_recon1(s->M, S, s->W, s->opts, s->stream);
+      if (e == hipSuccess) e = launch_recon2(s->M, S, s->W, s->opts, s->stream);
+      if (e == hipSuccess)
+        e = launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, rslot, s->flux_items, s->flux_block, s->stream);
+      if (e == hipSuccess) e = launch_drain(s->M, S, s->W, s->opts, s->stream);
+      if (e == hipSuccess) e = launch_update(s->M, S, S, O, s->W, A, s->upd_items, s->upd_block, s->stream);
+      return e;
+    };
+    auto fus = [&]() -> hipError_t {
+      hipError_t e = launch_recon1(s->M, S, s->W, s->opts, s->stream);
+      if (e == hipSuccess) e = launch_recon2(s->M, S, s->W, s->opts, s->stream);
+      if (e == hipSuccess) e = launch_fused(s->M, s->tiles, S, S, O, s->W, A, s->flux, s->ws, s->opts, s->stream);
+      return e;
+    };
+    double t[2] = {1e300, 1e300};
+    for (int pass = 0; pass < 3 && !rc; ++pass)  // pass 0 warms up
+      for (int k = 0; k < (fused ? 2 : 1) && !rc; ++k) {
+        hipError_t e = hipEventRecord(e0, s->stream);
+        for (int q = 0; q < reps && e == hipSuccess; ++q) e = k == 0 ? sep() : fus();
+        if (e == hipSuccess) e = hipEventRecord(e1, s->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float f = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
+        if (e != hipSuccess) rc = dev_fail(e, "autotune layout");
+        else if (pass > 0 && double(f) / reps < t[k]) t[k] = double(f) / reps;
+      }
+    cost[c] = t[0] + (fused ? t[1] : t[0]);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  int best = 0;
+  for (int c = 1; c < kLayoutCandidates; ++c)
+    if (cost[c] < cost[best]) best = c;
+  if (int r2 = swe_set_layout(s, cand[best])) return r2;
+  if (int r2 = swe_set_tuning(s, items[best][0], items[best][1])) return r2;
+  if (int r2 = swe_set_tuning_blocks(s, blocks[best][0], blocks[best][1])) return r2;
+  if (block) *block = cand[best];
+  if (cost_ms) for (int c = 0; c < kLayoutCandidates; ++c) cost_ms[c] = cost[c];
+  return SWE_OK;
+}
+
 // ComputeIntegrals of the current state: reconstruction, per-cell terms
 // and a fixed-order reduction (identical run to run).
 int swe_integrals(swe_solver* s, double* out3, double* per_cell) {
@@ -1270,6 +1406,7 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
     for (int64_t n = 0; n < nsteps; ++n)
       if (!(dts[n] > 0.)) return fail(SWE_ERR_ARG, "step sizes must be positive");
   if (s->M.pe) return fail(SWE_ERR_ARG, "periodic boundaries are not differentiated");
+  if (s->lay_b != 0) return fail(SWE_ERR_ARG, "the adjoint and tangent runs need the SoA state layout (swe_set_layout(s, 0))");
   if (s->opts & (SWE_OPT_HEAD_GRADIENT | SWE_OPT_HEAD_PARTWET2 | SWE_OPT_EXACT_BALANCE))
     return fail(SWE_ERR_ARG, "the adjoint does not support the HEAD-compatibility or exact-balance options");
   const int64_t K = checkpoint_every > 0 ? std::min(checkpoint_every, nsteps) : nsteps;
@@ -1408,7 +1545,7 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
         if (e == hipSuccess) e = launch_drain(s->M, Sq, Wq, s->opts, s->stream);
         StageArgs A{sc[q].a0, sc[q].a1, sc[q].coef, dtn(n), 0., s->cor, s->rbits, nullptr, friction(s)};
         if (e == hipSuccess) e = launch_update(s->M, Sq, Un, O, Wq, A, s->upd_items, s->upd_block, s->stream);
-        Sq = DevState{O.w, O.u, O.v};
+        Sq = DevState{O.w, O.u, O.v, O.mask};
       }
     }
     // the lengths of the segment's pass-2 lists, for the partially wet
@@ -1479,6 +1616,7 @@ int swe_tangent_run(swe_solver* s, int scheme, double dt, const double* dts, int
     for (int64_t n = 0; n < nsteps; ++n)
       if (!(dts[n] > 0.)) return fail(SWE_ERR_ARG, "step sizes must be positive");
   if (s->M.pe) return fail(SWE_ERR_ARG, "periodic boundaries are not differentiated");
+  if (s->lay_b != 0) return fail(SWE_ERR_ARG, "the adjoint and tangent runs need the SoA state layout (swe_set_layout(s, 0))");
   if (s->opts & (SWE_OPT_HEAD_GRADIENT | SWE_OPT_HEAD_PARTWET2 | SWE_OPT_EXACT_BALANCE))
     return fail(SWE_ERR_ARG, "the tangent run does not support the HEAD-compatibility or exact-balance options");
   if (int rc = set_device(s)) return rc;

@@ -54,7 +54,7 @@ struct Muscl {
 __device__ __forceinline__ Muscl load_muscl(const DevState& S, const DevWork& W, int c) {
   const int k = W.cls[c];
   const double w0 = W.mw[c], gx = W.mgx[c], gy = W.mgy[c];
-  const double u = S.u[c], v = S.v[c];
+  const double u = S.u[S.at(c)], v = S.v[S.at(c)];
   Muscl m{w0, gx, gy, u, v};
   if (k == 0) {
     m.u0 = 0.; m.v0 = 0.;

@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
     const V3 Pa = node(M, r.ep0), Pb = node(M, r.ep1);
     const Muscl mf = load_muscl(S, W, f), mt = load_muscl(S, W, tr);
     const double cxf = M.cx[f], cyf = M.cy[f], cxt = M.cx[tr], cyt = M.cy[tr];
-    const double hf = (t == kSolidWall && !M.eb) ? S.w[f] - M.bc[f] : 0.;
+    const double hf = (t == kSolidWall && !M.eb) ? S.w[S.at(f)] - M.bc[f] : 0.;
     const double le = edge_len(Pa, Pb);
     const V3 E = midpoint(Pa, Pb);
     double tx, ty;
@@ -130,17 +130,17 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
   {
     q.fl = fli;
     q.mu = load_muscl(S, W, i);
-    q.wi = S.w[i];
+    q.wi = S.w[S.at(i)];
     q.p[0] = node(M, tpi[0]);
     q.p[1] = node(M, tpi[1]);
     q.p[2] = node(M, tpi[2]);
-    w = S.w[i];
-    u = S.u[i];
-    v = S.v[i];
+    w = S.w[S.at(i)];
+    u = S.u[S.at(i)];
+    v = S.v[S.at(i)];
     if (comb) {
-      w0 = U0.w[i];
-      u0 = U0.u[i];
-      v0 = U0.v[i];
+      w0 = U0.w[U0.at(i)];
+      u0 = U0.u[U0.at(i)];
+      v0 = U0.v[U0.at(i)];
     }
     bci = M.bc[i];
     areai = M.area[i];
@@ -167,7 +167,7 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
       p1 = node(M, M.tp[T + xr]);
       p2 = node(M, M.tp[2 * T + xr]);
     }
-    sD[lr] = drain_dt(S.w[xr] - M.bc[xr], M.cflag[xr], cdr, p0, p1, p2, M.area[xr]);
+    sD[lr] = drain_dt(S.w[S.at(xr)] - M.bc[xr], M.cflag[xr], cdr, p0, p1, p2, M.area[xr]);
   }
   __syncthreads();
 
@@ -208,9 +208,9 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
   }
   double ow, ou, ov;
   cons_assign(bc, r0, r1, r2, ow, ou, ov);
-  out_store<NT>(&O.w[i], ow);
-  out_store<NT>(&O.u[i], ou);
-  out_store<NT>(&O.v[i], ov);
+  out_store<NT>(&O.w[O.at(i)], ow);
+  out_store<NT>(&O.u[O.at(i)], ou);
+  out_store<NT>(&O.v[O.at(i)], ov);
 }
 
 hipError_t launch_fused(const DevMesh& M, const DevTiles& P, const DevState& S, const DevState& U0,

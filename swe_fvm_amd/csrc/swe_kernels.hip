@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W
   // barrier: the cell's own record (which fills its LDS entry), its flags,
   // vertices and neighbour ids, and the neighbours outside the workgroup
   const CellRec ri = M.crec[is];
-  const double w = S.w[is];
+  const double w = S.w[S.at(is)];
   const uint32_t fl = M.cflag[is];
   int jn[3];
   bool inb[3];
@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W
     const int64_t lj = j - blk0;
     inb[kk] = lj >= 0 && lj < kBlock && j < M.c1;
     if (!inb[kk]) {  // the owner's workgroup reads the same record: an L2 hit mostly
-      wn[kk] = S.w[j];
+      wn[kk] = S.w[S.at(j)];
       const CellRec r = M.crec[j];
       rn[kk] = NRec{r.cx, r.cy, (r.fl & CELL_BND) ? INFINITY : r.zmax, r.bc};
     }
@@ -273,7 +273,7 @@ __device__ inline void partwet2_muscl(const DevMesh& M, const DevState& S, const
   }
   (void)i1;
   (void)i2;
-  double w = S.w[i];
+  double w = S.w[S.at(i)];
   double b23 = Q0.z, b12 = Q1.z, b13 = Q2.z;
   double w0 = w1, gx = 0., gy = 0.;  // fallback: pass-1 (PartWet1) result
   bool fallback = (w > b13) || (b13 - b23 < kTol);
@@ -413,7 +413,7 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, DevState S, DevWork W, d
     q.cyt = M.cy[tr];
     // the cell average enters only the wall flux: walls alone load it
     // (with SWE_OPT_EXACT_BALANCE the wall takes the inside edge depth)
-    q.hf = (t == kSolidWall && !M.eb) ? S.w[f] - M.bc[f] : 0.;
+    q.hf = (t == kSolidWall && !M.eb) ? S.w[S.at(f)] - M.bc[f] : 0.;
     q.inst = M.cinst ? M.cinst[f] : 0;
     q.on = e < M.e1 && (!M.cinst || M.bactive[q.inst]);
   }
@@ -485,7 +485,7 @@ __global__ void __launch_bounds__(256) k_drain(DevMesh M, DevState S, DevWork W)
   int64_t i = M.c0 + global_index();
   if (i >= M.c1) return;
   if (M.cinst && !M.bactive[M.cinst[i]]) return;  // finished ensemble member
-  double h = S.w[i] - M.bc[i];
+  double h = S.w[S.at(i)] - M.bc[i];
   double d;
   if (!is_wet(h)) {
     d = 0.;
@@ -519,7 +519,7 @@ __device__ __forceinline__ void rhs_load(const DevMesh& M, const DevState& S, co
                                          int i, RhsIn& q) {
   q.fl = M.cflag[i];
   q.mu = load_muscl(S, W, i);
-  q.wi = S.w[i];
+  q.wi = S.w[S.at(i)];
   q.p[0] = node(M, M.tp[i]);
   q.p[1] = node(M, M.tp[M.T + i]);
   q.p[2] = node(M, M.tp[2 * M.T + i]);
@@ -568,14 +568,14 @@ __global__ void __launch_bounds__(BS) k_update(DevMesh M, DevState S, DevState U
     const int64_t i = base + it * BS;
     const int is = (int)(i < M.c1 ? i : M.c0);  // out-of-range items read a valid cell and are dropped
     rhs_load(M, S, W, is, in[it]);
-    w[it] = S.w[is];
-    u[it] = S.u[is];
-    v[it] = S.v[is];
+    w[it] = S.w[S.at(is)];
+    u[it] = S.u[S.at(is)];
+    v[it] = S.v[S.at(is)];
     bc[it] = centroid(in[it].p[0], in[it].p[1], in[it].p[2]).z;
     if (comb) {
-      w0[it] = U0.w[is];
-      u0[it] = U0.u[is];
-      v0[it] = U0.v[is];
+      w0[it] = U0.w[U0.at(is)];
+      u0[it] = U0.u[U0.at(is)];
+      v0[it] = U0.v[U0.at(is)];
     }
   }
 #pragma unroll
@@ -587,9 +587,9 @@ __global__ void __launch_bounds__(BS) k_update(DevMesh M, DevState S, DevState U
       const int inst = M.cinst[i];
       if (!M.bactive[inst]) {
         // finished ensemble member: the stage leaves its state unchanged
-        O.w[i] = comb ? w0[it] : w[it];
-        O.u[i] = comb ? u0[it] : u[it];
-        O.v[i] = comb ? v0[it] : v[it];
+        O.w[O.at(i)] = comb ? w0[it] : w[it];
+        O.u[O.at(i)] = comb ? u0[it] : u[it];
+        O.v[O.at(i)] = comb ? v0[it] : v[it];
         continue;
       }
       dts = A.coef * M.bdt[inst];
@@ -613,9 +613,9 @@ __global__ void __launch_bounds__(BS) k_update(DevMesh M, DevState S, DevState U
     }
     double ow, ou, ov;
     cons_assign(bc[it], r0, r1, r2, ow, ou, ov);
-    out_store<NT>(&O.w[i], ow);
-    out_store<NT>(&O.u[i], ou);
-    out_store<NT>(&O.v[i], ov);
+    out_store<NT>(&O.w[O.at(i)], ow);
+    out_store<NT>(&O.u[O.at(i)], ou);
+    out_store<NT>(&O.v[O.at(i)], ov);
   }
 }
 
@@ -731,16 +731,16 @@ __device__ inline void cell_integrals(const DevMesh& M, const DevState& S, const
   const int k = W.cls[i];
   if (k == 0) return;
   const double A = M.area[i];
-  const double hh = S.w[i] - M.bc[i];
+  const double hh = S.w[S.at(i)] - M.bc[i];
   if (k == 2) {
-    const double uu = S.u[i], vv = S.v[i];
+    const double uu = S.u[S.at(i)], vv = S.v[S.at(i)];
     out[0] = A * hh;
     out[1] = A * (0.5 * hh * (uu * uu + vv * vv));
     out[2] = A * (0.5 * hh * hh + hh * M.bc[i]);
     return;
   }
   out[0] = A * hh;
-  const double w0 = W.mw[i], gx = W.mgx[i], gy = W.mgy[i], u0 = S.u[i], v0 = S.v[i];
+  const double w0 = W.mw[i], gx = W.mgx[i], gy = W.mgy[i], u0 = S.u[S.at(i)], v0 = S.v[S.at(i)];
   const double cx = M.cx[i], cy = M.cy[i];
   const V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[M.T + i]), p2 = node(M, M.tp[2 * M.T + i]);
   const int n = 10;
@@ -842,7 +842,7 @@ __global__ void __launch_bounds__(256) k_stats(DevMesh M, DevState S, double* pa
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   Stats v{0., INFINITY, -INFINITY, 0., 0.};
   if (i < M.T) {
-    const double w = S.w[i], u = S.u[i], vv = S.v[i];
+    const double w = S.w[S.at(i)], u = S.u[S.at(i)], vv = S.v[S.at(i)];
     if (!isfinite(w) || !isfinite(u) || !isfinite(vv)) {
       v.nonfinite = 1.;
     } else {
@@ -884,8 +884,8 @@ __global__ void k_reconstruct_one(DevMesh M, DevState S, DevWork W, int i, int k
   const CellRec ri = M.crec[i];
   const uint32_t fl = M.cflag[i];
   const V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[M.T + i]), p2 = node(M, M.tp[2 * M.T + i]);
-  const double w = S.w[i];
-  double o0 = w, o1 = S.u[i], o2 = S.v[i], gx = 0., gy = 0.;
+  const double w = S.w[S.at(i)];
+  double o0 = w, o1 = S.u[S.at(i)], o2 = S.v[S.at(i)], gx = 0., gy = 0.;
   const double lvl1 = partwet1_w(ri.zmin, ri.zmin, ri.zmax, ri.bc, w);
   if (kind == 0) {
     o0 = ri.bc; o1 = 0.; o2 = 0.;
@@ -900,7 +900,7 @@ __global__ void k_reconstruct_one(DevMesh M, DevState S, DevWork W, int i, int k
       int j = M.tt[kk * M.T + i];
       j = j < 0 ? i : j;  // a wall has no neighbour: the cell stands in, as in pass 1
       jn[kk] = j;
-      wn[kk] = S.w[j];
+      wn[kk] = S.w[S.at(j)];
       const CellRec r = M.crec[j];
       rn[kk] = NRec{r.cx, r.cy, (r.fl & CELL_BND) ? INFINITY : r.zmax, r.bc};
     }
@@ -993,6 +993,18 @@ hipError_t launch_recon2(const DevMesh& M, const DevState& S, const DevWork& W, 
   if (g.x > 1024) g.x = 1024;  // grid-stride over the list
   if (opts & 2) hipLaunchKernelGGL((k_recon2<true>), g, dim3(256), 0, st, M, S, W);
   else hipLaunchKernelGGL((k_recon2<false>), g, dim3(256), 0, st, M, S, W);
+  return hipGetLastError();
+}
+__global__ void __launch_bounds__(256) k_relayout(DevState S, DevOut O, int64_t T) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= T) return;
+  const int64_t a = S.at(i), b = O.at(i);
+  O.w[b] = S.w[a];
+  O.u[b] = S.u[a];
+  O.v[b] = S.v[a];
+}
+hipError_t launch_relayout(const DevState& S, const DevOut& O, int64_t T, hipStream_t st) {
+  if (T > 0) hipLaunchKernelGGL(k_relayout, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, st, S, O, T);
   return hipGetLastError();
 }
 hipError_t launch_flux(const DevMesh& M, const DevState& S, const DevWork& W, int flux, int ws,

@@ -103,16 +103,30 @@ struct DevGeomOut {
   CellRec* crec;
 };
 
+// Cell-state layout: the w, u, v of cell i sit at w[at(i)], u[at(i)],
+// v[at(i)] with at(i) = 3 i - 2 (i & mask).  The default mask (2^31 - 1)
+// gives at(i) = i: three separate arrays (SoA).  mask = B - 1 for a power of
+// two B gives blocks of B cells holding B values of w, then of u, then of v
+// (AoSoA; B = 1 is AoS), with u and v the pointers B and 2 B past w.
+constexpr uint32_t kLayoutSoA = 0x7fffffffu;
+constexpr int kLayoutMaxBlock = 64;  // largest block; allocations are padded to a multiple of it
+__host__ __device__ __forceinline__ int64_t state_at(int64_t i, uint32_t mask) {
+  return 3 * i - 2 * (int64_t)((uint32_t)i & mask);
+}
 struct DevState {
   const double* w;
   const double* u;
   const double* v;
+  uint32_t mask = kLayoutSoA;
+  __host__ __device__ __forceinline__ int64_t at(int64_t i) const { return state_at(i, mask); }
 };
 
 struct DevOut {
   double* w;
   double* u;
   double* v;
+  uint32_t mask = kLayoutSoA;
+  __host__ __device__ __forceinline__ int64_t at(int64_t i) const { return state_at(i, mask); }
 };
 
 struct DevWork {
@@ -186,6 +200,8 @@ struct StageArgs {
   double fric;         // linear friction coefficient (0: none)
 };
 
+// the state of cells [0, T) rewritten from S's layout into O's
+hipError_t launch_relayout(const DevState& S, const DevOut& O, int64_t T, hipStream_t st);
 hipError_t launch_setup_geometry(const DevMesh& M, const DevGeomOut& G, hipStream_t st);
 // zero_count false: the caller has zeroed W.pw_count (launch_slot_counts)
 hipError_t launch_recon1(const DevMesh& M, const DevState& S, const DevWork& W, int opts,

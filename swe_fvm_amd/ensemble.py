@@ -11,7 +11,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .core import SCHEMES, Fluxer, SolverError, _autotune, _check, _ptr, _set_tuning, _tuning
+from .core import (SCHEMES, Fluxer, SolverError, _autotune, _autotune_layout, _check, _layout, _ptr, _set_tuning,
+                   _tuning)
 
 
 class Ensemble:
@@ -83,6 +84,18 @@ class Ensemble:
         """Picks the fastest work items per thread on the current states
         (as SpaceDisc.autotune); a captured step graph is re-captured."""
         return _autotune(self._h, reps)
+
+    def set_layout(self, block=0):
+        """State layout of the batch (as SpaceDisc.set_layout)."""
+        _check(_lib.load().swe_set_layout(self._h, int(block)))
+
+    def layout(self):
+        return _layout(self._h)
+
+    def autotune_layout(self, reps=10):
+        """Work items, workgroup size and state layout (SoA or AoSoA) picked
+        on the current states (as SpaceDisc.autotune_layout)."""
+        return _autotune_layout(self._h, reps)
 
     def synchronize(self):
         _check(_lib.load().swe_synchronize(self._h))
