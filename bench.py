@@ -56,6 +56,11 @@ def host_cpu():
     return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
 
 
+def layout_name(block):
+    """The state layout of swe_set_layout: 0 = SoA, B = AoSoA blocks of B cells."""
+    return "soa" if block == 0 else f"aosoa{block}"
+
+
 def kernel_own_bytes(T, E, V):
     """Bytes of every array a kernel reads or writes, once each (int32
     indices, FP64 values), per launch: the kernel's own implementation
@@ -190,7 +195,7 @@ def ensemble_measure(n_members, cells, t_end, seq_steps, host_stepped=True, tune
           f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     fl = swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt]
     ens = Ensemble(fl, [d for d, _ in mem], [p for _, p in mem])
-    tune = ens.autotune(reps=tune_reps)
+    tune = ens.autotune_layout(reps=tune_reps)  # work items, workgroup size and state layout
     ens.run("SSPRK2", cfl=0.15, t_end=t_end, max_steps=2, graph=True)  # warm-up + capture
     ens.reset(0.0)
     ens.set_states([p for _, p in mem])
@@ -230,7 +235,9 @@ def ensemble_measure(n_members, cells, t_end, seq_steps, host_stepped=True, tune
                       "tuning": {"flux_edges_per_thread": tune["flux_items"],
                                  "update_cells_per_thread": tune["update_items"],
                                  "flux_block_threads": tune["flux_block"],
-                                 "update_block_threads": tune["update_block"]}}}
+                                 "update_block_threads": tune["update_block"],
+                                 "state_layout": layout_name(tune["layout"]),
+                                 "stage_ms_per_layout": tune["stage_ms"]}}}
     if host_stepped:
         hs_updates, hs_s = 0.0, 0.0
         for dom, prim in mem:
@@ -479,11 +486,12 @@ def main():
         sd.run("SSPRK2", args.warmup, dt=0.0, cfl=0.15)
     # work items per thread: the fastest bit-identical kernel instantiation
     tn = sd.tuning()
-    tune = {"flux_items": tn[0], "update_items": tn[1], "flux_block": tn[2], "update_block": tn[3]} \
-        if args.no_autotune else sd.autotune(reps=5)
+    tune = {"flux_items": tn[0], "update_items": tn[1], "flux_block": tn[2], "update_block": tn[3],
+            "layout": sd.layout(), "stage_ms": None} \
+        if args.no_autotune else sd.autotune_layout(reps=5)
     print(f"[bench] autotune: flux x{tune['flux_items']}/{tune['flux_block']} "
-          f"update x{tune['update_items']}/{tune['update_block']}",
-          file=sys.stderr, flush=True)
+          f"update x{tune['update_items']}/{tune['update_block']} layout {layout_name(tune['layout'])} "
+          f"(stage ms per layout {tune['stage_ms']})", file=sys.stderr, flush=True)
     sd.synchronize()
     t1 = time.perf_counter()
     ms = sd.time_run("SSPRK2", args.steps, dt=0.0, cfl=0.15)
@@ -570,7 +578,9 @@ def main():
                    "tuning": {"flux_edges_per_thread": tune["flux_items"],
                               "update_cells_per_thread": tune["update_items"],
                               "flux_block_threads": tune["flux_block"],
-                              "update_block_threads": tune["update_block"]}},
+                              "update_block_threads": tune["update_block"],
+                              "state_layout": layout_name(tune["layout"]),
+                              "stage_ms_per_layout": tune["stage_ms"]}},
         "roofline": {"bound": "hbm", "kernel": dom_k, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_cell": stage_b,
