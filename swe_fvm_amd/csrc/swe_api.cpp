@@ -130,7 +130,7 @@ int materialize_fluxes(swe_solver* s) {
 // a stage takes the fused kernel when its step size is known before the
 // flux pass: a fixed dt, or a later stage of a CFL step (slot > 0)
 bool use_fused(const swe_solver* s, double dt_fixed, int slot) {
-  return s->fused_ok && s->fused_on && s->n_inst == 0 && (dt_fixed > 0. || slot > 0);
+  return s->fused_ok && s->fused_on && (dt_fixed > 0. || slot > 0);
 }
 
 // One stage: ComputeInterfaceValues, ComputeFluxes, update loop.
@@ -343,8 +343,9 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
     s->M.pe = dpe;
   }
   // the fused kernel's tiles hold no periodic partners: periodic meshes keep
-  // the separate flux, draining and update kernels
-  if (n_inst == 0 && pe.empty()) {
+  // the separate flux, draining and update kernels (ensembles tile their
+  // stacked mesh; no edge joins two members)
+  if (pe.empty()) {
     TileTables tt_host;
     if (build_tiles(T, tt.data(), te.data(), flag.data(), et0.data(), et1.data(), ep0.data(), ep1.data(), tt_host)) {
       uint32_t *dcode, *drcode;
@@ -724,13 +725,20 @@ static int batch_step(swe_solver* s, int scheme, double cfl, double t_end) {
   }
   for (int q = 0; q < nst; ++q) {
     DevState S = state_of(s, st[q].in);
+    StageArgs A{st[q].a0, st[q].a1, st[q].coef, 0., cfl, s->cor, nullptr, nullptr, friction(s)};
+    if (q > 0 && use_fused(s, 0., q)) {  // the members' step sizes are known: one fused kernel
+      SWE_CHECK(launch_recon1(s->M, S, s->W, s->opts, s->stream), "recon1");
+      SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
+      SWE_CHECK(launch_fused(s->M, s->tiles, S, state_of(s, st[q].u0), out_of(s, st[q].out), s->W, A, s->flux, s->ws,
+                             s->opts, s->stream), "fused");
+      continue;
+    }
     SWE_CHECK(launch_batch_reset_r(s->B, q, s->stream), "batch_reset_r");
     SWE_CHECK(launch_recon1(s->M, S, s->W, s->opts, s->stream), "recon1");
     SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
     SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->B.rbits + q, s->flux_items, s->flux_block, s->stream), "flux");
     if (q == 0) SWE_CHECK(launch_batch_dt(s->B, cfl, t_end, s->stream), "batch_dt");
     SWE_CHECK(launch_drain(s->M, S, s->W, s->opts, s->stream), "drain");
-    StageArgs A{st[q].a0, st[q].a1, st[q].coef, 0., cfl, s->cor, nullptr, nullptr, friction(s)};
     SWE_CHECK(launch_update(s->M, S, state_of(s, st[q].u0), out_of(s, st[q].out), s->W, A, s->upd_items, s->upd_block, s->stream),
               "update");
   }
@@ -1020,7 +1028,7 @@ int swe_set_fused(swe_solver* s, int on, int* active) {
   if (int rc = set_device(s)) return rc;
   if (int rc = materialize_fluxes(s)) return rc;
   s->fused_on = on != 0;
-  if (active) *active = (s->fused_ok && s->fused_on && s->n_inst == 0) ? 1 : 0;
+  if (active) *active = (s->fused_ok && s->fused_on) ? 1 : 0;
   return SWE_OK;
 }
 
@@ -1190,7 +1198,7 @@ int swe_autotune_layout(swe_solver* s, int reps, int* block, double* cost_ms) {
   SWE_CHECK(hipEventCreate(&e0), "hipEventCreate");
   SWE_CHECK(hipEventCreate(&e1), "hipEventCreate");
   int rc = SWE_OK;
-  const bool fused = s->n_inst == 0 && use_fused(s, 1e-6, 1);
+  const bool fused = use_fused(s, 1e-6, 1);
   unsigned long long* rslot = (s->n_inst > 0 ? s->B.rbits : s->rbits) + 3;
   for (int c = 0; c < kLayoutCandidates && !rc; ++c) {
     rc = swe_set_layout(s, cand[c]);

@@ -47,11 +47,11 @@ static_assert(kFusedThreads == kTileCells || kFusedThreads == 2 * kTileCells, "t
 
 template <int FLUX, int WS, bool LEN, bool NT>
 __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(DevMesh M, DevTiles P, DevState S, DevState U0,
-                                                      DevOut O, DevWork W, StageArgs A) {
+                                                      DevOut O, DevWork W, StageArgs A, int64_t t0, int64_t t1) {
   __shared__ double sF0[kTileMaxEval], sF1[kTileMaxEval], sF2[kTileMaxEval];
   __shared__ double sD[kTileMaxLocal];
-  const int64_t tile = logical_block();
-  if (tile >= P.ntiles) return;  // padding workgroups of the XCD-aware grid
+  const int64_t tile = t0 + logical_block();
+  if (tile >= t1) return;  // padding workgroups of the XCD-aware grid
   const int64_t c0 = tile * kTileCells;
   const int nc = (int)(M.T - c0 < kTileCells ? M.T - c0 : kTileCells);
   const int rg0 = P.ring_off[tile], nr = P.ring_off[tile + 1] - rg0;
@@ -174,11 +174,24 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
   // ---- update of the tile's cells (k_update's operations; the fluxes and
   // neighbour draining steps are read from LDS edge by edge)
   if (!mine) return;
-  const double dt = stage_dt_fused(A);
+  const double dt = M.cinst ? 0. : stage_dt_fused(A);
   if (A.dts_log && tile == 0 && tid == 0) A.dts_log[0] = dt;
+  double dts = A.coef * dt;
+  if (M.cinst) {  // ensembles: the member's own step size (k_update's rule)
+    const int inst = M.cinst[i];
+    if (!M.bactive[inst]) {
+      // finished member, or a cell of a tile that straddles the edge of the
+      // active window: the stage leaves its state unchanged
+      const int64_t o = O.at(i);
+      O.w[o] = comb ? w0 : w;
+      O.u[o] = comb ? u0 : u;
+      O.v[o] = comb ? v0 : v;
+      return;
+    }
+    dts = A.coef * M.bdt[inst];
+  }
   q.dti = sD[tid];
   const double bc = centroid(q.p[0], q.p[1], q.p[2]).z;
-  const double dts = A.coef * dt;
   double R0, R1, R2;
   rhs_compute_by(
       q,
@@ -216,15 +229,20 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
 hipError_t launch_fused(const DevMesh& M, const DevTiles& P, const DevState& S, const DevState& U0,
                         const DevOut& O, const DevWork& W, const StageArgs& A, int flux, int ws, int opts,
                         hipStream_t st) {
-  if (M.cinst || M.c0 != 0 || M.c1 != M.T) return hipErrorInvalidValue;  // single meshes only
-  const dim3 g = grid_for_bs(P.ntiles, 1), b(kFusedThreads);
+  // the tiles that hold cells of the visited window [c0, c1): the whole mesh,
+  // or an ensemble's running members (cells of finished members in the
+  // first and last tile are copied unchanged, as every finished member's)
+  if (!M.cinst && (M.c0 != 0 || M.c1 != M.T)) return hipErrorInvalidValue;
+  const int64_t t0 = M.c0 / kTileCells, t1 = (M.c1 + kTileCells - 1) / kTileCells;
+  if (t1 <= t0) return hipGetLastError();
+  const dim3 g = grid_for_bs(t1 - t0, 1), b(kFusedThreads);
   const bool len = (opts & kOptDrainLength) != 0;
 #define SWE_FUSED(F, Wv)                                                                             \
   if (flux == F && ws == Wv) {                                                                       \
-    if (len && M.nt) hipLaunchKernelGGL((k_fused<F, Wv, true, true>), g, b, 0, st, M, P, S, U0, O, W, A);   \
-    else if (len) hipLaunchKernelGGL((k_fused<F, Wv, true, false>), g, b, 0, st, M, P, S, U0, O, W, A);     \
-    else if (M.nt) hipLaunchKernelGGL((k_fused<F, Wv, false, true>), g, b, 0, st, M, P, S, U0, O, W, A);    \
-    else hipLaunchKernelGGL((k_fused<F, Wv, false, false>), g, b, 0, st, M, P, S, U0, O, W, A);             \
+    if (len && M.nt) hipLaunchKernelGGL((k_fused<F, Wv, true, true>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1);   \
+    else if (len) hipLaunchKernelGGL((k_fused<F, Wv, true, false>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1);     \
+    else if (M.nt) hipLaunchKernelGGL((k_fused<F, Wv, false, true>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1);    \
+    else hipLaunchKernelGGL((k_fused<F, Wv, false, false>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1);             \
     return hipGetLastError();                                                                        \
   }
   SWE_FUSED(0, 0) SWE_FUSED(0, 1) SWE_FUSED(0, 2) SWE_FUSED(1, 0) SWE_FUSED(1, 1) SWE_FUSED(1, 2)
