@@ -178,13 +178,16 @@ def cpu_rows(fl):
     return rows
 
 
-def ensemble_measure(n_members, cells, t_end, seq_steps, host_stepped=True, tune_reps=3):
+def ensemble_measure(n_members, cells, t_end, seq_steps, host_stepped=True, tune_reps=3, chunk=0):
     """n_members seeded members (log-uniform sizes, ~cells in total),
     HLLC<Einfeldt>, SSP-RK2, per-member CFL 0.15 steps to t_end: the batched
     ensemble (one graph-captured step for all active members) against the
     same members advanced one after the other with the single-mesh solver
     (device-stepped: seq_steps steps per call; host-stepped: one
-    Solvers.SSPRK2 call per step, as the reference's drivers step)."""
+    Solvers.SSPRK2 call per step, as the reference's drivers step).  With
+    chunk > 0 the batched run goes chunk steps per call and reports each
+    chunk's active members and rate (where the run loses throughput as
+    members finish)."""
     import swe_fvm_amd as swe
     from swe_fvm_amd import workloads as W
     from swe_fvm_amd.ensemble import Ensemble
@@ -200,8 +203,26 @@ def ensemble_measure(n_members, cells, t_end, seq_steps, host_stepped=True, tune
     ens.reset(0.0)
     ens.set_states([p for _, p in mem])
     ens.synchronize()
+    chunks = []
     t1 = time.perf_counter()
-    done, act = ens.run("SSPRK2", cfl=0.15, t_end=t_end, graph=True)
+    if chunk > 0:
+        done, act, prev = 0, len(mem), np.zeros(len(mem), np.int64)
+        while act > 0:
+            tc = time.perf_counter()
+            d, act = ens.run("SSPRK2", cfl=0.15, t_end=t_end, max_steps=chunk, graph=True)
+            ens.synchronize()
+            wc = time.perf_counter() - tc
+            st = ens.times()[1]
+            upd = float(np.sum((st - prev) * T))
+            prev = st.copy()
+            done += d
+            chunks.append({"steps": done, "active_after": act, "cell_updates": upd, "seconds": wc,
+                           "rate": upd / wc})
+            print(f"[bench] chunk to step {done}: {act} active, {upd / wc:.3e}/s", file=sys.stderr, flush=True)
+            if d == 0:
+                break
+    else:
+        done, act = ens.run("SSPRK2", cfl=0.15, t_end=t_end, graph=True)
     ens.synchronize()
     wall_b = time.perf_counter() - t1
     tt, steps = ens.times()
@@ -229,6 +250,7 @@ def ensemble_measure(n_members, cells, t_end, seq_steps, host_stepped=True, tune
            "batched": {"steps": done, "members_active_at_end": act, "cell_updates": updates, "seconds": wall_b,
                        "t_end": t_end, "all_at_t_end": bool(np.all(tt == t_end)), "graph": True},
            "sequential": {"steps_per_member": seq_steps, "cell_updates": seq_updates, "seconds": seq_s},
+           "chunks": chunks or None,
            "config": {"workload": f"ensemble_{len(mem)}_members_T{int(T.sum())}", "members": len(mem),
                       "cells_min": int(T.min()), "cells_max": int(T.max()), "scheme": "SSPRK2",
                       "flux": "HLLC<Einfeldt>", "dt": "per-member CFL 0.15",
@@ -260,7 +282,7 @@ def ensemble_measure(n_members, cells, t_end, seq_steps, host_stepped=True, tune
 
 def bench_ensemble(args):
     """The full 256-member run (t_end = 1.0) as one JSON line."""
-    r = ensemble_measure(args.members, args.ensemble_cells, args.t_end, args.seq_steps)
+    r = ensemble_measure(args.members, args.ensemble_cells, args.t_end, args.seq_steps, chunk=args.chunk)
     r.update({"metric": "batched/sequential aggregate FP64 cell-updates/s", "higher_is_better": True,
               "dtype": "f64", "data": "synthetic", "n_gpus": 1})
     print(json.dumps(r))
@@ -417,6 +439,8 @@ def main():
     ap.add_argument("--ensemble-cells", type=int, default=20_000_000)
     ap.add_argument("--t-end", type=float, default=1.0)
     ap.add_argument("--seq-steps", type=int, default=40)
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="ensemble: run the batch in chunks of this many steps and report each chunk's rate")
     ap.add_argument("--adjoint", action="store_true",
                     help="bathymetry-inversion adjoint workload instead of the dam break")
     ap.add_argument("--adj-n", type=int, default=354, help="adjoint mesh size (T = 2 n^2)")
