@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W
   // vertices and neighbour ids, and the neighbours outside the workgroup
   const CellRec ri = M.crec[is];
   const double w = S.w[S.at(is)];
-  const uint32_t fl = M.cflag[is];
+  const uint32_t fl = ri.fl;  // the record holds the cell's flags
   int jn[3];
   bool inb[3];
   double wn[3];
