@@ -150,6 +150,8 @@ struct Topology {
                                          boundaryLines.data(), &t));
     return Topology(t);
   }
+  // a Topology owning the tables of a handle from the C ABI
+  static Topology adopt_handle(swe_topology* t) { return Topology(t); }
   // Gmsh 4.1 reader; geometry receives node (x, y, z)
   static Topology read_msh(const std::string& path, PointArray& geometry) {
     swe_topology* t = nullptr;
@@ -173,6 +175,20 @@ struct Topology {
   TriangTag TriangTriangs(NodeTag i) const { return {tt_(i, 0), tt_(i, 1), tt_(i, 2)}; }
   bool IsTriangleBoundary(NodeTag i) const {
     return IsEdgeBoundary(te_(i, 0)) || IsEdgeBoundary(te_(i, 1)) || IsEdgeBoundary(te_(i, 2));
+  }
+  // Boundary edges pairs(k, 0) and pairs(k, 1) become PERIODIC partners
+  // (swe_topology_set_periodic; the reference declares the tag only,
+  // include/Includes.h:21).  Copies of this Topology share its tables and
+  // see the pairing; solvers built before keep the tables they copied.
+  void SetPeriodic(const EdgeTagArray& pairs) {
+    swe_detail::check(swe_topology_set_periodic(h_.get(), pairs.rows(), pairs.data()));
+    swe_detail::check(swe_topology_copy(h_.get(), ep_.data(), et_.data(), tp_.data(), te_.data(), tt_.data()));
+  }
+  // the partner of each edge (-1: none)
+  std::vector<Idx> PeriodicPartners() const {
+    std::vector<Idx> p(static_cast<size_t>(NumEdges()));
+    swe_detail::check(swe_topology_periodic(h_.get(), p.data()));
+    return p;
   }
   const swe_topology* handle() const { return h_.get(); }
 
@@ -228,10 +244,43 @@ constexpr inline bool IsWet(double h) noexcept { return h > 1e-12; }
 inline Point DryState(double b) noexcept { return {b, 0., 0.}; }
 
 // Node geometry (x, y, bathymetry z) on a topology (owning copy).
+// ----------------------------------------------- include/StructTriangMesh.h
+// StructTriangMesh(ni, nj, h) (include/StructTriangMesh.h:4-15): ni x nj
+// squares of side h from (x0, y0), four CCW triangles per square around a
+// centre node (swe_mesh_struct).  The reference's class derives from a
+// TriangMesh holding the nodes, which HEAD no longer has; here the mesh owns
+// its node geometry (z = 0, to be set by the caller) and its Topology, and a
+// Domain is built from it.
+struct StructTriangMesh {
+  StructTriangMesh(size_t ni, size_t nj, double h, double x0 = 0., double y0 = 0.) : m_ni(ni), m_nj(nj) {
+    swe_topology* t = nullptr;
+    double* xyz = nullptr;
+    int64_t nn = 0;
+    swe_detail::check(swe_mesh_struct(int64_t(ni), int64_t(nj), h, x0, y0, &xyz, &nn, &t));
+    m_geometry.resize(nn);
+    std::memcpy(m_geometry.data(), xyz, sizeof(double) * size_t(3 * nn));
+    swe_free(xyz);
+    m_topology = std::make_shared<Topology>(Topology::adopt_handle(t));
+  }
+  size_t Ni() const { return m_ni; }
+  size_t Nj() const { return m_nj; }
+  PointArray& Geometry() { return m_geometry; }
+  const PointArray& Geometry() const { return m_geometry; }
+  Topology& GetTopology() { return *m_topology; }
+  const Topology& GetTopology() const { return *m_topology; }
+
+ private:
+  size_t m_ni, m_nj;
+  PointArray m_geometry;
+  std::shared_ptr<Topology> m_topology;
+};
+
 struct Domain {
   Domain(const PointArray& geometry, const Topology& topology) : m_geometry(geometry), m_topology(topology) {
     if (geometry.cols() != topology.NumNodes()) throw DomainError("geometry and topology disagree");
   }
+  // Domain{StructTriangMesh{n, n, l / n}} as the reference's drivers write it
+  explicit Domain(const StructTriangMesh& m) : Domain(m.Geometry(), m.GetTopology()) {}
   const Topology& GetTopology() const noexcept { return m_topology; }
   const PointArray& GetGeometry() const { return m_geometry; }
   size_t Size() const { return size_t(m_geometry.cols()); }
