@@ -288,7 +288,7 @@ def bench_ensemble(args):
     print(json.dumps(r))
 
 
-def adjoint_measure(B=32, n=354, N=2000, dt=2e-4, checkpoint=50, nfd=16, check_steps=100):
+def adjoint_measure(B=32, n=354, N=2000, dt=2e-4, checkpoint=50, nfd=16, check_steps=100, fused=True):
     """Bathymetry inversion, a batch of B members on the n x n-grid mesh
     (n = 354: 250,632 cells) stacked into one domain (every kernel advances
     all members in one launch), HLLC<Einfeldt>, SSP-RK2, fixed dt, N steps,
@@ -322,6 +322,7 @@ def adjoint_measure(B=32, n=354, N=2000, dt=2e-4, checkpoint=50, nfd=16, check_s
     prim = np.tile(prim1, (B, 1))
     z0 = np.tile(d["z"], B)
     sd = swe.SpaceDisc(fl, big, prim)
+    fused = sd.set_fused(fused)  # the forward's stages and the reverse sweep's recomputation
     # targets: one batched run over the members' true bathymetries
     sd.set_bathymetry(np.concatenate(d["z_true"]))
     sd.SetVolField(prim)
@@ -347,6 +348,7 @@ def adjoint_measure(B=32, n=354, N=2000, dt=2e-4, checkpoint=50, nfd=16, check_s
           file=sys.stderr, flush=True)
     # member 0 alone: its slice of the batch, bit for bit
     sd1 = swe.SpaceDisc(fl, dom1, prim1)
+    sd1.set_fused(fused)
     z1 = d["z"]
 
     def run1(zz, p0, nsteps):
@@ -409,7 +411,8 @@ def adjoint_measure(B=32, n=354, N=2000, dt=2e-4, checkpoint=50, nfd=16, check_s
         "tangent_duality": tangent, "fd_full_horizon": fd_full, "fd_short_horizon": fd_short,
         "config": {"workload": f"bathymetry_inversion_batch{B}_T{T1}", "members": B, "cells": B * T1,
                    "steps": N, "dt": dt, "gauges_per_member": int(len(g0)), "checkpoint_every": ck,
-                   "scheme": "SSPRK2", "flux": "HLLC<Einfeldt>", "gradient": "torch.autograd.Function"},
+                   "scheme": "SSPRK2", "flux": "HLLC<Einfeldt>", "gradient": "torch.autograd.Function",
+                   "fused_stages": fused},
         "dtype": "f64", "data": "synthetic", "n_gpus": 1,
     }
     return out
@@ -418,7 +421,7 @@ def adjoint_measure(B=32, n=354, N=2000, dt=2e-4, checkpoint=50, nfd=16, check_s
 def bench_adjoint(args):
     """The full inversion workload (2,000 steps) as one JSON line."""
     out = adjoint_measure(args.adj_batch, args.adj_n, args.adj_steps, args.adj_dt, args.adj_checkpoint,
-                          args.adj_fd, args.adj_check_steps)
+                          args.adj_fd, args.adj_check_steps, fused=not args.no_fused)
     print(json.dumps(out))
 
 

@@ -64,6 +64,10 @@ struct swe_solver {
 
 namespace {
 
+// the reverse sweep recomputes a stage with the storing fused kernel below
+// this size, with the separate flux / draining / update kernels above
+constexpr int64_t kFusedRecomputeMaxCells = int64_t(1) << 20;
+
 // entry points of the single-mesh step (one CFL slot per stage) refuse
 // ensemble handles, whose flux pass keeps one slot per member
 int single_mesh(const swe_solver* s) {
@@ -351,6 +355,7 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
       uint32_t *dcode, *drcode;
       int *droff, *drcell, *devoff;
       EdgeRec* dev;
+      int* devedge;
 #define ALLOC(ptr, n) if ((e = dalloc(s, &ptr, n)) != hipSuccess) return bail(e, "hipMalloc")
 #define H2D(dst, v)                                                                                      \
   if (!v.empty() && (e = hipMemcpy(dst, v.data(), sizeof(v[0]) * v.size(), hipMemcpyHostToDevice)) != hipSuccess) \
@@ -358,11 +363,13 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
       ALLOC(dcode, int64_t(tt_host.code.size())); ALLOC(droff, int64_t(tt_host.ring_off.size()));
       ALLOC(drcell, int64_t(tt_host.ring_cell.size())); ALLOC(drcode, int64_t(tt_host.ring_code.size()));
       ALLOC(devoff, int64_t(tt_host.ev_off.size())); ALLOC(dev, int64_t(tt_host.evrec.size()));
+      ALLOC(devedge, int64_t(tt_host.evedge.size()));
       H2D(dcode, tt_host.code); H2D(droff, tt_host.ring_off); H2D(drcell, tt_host.ring_cell);
       H2D(drcode, tt_host.ring_code); H2D(devoff, tt_host.ev_off); H2D(dev, tt_host.evrec);
+      H2D(devedge, tt_host.evedge);
 #undef H2D
 #undef ALLOC
-      s->tiles = DevTiles{dcode, droff, drcell, drcode, devoff, dev, tt_host.ntiles};
+      s->tiles = DevTiles{dcode, droff, drcell, drcode, devoff, dev, devedge, tt_host.ntiles};
       s->fused_ok = true;
     }
   }
@@ -1548,11 +1555,22 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
         const DevWork Wq = slot_work(m * ns + q);
         e = launch_recon1(s->M, Sq, Wq, s->opts, s->stream, false);
         if (e == hipSuccess) e = launch_recon2(s->M, Sq, Wq, s->opts, s->stream);
-        if (e == hipSuccess)  // fixed dt: the CFL reduction is not read
-          e = launch_flux(s->M, Sq, Wq, s->flux, s->ws, s->cor, s->rbits + 3, s->flux_items, s->flux_block, s->stream, false);
-        if (e == hipSuccess) e = launch_drain(s->M, Sq, Wq, s->opts, s->stream);
         StageArgs A{sc[q].a0, sc[q].a1, sc[q].coef, dtn(n), 0., s->cor, s->rbits, nullptr, friction(s)};
-        if (e == hipSuccess) e = launch_update(s->M, Sq, Un, O, Wq, A, s->upd_items, s->upd_block, s->stream);
+        if (use_fused(s, dtn(n), q) && T < kFusedRecomputeMaxCells) {
+          // one kernel: the stage output, and the fluxes and draining time
+          // steps the reverse stage reads, stored into the slot (on meshes
+          // below 2^20 cells, whose stages are launch-bound: above, the
+          // fused kernel's edge-scattered flux stores cost more than the two
+          // launches they save, 4 % of the reverse sweep at 8M cells)
+          if (e == hipSuccess)
+            e = launch_fused(s->M, s->tiles, Sq, Un, O, Wq, A, s->flux, s->ws, s->opts, s->stream, true);
+        } else {
+          if (e == hipSuccess)  // fixed dt: the CFL reduction is not read
+            e = launch_flux(s->M, Sq, Wq, s->flux, s->ws, s->cor, s->rbits + 3, s->flux_items, s->flux_block, s->stream,
+                            false);
+          if (e == hipSuccess) e = launch_drain(s->M, Sq, Wq, s->opts, s->stream);
+          if (e == hipSuccess) e = launch_update(s->M, Sq, Un, O, Wq, A, s->upd_items, s->upd_block, s->stream);
+        }
         Sq = DevState{O.w, O.u, O.v, O.mask};
       }
     }

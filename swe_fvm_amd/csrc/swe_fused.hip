@@ -47,7 +47,8 @@ static_assert(kFusedThreads == kTileCells || kFusedThreads == 2 * kTileCells, "t
 
 template <int FLUX, int WS, bool LEN, bool NT>
 __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(DevMesh M, DevTiles P, DevState S, DevState U0,
-                                                      DevOut O, DevWork W, StageArgs A, int64_t t0, int64_t t1) {
+                                                      DevOut O, DevWork W, StageArgs A, int64_t t0, int64_t t1,
+                                                      int store) {
   __shared__ double sF0[kTileMaxEval], sF1[kTileMaxEval], sF2[kTileMaxEval];
   __shared__ double sD[kTileMaxLocal];
   const int64_t tile = t0 + logical_block();
@@ -117,6 +118,12 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
     sF0[j] = F[0];
     sF1[j] = F[1];
     sF2[j] = F[2];
+    if (store && f >= c0 && f < c0 + nc) {  // the edge's one owning evaluation: its et0 cell is ours
+      const int e = P.evedge[e0 + j];
+      out_store<NT>(&W.f0[e], F[0]);
+      out_store<NT>(&W.f1[e], F[1]);
+      out_store<NT>(&W.f2[e], F[2]);
+    }
   }
   __syncthreads();
 
@@ -159,7 +166,11 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
     }
     return sum > kTol ? area * h / sum : INFINITY;
   };
-  if (mine) sD[tid] = drain_dt(w - bci, q.fl, code, q.p[0], q.p[1], q.p[2], areai);
+  if (mine) {
+    const double d = drain_dt(w - bci, q.fl, code, q.p[0], q.p[1], q.p[2], areai);
+    sD[tid] = d;
+    if (store) out_store<NT>(&W.dtd[i], d);
+  }
   if (lr < nl) {
     V3 p0, p1, p2;
     if (LEN) {
@@ -228,7 +239,7 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
 
 hipError_t launch_fused(const DevMesh& M, const DevTiles& P, const DevState& S, const DevState& U0,
                         const DevOut& O, const DevWork& W, const StageArgs& A, int flux, int ws, int opts,
-                        hipStream_t st) {
+                        hipStream_t st, bool store) {
   // the tiles that hold cells of the visited window [c0, c1): the whole mesh,
   // or an ensemble's running members (cells of finished members in the
   // first and last tile are copied unchanged, as every finished member's)
@@ -239,10 +250,10 @@ hipError_t launch_fused(const DevMesh& M, const DevTiles& P, const DevState& S, 
   const bool len = (opts & kOptDrainLength) != 0;
 #define SWE_FUSED(F, Wv)                                                                             \
   if (flux == F && ws == Wv) {                                                                       \
-    if (len && M.nt) hipLaunchKernelGGL((k_fused<F, Wv, true, true>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1);   \
-    else if (len) hipLaunchKernelGGL((k_fused<F, Wv, true, false>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1);     \
-    else if (M.nt) hipLaunchKernelGGL((k_fused<F, Wv, false, true>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1);    \
-    else hipLaunchKernelGGL((k_fused<F, Wv, false, false>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1);             \
+    if (len && M.nt) hipLaunchKernelGGL((k_fused<F, Wv, true, true>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1, store ? 1 : 0);   \
+    else if (len) hipLaunchKernelGGL((k_fused<F, Wv, true, false>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1, store ? 1 : 0);     \
+    else if (M.nt) hipLaunchKernelGGL((k_fused<F, Wv, false, true>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1, store ? 1 : 0);    \
+    else hipLaunchKernelGGL((k_fused<F, Wv, false, false>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1, store ? 1 : 0);             \
     return hipGetLastError();                                                                        \
   }
   SWE_FUSED(0, 0) SWE_FUSED(0, 1) SWE_FUSED(0, 2) SWE_FUSED(1, 0) SWE_FUSED(1, 1) SWE_FUSED(1, 2)

@@ -13,7 +13,10 @@
 //   code = slot << 16 | nb
 // with nb the local id of the neighbour across edge k, or kNbWall /
 // kNbFree (boundary edge of that tag) or kNbOut (a ring cell's neighbour
-// outside the tile: its global id is tt's).
+// outside the tile: its global id is tt's).  Each evaluation also records
+// its edge's index: a storing launch writes an edge's flux from the one
+// evaluation whose et0 cell is a tile cell (the edge's et0 cell evaluates
+// it in that cell's own tile, and no other tile holds that cell).
 #include <algorithm>
 #include <cstdint>
 #include <vector>
@@ -77,6 +80,7 @@ bool build_tiles(int64_t T, const int* tt, const int* te, const uint32_t* cflag,
           out.ev.push_back(uint16_t((l << 2) | k));
           const int e = te[size_t(k * T + x)];
           out.evrec.push_back(EdgeRec{et0[e], et1[e], ep0[e], ep1[e]});
+          out.evedge.push_back(e);
         }
       }
     }

@@ -31,6 +31,7 @@ struct TileTables {
   std::vector<int> ev_off;          // [ntiles + 1]
   std::vector<uint16_t> ev;         // evaluations: local cell << 2 | k
   std::vector<EdgeRec> evrec;       // per evaluation: the edge's et0, et1, ep0, ep1
+  std::vector<int> evedge;          // per evaluation: the edge
   int max_eval = 0, max_ring = 0;
 };
 
@@ -47,6 +48,7 @@ struct DevTiles {
   const uint32_t* ring_code;
   const int* ev_off;
   const EdgeRec* evrec;  // et0, et1, ep0, ep1 of each evaluated edge
+  const int* evedge;     // the edge of each evaluation (for the fluxes a storing launch writes)
   int64_t ntiles;
 };
 

@@ -6,7 +6,10 @@
 //   - each edge is evaluated once per tile;
 //   - every (local cell, k) code points at the slot of its own edge and its
 //     neighbour code at the cell across that edge (or the boundary tag);
-//   - LDS capacities hold.
+//   - LDS capacities hold;
+//   - every edge has exactly one evaluation whose et0 cell is a tile cell
+//     (the flux a storing launch writes), and each evaluation's edge index
+//     is its edge.
 // Built with g++ into a shared library by tests/test_tiles_host.py.
 #include <cstdint>
 #include <cstdio>
@@ -29,6 +32,8 @@ extern "C" int tiles_check(int64_t T, const int* tt, const int* te, const uint32
   if (!build_tiles(T, tt, te, cflag, et0, et1, ep0, ep1, P)) return bad("build_tiles refused the mesh");
   if (P.code.size() != size_t(3 * T)) return bad("code size");
   int64_t evals = 0, ring = 0;
+  std::vector<int> owners(size_t(E), 0);  // per edge: tiles whose storing launch writes its flux
+  std::vector<int64_t> owned;
   for (int64_t tile = 0; tile < P.ntiles; ++tile) {
     const int64_t c0 = tile * kTileCells;
     const int nc = int(T - c0 < kTileCells ? T - c0 : kTileCells);
@@ -36,7 +41,7 @@ extern "C" int tiles_check(int64_t T, const int* tt, const int* te, const uint32
     const int e0 = P.ev_off[size_t(tile)], ne = P.ev_off[size_t(tile) + 1] - e0;
     if (nr < 0 || ne < 0 || ne > kTileMaxEval || kTileCells + nr > kTileMaxLocal) return bad("capacity");
     if (size_t(rg0 + nr) > P.ring_cell.size() || size_t(3 * (rg0 + nr)) > P.ring_code.size() ||
-        size_t(e0 + ne) > P.ev.size() || P.evrec.size() != P.ev.size())
+        size_t(e0 + ne) > P.ev.size() || P.evrec.size() != P.ev.size() || P.evedge.size() != P.ev.size())
       return bad("table range");
     const int nl = kTileCells + nr;
     auto valid_local = [&](int l) { return (l >= 0 && l < nc) || (l >= kTileCells && l < nl); };
@@ -74,6 +79,9 @@ extern "C" int tiles_check(int64_t T, const int* tt, const int* te, const uint32
       if (!seen.insert(e).second) return bad("edge evaluated twice in a tile");
       const EdgeRec& rec = P.evrec[size_t(e0 + j)];
       if (rec.et0 != f || rec.et1 != et1[e] || rec.ep0 < 0 || rec.ep1 < 0) return bad("edge record");
+      if (P.evedge[size_t(e0 + j)] != e) return bad("evaluation's edge index");
+      // the evaluation a storing launch writes: its et0 cell is a tile cell
+      if (f >= c0 && f < c0 + nc) owned.push_back(e);
       slot_edge[size_t(j)] = e;
     }
     for (int l = 0; l < nl; ++l) {
@@ -98,7 +106,11 @@ extern "C" int tiles_check(int64_t T, const int* tt, const int* te, const uint32
     }
     evals += ne;
     ring += nr;
+    for (int64_t e : owned) owners[size_t(e)]++;
+    owned.clear();
   }
+  for (int64_t e = 0; e < E; ++e)
+    if (owners[size_t(e)] != 1) return bad("an edge's flux is written by " + std::to_string(owners[size_t(e)]) + " tiles");
   stats[0] = P.ntiles;
   stats[1] = evals;
   stats[2] = ring;
