@@ -82,6 +82,9 @@ def kernel_own_bytes(T, E, V):
         # area, edmin, nodes in; state out (fluxes and draining steps stay
         # in LDS)
         "fused": T * (24 + 24 + 12 + 12 + 4 + 24 + 1 + 16 + 8 + 8) + E * 8 + nodes + T * 24,
+        # as the fused kernel without the per-edge MUSCL / centroid gathers:
+        # the tile's listed edges' fluxes (one edge index and 24 B each) in
+        "fused_ff": T * (24 + 24 + 12 + 12 + 4 + 24 + 1 + 16) + E * (4 + 24) + nodes + T * 24,
     }
 
 
@@ -531,12 +534,13 @@ def main():
     # per-kernel average launch durations (events bracket each launch)
     kms_total, kms = sd.time_run("SSPRK2", args.kernel_steps, dt=0.0, cfl=0.15, per_kernel=True)
     own = kernel_own_bytes(T, E, V)
-    # launches per SSP-RK2 step: two reconstructions; the separate flux,
-    # draining and update kernels run in stage 1 (CFL mode), the fused one
-    # in stage 2 (both stages separate without it)
-    per_step = {"recon1": 2, "recon2": 2, "flux": 1 if fused else 2, "drain": 1 if fused else 2,
-                "update": 1 if fused else 2, "fused": 1}
-    names = ["recon1", "recon2", "flux", "drain", "update", "fused"]
+    # launches per SSP-RK2 step: two reconstructions; in stage 1 (CFL mode)
+    # the separate flux kernel, then the tile kernel reading its fluxes
+    # (fused_ff), in stage 2 the fused flux-draining-update tile kernel
+    # (without tiles: flux, draining and update kernels in both stages)
+    per_step = {"recon1": 2, "recon2": 2, "flux": 1 if fused else 2, "drain": 2, "update": 2, "fused": 1,
+                "fused_ff": 1}
+    names = ["recon1", "recon2", "flux", "drain", "update", "fused", "fused_ff"]
     per = {k: {"avg_ms": float(kms[q]), "launches_per_step": per_step[k], "own_array_bytes": own[k],
                "own_array_gbs": own[k] / (kms[q] * 1e-3) / 1e9}
            for q, k in enumerate(names) if kms[q] > 0}

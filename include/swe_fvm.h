@@ -250,9 +250,10 @@ int swe_batch_info(swe_solver* s, int64_t* n, int64_t* cell_off, int64_t* edge_o
 int swe_batch_time(swe_solver* s, double* t, int64_t* steps);
 
 /* Time, with HIP events on the solver's stream, nsteps steps of swe_run:
- * *ms = total milliseconds; kernel_ms (optional, 6 doubles) = average
+ * *ms = total milliseconds; kernel_ms (optional, 7 doubles) = average
  * per-launch duration of the reconstruction, wet/dry, flux, draining,
- * update and fused flux-draining-update kernels (0 for a kernel the run did
+ * update, fused flux-draining-update and fused draining-update (after the
+ * flux pass of a CFL step's first stage) kernels (0 for a kernel the run did
  * not use), measured by bracketing each launch with events. */
 int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nsteps, double* ms,
                  double* kernel_ms);
@@ -373,8 +374,9 @@ int swe_autotune(swe_solver* s, int reps, int* flux_items, int* update_items, do
 int swe_set_layout(swe_solver* s, int block);
 int swe_get_layout(const swe_solver* s, int* block);
 /* Tunes the layout with the work-item variants: for each candidate layout
- * (SoA, 64, 8, 1) swe_autotune, then one separate stage and, for a single
- * mesh that fuses its later stages, one fused stage timed on the live state;
+ * (SoA, 64, 8, 1) swe_autotune, then the first stage of a CFL step and, for
+ * a solver that fuses its later stages, one fused stage timed on the live
+ * state;
  * keeps the layout and variants of least cost.  cost_ms (may be null)
  * receives SWE_LAYOUT_CANDIDATES costs in that order. */
 #define SWE_LAYOUT_CANDIDATES 4

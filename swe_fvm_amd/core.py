@@ -581,11 +581,12 @@ class SpaceDisc:
         """Milliseconds of nsteps steps (HIP events on the solver stream).
 
         per_kernel: also return the average per-launch milliseconds of the
-        recon1, recon2, flux, drain, update and fused flux-draining-update
-        kernels (0 for a kernel that did not run)."""
+        recon1, recon2, flux, drain, update, fused flux-draining-update and
+        fused draining-update (after a CFL step's flux pass) kernels (0 for
+        a kernel that did not run)."""
         ms = ctypes.c_double()
         if per_kernel:
-            k = np.zeros(6, np.float64)
+            k = np.zeros(7, np.float64)
             _check(_lib.load().swe_time_run(self._h, SCHEMES[scheme], float(dt), float(cfl),
                                             int(nsteps), ctypes.byref(ms), _ptr(k)))
             return ms.value, k
@@ -594,8 +595,9 @@ class SpaceDisc:
         return ms.value
 
     def set_fused(self, on=True):
-        """Uses (default) or not the fused flux-draining-update kernel for
-        the stages whose step size is known; returns whether it is used.
+        """Uses (default) or not the fused tile kernel: flux-draining-update
+        for the stages whose step size is known, draining-update after the
+        flux pass of a CFL step's first stage; returns whether it is used.
         Results are identical either way."""
         act = ctypes.c_int()
         _check(_lib.load().swe_set_fused(self._h, 1 if on else 0, ctypes.byref(act)))

@@ -136,6 +136,10 @@ int materialize_fluxes(swe_solver* s) {
 bool use_fused(const swe_solver* s, double dt_fixed, int slot) {
   return s->fused_ok && s->fused_on && (dt_fixed > 0. || slot > 0);
 }
+// otherwise (the first stage of a CFL step) the separate flux pass finds the
+// step size, and the draining and update phases still run as one tile
+// kernel reading its fluxes
+bool use_fused_after_flux(const swe_solver* s) { return s->fused_ok && s->fused_on; }
 
 // One stage: ComputeInterfaceValues, ComputeFluxes, update loop.
 int run_stage(swe_solver* s, int in, int u0, int out, double a0, double a1, double coef,
@@ -153,8 +157,13 @@ int run_stage(swe_solver* s, int in, int u0, int out, double a0, double a1, doub
   } else {
     SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->rbits + slot, s->flux_items, s->flux_block, s->stream), "flux");
     s->flux_stale = false;
-    SWE_CHECK(launch_drain(s->M, S, s->W, s->opts, s->stream), "drain");
-    SWE_CHECK(launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->upd_items, s->upd_block, s->stream), "update");
+    if (use_fused_after_flux(s)) {
+      SWE_CHECK(launch_fused(s->M, s->tiles, S, state_of(s, u0), out_of(s, out), s->W, A, s->flux, s->ws, s->opts,
+                             s->stream, false, true), "fused from fluxes");
+    } else {
+      SWE_CHECK(launch_drain(s->M, S, s->W, s->opts, s->stream), "drain");
+      SWE_CHECK(launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->upd_items, s->upd_block, s->stream), "update");
+    }
   }
   s->last_r_slot = slot;
   return SWE_OK;
@@ -745,6 +754,11 @@ static int batch_step(swe_solver* s, int scheme, double cfl, double t_end) {
     SWE_CHECK(launch_recon2(s->M, S, s->W, s->opts, s->stream), "recon2");
     SWE_CHECK(launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, s->B.rbits + q, s->flux_items, s->flux_block, s->stream), "flux");
     if (q == 0) SWE_CHECK(launch_batch_dt(s->B, cfl, t_end, s->stream), "batch_dt");
+    if (use_fused_after_flux(s)) {
+      SWE_CHECK(launch_fused(s->M, s->tiles, S, state_of(s, st[q].u0), out_of(s, st[q].out), s->W, A, s->flux, s->ws,
+                             s->opts, s->stream, false, true), "fused from fluxes");
+      continue;
+    }
     SWE_CHECK(launch_drain(s->M, S, s->W, s->opts, s->stream), "drain");
     SWE_CHECK(launch_update(s->M, S, state_of(s, st[q].u0), out_of(s, st[q].out), s->W, A, s->upd_items, s->upd_block, s->stream),
               "update");
@@ -947,9 +961,9 @@ int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nstep
   } else {
     // bracket each kernel of each stage with its own event pair: recon1,
     // recon2, then flux, drain, update or the fused kernel
-    const int nk = 6;
-    double acc[nk] = {0, 0, 0, 0, 0, 0};
-    int64_t cnt[nk] = {0, 0, 0, 0, 0, 0};
+    const int nk = 7;
+    double acc[nk] = {0, 0, 0, 0, 0, 0, 0};
+    int64_t cnt[nk] = {0, 0, 0, 0, 0, 0, 0};
     std::vector<hipEvent_t> ev(2 * nk, nullptr);
     for (auto& x : ev)
       if (!rc && hipEventCreate(&x) != hipSuccess) rc = fail(SWE_ERR_DEVICE, "hipEventCreate");
@@ -971,7 +985,7 @@ int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nstep
         else { in = c; u0 = a; out = a; a0 = 1. / 3.; a1 = 2. / 3.; coef = 2. / 3.; }
         DevState S = state_of(s, in);
         StageArgs A{a0, a1, coef, dt, cfl, s->cor, s->rbits, st == 0 ? s->dts + k : nullptr, friction(s)};
-        bool used[nk] = {false, false, false, false, false, false};
+        bool used[nk] = {false, false, false, false, false, false, false};
         hipError_t e = hipSuccess;
         auto timed = [&](int q, auto&& launch) {
           if (e == hipSuccess) e = hipEventRecord(ev[size_t(2 * q)], s->stream);
@@ -995,11 +1009,18 @@ int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nstep
                                s->stream);
           });
           s->flux_stale = false;
-          timed(3, [&] { return launch_drain(s->M, S, s->W, s->opts, s->stream); });
-          timed(4, [&] {
-            return launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->upd_items, s->upd_block,
-                                 s->stream);
-          });
+          if (use_fused_after_flux(s)) {
+            timed(6, [&] {
+              return launch_fused(s->M, s->tiles, S, state_of(s, u0), out_of(s, out), s->W, A, s->flux, s->ws,
+                                  s->opts, s->stream, false, true);
+            });
+          } else {
+            timed(3, [&] { return launch_drain(s->M, S, s->W, s->opts, s->stream); });
+            timed(4, [&] {
+              return launch_update(s->M, S, state_of(s, u0), out_of(s, out), s->W, A, s->upd_items, s->upd_block,
+                                   s->stream);
+            });
+          }
         }
         if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
         if (e != hipSuccess) { rc = dev_fail(e, "kernel"); break; }
@@ -1189,10 +1210,12 @@ int swe_get_layout(const swe_solver* s, int* block) {
 // The layout dimension of the autotuner: for each candidate layout the
 // work-item and workgroup variants are tuned (swe_autotune), then one stage
 // of each kind the solver runs is timed on the live state with them: the
-// separate reconstruction / flux / draining / update stage, and for a
-// single mesh that fuses its later stages also the reconstruction + fused
-// stage.  The layout (with its variants) of the least cost, separate stage
-// + (fused or separate) stage (an SSP-RK2 step in CFL mode), is kept.
+// first stage of a CFL step (reconstruction, flux pass, then the tile kernel
+// reading the fluxes, or the separate draining and update kernels where no
+// tiles are built), and for a solver that fuses its later stages also the
+// reconstruction + fused stage.  The layout (with its variants) of the least
+// cost, first stage + (fused or first-stage kind) stage (an SSP-RK2 step in
+// CFL mode), is kept.
 int swe_autotune_layout(swe_solver* s, int reps, int* block, double* cost_ms) {
   if (!s) return fail(SWE_ERR_ARG, "null solver");
   if (reps < 1) return fail(SWE_ERR_ARG, "reps must be >= 1");
@@ -1220,8 +1243,10 @@ int swe_autotune_layout(swe_solver* s, int reps, int* block, double* cost_ms) {
       if (e == hipSuccess) e = launch_recon2(s->M, S, s->W, s->opts, s->stream);
       if (e == hipSuccess)
         e = launch_flux(s->M, S, s->W, s->flux, s->ws, s->cor, rslot, s->flux_items, s->flux_block, s->stream);
-      if (e == hipSuccess) e = launch_drain(s->M, S, s->W, s->opts, s->stream);
-      if (e == hipSuccess) e = launch_update(s->M, S, S, O, s->W, A, s->upd_items, s->upd_block, s->stream);
+      if (e == hipSuccess && fused)
+        e = launch_fused(s->M, s->tiles, S, S, O, s->W, A, s->flux, s->ws, s->opts, s->stream, false, true);
+      if (e == hipSuccess && !fused) e = launch_drain(s->M, S, s->W, s->opts, s->stream);
+      if (e == hipSuccess && !fused) e = launch_update(s->M, S, S, O, s->W, A, s->upd_items, s->upd_block, s->stream);
       return e;
     };
     auto fus = [&]() -> hipError_t {

@@ -17,6 +17,13 @@
 // CFL step of an earlier flux pass of the same step (stages 2 and 3).  The
 // pass's own CFL length-to-wavespeed minimum is not taken: the solver
 // recomputes it with the fluxes when a getter asks for them.
+//
+// The first stage of a CFL step learns its step size from its own flux pass,
+// so the separate flux kernel runs first; the FF instance of this kernel
+// then loads the listed edges' fluxes from memory instead of evaluating them
+// (the same values: the separate flux kernel is the same device functions
+// in the same orientation) and runs the draining and update phases as
+// usual, in place of the separate draining and update kernels.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -45,7 +52,7 @@ __device__ __forceinline__ double stage_dt_fused(const StageArgs& A) {
 constexpr int kFusedThreads = SWE_FUSED_THREADS;
 static_assert(kFusedThreads == kTileCells || kFusedThreads == 2 * kTileCells, "threads per tile");
 
-template <int FLUX, int WS, bool LEN, bool NT>
+template <int FLUX, int WS, bool LEN, bool NT, bool FF>
 __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(DevMesh M, DevTiles P, DevState S, DevState U0,
                                                       DevOut O, DevWork W, StageArgs A, int64_t t0, int64_t t1,
                                                       int store) {
@@ -87,6 +94,77 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
     for (int k = 0; k < 3; ++k) cdr[k] = P.ring_code[3 * (rg0 + tid) + k];
   }
 
+  // the own cell's update and draining inputs, and the ring cell's
+  // draining inputs (loaded before the flux phase by the FF instance, whose
+  // flux phase is loads only; after it by the evaluating one, whose flux
+  // phase needs the registers)
+  RhsCell q;
+  double w, u, v, w0 = 0., u0 = 0., v0 = 0., bci, areai;
+  const bool comb = A.a0 != 0.;
+  auto load_own = [&]() {
+    q.fl = fli;
+    q.mu = load_muscl(S, W, i);
+    q.wi = S.w[S.at(i)];
+    q.p[0] = node(M, tpi[0]);
+    q.p[1] = node(M, tpi[1]);
+    q.p[2] = node(M, tpi[2]);
+    w = S.w[S.at(i)];
+    u = S.u[S.at(i)];
+    v = S.v[S.at(i)];
+    if (comb) {
+      w0 = U0.w[U0.at(i)];
+      u0 = U0.u[U0.at(i)];
+      v0 = U0.v[U0.at(i)];
+    }
+    bci = M.bc[i];
+    areai = M.area[i];
+  };
+  double hr = 0., arear = 0.;
+  uint32_t flr = 0u;
+  V3 rp0{0., 0., 0.}, rp1{0., 0., 0.}, rp2{0., 0., 0.};
+  auto load_ring = [&]() {
+    if (lr < nl) {
+      if (LEN) {
+        rp0 = node(M, M.tp[xr]);
+        rp1 = node(M, M.tp[T + xr]);
+        rp2 = node(M, M.tp[2 * T + xr]);
+      }
+      hr = S.w[S.at(xr)] - M.bc[xr];
+      flr = M.cflag[xr];
+      arear = M.area[xr];
+    }
+  };
+  if constexpr (FF) {
+    load_own();
+    load_ring();
+    // fluxes of the stage's flux pass, written by the separate flux kernel;
+    // ne <= kTileMaxEval: at most kFE evaluations per thread, every load of
+    // them issued before the first LDS write
+    constexpr int kFE = (kTileMaxEval + kFusedThreads - 1) / kFusedThreads;
+    int ev[kFE];
+#pragma unroll
+    for (int it = 0; it < kFE; ++it) {
+      const int j = tid + it * kFusedThreads;
+      ev[it] = j < ne ? P.evedge[e0 + j] : -1;
+    }
+    double f0[kFE], f1[kFE], f2[kFE];
+#pragma unroll
+    for (int it = 0; it < kFE; ++it) {
+      const int e = ev[it] < 0 ? 0 : ev[it];
+      f0[it] = W.f0[e];
+      f1[it] = W.f1[e];
+      f2[it] = W.f2[e];
+    }
+#pragma unroll
+    for (int it = 0; it < kFE; ++it) {
+      const int j = tid + it * kFusedThreads;
+      if (j < ne) {
+        sF0[j] = f0[it];
+        sF1[j] = f1[it];
+        sF2[j] = f2[it];
+      }
+    }
+  } else {
   // the next evaluation's edge record is loaded before this one's gathers
   EdgeRec rn = tid < ne ? P.evrec[e0 + tid] : EdgeRec{0, 0, 0, 0};
   for (int j = tid; j < ne; j += kFusedThreads) {
@@ -125,33 +203,14 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
       out_store<NT>(&W.f2[e], F[2]);
     }
   }
+  }
   __syncthreads();
 
   // ---- draining time steps of every local cell (k_drain's operations).
   // The thread's own cell (tid < nc) is also the cell it updates: its
   // update inputs are loaded here with its draining inputs, so one latency
   // covers both phases' loads.
-  RhsCell q;
-  double w, u, v, w0 = 0., u0 = 0., v0 = 0., bci, areai;
-  const bool comb = A.a0 != 0.;
-  {
-    q.fl = fli;
-    q.mu = load_muscl(S, W, i);
-    q.wi = S.w[S.at(i)];
-    q.p[0] = node(M, tpi[0]);
-    q.p[1] = node(M, tpi[1]);
-    q.p[2] = node(M, tpi[2]);
-    w = S.w[S.at(i)];
-    u = S.u[S.at(i)];
-    v = S.v[S.at(i)];
-    if (comb) {
-      w0 = U0.w[U0.at(i)];
-      u0 = U0.u[U0.at(i)];
-      v0 = U0.v[U0.at(i)];
-    }
-    bci = M.bc[i];
-    areai = M.area[i];
-  }
+  if constexpr (!FF) load_own();
   // the draining step of a cell from its depth, edges and flux slots
   auto drain_dt = [&](double h, uint32_t fl, const uint32_t* cd, const V3& p0, const V3& p1, const V3& p2,
                       double area) -> double {
@@ -171,15 +230,8 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
     sD[tid] = d;
     if (store) out_store<NT>(&W.dtd[i], d);
   }
-  if (lr < nl) {
-    V3 p0, p1, p2;
-    if (LEN) {
-      p0 = node(M, M.tp[xr]);
-      p1 = node(M, M.tp[T + xr]);
-      p2 = node(M, M.tp[2 * T + xr]);
-    }
-    sD[lr] = drain_dt(S.w[S.at(xr)] - M.bc[xr], M.cflag[xr], cdr, p0, p1, p2, M.area[xr]);
-  }
+  if constexpr (!FF) load_ring();
+  if (lr < nl) sD[lr] = drain_dt(hr, flr, cdr, rp0, rp1, rp2, arear);
   __syncthreads();
 
   // ---- update of the tile's cells (k_update's operations; the fluxes and
@@ -239,7 +291,7 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
 
 hipError_t launch_fused(const DevMesh& M, const DevTiles& P, const DevState& S, const DevState& U0,
                         const DevOut& O, const DevWork& W, const StageArgs& A, int flux, int ws, int opts,
-                        hipStream_t st, bool store) {
+                        hipStream_t st, bool store, bool from_fluxes) {
   // the tiles that hold cells of the visited window [c0, c1): the whole mesh,
   // or an ensemble's running members (cells of finished members in the
   // first and last tile are copied unchanged, as every finished member's)
@@ -248,12 +300,20 @@ hipError_t launch_fused(const DevMesh& M, const DevTiles& P, const DevState& S, 
   if (t1 <= t0) return hipGetLastError();
   const dim3 g = grid_for_bs(t1 - t0, 1), b(kFusedThreads);
   const bool len = (opts & kOptDrainLength) != 0;
+  const int sto = store ? 1 : 0;
+  if (from_fluxes) {  // the flux scheme does not enter: one instance per drain / store variant
+    if (len && M.nt) hipLaunchKernelGGL((k_fused<0, 0, true, true, true>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1, sto);
+    else if (len) hipLaunchKernelGGL((k_fused<0, 0, true, false, true>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1, sto);
+    else if (M.nt) hipLaunchKernelGGL((k_fused<0, 0, false, true, true>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1, sto);
+    else hipLaunchKernelGGL((k_fused<0, 0, false, false, true>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1, sto);
+    return hipGetLastError();
+  }
 #define SWE_FUSED(F, Wv)                                                                             \
   if (flux == F && ws == Wv) {                                                                       \
-    if (len && M.nt) hipLaunchKernelGGL((k_fused<F, Wv, true, true>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1, store ? 1 : 0);   \
-    else if (len) hipLaunchKernelGGL((k_fused<F, Wv, true, false>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1, store ? 1 : 0);     \
-    else if (M.nt) hipLaunchKernelGGL((k_fused<F, Wv, false, true>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1, store ? 1 : 0);    \
-    else hipLaunchKernelGGL((k_fused<F, Wv, false, false>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1, store ? 1 : 0);             \
+    if (len && M.nt) hipLaunchKernelGGL((k_fused<F, Wv, true, true, false>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1, sto);   \
+    else if (len) hipLaunchKernelGGL((k_fused<F, Wv, true, false, false>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1, sto);     \
+    else if (M.nt) hipLaunchKernelGGL((k_fused<F, Wv, false, true, false>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1, sto);    \
+    else hipLaunchKernelGGL((k_fused<F, Wv, false, false, false>), g, b, 0, st, M, P, S, U0, O, W, A, t0, t1, sto);             \
     return hipGetLastError();                                                                        \
   }
   SWE_FUSED(0, 0) SWE_FUSED(0, 1) SWE_FUSED(0, 2) SWE_FUSED(1, 0) SWE_FUSED(1, 1) SWE_FUSED(1, 2)

@@ -272,11 +272,13 @@ hipError_t launch_adjoint_finish(const DevMesh& M, const DevAdj& J, double* zbar
 // step size must be known (A.dt_fixed > 0, or the CFL r of an earlier pass
 // in A.rbits[0], or an ensemble's per-member steps).  W's fluxes and
 // draining time steps are written only with store (each edge's flux and each
-// cell's draining step once); this pass's CFL minimum never is.
+// cell's draining step once); this pass's CFL minimum never is.  With
+// from_fluxes the stage's fluxes are already in W (the separate flux pass
+// of a CFL step's first stage) and are read instead of evaluated.
 struct DevTiles;
 hipError_t launch_fused(const DevMesh& M, const DevTiles& P, const DevState& S, const DevState& U0,
                         const DevOut& O, const DevWork& W, const StageArgs& A, int flux, int ws, int opts,
-                        hipStream_t st, bool store = false);
+                        hipStream_t st, bool store = false, bool from_fluxes = false);
 // The allocation-free integrals keep their block partials in a static
 // per-GPU array of 3 * kScratchBlocks doubles (meshes up to kScratchBlocks *
 // 256 cells); scratch_address gives its device address on the current
