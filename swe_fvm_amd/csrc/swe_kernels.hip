@@ -56,6 +56,7 @@ __global__ void __launch_bounds__(256) k_setup_cells(DevMesh M, DevGeomOut G) {
   r.fl = M.cflag[i];
   r.pad = 0;
   G.crec[i] = r;
+  G.nrec[i] = NRec{r.cx, r.cy, (r.fl & CELL_BND) ? INFINITY : r.zmax, r.bc};
 }
 
 // min over the edge's two cells of 2 area / length (the flux's CFL length)
@@ -155,9 +156,9 @@ __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W
   // every load that does not need the LDS window is issued before the
   // barrier: the cell's own record (which fills its LDS entry), its flags,
   // vertices and neighbour ids, and the neighbours outside the workgroup
-  const CellRec ri = M.crec[is];
+  const NRec ri = M.nrec[is];
   const double w = S.w[S.at(is)];
-  const uint32_t fl = ri.fl;  // the record holds the cell's flags
+  const uint32_t fl = M.cflag[is];
   int jn[3];
   bool inb[3];
   double wn[3];
@@ -171,14 +172,13 @@ __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W
     inb[kk] = lj >= 0 && lj < kBlock && j < M.c1;
     if (!inb[kk]) {  // the owner's workgroup reads the same record: an L2 hit mostly
       wn[kk] = S.w[S.at(j)];
-      const CellRec r = M.crec[j];
-      rn[kk] = NRec{r.cx, r.cy, (r.fl & CELL_BND) ? INFINITY : r.zmax, r.bc};
+      rn[kk] = M.nrec[j];
     }
   }
   const V3 p0 = node(M, M.tp[is]), p1 = node(M, M.tp[M.T + is]), p2 = node(M, M.tp[2 * M.T + is]);
   if (live) {
     s_w[threadIdx.x] = w;
-    s_r[threadIdx.x] = NRec{ri.cx, ri.cy, (ri.fl & CELL_BND) ? INFINITY : ri.zmax, ri.bc};
+    s_r[threadIdx.x] = ri;
   }
   __syncthreads();
   if (!live) return;
@@ -191,18 +191,20 @@ __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W
       rn[kk] = s_r[lj];
     }
   const double bc = ri.bc;
-  const double zmx = ri.zmax, zmn = ri.zmin;
   const double cx = ri.cx, cy = ri.cy;
   double w0 = w, gx = 0., gy = 0.;
+  double zmx = 0.;  // the highest vertex, read where pass 1 or 2 needs it
   int8_t k;
   if (!is_wet(w - bc)) {
     k = 0;
     w0 = bc;
     gradient(p0, p1, p2, gx, gy);
-  } else if ((fl & CELL_BND) || !(zmx < w)) {
+  } else if (!(ri.zmx < w)) {  // a boundary cell (zmx = +inf) or a vertex at or above w
     k = 1;
     // b13 = zmax, b23 = zmin enter partwet1_w only through max / min
-    w0 = partwet1_w(zmn, zmn, zmx, bc, w);
+    const CellRec rf = M.crec[i];
+    zmx = rf.zmax;
+    w0 = partwet1_w(rf.zmin, rf.zmin, zmx, bc, w);
     W.wrec1[i] = w0;
   } else {
     k = 2;

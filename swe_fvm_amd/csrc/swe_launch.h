@@ -12,8 +12,9 @@
 //   static geometry (computed once by the setup kernels with the step's own
 //   formulas, so every use is bit-identical to recomputing it):
 //     cells  cx, cy, bc (centroid x, y, z), area, and an AoS copy
-//            crec {cx, cy, bc, zmin, zmax, cflag} (48 B) for the
-//            reconstruction's neighbour gathers
+//            crec {cx, cy, bc, zmin, zmax, cflag} (48 B) and its 32-byte
+//            part nrec {cx, cy, zmax or +inf on boundary cells, bc} for the
+//            reconstruction's own-cell and neighbour reads
 //     edges  edmin (min over the two cells of 2 area / length; the inside
 //            cell alone on a free-flow edge, 0 on a wall)
 //   Edge lengths, midpoints and tangents are recomputed from px, py, pz.
@@ -84,6 +85,10 @@ struct DevMesh {
   // SWE_OPT_EXACT_BALANCE: the flux and RHS groupings that keep equal edge
   // states at rest to the bit
   int eb;
+  // the 32-byte part of crec the reconstruction reads for every cell and
+  // neighbour (highest vertex +inf on boundary cells); crec itself is read
+  // only where zmin or the true zmax of a boundary cell enter
+  const NRec* nrec;
 };
 
 // per-instance control of a batched ensemble
@@ -101,6 +106,7 @@ struct DevBatch {
 struct DevGeomOut {
   double *cx, *cy, *bc, *area, *edmin;
   CellRec* crec;
+  NRec* nrec;
 };
 
 // Cell-state layout: the w, u, v of cell i sit at w[at(i)], u[at(i)],
