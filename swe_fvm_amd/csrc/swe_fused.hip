@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
   // flux phase is loads only; after it by the evaluating one, whose flux
   // phase needs the registers)
   RhsCell q;
-  double w, u, v, w0 = 0., u0 = 0., v0 = 0., bci, areai;
+  double w, u, v, w0 = 0., u0 = 0., v0 = 0.;
   const bool comb = A.a0 != 0.;
   auto load_own = [&]() {
     q.fl = fli;
@@ -116,8 +116,6 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
       u0 = U0.u[U0.at(i)];
       v0 = U0.v[U0.at(i)];
     }
-    bci = M.bc[i];
-    areai = M.area[i];
   };
   double hr = 0., arear = 0.;
   uint32_t flr = 0u;
@@ -226,7 +224,10 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
     return sum > kTol ? area * h / sum : INFINITY;
   };
   if (mine) {
-    const double d = drain_dt(w - bci, q.fl, code, q.p[0], q.p[1], q.p[2], areai);
+    // the centroid bathymetry and area from the vertices, as the setup
+    // kernel computes M.bc and M.area (the same functions: the same bits)
+    const double d = drain_dt(w - centroid(q.p[0], q.p[1], q.p[2]).z, q.fl, code, q.p[0], q.p[1], q.p[2],
+                              tri_area(q.p[0], q.p[1], q.p[2]));
     sD[tid] = d;
     if (store) out_store<NT>(&W.dtd[i], d);
   }
