@@ -172,7 +172,8 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
     const int tr = t < 0 ? f : t;  // boundary edges read their own cell twice: R = L
     const V3 Pa = node(M, r.ep0), Pb = node(M, r.ep1);
     const Muscl mf = load_muscl(S, W, f), mt = load_muscl(S, W, tr);
-    const double cxf = M.cx[f], cyf = M.cy[f], cxt = M.cx[tr], cyt = M.cy[tr];
+    const CXY cf = M.cxy[f], ct = M.cxy[tr];
+    const double cxf = cf.x, cyf = cf.y, cxt = ct.x, cyt = ct.y;
     const double hf = (t == kSolidWall && !M.eb) ? S.w[S.at(f)] - M.bc[f] : 0.;
     const double le = edge_len(Pa, Pb);
     const V3 E = midpoint(Pa, Pb);

@@ -57,6 +57,7 @@ __global__ void __launch_bounds__(256) k_setup_cells(DevMesh M, DevGeomOut G) {
   r.pad = 0;
   G.crec[i] = r;
   G.nrec[i] = NRec{r.cx, r.cy, (r.fl & CELL_BND) ? INFINITY : r.zmax, r.bc};
+  G.cxy[i] = CXY{c.x, c.y};
 }
 
 // min over the edge's two cells of 2 area / length (the flux's CFL length)
@@ -409,10 +410,11 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, DevState S, DevWork W, d
     q.t = t;
     q.mf = load_muscl(S, W, f);
     q.mt = load_muscl(S, W, tr);
-    q.cxf = M.cx[f];
-    q.cyf = M.cy[f];
-    q.cxt = M.cx[tr];
-    q.cyt = M.cy[tr];
+    const CXY cf = M.cxy[f], ct = M.cxy[tr];
+    q.cxf = cf.x;
+    q.cyf = cf.y;
+    q.cxt = ct.x;
+    q.cyt = ct.y;
     // the cell average enters only the wall flux: walls alone load it
     // (with SWE_OPT_EXACT_BALANCE the wall takes the inside edge depth)
     q.hf = (t == kSolidWall && !M.eb) ? S.w[S.at(f)] - M.bc[f] : 0.;

@@ -14,7 +14,8 @@
 //     cells  cx, cy, bc (centroid x, y, z), area, and an AoS copy
 //            crec {cx, cy, bc, zmin, zmax, cflag} (48 B) and its 32-byte
 //            part nrec {cx, cy, zmax or +inf on boundary cells, bc} for the
-//            reconstruction's own-cell and neighbour reads
+//            reconstruction's own-cell and neighbour reads, and cxy (cx, cy
+//            interleaved) for the flux evaluations
 //     edges  edmin (min over the two cells of 2 area / length; the inside
 //            cell alone on a free-flow edge, 0 on a wall)
 //   Edge lengths, midpoints and tangents are recomputed from px, py, pz.
@@ -43,6 +44,12 @@ struct __attribute__((aligned(16))) CellRec {
 // staged in LDS by k_recon1
 struct __attribute__((aligned(32))) NRec {
   double cx, cy, zmx, bc;
+};
+
+// a cell's centroid (x, y) in one 16-byte load, for the flux evaluations'
+// gathers of both cells of an edge
+struct __attribute__((aligned(16))) CXY {
+  double x, y;
 };
 
 struct DevMesh {
@@ -89,6 +96,8 @@ struct DevMesh {
   // neighbour (highest vertex +inf on boundary cells); crec itself is read
   // only where zmin or the true zmax of a boundary cell enter
   const NRec* nrec;
+  // (cx, cy) interleaved
+  const CXY* cxy;
 };
 
 // per-instance control of a batched ensemble
@@ -107,6 +116,7 @@ struct DevGeomOut {
   double *cx, *cy, *bc, *area, *edmin;
   CellRec* crec;
   NRec* nrec;
+  CXY* cxy;
 };
 
 // Cell-state layout: the w, u, v of cell i sit at w[at(i)], u[at(i)],
