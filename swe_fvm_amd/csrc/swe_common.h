@@ -33,7 +33,8 @@ __device__ __forceinline__ int64_t global_index() {
 }
 
 __device__ __forceinline__ V3 node(const DevMesh& M, int n) {
-  return {M.px[n], M.py[n], M.pz[n]};
+  const NodeRec r = M.pn[n];
+  return {r.x, r.y, r.z};
 }
 __device__ __forceinline__ int ep0_local(uint32_t fl, int k) { return (fl >> (4 + 4 * k)) & 3; }
 __device__ __forceinline__ int ep1_local(uint32_t fl, int k) { return (fl >> (6 + 4 * k)) & 3; }

@@ -40,6 +40,13 @@ namespace swe {
 
 // -------------------------------------------------------------- setup --
 // Static geometry with the formulas of Domain (src/Bathymetry.cpp:24-90).
+// the interleaved node records, before the cell and edge setup reads them
+__global__ void __launch_bounds__(256) k_setup_nodes(DevMesh M, DevGeomOut G) {
+  int64_t n = global_index();
+  if (n >= M.V) return;
+  G.pn[n] = NodeRec{M.px[n], M.py[n], M.pz[n], 0.};
+}
+
 __global__ void __launch_bounds__(256) k_setup_cells(DevMesh M, DevGeomOut G) {
   int64_t i = global_index();
   if (i >= M.T) return;
@@ -974,6 +981,7 @@ hipError_t scratch_address(double** out) {
 // -------------------------------------------------------------- launch --
 
 hipError_t launch_setup_geometry(const DevMesh& M, const DevGeomOut& G, hipStream_t st) {
+  hipLaunchKernelGGL(k_setup_nodes, grid_for(M.V), dim3(256), 0, st, M, G);
   hipLaunchKernelGGL(k_setup_cells, grid_for(M.T), dim3(256), 0, st, M, G);
   hipLaunchKernelGGL(k_setup_edges, grid_for(M.E), dim3(256), 0, st, M, G);
   return hipGetLastError();

@@ -1,7 +1,7 @@
 // Device data layout and kernel launchers of the explicit step.
 //
 // HBM layout (structure of arrays, int32 indices, Morton-ordered cells):
-//   nodes  px, py, pz                     double[V]
+//   nodes  px, py, pz                     double[V], and pn {x, y, z, 0} (32 B)
 //   cells  tp, te, tt                     int32[3][T]   (column k at k*T)
 //          cflag                          uint32[T]
 //            bits 0-2  et0(te[k]) == cell (flux orientation of edge k)
@@ -18,7 +18,7 @@
 //            interleaved) for the flux evaluations
 //     edges  edmin (min over the two cells of 2 area / length; the inside
 //            cell alone on a free-flow edge, 0 on a wall)
-//   Edge lengths, midpoints and tangents are recomputed from px, py, pz.
+//   Edge lengths, midpoints and tangents are recomputed from the nodes.
 //   state  w, u, v                        double[T] per buffer
 //   work   MUSCL (mw, mgx, mgy): pass 1, overwritten in place by pass 2 for
 //          partially wet cells, whose pass-1 level is kept in wrec1 for the
@@ -50,6 +50,11 @@ struct __attribute__((aligned(32))) NRec {
 // gathers of both cells of an edge
 struct __attribute__((aligned(16))) CXY {
   double x, y;
+};
+
+// a node's coordinates (x, y, z) in one aligned 32-byte record
+struct __attribute__((aligned(32))) NodeRec {
+  double x, y, z, pad;
 };
 
 struct DevMesh {
@@ -98,6 +103,8 @@ struct DevMesh {
   const NRec* nrec;
   // (cx, cy) interleaved
   const CXY* cxy;
+  // (px, py, pz) interleaved, kept equal to them by the setup kernels
+  const NodeRec* pn;
 };
 
 // per-instance control of a batched ensemble
@@ -117,6 +124,7 @@ struct DevGeomOut {
   CellRec* crec;
   NRec* nrec;
   CXY* cxy;
+  NodeRec* pn;
 };
 
 // Cell-state layout: the w, u, v of cell i sit at w[at(i)], u[at(i)],
