@@ -325,6 +325,7 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
   ALLOC(G.nrec, T);
   ALLOC(G.cxy, T);
   ALLOC(G.pn, V);
+  ALLOC(G.cv, T);
 #undef ALLOC
 #define H2D(dst, src, n)                                                                     \
   if ((e = hipMemcpy(dst, src, sizeof(*(src)) * size_t(n), hipMemcpyHostToDevice)) != hipSuccess) \
@@ -349,7 +350,7 @@ static int create_impl(const Topo& m, const double* xyz, int flux, int wavespeed
   // ensembles below
   s->M = DevMesh{dpx, dpy, dpz, dtp, dte, dtt, dfl, dep0, dep1, det0, det1, doff, dn2c, T, E, V,
                  G.cx, G.cy, G.bc, G.area, G.edmin, G.crec, nullptr, nullptr, nullptr, 0, T, 0, E,
-                 T >= (int64_t(1) << 20) ? 1 : 0, nullptr, (options & SWE_OPT_EXACT_BALANCE) ? 1 : 0, G.nrec, G.cxy, G.pn};
+                 T >= (int64_t(1) << 20) ? 1 : 0, nullptr, (options & SWE_OPT_EXACT_BALANCE) ? 1 : 0, G.nrec, G.cxy, G.pn, G.cv};
   s->G = G;
   if (!pe.empty()) {
     int* dpe;

@@ -77,13 +77,11 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
   const bool mine = tid < nc;
   const int i = (int)(c0 + (mine ? tid : 0));
   uint32_t code[3];
-  int tpi[3];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    code[k] = P.code[k * T + i];
-    tpi[k] = M.tp[k * T + i];
-  }
-  const uint32_t fli = M.cflag[i];
+  for (int k = 0; k < 3; ++k) code[k] = P.code[k * T + i];
+  const CVtx cvi = M.cv[i];
+  const int tpi[3] = {cvi.v0, cvi.v1, cvi.v2};
+  const uint32_t fli = cvi.fl;
   const int nl = kTileCells + nr;
   const int lr = kTileCells + tid;
   int xr = 0;
@@ -122,13 +120,14 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
   V3 rp0{0., 0., 0.}, rp1{0., 0., 0.}, rp2{0., 0., 0.};
   auto load_ring = [&]() {
     if (lr < nl) {
+      const CVtx cvr = M.cv[xr];
       if (LEN) {
-        rp0 = node(M, M.tp[xr]);
-        rp1 = node(M, M.tp[T + xr]);
-        rp2 = node(M, M.tp[2 * T + xr]);
+        rp0 = node(M, cvr.v0);
+        rp1 = node(M, cvr.v1);
+        rp2 = node(M, cvr.v2);
       }
       hr = S.w[S.at(xr)] - M.bc[xr];
-      flr = M.cflag[xr];
+      flr = cvr.fl;
       arear = M.area[xr];
     }
   };

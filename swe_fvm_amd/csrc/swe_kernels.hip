@@ -65,6 +65,7 @@ __global__ void __launch_bounds__(256) k_setup_cells(DevMesh M, DevGeomOut G) {
   G.crec[i] = r;
   G.nrec[i] = NRec{r.cx, r.cy, (r.fl & CELL_BND) ? INFINITY : r.zmax, r.bc};
   G.cxy[i] = CXY{c.x, c.y};
+  G.cv[i] = CVtx{M.tp[i], M.tp[M.T + i], M.tp[2 * M.T + i], M.cflag[i]};
 }
 
 // min over the edge's two cells of 2 area / length (the flux's CFL length)
@@ -166,7 +167,8 @@ __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W
   // vertices and neighbour ids, and the neighbours outside the workgroup
   const NRec ri = M.nrec[is];
   const double w = S.w[S.at(is)];
-  const uint32_t fl = M.cflag[is];
+  const CVtx cvi = M.cv[is];
+  const uint32_t fl = cvi.fl;
   int jn[3];
   bool inb[3];
   double wn[3];
@@ -183,7 +185,7 @@ __global__ void __launch_bounds__(256) k_recon1(DevMesh M, DevState S, DevWork W
       rn[kk] = M.nrec[j];
     }
   }
-  const V3 p0 = node(M, M.tp[is]), p1 = node(M, M.tp[M.T + is]), p2 = node(M, M.tp[2 * M.T + is]);
+  const V3 p0 = node(M, cvi.v0), p1 = node(M, cvi.v1), p2 = node(M, cvi.v2);
   if (live) {
     s_w[threadIdx.x] = w;
     s_r[threadIdx.x] = ri;

@@ -57,6 +57,12 @@ struct __attribute__((aligned(32))) NodeRec {
   double x, y, z, pad;
 };
 
+// a cell's vertex ids and flags (tp[0..2][i], cflag[i]) in one 16-byte load
+struct __attribute__((aligned(16))) CVtx {
+  int v0, v1, v2;
+  uint32_t fl;
+};
+
 struct DevMesh {
   const double* px;
   const double* py;
@@ -105,6 +111,8 @@ struct DevMesh {
   const CXY* cxy;
   // (px, py, pz) interleaved, kept equal to them by the setup kernels
   const NodeRec* pn;
+  // (tp[0][i], tp[1][i], tp[2][i], cflag[i]) interleaved
+  const CVtx* cv;
 };
 
 // per-instance control of a batched ensemble
@@ -125,6 +133,7 @@ struct DevGeomOut {
   NRec* nrec;
   CXY* cxy;
   NodeRec* pn;
+  CVtx* cv;
 };
 
 // Cell-state layout: the w, u, v of cell i sit at w[at(i)], u[at(i)],
