@@ -397,7 +397,7 @@ def adjoint_measure(B=32, n=354, N=2000, dt=2e-4, checkpoint=50, nfd=16, check_s
         lhs = float(np.sum(ybar * tdot))
         rhs = float(np.sum(gp[:T1] * dp) + np.sum(gz[:V1] * dz))
         dual.append({"tangent": lhs, "adjoint": rhs, "rel": abs(lhs - rhs) / max(abs(lhs), 1e-300)})
-    tangent = {"steps": N, "directions": len(dual), "max_rel": max(r["rel"] for r in dual),
+    tangent = {"steps": N, "directions": len(dual), "max_rel": max((r["rel"] for r in dual), default=None),
                "tangent_run_ms": tl_ms / max(len(dual), 1), "results": dual}
     fd_full = fd_checks(N, f0, gz[:V1], gp[:T1])
     nc = check_steps

@@ -1597,8 +1597,13 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
           if (e == hipSuccess)  // fixed dt: the CFL reduction is not read
             e = launch_flux(s->M, Sq, Wq, s->flux, s->ws, s->cor, s->rbits + 3, s->flux_items, s->flux_block, s->stream,
                             false);
-          if (e == hipSuccess) e = launch_drain(s->M, Sq, Wq, s->opts, s->stream);
-          if (e == hipSuccess) e = launch_update(s->M, Sq, Un, O, Wq, A, s->upd_items, s->upd_block, s->stream);
+          if (use_fused_after_flux(s)) {  // draining (stored) and update in one tile kernel
+            if (e == hipSuccess)
+              e = launch_fused(s->M, s->tiles, Sq, Un, O, Wq, A, s->flux, s->ws, s->opts, s->stream, true, true);
+          } else {
+            if (e == hipSuccess) e = launch_drain(s->M, Sq, Wq, s->opts, s->stream);
+            if (e == hipSuccess) e = launch_update(s->M, Sq, Un, O, Wq, A, s->upd_items, s->upd_block, s->stream);
+          }
         }
         Sq = DevState{O.w, O.u, O.v, O.mask};
       }

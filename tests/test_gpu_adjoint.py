@@ -818,13 +818,15 @@ def test_inversion_batch_against_differences_of_the_oracle():
     assert checked >= 3
 
 
-@pytest.mark.parametrize("scheme", ["Euler", "SSPRK2", "SSPRK3"])
-def test_adjoint_same_with_fused_recomputation(scheme):
+@pytest.mark.parametrize("scheme,n", [("Euler", 24), ("SSPRK2", 24), ("SSPRK3", 24), ("SSPRK2", 730)])
+def test_adjoint_same_with_fused_recomputation(scheme, n):
     """The reverse sweep's recomputation runs the fused kernel storing each
-    stage's fluxes and draining steps (one writer per edge and cell), or the
+    stage's fluxes and draining steps (one writer per edge and cell; below
+    2^20 cells), the separate flux kernel and then the tile kernel storing
+    the draining steps (above: n = 730 gives 1,065,800 cells), or the
     separate flux / draining / update kernels: the gradients are the same
     bit for bit, on a wet/dry mesh whose draining steps are active."""
-    xyz, topo, prim = C.wet_dry_mesh(24, seed=5)
+    xyz, topo, prim = C.wet_dry_mesh(n, seed=5)
     dom = swe.Domain(xyz, topo)
     z = xyz[:, 2].copy()
     rng = np.random.default_rng(4)
@@ -834,9 +836,12 @@ def test_adjoint_same_with_fused_recomputation(scheme):
     for fused in (True, False):
         sd = swe.SpaceDisc(swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt], dom, prim, cor=0.2)
         assert sd.set_fused(fused) == fused
-        J, s0, zb, _ = AD.run_and_adjoint(sd, prim, z, 7, fn, scheme=scheme, dt=2e-3, checkpoint_every=3)
+        # dt 2e-3 at n = 24, scaled with the cell size
+        J, s0, zb, _ = AD.run_and_adjoint(sd, prim, z, 7, fn, scheme=scheme, dt=0.048 / n, checkpoint_every=3)
         out.append((J, s0, zb, sd.GetVolField()))
     (Ja, sa, za, fa), (Jb, sb, zb_, fb) = out
-    print(f"{scheme}: fused recomputation bitwise={np.array_equal(sa, sb) and np.array_equal(za, zb_)}", flush=True)
+    assert np.all(np.isfinite(sa)) and np.all(np.isfinite(za))
+    print(f"{scheme} T={prim.shape[0]}: fused recomputation bitwise={np.array_equal(sa, sb) and np.array_equal(za, zb_)}",
+          flush=True)
     assert Ja == Jb and np.array_equal(fa, fb)
     assert np.array_equal(sa, sb) and np.array_equal(za, zb_)
