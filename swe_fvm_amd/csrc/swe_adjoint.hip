@@ -176,10 +176,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   if (i64 >= M.T) return;
   const int i = (int)i64;
   const int64_t T = M.T;
-  const uint32_t fl = M.cflag[i];
+  const CVtx cvi = M.cv[i];
+  const uint32_t fl = cvi.fl;
   const int cls = W.cls[i];  // loads come before the first store (see below)
   const Muscl mu = load_muscl(S, W, i);
-  const V3 p[3] = {node(M, M.tp[i]), node(M, M.tp[T + i]), node(M, M.tp[2 * T + i])};
+  const V3 p[3] = {node(M, cvi.v0), node(M, cvi.v1), node(M, cvi.v2)};
   const V3 c = centroid(p[0], p[1], p[2]);
   const double bc = c.z;
   const double i_area = 1. / tri_area(p[0], p[1], p[2]);
@@ -475,20 +476,22 @@ __global__ void __launch_bounds__(256) k_adj_flux(DevMesh M, DevState S, DevWork
   if (t == kSolidWall) {
     // ElemFlux(n, (h, 0, 0)) = (0, h^2/2 n) of the cell average: the depth
     // adjoint travels in side 0's w slot
+    const CXY cf = M.cxy[f];
     double tx, ty;
-    tangent(Pa, Pb, le, V3{M.cx[f], M.cy[f], 0.}, tx, ty);
+    tangent(Pa, Pb, le, V3{cf.x, cf.y, 0.}, tx, ty);
     const double nx = ty, ny = -tx;
     const double hf = S.w[f] - M.bc[f];
     if (is_wet(hf)) g[0] = (Fb[1] * nx + Fb[2] * ny) * hf;
   } else {
     const int tr = t < 0 ? f : t;
     const Muscl mf = load_muscl(S, W, f), mt = load_muscl(S, W, tr);
+    const CXY cf = M.cxy[f], ct = M.cxy[tr];
     const V3 E = midpoint(Pa, Pb);
     double tx, ty;
-    tangent(Pa, Pb, le, V3{M.cx[f], M.cy[f], 0.}, tx, ty);
+    tangent(Pa, Pb, le, V3{cf.x, cf.y, 0.}, tx, ty);
     const double nx = ty, ny = -tx;
-    const Slot L = slot_value(mf, M.cx[f], M.cy[f], E);
-    const Slot R = slot_value(mt, M.cx[tr], M.cy[tr], E);
+    const Slot L = slot_value(mf, cf.x, cf.y, E);
+    const Slot R = slot_value(mt, ct.x, ct.y, E);
     // reverse mode of the Riemann solver (swe_flux_adj.h); the edge
     // bathymetry enters it only through hl = L.w - z and hr = R.w - z
     // (include/Fluxes.h:18-21,63-66): dF/dz = -(dF/dL.w + dF/dR.w)
@@ -592,7 +595,8 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
   if (i64 >= M.T) return;
   const int i = (int)i64;
   const int64_t T = M.T, Ecnt = M.E;
-  const uint32_t fl = M.cflag[i];
+  const CVtx cvi = M.cv[i];
+  const uint32_t fl = cvi.fl;
   const int cls = W.cls[i];
   // the update's gradient adjoint (its level and velocity parts it applied
   // itself); partially wet cells go through k_adj_pw's result
@@ -624,8 +628,9 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
     wb += dhi;
     bcb -= dhi;
   }
-  const V3 p0 = node(M, M.tp[i]), p1 = node(M, M.tp[T + i]), p2 = node(M, M.tp[2 * T + i]);
-  const double cxi = M.cx[i], cyi = M.cy[i];
+  const V3 p0 = node(M, cvi.v0), p1 = node(M, cvi.v1), p2 = node(M, cvi.v2);
+  const CXY ci = M.cxy[i];
+  const double cxi = ci.x, cyi = ci.y;
   {
     // the slot values of the cell's three edges (AtPoint at the midpoints)
     const Muscl mu = load_muscl(S, W, i);
@@ -703,24 +708,25 @@ __global__ void __launch_bounds__(256) k_adj_recon(DevMesh M, DevState S, DevWor
       bool any_dry = false;
       for (int k = 0; k < 3; ++k) {
         const int j = M.tt[k * T + i];  // full-wet cells are interior
-        const CellRec rj = M.crec[j];
+        const NRec rj = M.nrec[j];  // highest vertex +inf on boundary cells
         const double wj = S.w[j];
         E[k] = edge_mid(p0, p1, p2, fl, k);
         kind[k] = 0;
-        if (!(rj.fl & CELL_BND) && rj.zmax < wj) {
+        if (rj.zmx < wj) {
           pts[k] = {rj.cx, rj.cy, wj};
         } else if (!is_wet(wj - rj.bc)) {
           any_dry = true;
           pts[k] = {0., 0., 0.};
         } else {
           kind[k] = 1;
-          const double w0j = partwet1_w(rj.zmin, rj.zmin, rj.zmax, rj.bc, wj);
+          const CellRec rf = M.crec[j];  // partially wet neighbour: the full record
+          const double w0j = partwet1_w(rf.zmin, rf.zmin, rf.zmax, rf.bc, wj);
           const double dx = E[k].x - rj.cx, dy = E[k].y - rj.cy;
           double a = w0j + (0. * dx + 0. * dy);
           if (!((a - E[k].z) >= 0)) {
             a = E[k].z;
             kind[k] = 2;  // clamped to the edge bathymetry
-          } else if (!(wj >= rj.zmax)) {
+          } else if (!(wj >= rf.zmax)) {
             kind[k] = 3;  // cube-root / bisection level of the neighbour
           }
           pts[k] = {E[k].x, E[k].y, 0.5 * (w + a)};
