@@ -66,25 +66,27 @@ def kernel_own_bytes(T, E, V):
     indices, FP64 values), per launch: the kernel's own implementation
     arrays, intermediates included (not the compulsory bytes of a
     cell-update, which the roofline uses)."""
-    nodes = 24.0 * V
+    nodes = 32.0 * V  # the interleaved node records
     return {
-        # w, tp, tt, cflag, nodes in; MUSCL (w0, gx, gy) + class out
-        "recon1": T * (8 + 12 + 12 + 4) + nodes + T * (24 + 1),
+        # w, 32-byte record, vertex ids + flags, neighbour ids, nodes in;
+        # MUSCL (w0, gx, gy) + class out
+        "recon1": T * (8 + 32 + 16 + 12) + nodes + T * (24 + 1),
         # class of every cell (pass 2 itself is sparse)
         "recon2": T * 1.0,
-        # edge ends/cells, per-cell MUSCL + class + w, u, v + tp, nodes in; F out
-        "flux": E * 16 + T * (24 + 1 + 24 + 12) + nodes + E * 24,
+        # edge ends/cells, CFL length, per-cell MUSCL + class + u, v +
+        # centroid, nodes in; F out
+        "flux": E * (16 + 8) + T * (24 + 1 + 16 + 16) + nodes + E * 24,
         # w, tp, te, cflag, F0, nodes in; dtd out
         "drain": T * (8 + 12 + 12 + 4) + E * 8 + nodes + T * 8,
         # state (+ U0), tp, te, tt, cflag, MUSCL, class, F, dtd, nodes in; state out
         "update": T * (24 + 24 + 36 + 4 + 24 + 1 + 8) + E * 24 + nodes + T * 24,
-        # state (+ U0), tp, tile codes, cflag, MUSCL, class, centroids, bc,
-        # area, edmin, nodes in; state out (fluxes and draining steps stay
-        # in LDS)
-        "fused": T * (24 + 24 + 12 + 12 + 4 + 24 + 1 + 16 + 8 + 8) + E * 8 + nodes + T * 24,
+        # state (+ U0), vertex ids + flags, tile codes, MUSCL, class,
+        # centroids, edge records, nodes in; state out (fluxes and draining
+        # steps stay in LDS)
+        "fused": T * (24 + 24 + 16 + 12 + 24 + 1 + 16) + E * 16 + nodes + T * 24,
         # as the fused kernel without the per-edge MUSCL / centroid gathers:
-        # the tile's listed edges' fluxes (one edge index and 24 B each) in
-        "fused_ff": T * (24 + 24 + 12 + 12 + 4 + 24 + 1 + 16) + E * (4 + 24) + nodes + T * 24,
+        # each edge's flux (24 B) and evaluation index in
+        "fused_ff": T * (24 + 24 + 16 + 12 + 24 + 1) + E * (4 + 24) + nodes + T * 24,
     }
 
 
