@@ -95,3 +95,35 @@ def test_fused_state_after_interface_getters():
         x.run("SSPRK2", 5, dt=0.0, cfl=0.15)
     assert np.array_equal(a.GetVolField(), b.GetVolField())
     assert np.array_equal(a.GetFluxes(), b.GetFluxes())
+
+
+@pytest.mark.parametrize("scheme", ["Euler", "SSPRK2"])
+@pytest.mark.parametrize("fused", [True, False])
+def test_timed_run_matches_run_and_names_its_kernels(scheme, fused):
+    """swe_time_run launches a stage's kernels itself, each between events:
+    the state and step sizes after it equal those of swe_run bit for bit,
+    and the per-kernel times are non-zero exactly for the kernels the path
+    runs (CFL mode: the flux pass then the tile kernel reading its fluxes in
+    a step's first stage, the fused kernel in the second; without tiles the
+    separate draining and update kernels)."""
+    xyz, topo, prim = C.wet_dry_mesh(32, seed=24)
+    fl = swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt]
+    dom = swe.Domain(xyz, topo)
+    a = swe.SpaceDisc(fl, dom, prim, cor=0.1)
+    b = swe.SpaceDisc(fl, dom, prim, cor=0.1)
+    for x in (a, b):
+        assert x.set_fused(fused) == fused
+    a.run(scheme, 4, dt=0.0, cfl=0.15)
+    ms, k = b.time_run(scheme, 4, dt=0.0, cfl=0.15, per_kernel=True)
+    assert ms > 0 and k.shape == (7,)
+    assert np.array_equal(a.GetVolField(), b.GetVolField())
+    assert np.array_equal(a.dts(4), b.dts(4))
+    names = ["recon1", "recon2", "flux", "drain", "update", "fused", "fused_ff"]
+    ran = {n for n, v in zip(names, k) if v > 0}
+    if not fused:
+        want = {"recon1", "recon2", "flux", "drain", "update"}
+    elif scheme == "Euler":
+        want = {"recon1", "recon2", "flux", "fused_ff"}
+    else:
+        want = {"recon1", "recon2", "flux", "fused_ff", "fused"}
+    assert ran == want, (ran, k)
