@@ -268,13 +268,14 @@ int swe_time_run(swe_solver* s, int scheme, double dt, double cfl, int64_t nstep
  * a fixed-order tree (identical run to run); per_cell (optional, 3 T
  * doubles) receives the cell terms.  Single-mesh handles only. */
 int swe_integrals(swe_solver* s, double* out3, double* per_cell);
-/* Switches the fused flux-draining-update kernel off (on = 0) or back on
- * (no reference counterpart).  When on (the default, on meshes whose tiles
- * fit the kernel), every stage whose step size is known before its flux
- * pass (fixed dt; stages 2 and 3 of a CFL step) runs as reconstruction +
- * one fused kernel instead of reconstruction + flux + draining + update;
- * results are identical bit for bit.  *active (may be null) receives
- * whether the kernel will be used. */
+/* Switches the tile kernel off (on = 0) or back on (no reference
+ * counterpart).  When on (the default, on meshes whose tiles fit the
+ * kernel), every stage whose step size is known before its flux pass (fixed
+ * dt; stages 2 and 3 of a CFL step) runs as reconstruction + one fused
+ * kernel instead of reconstruction + flux + draining + update, and the first
+ * stage of a CFL step as reconstruction + flux + one kernel for the
+ * draining and the update; results are identical bit for bit.  *active (may
+ * be null) receives whether the kernel will be used. */
 int swe_set_fused(swe_solver* s, int on, int* active);
 /* Allocation-free swe_integrals.  No counterpart in the reference: for
  * callers that cannot allocate device memory (code under HIP graph capture,
