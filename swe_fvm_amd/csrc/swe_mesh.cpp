@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <fstream>
 #include <map>
 #include <sstream>
@@ -334,16 +335,20 @@ void struct_mesh(int64_t ni, int64_t nj, double h, double x0, double y0, std::ve
 }
 
 // Gmsh 4.1 ASCII: $Nodes blocks (tags then coordinates), $Elements blocks
-// (type 1 lines, type 2 triangles).  Node index = rank of its tag.
-int read_msh(const char* path, std::vector<double>& xyz, std::vector<int64_t>& tri,
-             std::vector<int64_t>& bnd) {
-  std::ifstream in(path);
-  if (!in) return fail(SWE_ERR_ARG, std::string("cannot open ") + path);
+// (type 1 lines, type 2 triangles).  Node index = rank of its tag.  Every
+// header and block count is checked against the section's totals and each
+// section must end with its $End marker, so a truncated or malformed file is
+// an error, not a partial mesh.
+static bool end_marker(std::istream& in, const char* want) {
+  std::string s;
+  in >> s;
+  return bool(in) && s == want;
+}
+static int read_msh_sections(std::istream& in, std::map<int64_t, std::array<double, 3>>& nodes,
+                             std::vector<std::pair<int64_t, int64_t>>& lines,
+                             std::vector<std::array<int64_t, 3>>& tris) {
   std::string line;
-  std::map<int64_t, std::array<double, 3>> nodes;
-  std::vector<std::pair<int64_t, int64_t>> lines;
-  std::vector<std::array<int64_t, 3>> tris;
-  bool fmt_ok = false;
+  bool fmt_ok = false, have_nodes = false, have_elements = false;
   while (std::getline(in, line)) {
     if (line.rfind("$MeshFormat", 0) == 0) {
       std::getline(in, line);
@@ -353,11 +358,15 @@ int read_msh(const char* path, std::vector<double>& xyz, std::vector<int64_t>& t
         return fail(SWE_ERR_ARG, "binary Gmsh files are not read");
       fmt_ok = true;
     } else if (line.rfind("$Nodes", 0) == 0) {
-      int64_t nblocks, nn, mn, mx;
+      int64_t nblocks = -1, nn = -1, mn = 0, mx = 0;
       in >> nblocks >> nn >> mn >> mx;
+      if (!in || nblocks < 0 || nn < 0) return fail(SWE_ERR_ARG, "malformed $Nodes header");
+      int64_t got = 0;
       for (int64_t b = 0; b < nblocks; ++b) {
-        int64_t ed, et, par, cnt;
+        int64_t ed = -1, et = 0, par = -1, cnt = -1;
         in >> ed >> et >> par >> cnt;
+        if (!in || ed < 0 || ed > 3 || par < 0 || cnt < 0 || cnt > nn - got)
+          return fail(SWE_ERR_ARG, "malformed $Nodes block");
         std::vector<int64_t> tags(static_cast<size_t>(cnt));
         for (auto& t : tags) in >> t;
         for (int64_t k = 0; k < cnt; ++k) {
@@ -369,13 +378,22 @@ int read_msh(const char* path, std::vector<double>& xyz, std::vector<int64_t>& t
           }
           nodes[tags[size_t(k)]] = p;
         }
+        if (!in) return fail(SWE_ERR_ARG, "truncated $Nodes section");
+        got += cnt;
       }
+      if (got != nn) return fail(SWE_ERR_ARG, "$Nodes blocks do not add up to the declared node count");
+      if (!end_marker(in, "$EndNodes")) return fail(SWE_ERR_ARG, "truncated $Nodes section (no $EndNodes)");
+      have_nodes = true;
     } else if (line.rfind("$Elements", 0) == 0) {
-      int64_t nblocks, ne, mn, mx;
+      int64_t nblocks = -1, ne = -1, mn = 0, mx = 0;
       in >> nblocks >> ne >> mn >> mx;
+      if (!in || nblocks < 0 || ne < 0) return fail(SWE_ERR_ARG, "malformed $Elements header");
+      int64_t got = 0;
       for (int64_t b = 0; b < nblocks; ++b) {
-        int64_t ed, et, typ, cnt;
+        int64_t ed = -1, et = 0, typ = -1, cnt = -1;
         in >> ed >> et >> typ >> cnt;
+        if (!in || ed < 0 || ed > 3 || cnt < 0 || cnt > ne - got)
+          return fail(SWE_ERR_ARG, "malformed $Elements block");
         int nper = (typ == 1) ? 2 : (typ == 2) ? 3 : (typ == 15) ? 1 : (typ == 3) ? 4 : (typ == 4) ? 4 : -1;
         if (nper < 0) return fail(SWE_ERR_ARG, "unsupported Gmsh element type");
         for (int64_t k = 0; k < cnt; ++k) {
@@ -385,11 +403,33 @@ int read_msh(const char* path, std::vector<double>& xyz, std::vector<int64_t>& t
           if (typ == 1) lines.push_back({v[0], v[1]});
           else if (typ == 2) tris.push_back({v[0], v[1], v[2]});
         }
+        if (!in) return fail(SWE_ERR_ARG, "truncated $Elements section");
+        got += cnt;
       }
+      if (got != ne) return fail(SWE_ERR_ARG, "$Elements blocks do not add up to the declared element count");
+      if (!end_marker(in, "$EndElements"))
+        return fail(SWE_ERR_ARG, "truncated $Elements section (no $EndElements)");
+      have_elements = true;
     }
     if (!in) break;
   }
   if (!fmt_ok) return fail(SWE_ERR_ARG, "missing $MeshFormat");
+  if (!have_nodes || !have_elements) return fail(SWE_ERR_ARG, "missing $Nodes or $Elements section");
+  return SWE_OK;
+}
+
+int read_msh(const char* path, std::vector<double>& xyz, std::vector<int64_t>& tri,
+             std::vector<int64_t>& bnd) {
+  std::ifstream in(path);
+  if (!in) return fail(SWE_ERR_ARG, std::string("cannot open ") + path);
+  std::map<int64_t, std::array<double, 3>> nodes;
+  std::vector<std::pair<int64_t, int64_t>> lines;
+  std::vector<std::array<int64_t, 3>> tris;
+  try {
+    if (int rc = read_msh_sections(in, nodes, lines, tris)) return rc;
+  } catch (const std::exception& e) {  // e.g. an allocation for counts no memory holds
+    return fail(SWE_ERR_ARG, std::string("malformed Gmsh file: ") + e.what());
+  }
   std::unordered_map<int64_t, int64_t> idx;
   xyz.clear();
   int64_t k = 0;

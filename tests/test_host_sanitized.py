@@ -1,0 +1,43 @@
+"""The host-only C++ of the library (the mesh generators, the Gmsh reader,
+the topology builder and checks, swe_mesh.cpp; the fused kernel's tile
+tables, swe_tiles.cpp) and the tile checker, built with g++ under
+AddressSanitizer and UBSan and driven by tests/host_asan_driver.py in a
+process of its own: generated, structured and file meshes, a refused cell
+order, and the error paths (indices out of range, bad periodic pairs,
+missing and truncated files) run without a sanitizer report."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _runtime(name):
+    res = subprocess.run(["g++", f"-print-file-name={name}"], capture_output=True, text=True)
+    path = res.stdout.strip()
+    return path if res.returncode == 0 and os.path.isabs(path) and os.path.exists(path) else None
+
+
+def test_host_code_under_address_and_ub_sanitizers(tmp_path, golden):
+    asan = _runtime("libasan.so")
+    if asan is None:
+        pytest.skip("no AddressSanitizer runtime for g++ on this host")
+    lib = str(tmp_path / "libhost_asan.so")
+    cmd = ["g++", "-std=c++17", "-O1", "-g", "-shared", "-fPIC", "-fsanitize=address,undefined",
+           "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-DSWE_TILE_CELLS=256",
+           os.path.join(ROOT, "tests", "cpp", "tiles_check.cpp"),
+           os.path.join(ROOT, "swe_fvm_amd", "csrc", "swe_tiles.cpp"),
+           os.path.join(ROOT, "swe_fvm_amd", "csrc", "swe_mesh.cpp"), "-o", lib]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1:abort_on_error=0",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
+    run = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "host_asan_driver.py"), lib,
+                          golden.bowl_path(), golden.msh_path("basic.msh.gz"), str(tmp_path)],
+                         capture_output=True, text=True, env=env, timeout=300)
+    print(run.stdout)
+    assert "AddressSanitizer" not in run.stderr and "runtime error" not in run.stderr, run.stderr[-4000:]
+    assert run.returncode == 0, run.stderr[-4000:]
+    assert run.stdout.strip().endswith("ok")
