@@ -88,6 +88,8 @@ int build_topology(int64_t V, int64_t T, const int64_t* tri, int64_t nb, const i
     }
   }
   m.E = int64_t(m.ep.size() / 2);
+  for (int64_t e = 0; e < m.E; ++e)  // a boundary line is an edge of some triangle
+    if (m.et[2 * e] < 0) return fail(SWE_ERR_DOMAIN, "boundary line that is not an edge of any triangle");
   for (int64_t t = 0; t < T; ++t)
     for (int k = 0; k < 3; ++k) {
       int64_t e = m.te[3 * t + k];

@@ -138,4 +138,38 @@ for name, body in bad_files.items():
     rc = L.swe_topology_read_msh(path.encode(), ctypes.byref(xyz), ctypes.byref(nv), ctypes.byref(t))
     print(f"{name}: rc={rc}", L.swe_last_error().decode(), flush=True)
     assert rc != 0, name
+
+# seeded mutations of the small mesh file (a line deleted or duplicated, a
+# token replaced by another integer, a cut): each is read without a
+# sanitizer report; the accepted ones also go through the tile checker
+small = open(basic).read().split("\n")
+rng = np.random.default_rng(7)
+accepted = 0
+for it in range(300):
+    lines = list(small)
+    kind = it % 4
+    i = int(rng.integers(0, len(lines)))
+    if kind == 0:
+        del lines[i]
+    elif kind == 1:
+        lines.insert(i, lines[i])
+    elif kind == 2:
+        toks = lines[i].split()
+        if toks:
+            toks[int(rng.integers(0, len(toks)))] = str(int(rng.choice([-1, 0, 1, 2, 3, 7, 99, 2 ** 31, -2 ** 40])))
+            lines[i] = " ".join(toks)
+    body = "\n".join(lines)
+    if kind == 3:
+        body = body[:int(rng.integers(0, len(body)))]
+    path = os.path.join(tmp, "fuzz.msh")
+    with open(path, "w") as f:
+        f.write(body)
+    xyz, nv, t = P(), I64(), P()
+    rc = L.swe_topology_read_msh(path.encode(), ctypes.byref(xyz), ctypes.byref(nv), ctypes.byref(t))
+    if rc == 0:
+        accepted += 1
+        L.swe_free(xyz)
+        check_tiles(f"mutation {it}", t, refused_ok=True) if L.swe_topology_num_elements(t) > 0 else None
+        L.swe_topology_destroy(t)
+print(f"mutations: {accepted} of 300 accepted", flush=True)
 print("ok", flush=True)

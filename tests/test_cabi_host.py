@@ -204,3 +204,23 @@ def test_empty_and_out_of_range_tables_are_rejected_before_the_device():
     tri[0, 1] = tri[0, 0]
     with pytest.raises(ValueError):
         swe.Topology.build(V, tri, np.zeros((0, 2), np.int64))
+
+
+def test_boundary_line_off_the_triangulation_is_rejected(golden):
+    """A boundary line that is no triangle's edge would become an edge
+    without a cell: the builder refuses it; a line along an interior edge is
+    an ordinary interior edge numbered among the boundary edges; a mesh
+    file whose counts or end markers are wrong is an error, not a partial
+    mesh."""
+    tri = np.array([[0, 1, 2], [0, 2, 3]], np.int64)  # two triangles of a square
+    with pytest.raises(swe.DomainError, match="not an edge of any triangle"):
+        swe.Topology.build(4, tri, np.array([[0, 1], [1, 3]], np.int64))
+    t = swe.Topology.build(4, tri, np.array([[0, 2]], np.int64)).tables()
+    assert list(t["ep"][0]) == [0, 2] and t["et"][0, 1] >= 0  # interior, numbered first
+    text = open(golden.bowl_path()).read()
+    path = golden.bowl_path() + ".cut.msh"
+    with open(path, "w") as f:
+        f.write(text[: text.index("$EndElements")])
+    with pytest.raises(ValueError, match="truncated"):
+        swe.read_msh(path)
+    os.remove(path)

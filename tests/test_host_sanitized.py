@@ -3,8 +3,9 @@ the topology builder and checks, swe_mesh.cpp; the fused kernel's tile
 tables, swe_tiles.cpp) and the tile checker, built with g++ under
 AddressSanitizer and UBSan and driven by tests/host_asan_driver.py in a
 process of its own: generated, structured and file meshes, a refused cell
-order, and the error paths (indices out of range, bad periodic pairs,
-missing and truncated files) run without a sanitizer report."""
+order, the error paths (indices out of range, bad periodic pairs, missing,
+truncated and miscounted files) and 300 seeded mutations of a mesh file run
+without a sanitizer report."""
 import os
 import subprocess
 import sys
@@ -32,7 +33,7 @@ def test_host_code_under_address_and_ub_sanitizers(tmp_path, golden):
            os.path.join(ROOT, "swe_fvm_amd", "csrc", "swe_mesh.cpp"), "-o", lib]
     res = subprocess.run(cmd, capture_output=True, text=True)
     assert res.returncode == 0, res.stderr
-    env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1:abort_on_error=0",
+    env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1:abort_on_error=0:allocator_may_return_null=1",
                UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
     run = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "host_asan_driver.py"), lib,
                           golden.bowl_path(), golden.msh_path("basic.msh.gz"), str(tmp_path)],
