@@ -348,8 +348,10 @@ def adjoint_measure(B=32, n=354, N=2000, dt=2e-4, checkpoint=50, nfd=16, check_s
     loss = ((out[gidx, 0] - tgt) ** 2).sum()
     loss.backward()
     ms = sd.last_adjoint_ms
+    split = sd.adjoint_timing()  # the reverse sweep's recomputation and reverse stages
     gp, gz = p.grad.numpy(), zt.grad.numpy()
-    print(f"[bench] forward {ms_fwd:.1f} ms, adjoint sweeps {ms[0]:.1f} + {ms[1]:.1f} ms",
+    print(f"[bench] forward {ms_fwd:.1f} ms, adjoint sweeps {ms[0]:.1f} + {ms[1]:.1f} ms "
+          f"(recomputation {split['recompute_ms']:.1f}, reverse stages {split['reverse_stages_ms']:.1f})",
           file=sys.stderr, flush=True)
     # member 0 alone: its slice of the batch, bit for bit
     sd1 = swe.SpaceDisc(fl, dom1, prim1)
@@ -411,6 +413,11 @@ def adjoint_measure(B=32, n=354, N=2000, dt=2e-4, checkpoint=50, nfd=16, check_s
         "value": ms[1] / ms_fwd, "unit": "ratio", "higher_is_better": False,
         "gradient_cost_ratio": (ms[0] + ms[1]) / ms_fwd,
         "forward_ms": ms_fwd, "adjoint_forward_sweep_ms": ms[0], "adjoint_reverse_sweep_ms": ms[1],
+        # the reverse sweep = recomputation of the segments from their
+        # checkpoints (forward work) + the reverse stages proper
+        "reverse_sweep_recompute_ms": split["recompute_ms"],
+        "reverse_sweep_reverse_stages_ms": split["reverse_stages_ms"],
+        "reverse_stages_per_forward": split["reverse_stages_ms"] / ms_fwd,
         "forward_cell_updates_per_s": B * T1 * N / (ms_fwd * 1e-3),
         "loss": float(loss.item()), "member0_slice_bitwise": slice_same,
         "tangent_duality": tangent, "fd_full_horizon": fd_full, "fd_short_horizon": fd_short,

@@ -327,6 +327,13 @@ int swe_state_stats(swe_solver* s, double* out5);
 int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int64_t nsteps,
                     int64_t checkpoint_every, const double* final_bar, double* state0_bar,
                     double* z_bar, double* ms);
+/* Timing of the last swe_adjoint_run on this handle (no reference
+ * counterpart): out3 = milliseconds of the forward sweep, of the reverse
+ * sweep's recomputation of its segments from their checkpoints (forward
+ * work that checkpointing costs), and of the rest of the reverse sweep
+ * (the reverse stages; the two last add up to swe_adjoint_run's ms[1]);
+ * zeros before the first run. */
+int swe_adjoint_timing(const swe_solver* s, double* out3);
 /* Tangent-linear run (no reference counterpart): the directional
  * derivative of nsteps steps of fixed sizes (dt > 0, or dts[nsteps]) along
  * state0_dot (T x 3 row-major tangent of the initial w, u, v) and z_dot (V

@@ -69,6 +69,7 @@ SIGNATURES = [
     ("swe_get_muscl", _int, [_p, _p, _p, _p]),
     ("swe_state_stats", _int, [_p, _p]),
     ("swe_adjoint_run", _int, [_p, _int, _d, _p, _i64, _i64, _p, _p, _p, _p]),
+    ("swe_adjoint_timing", _int, [_p, _p]),
     ("swe_tangent_run", _int, [_p, _int, _d, _p, _i64, _p, _p, _p, _p]),
     ("swe_set_bathymetry", _int, [_p, _p]),
     ("swe_set_tuning", _int, [_p, _int, _int]),

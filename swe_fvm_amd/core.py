@@ -530,6 +530,15 @@ class SpaceDisc:
                                            _ptr(zb) if want_z else None, _ptr(ms)))
         return s0, zb, ms
 
+    def adjoint_timing(self):
+        """Milliseconds of the last adjoint_run: forward sweep, the reverse
+        sweep's recomputation of its segments from their checkpoints, and
+        the rest of the reverse sweep (the reverse stages)."""
+        out = np.zeros(3, np.float64)
+        _check(_lib.load().swe_adjoint_timing(self._h, _ptr(out)))
+        return {"forward_sweep_ms": float(out[0]), "recompute_ms": float(out[1]),
+                "reverse_stages_ms": float(out[2])}
+
     def tangent_run(self, state0_dot, nsteps, z_dot=None, scheme="SSPRK2", dt=None, dts=None):
         """Runs nsteps steps from the current state with fixed step sizes,
         carrying the tangent (state0_dot: T x 3, z_dot: V or None) through

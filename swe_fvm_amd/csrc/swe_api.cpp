@@ -42,6 +42,9 @@ struct swe_solver {
   double* dts = nullptr;
   int64_t dts_cap = 0;
   int64_t last_nsteps = 0;
+  // the last swe_adjoint_run: forward sweep, the reverse sweep's
+  // recomputation, the rest of the reverse sweep (milliseconds)
+  double adj_ms[3] = {0., 0., 0.};
   // batched ensemble (n_inst > 0)
   int n_inst = 0;
   DevBatch B{};
@@ -1564,10 +1567,13 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
   double* ob = bar_a;
   double* sb = bar_b;
   const double* wpend = nullptr;
+  SweepTimer rt;  // each segment's recomputation, read after its synchronisation
+  double recompute_ms = 0.;
   for (int64_t g = nck - 1; g >= 0 && !rc; --g) {
     const int64_t n0 = g * K, n1 = std::min(nsteps, n0 + K);
     // recompute the segment keeping step inputs, stage outputs and every
     // stage's intermediates; the last segment also produces U_N
+    rt.mark(0, s->stream);
     e = copy_state(st3(Buf3{ck + g * 3 * T}, T), out3(Buf3{seg}, T), T, s->stream);
     // the slots' pass-2 counters, zeroed in one launch
     const int nslot = int((n1 - n0) * ns);
@@ -1608,6 +1614,7 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
         Sq = DevState{O.w, O.u, O.v, O.mask};
       }
     }
+    rt.mark(1, s->stream);
     // the lengths of the segment's pass-2 lists, for the partially wet
     // kernels of the reverse stages (one synchronisation per segment)
     if (e == hipSuccess) e = launch_slot_counts(slot_work(0).pw_count, slot_bytes, nslot, cnt, s->stream);
@@ -1615,6 +1622,7 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
       e = hipMemcpyAsync(hcnt.data(), cnt, sizeof(unsigned int) * size_t(nslot), hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) { rc = dev_fail(e, "adjoint recomputation"); break; }
+    recompute_ms += rt.ms(0, 1);
     if (g == nck - 1) s->cur = (s->cur + 1) % 3;  // U_N is the current state
     // reverse the steps of the segment: ob holds the adjoint of the stage
     // output, its w part owing the neighbour terms wpend of the stage after
@@ -1652,10 +1660,19 @@ int swe_adjoint_run(swe_solver* s, int scheme, double dt, const double* dts, int
         ms[0] = tm.ms(0, 1);
         ms[1] = tm.ms(1, 2);
       }
+      s->adj_ms[0] = tm.ms(0, 1);
+      s->adj_ms[1] = recompute_ms;
+      s->adj_ms[2] = tm.ms(1, 2) - recompute_ms;
     }
   }
   release();
   return rc;
+}
+
+int swe_adjoint_timing(const swe_solver* s, double* out3) {
+  if (!s || !out3) return fail(SWE_ERR_ARG, "null argument");
+  for (int k = 0; k < 3; ++k) out3[k] = s->adj_ms[k];
+  return SWE_OK;
 }
 
 // Tangent-linear run (no reference counterpart): the directional

@@ -845,3 +845,19 @@ def test_adjoint_same_with_fused_recomputation(scheme, n):
           flush=True)
     assert Ja == Jb and np.array_equal(fa, fb)
     assert np.array_equal(sa, sb) and np.array_equal(za, zb_)
+
+
+def test_adjoint_timing_splits_the_reverse_sweep():
+    """swe_adjoint_timing: the forward sweep as adjoint_run reports it, and
+    the reverse sweep split into the segments' recomputation and the
+    reverse stages, both positive, adding up to the reverse sweep."""
+    xyz, topo, prim = C.wet_dry_mesh(20, seed=6)
+    sd = swe.SpaceDisc(swe.Fluxes.HLLC[swe.Wavespeeds.Einfeldt], swe.Domain(xyz, topo), prim)
+    assert sd.adjoint_timing() == {"forward_sweep_ms": 0.0, "recompute_ms": 0.0, "reverse_stages_ms": 0.0}
+    bar = np.random.default_rng(2).standard_normal(prim.shape)
+    _, _, ms = sd.adjoint_run(bar, 9, dt=1e-3, checkpoint_every=4)
+    t = sd.adjoint_timing()
+    print("adjoint timing:", ms, t, flush=True)
+    assert t["forward_sweep_ms"] == ms[0]
+    assert t["recompute_ms"] > 0 and t["reverse_stages_ms"] > 0
+    assert abs(t["recompute_ms"] + t["reverse_stages_ms"] - ms[1]) <= 1e-9 * ms[1]
