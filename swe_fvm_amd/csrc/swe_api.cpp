@@ -68,7 +68,8 @@ struct swe_solver {
 namespace {
 
 // the reverse sweep recomputes a stage with the storing fused kernel below
-// this size, with the separate flux / draining / update kernels above
+// this size, with the separate flux kernel and then the tile kernel reading
+// its fluxes (storing the draining steps) above
 constexpr int64_t kFusedRecomputeMaxCells = int64_t(1) << 20;
 
 // entry points of the single-mesh step (one CFL slot per stage) refuse

@@ -75,6 +75,11 @@ def test_fused_head_options_and_reordered_file_mesh(golden):
         a.run("SSPRK3", 6, dt=2e-3)
         b.run("SSPRK3", 6, dt=2e-3)
         assert np.array_equal(a.GetVolField(), b.GetVolField()), opts
+        # CFL steps: the first stage's flux pass and tile kernel reading it
+        a.run("SSPRK3", 4, dt=0.0, cfl=0.15)
+        b.run("SSPRK3", 4, dt=0.0, cfl=0.15)
+        assert np.array_equal(a.GetVolField(), b.GetVolField()), opts
+        assert np.array_equal(a.dts(4), b.dts(4)), opts
 
 
 def test_fused_state_after_interface_getters():

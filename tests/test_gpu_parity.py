@@ -196,8 +196,9 @@ def test_large_properties_determinism_mass_positivity():
 @pytest.mark.parametrize("flux,wave", [("HLL", "Einfeldt"), ("HLLC", "Einfeldt"), ("HLLC", "Davis")])
 def test_exact_balance_matches_oracle(scheme, flux, wave):
     """SWE_OPT_EXACT_BALANCE on a wet/dry mesh with Coriolis: CFL steps (the
-    separate flux / draining / update kernels for the first stage) and
-    fixed-dt steps (the fused tile kernel) against the oracle, bit for bit."""
+    flux kernel, then the tile kernel reading its fluxes, for the first
+    stage) and fixed-dt steps (the fused tile kernel) against the oracle,
+    bit for bit."""
     xyz, topo, prim = C.wet_dry_mesh(24, seed=11)
     tp = topo.tables()["tp"]
     for dt in (0.0, 2e-3):
