@@ -162,45 +162,45 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
       }
     }
   } else {
-  // the next evaluation's edge record is loaded before this one's gathers
-  EdgeRec rn = tid < ne ? P.evrec[e0 + tid] : EdgeRec{0, 0, 0, 0};
-  for (int j = tid; j < ne; j += kFusedThreads) {
-    const EdgeRec r = rn;
-    if (j + kFusedThreads < ne) rn = P.evrec[e0 + j + kFusedThreads];
-    const int f = r.et0, t = r.et1;
-    const int tr = t < 0 ? f : t;  // boundary edges read their own cell twice: R = L
-    const V3 Pa = node(M, r.ep0), Pb = node(M, r.ep1);
-    const Muscl mf = load_muscl(S, W, f), mt = load_muscl(S, W, tr);
-    const CXY cf = M.cxy[f], ct = M.cxy[tr];
-    const double cxf = cf.x, cyf = cf.y, cxt = ct.x, cyt = ct.y;
-    const double hf = (t == kSolidWall && !M.eb) ? S.w[S.at(f)] - M.bc[f] : 0.;
-    const double le = edge_len(Pa, Pb);
-    const V3 E = midpoint(Pa, Pb);
-    double tx, ty;
-    tangent(Pa, Pb, le, V3{cxf, cyf, 0.}, tx, ty);
-    const double nx = ty, ny = -tx;
-    double F[3];
-    double lw = 1.0;  // the CFL candidate is not taken here (unused)
-    if (t == kSolidWall) {
-      // SWE_OPT_EXACT_BALANCE: the inside edge depth instead of the cell mean
-      const double hw = M.eb ? slot_value(mf, cxf, cyf, E).w - E.z : hf;
-      elem_flux(nx, ny, hw, 0., 0., F[0], F[1], F[2]);
-    } else {
-      const Slot L = slot_value(mf, cxf, cyf, E);
-      const Slot R = slot_value(mt, cxt, cyt, E);
-      if (FLUX == 0) flux_hll<WS>(nx, ny, E.z, L, R, 1.0, A.cor, lw, F, M.eb != 0);
-      else flux_hllc<WS>(tx, ty, E.z, L, R, 1.0, A.cor, lw, F, M.eb != 0);
+    // the next evaluation's edge record is loaded before this one's gathers
+    EdgeRec rn = tid < ne ? P.evrec[e0 + tid] : EdgeRec{0, 0, 0, 0};
+    for (int j = tid; j < ne; j += kFusedThreads) {
+      const EdgeRec r = rn;
+      if (j + kFusedThreads < ne) rn = P.evrec[e0 + j + kFusedThreads];
+      const int f = r.et0, t = r.et1;
+      const int tr = t < 0 ? f : t;  // boundary edges read their own cell twice: R = L
+      const V3 Pa = node(M, r.ep0), Pb = node(M, r.ep1);
+      const Muscl mf = load_muscl(S, W, f), mt = load_muscl(S, W, tr);
+      const CXY cf = M.cxy[f], ct = M.cxy[tr];
+      const double cxf = cf.x, cyf = cf.y, cxt = ct.x, cyt = ct.y;
+      const double hf = (t == kSolidWall && !M.eb) ? S.w[S.at(f)] - M.bc[f] : 0.;
+      const double le = edge_len(Pa, Pb);
+      const V3 E = midpoint(Pa, Pb);
+      double tx, ty;
+      tangent(Pa, Pb, le, V3{cxf, cyf, 0.}, tx, ty);
+      const double nx = ty, ny = -tx;
+      double F[3];
+      double lw = 1.0;  // the CFL candidate is not taken here (unused)
+      if (t == kSolidWall) {
+        // SWE_OPT_EXACT_BALANCE: the inside edge depth instead of the cell mean
+        const double hw = M.eb ? slot_value(mf, cxf, cyf, E).w - E.z : hf;
+        elem_flux(nx, ny, hw, 0., 0., F[0], F[1], F[2]);
+      } else {
+        const Slot L = slot_value(mf, cxf, cyf, E);
+        const Slot R = slot_value(mt, cxt, cyt, E);
+        if (FLUX == 0) flux_hll<WS>(nx, ny, E.z, L, R, 1.0, A.cor, lw, F, M.eb != 0);
+        else flux_hllc<WS>(tx, ty, E.z, L, R, 1.0, A.cor, lw, F, M.eb != 0);
+      }
+      sF0[j] = F[0];
+      sF1[j] = F[1];
+      sF2[j] = F[2];
+      if (store && f >= c0 && f < c0 + nc) {  // the edge's one owning evaluation: its et0 cell is ours
+        const int e = P.evedge[e0 + j];
+        out_store<NT>(&W.f0[e], F[0]);
+        out_store<NT>(&W.f1[e], F[1]);
+        out_store<NT>(&W.f2[e], F[2]);
+      }
     }
-    sF0[j] = F[0];
-    sF1[j] = F[1];
-    sF2[j] = F[2];
-    if (store && f >= c0 && f < c0 + nc) {  // the edge's one owning evaluation: its et0 cell is ours
-      const int e = P.evedge[e0 + j];
-      out_store<NT>(&W.f0[e], F[0]);
-      out_store<NT>(&W.f1[e], F[1]);
-      out_store<NT>(&W.f2[e], F[2]);
-    }
-  }
   }
   __syncthreads();
 
