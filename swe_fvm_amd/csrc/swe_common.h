@@ -27,6 +27,15 @@ __device__ __forceinline__ int logical_block() {
   int b = blockIdx.x;
   return (b & 7) * per + (b >> 3);
 }
+// The same ranges, each walked from its far end.  A kernel launched after
+// one that walked them forward starts on the lines the other touched last,
+// which the Infinity Cache still holds (k_flux after the reconstruction,
+// the evaluating tile kernel after the second reconstruction).
+__device__ __forceinline__ int logical_block_reversed() {
+  int per = gridDim.x >> 3;
+  int b = blockIdx.x;
+  return (b & 7) * per + (per - 1 - (b >> 3));
+}
 constexpr int kBlock = 256;  // every launch uses 256-thread workgroups
 __device__ __forceinline__ int64_t global_index() {
   return (int64_t)logical_block() * kBlock + threadIdx.x;

@@ -58,7 +58,9 @@ __global__ void __launch_bounds__(kFusedThreads, SWE_FUSED_MIN_WAVES) k_fused(De
                                                       int store) {
   __shared__ double sF0[kTileMaxEval], sF1[kTileMaxEval], sF2[kTileMaxEval];
   __shared__ double sD[kTileMaxLocal];
-  const int64_t tile = t0 + logical_block();
+  // the evaluating instance walks the tiles backwards (it follows a
+  // forward reconstruction); the FF instance forwards (it follows k_flux)
+  const int64_t tile = t0 + (FF ? logical_block() : logical_block_reversed());
   if (tile >= t1) return;  // padding workgroups of the XCD-aware grid
   const int64_t c0 = tile * kTileCells;
   const int nc = (int)(M.T - c0 < kTileCells ? M.T - c0 : kTileCells);

@@ -391,7 +391,8 @@ struct FluxIn {
 template <int FLUX, int WS, int NI, bool CFL, int BS, bool NT>
 __global__ void __launch_bounds__(BS) k_flux(DevMesh M, DevState S, DevWork W, double cor,
                                               unsigned long long* rbits) {
-  const int64_t base = M.e0 + (int64_t)logical_block() * (BS * NI) + threadIdx.x;
+  // edges walked backwards: the reconstruction before walks forwards
+  const int64_t base = M.e0 + (int64_t)logical_block_reversed() * (BS * NI) + threadIdx.x;
   FluxIn in[NI];
 #pragma unroll
   for (int it = 0; it < NI; ++it) {
