@@ -458,7 +458,10 @@ __device__ inline void side_flux_bar(const DevMesh& M, const DevState& S, const 
 // reconstruction.
 template <int FLUX, int WS, bool LEN, bool NT>
 __global__ void __launch_bounds__(256) k_adj_flux(DevMesh M, DevState S, DevWork W, DevAdj J, double dts) {
-  const int64_t e64 = global_index();  // XCD-aware, as the forward kernels
+  // XCD-aware as the forward kernels, each range walked backwards: the
+  // update adjoint before it and the reconstruction adjoint after it walk
+  // forwards, so each kernel starts on the lines the previous touched last
+  const int64_t e64 = (int64_t)logical_block_reversed() * kBlock + threadIdx.x;
   if (e64 >= M.E) return;
   const int e = (int)e64;
   const int64_t Ecnt = M.E;
