@@ -125,7 +125,7 @@ def cpu_baseline(dom, prim, flux, wave, reps=3):
 
 
 def cpu_rows(fl):
-    """BASELINE.md's CPU rows: the oracle (1 thread) against the GPU on the
+    """The CPU-against-GPU rows: the oracle (1 thread) against the GPU on the
     Thacker configuration (40,000 cells, one period, CFL 0.5, in full) and
     on the 1M-cell lake at rest (3 steps sampled, HLLC<Einfeldt>)."""
     import swe_fvm_amd as swe
@@ -133,7 +133,7 @@ def cpu_rows(fl):
     from swe_fvm_amd import workloads as W
     from swe_fvm_amd.ensemble import Ensemble
     rows = []
-    # C1: the GPU runs to the period with per-step CFL dt (the last step
+    # Thacker, 40,000 cells: the GPU runs to the period with per-step CFL dt (the last step
     # clipped); the oracle replays the same steps
     dom, prim, th = W.thacker(n=100, quad=100)
     T = prim.shape[0]
@@ -166,7 +166,7 @@ def cpu_rows(fl):
                  "note": "GPU: one graph-captured ensemble step per step (launch-bound at 40k cells); CPU time "
                          "includes one extra interface-value and flux pass per step for the clipped-step replay"})
     del ens
-    # C2: 3 steps sampled on the CPU, 100 on the GPU
+    # lake at rest, 999,698 cells: 3 steps sampled on the CPU, 100 on the GPU
     dom, prim = W.lake_at_rest(n=707, seed=1)
     T = prim.shape[0]
     orc = O.OracleSolver(dom.geometry, dom.topology.tables(), "HLLC", "Einfeldt")
@@ -494,7 +494,7 @@ def main():
         print(json.dumps({"copy_bytes_per_launch": 2 * a.numel() * 8, "launches": 5}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus above 1 is not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
@@ -578,7 +578,7 @@ def main():
     if not args.no_extras:
         t0 = time.time()
         extras["ensemble"] = ensemble_measure(256, 20_000_000, args.ens_t_end, 10, host_stepped=False, tune_reps=2)
-        extras["ensemble"]["note"] = (f"config 4 shortened to t_end = {args.ens_t_end} (the full run to t = 1.0: "
+        extras["ensemble"]["note"] = (f"the 256-member ensemble run shortened to t_end = {args.ens_t_end} (the full run to t = 1.0: "
                                       f"python bench.py --ensemble)")
         gc.collect()
         print(f"[bench] ensemble {extras['ensemble']['value']:.2f}x in {time.time() - t0:.0f}s",
@@ -588,7 +588,7 @@ def main():
                               check_steps=50)
         for key in ("fd_full_horizon", "fd_short_horizon", "tangent_duality"):
             adj[key].pop("results", None)
-        adj["note"] = (f"config 5 shortened to {args.adj_short_steps} steps, checkpoints every 50 "
+        adj["note"] = (f"the 32-member inversion shortened to {args.adj_short_steps} steps, checkpoints every 50 "
                        f"(the full 2,000 steps: python bench.py --adjoint)")
         extras["adjoint"] = adj
         gc.collect()

@@ -1,5 +1,5 @@
 """bench.py's command-line contract on the host: more than one GPU is
-refused with a message and a nonzero status (this build runs on one GPU),
+refused with a message and a nonzero status (the benchmark runs on one GPU),
 and the options the measurement commands under profiles/ use exist."""
 import json
 import os
