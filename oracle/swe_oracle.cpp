@@ -1,7 +1,7 @@
 // ORACLE — test infrastructure only.
 //
 // A scalar C++17 CPU restatement of SWE_FVM's explicit finite-volume step
-// (no Eigen).  It is the checker that the HIP product path is compared
+// (no Eigen).  It is what the HIP product path's results are compared
 // against; it is imported only by tests/, __graft_entry__.smoke() and the
 // cpu_baseline leg of bench.py.  The product library never links it.
 //
