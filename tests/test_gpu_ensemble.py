@@ -127,3 +127,32 @@ def test_euler_graph_and_plain_runs_mixed():
         sd = swe.SpaceDisc(FL, dom, prim)
         sd.run("Euler", nsteps, dt=0.0, cfl=0.15)
         assert np.array_equal(got[k], sd.GetVolField()), k
+
+
+def test_full_size_ensemble_members_equal_single_runs():
+    """The benchmark's 256-member ensemble at full size (10k-340k cells per
+    member, 19,999,564 cells in all), 8 graph-captured SSP-RK2 steps with
+    per-member CFL step sizes: the smallest, a middle and the largest member
+    equal their own single-mesh runs bit for bit, and so do their times."""
+    import gc
+
+    print("[full size] building the 256 members", flush=True)
+    mem = W.ensemble(n_inst=256, total=20_000_000, seed=11)
+    assert sum(p.shape[0] for _, p in mem) == 19_999_564
+    ens = Ensemble(FL, [d for d, _ in mem], [p for _, p in mem])
+    nsteps = 8
+    done, act = ens.run("SSPRK2", cfl=0.15, t_end=1e9, max_steps=nsteps, graph=True)
+    assert done == nsteps and act == len(mem)
+    got = ens.states()
+    t, steps = ens.times()
+    assert np.all(steps == nsteps)
+    del ens
+    gc.collect()
+    print("[full size] batched run done", flush=True)
+    for k in (0, len(mem) // 2, len(mem) - 1):
+        dom, prim = mem[k]
+        sd = swe.SpaceDisc(FL, dom, prim)
+        sd.run("SSPRK2", nsteps, dt=0.0, cfl=0.15)
+        assert np.array_equal(got[k], sd.GetVolField()), k
+        assert t[k] == np.sum(sd.dts(nsteps)) or abs(t[k] - np.sum(sd.dts(nsteps))) <= 1e-15 * t[k]
+        del sd
